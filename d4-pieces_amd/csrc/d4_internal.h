@@ -1,0 +1,104 @@
+// d4_internal.h -- shared declarations of the host engine and the gfx950
+// kernels of the row-by-row tiling search.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "d4.h"
+
+namespace d4 {
+
+constexpr uint64_t EMPTY_KEY = ~0ull;
+constexpr int MAX_CAND = 64;     // (piece, transform) pairs: one u64 mask
+constexpr int MAX_PIECES = 32;
+constexpr int MAX_FIT_ROWS = 6;  // tallest piece
+constexpr int MAX_ROWS = 64;     // board height
+
+// Counter slots of the per-step device counter block.
+enum : int {
+    C32_WORK = 0,      // dynamic record fetch (extend / next_size)
+    C32_OVERFLOW = 1,  // hash probe limit hit
+    C32_EDGES = 2,     // edges written (enum mode)
+    C32_EDGE_OVF = 3,  // edge buffer overflow
+    C32_OUT = 4,       // compaction output cursor
+    C32_N = 8,
+};
+enum : int {
+    C64_CHILDREN = 0,
+    C64_NEW = 1,
+    C64_PRUNED = 2,
+    C64_NEXTSIZE = 3,
+    C64_N = 4,
+};
+
+// Arguments of the extend kernel (passed by value).
+struct ExtendArgs {
+    const uint64_t *in_key;
+    const uint64_t *in_cnt;
+    uint32_t n_in;
+    uint32_t row;
+    // library tables (global copies; staged into LDS)
+    const uint64_t *fit;     // [8][fit_rows][256] candidate masks
+    const uint64_t *place;   // [8][64] placed shape per anchor column
+    const uint64_t *range;   // [64] mask of the candidate's piece
+    const uint32_t *cinfo;   // [64] counter offset | field mask << 8 | count << 16
+    const uint32_t *pinfo;   // [n_pieces] same packing per piece
+    const uint64_t *prange;  // [n_pieces]
+    uint32_t fit_rows;
+    uint32_t n_pieces;
+    // instances
+    const uint64_t *bw;      // [n_inst][bw_stride] board window of row r
+    uint32_t bw_stride;
+    const uint32_t *hgt;     // [n_inst]
+    unsigned long long *result;  // [n_inst] terminal counts
+    // key layout
+    uint32_t occ_bits;
+    uint32_t inst_shift;     // 64 when there is a single instance
+    uint64_t occ_mask;
+    uint32_t used_mask;
+    // dedup table
+    unsigned long long *tab_key;
+    unsigned long long *tab_cnt;
+    uint64_t tab_mask;
+    uint32_t max_probe;
+    // counters
+    uint32_t *c32;
+    unsigned long long *c64;
+    // enum-mode edges
+    uint32_t *e_parent;
+    uint64_t *e_desc;
+    uint64_t *e_slot;
+    uint32_t e_cap;
+    // pruning
+    uint32_t gcd;
+    uint32_t min_size;
+};
+
+// Launchers (d4_kernels.hip).
+void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
+                       uint64_t n, hipStream_t st);
+void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
+                           hipStream_t st);
+void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
+                   int n_cu, hipStream_t st);
+void launch_compact(const unsigned long long *tab_key,
+                    const unsigned long long *tab_cnt, uint64_t m,
+                    uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
+                    uint32_t *slot_uid, int n_cu, hipStream_t st);
+void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
+                        uint32_t *e_child, uint32_t n, hipStream_t st);
+void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
+                      const uint64_t *bw, uint32_t bw_stride,
+                      const uint32_t *hgt, uint32_t inst_shift,
+                      uint64_t occ_mask, const unsigned long long *comp,
+                      uint32_t *c32, unsigned long long *c64, int n_cu,
+                      hipStream_t st);
+size_t extend_lds_bytes(uint32_t fit_rows);
+
+// Error plumbing (d4_api.cpp).
+void set_error(const std::string &msg);
+
+}  // namespace d4

@@ -1,0 +1,451 @@
+// d4_kernels.hip -- gfx950 kernels of the row-by-row polyomino tiling search.
+//
+// One row step turns the frontier of row r (states: occupancy of the rows
+// below r by pieces already placed + per-piece use counters, with a u64
+// tiling count) into the frontier of row r+1:
+//
+//   k_extend  each lane takes one parent record and enumerates, depth first,
+//             every way to fill the remaining open cells of row r with
+//             pieces anchored at the lowest open cell (the reference CPU
+//             searcher's order, src/Piece.cpp:82-98).  Candidates at an
+//             anchor come from byte-sliced fit tables staged in LDS:
+//             fit[c][j][byte] = mask of (piece, transform) pairs anchored at
+//             column c whose row-j cells lie inside `byte`, so one AND per
+//             window row (the first fit_rows rows) leaves the placements
+//             that fit there; a full-shape test on the chosen one covers the
+//             rows below.  Every
+//             completed row is a child: it is checked for enclosed regions
+//             that no piece set can fill (connectivity pruning) and merged
+//             into an open-addressing hash table keyed by the packed state,
+//             summing counts (dedup).  In enum mode each child also writes a
+//             back-pointer edge, slots claimed by wave ballot + prefix count
+//             with one atomic per wave.
+//   k_compact scans the hash table and writes the unique states densely
+//             (wave ballot + mbcnt, one atomic per wave).
+//
+// This replaces the reference's tiled_row_search / impl<H> / host Sorter
+// pipeline (src/kernel.cuh:12-160, src/sorter.cpp:156-214): there the L x R
+// half-row cross product is tested per record, every child takes one
+// atomicAdd, and dedup runs on the host over PCIe.
+#include "d4_internal.h"
+
+namespace d4 {
+
+namespace {
+
+constexpr int EXT_THREADS = 1024;
+constexpr int CMP_THREADS = 256;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Wave-aggregated slot allocation: every active lane calls it; lanes with
+// pred get consecutive indices from *ctr with one atomic for the wave.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t *ctr, bool pred) {
+    uint64_t m = __ballot(pred);
+    uint64_t act = __ballot(1);
+    uint32_t leader = (uint32_t)__builtin_ctzll(act);
+    uint32_t base = 0;
+    if (lane_id() == leader && m)
+        base = atomicAdd(ctr, (uint32_t)__builtin_popcountll(m));
+    base = __shfl(base, (int)leader);
+    return base + mbcnt(m);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t lo = __shfl_xor((uint32_t)v, o);
+        uint32_t hi = __shfl_xor((uint32_t)(v >> 32), o);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 31;
+    x *= 0x7fb5d329728ea185ull;
+    x ^= x >> 27;
+    x *= 0x81dadef4bc2dd44dull;
+    x ^= x >> 33;
+    return x;
+}
+
+// Insert (key, cnt) into the table: returns 1 when the key is new, 0 when it
+// merged into an existing slot, -1 when the probe limit was reached.  Keys
+// only ever change EMPTY -> key, so a stale plain read is harmless: EMPTY
+// leads to the CAS, which returns the true value.
+__device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
+                                            uint64_t cnt, uint64_t &slot) {
+    uint64_t h = mix64(key) & a.tab_mask;
+    for (uint32_t p = 0; p < a.max_probe; p++) {
+        uint64_t cur = __hip_atomic_load(a.tab_key + h, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == EMPTY_KEY) {
+            cur = atomicCAS(a.tab_key + h, (unsigned long long)EMPTY_KEY,
+                            (unsigned long long)key);
+            if (cur == EMPTY_KEY) {
+                atomicAdd(a.tab_cnt + h, (unsigned long long)cnt);
+                slot = h;
+                return 1;
+            }
+        }
+        if (cur == key) {
+            atomicAdd(a.tab_cnt + h, (unsigned long long)cnt);
+            slot = h;
+            return 0;
+        }
+        h = (h + 1) & a.tab_mask;
+    }
+    return -1;
+}
+
+constexpr uint64_t COL0 = 0x0101010101010101ull;
+constexpr uint64_t COL7 = 0x8080808080808080ull;
+
+__device__ __forceinline__ uint64_t grow4(uint64_t x) {
+    return x | (x << 8) | (x >> 8) | ((x << 1) & ~COL0) | ((x >> 1) & ~COL7);
+}
+
+// Connectivity pruning of a child's open window E (8 rows, the last of which
+// continues into rows not in the window).  Every 4-connected open region
+// that does not reach the last row is closed: its area must be a multiple of
+// the gcd of the piece sizes and at least the smallest piece, or no tiling
+// can complete it.  (Shape::extend1 / connected / sml_island semantics,
+// src/Shape.cpp:81-123, applied to the live board at run time instead of the
+// reference's static table-time island test, src/frow.cpp:78-100.)
+__device__ __forceinline__ bool viable(uint64_t E, uint32_t g,
+                                       uint32_t min_size) {
+    uint64_t x = E & 0xff00000000000000ull;
+    for (;;) {
+        uint64_t y = grow4(x) & E;
+        if (y == x) break;
+        x = y;
+    }
+    uint64_t closed = E & ~x;
+    while (closed) {
+        uint64_t comp = closed & (~closed + 1ull);
+        for (;;) {
+            uint64_t y = grow4(comp) & closed;
+            if (y == comp) break;
+            comp = y;
+        }
+        uint32_t area = (uint32_t)__builtin_popcountll(comp);
+        if (area < min_size || area % g) return false;
+        closed &= ~comp;
+    }
+    return true;
+}
+
+__device__ __forceinline__ uint64_t fitmask(const uint64_t *s_fit,
+                                            uint32_t rows, uint32_t c,
+                                            uint64_t E) {
+    const uint64_t *f = s_fit + (size_t)c * rows * 256u;
+    uint64_t m = f[E & 0xffu];
+    for (uint32_t j = 1; j < rows; j++)
+        m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
+    return m;
+}
+
+template <bool ENUM, bool PRUNE>
+__global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
+    extern __shared__ uint64_t lds[];
+    const uint32_t nfit = 8u * a.fit_rows * 256u;
+    uint64_t *s_fit = lds;
+    uint64_t *s_place = s_fit + nfit;
+    uint64_t *s_range = s_place + 8 * MAX_CAND;
+    uint64_t *s_prange = s_range + MAX_CAND;
+    uint32_t *s_cinfo = reinterpret_cast<uint32_t *>(s_prange + MAX_PIECES);
+    uint32_t *s_pinfo = s_cinfo + MAX_CAND;
+    for (uint32_t i = threadIdx.x; i < nfit; i += blockDim.x) s_fit[i] = a.fit[i];
+    for (uint32_t i = threadIdx.x; i < 8 * MAX_CAND; i += blockDim.x)
+        s_place[i] = a.place[i];
+    if (threadIdx.x < MAX_CAND) {
+        s_range[threadIdx.x] = a.range[threadIdx.x];
+        s_cinfo[threadIdx.x] = a.cinfo[threadIdx.x];
+    }
+    if (threadIdx.x < MAX_PIECES) {
+        s_prange[threadIdx.x] = threadIdx.x < a.n_pieces ? a.prange[threadIdx.x] : 0;
+        s_pinfo[threadIdx.x] = threadIdx.x < a.n_pieces ? a.pinfo[threadIdx.x] : 0;
+    }
+    __syncthreads();
+
+    const uint32_t lane = lane_id();
+    const uint32_t rows = a.fit_rows;
+    uint64_t n_children = 0, n_new = 0, n_pruned = 0;
+
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(a.c32 + C32_WORK, 64u);
+        base = __shfl(base, 0);
+        if (base >= a.n_in) break;
+        const uint32_t idx = base + lane;
+        bool live = idx < a.n_in;
+        uint64_t key = live ? a.in_key[idx] : 0ull;
+        uint64_t cnt = live ? a.in_cnt[idx] : 0ull;
+        uint32_t inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
+        uint64_t occ = key & a.occ_mask;
+        uint32_t used = (uint32_t)(key >> a.occ_bits) & a.used_mask;
+        if (live && a.row >= a.hgt[inst]) {
+            // terminal state of a finished instance: every cell is covered
+            atomicAdd(a.result + inst, (unsigned long long)cnt);
+            live = false;
+        }
+        uint64_t BW = 0, BWn = 0;
+        if (live) {
+            BW = a.bw[(size_t)inst * a.bw_stride + a.row];
+            BWn = a.bw[(size_t)inst * a.bw_stride + a.row + 1];
+        }
+        uint64_t E = BW & ~occ;
+        uint64_t avail = 0;
+        if (live)
+            for (uint32_t p = 0; p < a.n_pieces; p++) {
+                uint32_t pi = s_pinfo[p];
+                uint32_t f = (used >> (pi & 0xffu)) & ((pi >> 8) & 0xffu);
+                if (f < (pi >> 16)) avail |= s_prange[p];
+            }
+
+        // DFS over the completions of row r.  Stack: desc holds k+1 per
+        // level (one byte each), anc the anchor column (3 bits each).
+        uint32_t d = 0, c = 0, anc = 0;
+        uint64_t desc = 0, ok = 0;
+        bool searching = false, pass = false;
+        if (live) {
+            if (E & 0xffu) {
+                c = (uint32_t)__builtin_ctzll(E & 0xffu);
+                ok = fitmask(s_fit, rows, c, E) & avail;
+                searching = true;
+            } else {
+                pass = true;  // row already covered by pieces from above
+            }
+        }
+
+        while (__any(searching || pass)) {
+            bool emit = false;
+            uint64_t E2 = 0, desc2 = 0;
+            uint32_t used2 = 0;
+            if (pass) {
+                emit = true;
+                E2 = E;
+                used2 = used;
+                pass = false;
+            } else if (searching) {
+                if (ok) {
+                    uint32_t k = (uint32_t)__builtin_ctzll(ok);
+                    ok &= ok - 1;
+                    uint64_t pl = s_place[c * MAX_CAND + k];
+                    uint32_t ci = s_cinfo[k];
+                    E2 = E & ~pl;
+                    uint32_t inc = 1u << (ci & 0xffu);
+                    if (pl & ~E) {
+                        // fails in a window row the fit table does not cover
+                    } else if (!(E2 & 0xffu)) {
+                        emit = true;
+                        used2 = used + inc;
+                        desc2 = desc | ((uint64_t)(k + 1) << (8 * d));
+                    } else {
+                        E = E2;
+                        used += inc;
+                        if (((used >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
+                            avail &= ~s_range[k];
+                        desc |= (uint64_t)(k + 1) << (8 * d);
+                        anc |= c << (3 * d);
+                        d++;
+                        c = (uint32_t)__builtin_ctzll(E & 0xffu);
+                        ok = fitmask(s_fit, rows, c, E) & avail;
+                    }
+                } else if (d == 0) {
+                    searching = false;
+                } else {
+                    d--;
+                    uint32_t k = (uint32_t)((desc >> (8 * d)) & 0xffu) - 1u;
+                    c = (anc >> (3 * d)) & 7u;
+                    desc &= ~(0xffull << (8 * d));
+                    anc &= ~(7u << (3 * d));
+                    E |= s_place[c * MAX_CAND + k];
+                    used -= 1u << (s_cinfo[k] & 0xffu);
+                    avail |= s_range[k];
+                    ok = fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1);
+                }
+            }
+            bool keep = emit;
+            if (emit && PRUNE) {
+                uint64_t Ec = (E2 >> 8) | (BWn & 0xff00000000000000ull);
+                keep = viable(Ec, a.gcd, a.min_size);
+                n_pruned += keep ? 0 : 1;
+            }
+            uint64_t slot = 0;
+            if (keep) {
+                n_children++;
+                uint64_t occ2 = (BW & ~E2) >> 8;
+                uint64_t key2 = occ2 | ((uint64_t)used2 << a.occ_bits);
+                if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
+                int r = table_insert(a, key2, cnt, slot);
+                if (r < 0) {
+                    atomicOr(a.c32 + C32_OVERFLOW, 1u);
+                    keep = false;
+                } else {
+                    n_new += (uint64_t)r;
+                }
+            }
+            if (ENUM) {
+                uint32_t e = wave_alloc(a.c32 + C32_EDGES, keep);
+                if (keep) {
+                    if (e < a.e_cap) {
+                        a.e_parent[e] = idx;
+                        a.e_desc[e] = desc2;
+                        a.e_slot[e] = slot;
+                    } else {
+                        atomicOr(a.c32 + C32_EDGE_OVF, 1u);
+                    }
+                }
+            }
+        }
+    }
+
+    n_children = wave_sum_u64(n_children);
+    n_new = wave_sum_u64(n_new);
+    n_pruned = wave_sum_u64(n_pruned);
+    if (lane == 0) {
+        if (n_children) atomicAdd(a.c64 + C64_CHILDREN, (unsigned long long)n_children);
+        if (n_new) atomicAdd(a.c64 + C64_NEW, (unsigned long long)n_new);
+        if (n_pruned) atomicAdd(a.c64 + C64_PRUNED, (unsigned long long)n_pruned);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_table(unsigned long long *key,
+                                                    unsigned long long *cnt,
+                                                    uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        key[i] = EMPTY_KEY;
+        cnt[i] = 0;
+    }
+}
+
+__global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
+    if (threadIdx.x < C32_N) c32[threadIdx.x] = 0;
+    if (threadIdx.x < C64_N) c64[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(CMP_THREADS) void k_compact(
+    const unsigned long long *tab_key, const unsigned long long *tab_cnt,
+    uint64_t m, uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
+    uint32_t *slot_uid) {
+    const uint64_t waves = (uint64_t)gridDim.x * (CMP_THREADS / 64);
+    const uint64_t w = blockIdx.x * (uint64_t)(CMP_THREADS / 64) + (threadIdx.x >> 6);
+    for (uint64_t b = w * 64; b < m; b += waves * 64) {
+        uint64_t i = b + lane_id();
+        uint64_t k = i < m ? tab_key[i] : EMPTY_KEY;
+        bool valid = k != EMPTY_KEY;
+        uint32_t o = wave_alloc(c32 + C32_OUT, valid);
+        if (valid) {
+            out_key[o] = k;
+            out_cnt[o] = tab_cnt[i];
+            if (slot_uid) slot_uid[i] = o;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_remap_edges(const uint64_t *e_slot,
+                                                     const uint32_t *slot_uid,
+                                                     uint32_t *e_child,
+                                                     uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x)
+        e_child[i] = slot_uid[e_slot[i]];
+}
+
+__global__ __launch_bounds__(256) void k_next_size(
+    const uint64_t *key, uint32_t n, uint32_t row, const uint64_t *bw,
+    uint32_t bw_stride, const uint32_t *hgt, uint32_t inst_shift,
+    uint64_t occ_mask, const unsigned long long *comp, uint32_t *c32,
+    unsigned long long *c64) {
+    uint64_t acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x) {
+        uint64_t k = key[i];
+        uint32_t inst = inst_shift < 64 ? (uint32_t)(k >> inst_shift) : 0u;
+        if (row >= hgt[inst]) continue;
+        uint64_t E = bw[(size_t)inst * bw_stride + row] & ~(k & occ_mask);
+        acc += comp[E & 0xffu];
+    }
+    acc = wave_sum_u64(acc);
+    if (lane_id() == 0 && acc) atomicAdd(c64 + C64_NEXTSIZE, (unsigned long long)acc);
+}
+
+}  // namespace
+
+size_t extend_lds_bytes(uint32_t fit_rows) {
+    return sizeof(uint64_t) * (8u * fit_rows * 256u + 8u * MAX_CAND + MAX_CAND + MAX_PIECES) +
+           sizeof(uint32_t) * (MAX_CAND + MAX_PIECES);
+}
+
+void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
+                       uint64_t n, hipStream_t st) {
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (!blocks) blocks = 1;
+    hipLaunchKernelGGL(k_fill_table, dim3((uint32_t)blocks), dim3(256), 0, st, key, cnt, n);
+}
+
+void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c32, c64);
+}
+
+void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, int n_cu,
+                   hipStream_t st) {
+    size_t lds = extend_lds_bytes(a.fit_rows);
+    uint64_t waves = ((uint64_t)a.n_in + 63) / 64;
+    uint64_t blocks = (waves + EXT_THREADS / 64 - 1) / (EXT_THREADS / 64);
+    uint64_t cap = (uint64_t)n_cu * (lds <= 80 * 1024 ? 2 : 1);
+    if (blocks > cap) blocks = cap;
+    if (!blocks) blocks = 1;
+    dim3 g((uint32_t)blocks), b(EXT_THREADS);
+    if (enum_mode) {
+        if (prune) hipLaunchKernelGGL((k_extend<true, true>), g, b, lds, st, a);
+        else hipLaunchKernelGGL((k_extend<true, false>), g, b, lds, st, a);
+    } else {
+        if (prune) hipLaunchKernelGGL((k_extend<false, true>), g, b, lds, st, a);
+        else hipLaunchKernelGGL((k_extend<false, false>), g, b, lds, st, a);
+    }
+}
+
+void launch_compact(const unsigned long long *tab_key,
+                    const unsigned long long *tab_cnt, uint64_t m,
+                    uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
+                    uint32_t *slot_uid, int n_cu, hipStream_t st) {
+    uint64_t blocks = (m + CMP_THREADS - 1) / CMP_THREADS;
+    uint64_t cap = (uint64_t)n_cu * 8;
+    if (blocks > cap) blocks = cap;
+    if (!blocks) blocks = 1;
+    hipLaunchKernelGGL(k_compact, dim3((uint32_t)blocks), dim3(CMP_THREADS), 0, st,
+                       tab_key, tab_cnt, m, out_key, out_cnt, c32, slot_uid);
+}
+
+void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
+                        uint32_t *e_child, uint32_t n, hipStream_t st) {
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (!blocks) blocks = 1;
+    hipLaunchKernelGGL(k_remap_edges, dim3(blocks), dim3(256), 0, st, e_slot, slot_uid,
+                       e_child, n);
+}
+
+void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
+                      const uint64_t *bw, uint32_t bw_stride,
+                      const uint32_t *hgt, uint32_t inst_shift,
+                      uint64_t occ_mask, const unsigned long long *comp,
+                      uint32_t *c32, unsigned long long *c64, int n_cu,
+                      hipStream_t st) {
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > (uint32_t)n_cu * 4) blocks = (uint32_t)n_cu * 4;
+    if (!blocks) blocks = 1;
+    hipLaunchKernelGGL(k_next_size, dim3(blocks), dim3(256), 0, st, key, n, row, bw,
+                       bw_stride, hgt, inst_shift, occ_mask, comp, c32, c64);
+}
+
+}  // namespace d4

@@ -1,0 +1,292 @@
+"""Python host mirror of the d4 C ABI (include/d4.h).
+
+The product is libd4.so (gfx950 HIP kernels + C++ host engine).  This module
+is a thin ctypes layer over it with the reference's names, so that tests and
+benchmarks read like the reference's own driver code:
+
+  CudaSearcher     <- struct CudaSearcher       src/searcher_cuda.h:10-38
+                      (get_height, size, next_size, search_GPU)
+  PieceLibrary     <- g_nme/g_sym + compute_frow_on_cpu + transfer_frow_to_gpu
+                      src/frow.h:8-12, src/frow.cpp:39-166
+  free_polyominoes <- known_shapes[n]           src/known1.cpp:3-14
+  Batch            <- SearcherFactory::run over many board configs,
+                      src/searcher.cpp:86-107, as one shared device frontier
+
+There is no fallback: if libd4.so is missing or no HIP device is visible the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence, Tuple
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libd4.so")
+
+D4_OK, D4_EINVAL, D4_ENOMEM, D4_EDEVICE, D4_ESTATE, D4_ERANGE = 0, -1, -2, -3, -4, -5
+MODE_COUNT, MODE_ENUM = 0, 1
+FLAG_NO_PRUNE = 1
+
+
+class D4Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"d4 error {code}: {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("rows_done", ctypes.c_uint64),
+        ("records_extended", ctypes.c_uint64),
+        ("children", ctypes.c_uint64),
+        ("unique", ctypes.c_uint64),
+        ("pruned", ctypes.c_uint64),
+        ("peak_frontier", ctypes.c_uint64),
+        ("retries", ctypes.c_uint64),
+        ("gpu_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+# (name, restype, argtypes) of every symbol include/d4.h declares
+_P = ctypes.c_void_p
+_U8P = ctypes.POINTER(ctypes.c_uint8)
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+_SZP = ctypes.POINTER(ctypes.c_size_t)
+SIGNATURES = [
+    ("d4_last_error", ctypes.c_char_p, []),
+    ("d4_device_count", ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    ("d4_free_polyominoes", ctypes.c_size_t, [ctypes.c_uint, _U64P, ctypes.c_size_t]),
+    ("d4_library_create", ctypes.c_int,
+     [_U64P, _U32P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
+    ("d4_library_destroy", None, [_P]),
+    ("d4_library_info", ctypes.c_int, [_P, _U32P, _U32P]),
+    ("d4_searcher_create_ea", ctypes.c_int,
+     [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
+    ("d4_searcher_create", ctypes.c_int,
+     [_P, _U8P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
+    ("d4_searcher_destroy", None, [_P]),
+    ("d4_searcher_height", ctypes.c_uint32, [_P]),
+    ("d4_searcher_size", ctypes.c_uint64, [_P]),
+    ("d4_searcher_next_size", ctypes.c_uint64, [_P]),
+    ("d4_searcher_step", ctypes.c_int, [_P]),
+    ("d4_searcher_run", ctypes.c_int, [_P]),
+    ("d4_searcher_count", ctypes.c_int, [_P, _U64P]),
+    ("d4_searcher_solutions", ctypes.c_int,
+     [_P, ctypes.c_int, _U8P, ctypes.c_size_t, _SZP, _U64P]),
+    ("d4_searcher_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    ("d4_batch_create", ctypes.c_int,
+     [_P, ctypes.c_uint32, _U8P, ctypes.c_uint32, _U32P, ctypes.c_uint32, ctypes.c_uint32,
+      ctypes.POINTER(_P)]),
+    ("d4_batch_destroy", None, [_P]),
+    ("d4_batch_run", ctypes.c_int, [_P]),
+    ("d4_batch_counts", ctypes.c_int, [_P, _U64P]),
+    ("d4_batch_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    ("d4_batch_solutions", ctypes.c_int,
+     [_P, ctypes.c_uint32, ctypes.c_int, _U8P, ctypes.c_size_t, _SZP, _U64P]),
+]
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libd4.so (built by `make -C d4-pieces_amd`); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `make -C d4-pieces_amd` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != D4_OK:
+        msg = load().d4_last_error()
+        raise D4Error(rc, msg.decode() if msg else "")
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(load().d4_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def free_polyominoes(n: int) -> List[int]:
+    """Canonical free polyominoes with n cells, ascending (known_shapes[n])."""
+    lib = load()
+    total = lib.d4_free_polyominoes(n, None, 0)
+    buf = (ctypes.c_uint64 * max(total, 1))()
+    lib.d4_free_polyominoes(n, buf, total)
+    return list(buf[:total])
+
+
+def _u64arr(v):
+    return (ctypes.c_uint64 * len(v))(*v)
+
+
+def _u32arr(v):
+    return (ctypes.c_uint32 * len(v))(*v)
+
+
+def _u8arr(v):
+    return (ctypes.c_uint8 * max(len(v), 1))(*v)
+
+
+Step = Tuple[int, int, int, int]  # (piece_id, trs_id, y, x)
+
+
+def decode_solutions(buf: bytes) -> List[List[Step]]:
+    out, i = [], 0
+    while i < len(buf):
+        n = buf[i]
+        i += 1
+        out.append([tuple(buf[i + 4 * j: i + 4 * j + 4]) for j in range(n)])
+        i += 4 * n
+    return out
+
+
+class PieceLibrary:
+    """Piece set + placement tables on the current device."""
+
+    def __init__(self, shapes: Sequence[int], counts: Optional[Sequence[int]] = None,
+                 sym: int = 0xFF, width: int = 8):
+        lib = load()
+        counts = list(counts) if counts is not None else [1] * len(shapes)
+        self.shapes, self.counts, self.sym, self.width = list(shapes), counts, sym, width
+        h = _P()
+        _check(lib.d4_library_create(_u64arr(self.shapes), _u32arr(counts), len(shapes),
+                                     sym, width, ctypes.byref(h)))
+        self._h = h
+        nc, mh = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib.d4_library_info(h, ctypes.byref(nc), ctypes.byref(mh)))
+        self.n_cand, self.max_height = nc.value, mh.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.d4_library_destroy(self._h)
+            self._h = None
+
+
+def _rows_arg(rows: Sequence[int]):
+    return _u8arr(list(rows))
+
+
+class CudaSearcher:
+    """One board, advanced one row per search_GPU() (CudaSearcher)."""
+
+    def __init__(self, library: PieceLibrary, rows: Optional[Sequence[int]] = None,
+                 empty_area: Optional[int] = None, mode: int = MODE_COUNT,
+                 prune: bool = True):
+        lib = load()
+        self.library = library
+        flags = 0 if prune else FLAG_NO_PRUNE
+        h = _P()
+        if empty_area is not None:
+            _check(lib.d4_searcher_create_ea(library._h, empty_area, mode, flags,
+                                             ctypes.byref(h)))
+        else:
+            rows = list(rows or [])
+            _check(lib.d4_searcher_create(library._h, _rows_arg(rows), len(rows), mode,
+                                          flags, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.d4_searcher_destroy(self._h)
+            self._h = None
+
+    def get_height(self) -> int:
+        return load().d4_searcher_height(self._h)
+
+    def size(self) -> int:
+        return load().d4_searcher_size(self._h)
+
+    def next_size(self) -> int:
+        return load().d4_searcher_next_size(self._h)
+
+    def search_GPU(self) -> None:
+        _check(load().d4_searcher_step(self._h))
+
+    def run(self) -> "CudaSearcher":
+        _check(load().d4_searcher_run(self._h))
+        return self
+
+    def count(self) -> int:
+        v = ctypes.c_uint64()
+        _check(load().d4_searcher_count(self._h, ctypes.byref(v)))
+        return v.value
+
+    def solutions(self, single: bool = False) -> List[List[Step]]:
+        lib = load()
+        ln, ns = ctypes.c_size_t(), ctypes.c_uint64()
+        _check(lib.d4_searcher_solutions(self._h, int(single), None, 0, ctypes.byref(ln),
+                                         ctypes.byref(ns)))
+        buf = (ctypes.c_uint8 * max(ln.value, 1))()
+        _check(lib.d4_searcher_solutions(self._h, int(single), buf, ln.value,
+                                         ctypes.byref(ln), ctypes.byref(ns)))
+        return decode_solutions(bytes(buf[:ln.value]))
+
+    def stats(self) -> dict:
+        st = Stats()
+        _check(load().d4_searcher_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+
+class Batch:
+    """Many boards sharing one library, advanced together in one frontier."""
+
+    def __init__(self, library: PieceLibrary, boards: Sequence[Sequence[int]],
+                 mode: int = MODE_COUNT, prune: bool = True):
+        lib = load()
+        self.library = library
+        self.n = len(boards)
+        stride = max([len(b) for b in boards] + [1])
+        flat = []
+        for b in boards:
+            flat.extend(list(b) + [0] * (stride - len(b)))
+        heights = [len(b) for b in boards]
+        h = _P()
+        _check(lib.d4_batch_create(library._h, self.n, _u8arr(flat), stride, _u32arr(heights),
+                                   mode, 0 if prune else FLAG_NO_PRUNE, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.d4_batch_destroy(self._h)
+            self._h = None
+
+    def run(self) -> "Batch":
+        _check(load().d4_batch_run(self._h))
+        return self
+
+    def counts(self) -> List[int]:
+        buf = (ctypes.c_uint64 * self.n)()
+        _check(load().d4_batch_counts(self._h, buf))
+        return list(buf)
+
+    def solutions(self, i: int, single: bool = False) -> List[List[Step]]:
+        lib = load()
+        ln, ns = ctypes.c_size_t(), ctypes.c_uint64()
+        _check(lib.d4_batch_solutions(self._h, i, int(single), None, 0, ctypes.byref(ln),
+                                      ctypes.byref(ns)))
+        buf = (ctypes.c_uint8 * max(ln.value, 1))()
+        _check(lib.d4_batch_solutions(self._h, i, int(single), buf, ln.value,
+                                      ctypes.byref(ln), ctypes.byref(ns)))
+        return decode_solutions(bytes(buf[:ln.value]))
+
+    def stats(self) -> dict:
+        st = Stats()
+        _check(load().d4_batch_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()

@@ -1,0 +1,148 @@
+"""Seeded synthetic workloads for the tiling search.
+
+Every random choice comes from std::mt19937_64 (the generator the reference's
+benches use, bmark/sn.cu:54, bmark/push_nm.cu:229), restated here so the
+instances are reproducible from the seed alone, with a portable bounded-int
+mapping (lo + floor(x * n / 2^64)) instead of std::uniform_int_distribution,
+whose output differs between standard libraries.
+
+Workloads (boards are lists of 8-bit row masks, bit c = column c open):
+  plain_6x10()        12 free pentominoes x1 on 6 wide x 10 rows
+  seeded_8x8(n, seed) 8x8 with 4 blocked cells, every open region's area a
+                      multiple of 5 (first n accepted draws)
+  large_8x15()        12 free pentominoes x2 on 8 wide x 15 rows
+  request_stream(...) Poisson arrivals of 8x8 boards with 30-64 open cells
+                      (same region filter), 70% first-solution
+"""
+from __future__ import annotations
+
+from typing import List, Sequence, Tuple
+
+MASK64 = (1 << 64) - 1
+
+
+class MT19937_64:
+    """std::mt19937_64 (Matsumoto-Nishimura 64-bit Mersenne Twister)."""
+
+    N, M = 312, 156
+    A = 0xB5026F5AA96619E9
+    UPPER, LOWER = 0xFFFFFFFF80000000, 0x7FFFFFFF
+
+    def __init__(self, seed: int = 5489):
+        mt = [0] * self.N
+        mt[0] = seed & MASK64
+        for i in range(1, self.N):
+            mt[i] = (6364136223846793005 * (mt[i - 1] ^ (mt[i - 1] >> 62)) + i) & MASK64
+        self.mt, self.idx = mt, self.N
+
+    def _twist(self) -> None:
+        mt, N, M = self.mt, self.N, self.M
+        for i in range(N):
+            x = (mt[i] & self.UPPER) | (mt[(i + 1) % N] & self.LOWER)
+            xa = x >> 1
+            if x & 1:
+                xa ^= self.A
+            mt[i] = mt[(i + M) % N] ^ xa
+        self.idx = 0
+
+    def __call__(self) -> int:
+        if self.idx >= self.N:
+            self._twist()
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & MASK64
+
+    def randint(self, lo: int, hi: int) -> int:
+        """Uniform integer in [lo, hi]."""
+        return lo + ((self() * (hi - lo + 1)) >> 64)
+
+    def uniform(self) -> float:
+        return (self() >> 11) * (1.0 / (1 << 53))
+
+
+def pentominoes() -> List[int]:
+    from d4pieces import free_polyominoes
+    return free_polyominoes(5)
+
+
+def regions(rows: Sequence[int], width: int = 8) -> List[int]:
+    """Areas of the 4-connected open regions of a board."""
+    cells = {(r, c) for r, v in enumerate(rows) for c in range(width) if (v >> c) & 1}
+    areas = []
+    while cells:
+        stack = [cells.pop()]
+        a = 0
+        while stack:
+            r, c = stack.pop()
+            a += 1
+            for q in ((r - 1, c), (r + 1, c), (r, c - 1), (r, c + 1)):
+                if q in cells:
+                    cells.remove(q)
+                    stack.append(q)
+        areas.append(a)
+    return areas
+
+
+def plain_6x10() -> Tuple[List[int], List[int], List[int], int]:
+    """(shapes, counts, rows, width) of 12 pentominoes x1 on 6x10."""
+    return pentominoes(), [1] * 12, [0x3F] * 10, 6
+
+
+def large_8x15() -> Tuple[List[int], List[int], List[int], int]:
+    """(shapes, counts, rows, width) of 12 pentominoes x2 on 8x15."""
+    return pentominoes(), [2] * 12, [0xFF] * 15, 8
+
+
+def _blocked_board(rng: MT19937_64, n_blocked: int) -> List[int]:
+    cells: List[int] = []
+    while len(cells) < n_blocked:
+        v = rng.randint(0, 63)
+        if v not in cells:
+            cells.append(v)
+    rows = [0xFF] * 8
+    for v in cells:
+        rows[v // 8] &= 0xFF ^ (1 << (v % 8))
+    return rows
+
+
+def seeded_8x8(n: int = 256, seed: int = 2) -> List[List[int]]:
+    """First n 8x8 boards with 4 distinct uniform blocked cells whose open
+    regions all have area divisible by 5."""
+    rng = MT19937_64(seed)
+    out: List[List[int]] = []
+    while len(out) < n:
+        rows = _blocked_board(rng, 4)
+        if all(a % 5 == 0 for a in regions(rows)):
+            out.append(rows)
+    return out
+
+
+def centre_hole_8x8() -> List[int]:
+    """8x8 minus the central 2x2 (Scott 1958: 520 pentomino tilings, 65 x 8)."""
+    return [0xFF, 0xFF, 0xFF, 0xE7, 0xE7, 0xFF, 0xFF, 0xFF]
+
+
+def request_stream(n: int, rate: float, seed: int = 4):
+    """n requests with Poisson(rate) arrivals: list of (t_arrival_s, rows,
+    single).  Boards are 8x8 with U{30..64} open cells at uniform positions,
+    redrawn until every open region's area is a multiple of 5; 70% of the
+    requests ask for the first solution only."""
+    import math
+    rng = MT19937_64(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        u = rng.uniform()
+        t += -math.log(1.0 - u) / rate
+        while True:
+            free = rng.randint(30, 64)
+            rows = _blocked_board(rng, 64 - free)
+            if all(a % 5 == 0 for a in regions(rows)):
+                break
+        single = rng.uniform() < 0.7
+        out.append((t, rows, single))
+    return out

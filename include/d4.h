@@ -1,0 +1,167 @@
+/*
+ * d4.h -- C ABI of the MI355X (gfx950) row-by-row polyomino tiling search.
+ *
+ * Drop-in counterpart of the reference's GPU row driver
+ *   struct CudaSearcher                       src/searcher_cuda.h:10-38
+ * plus the set-up it needs (piece library, placement tables, device upload:
+ *   compute_frow_on_cpu / transfer_frow_to_gpu / prepare_kernels,
+ *   src/frow.h:10-12, src/kernel.h:80) and the terminal stage the reference
+ * left unfinished (src/searcher.cpp:36-65).
+ *
+ * Conventions: every call returns D4_OK (0) or a negative D4_E* code and
+ * never throws; d4_last_error() gives the calling thread's last message.
+ * Handles are owned by the caller and freed with the matching _destroy;
+ * device memory is owned by the library.  A handle is used by one thread at
+ * a time ("NOT thread-safe", src/searcher.hpp:15-16); distinct handles may be
+ * used from distinct threads.
+ *
+ * Board encoding: row-major, 8 bits per row (bit c of rows[r] = cell (r, c)
+ * is open / must be covered), width <= 8, height <= 64.  This is the
+ * reference's Shape<8> layout (bit = row*8 + col, src/Shape.hpp:20-23)
+ * extended downwards.
+ */
+#ifndef D4_H
+#define D4_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    D4_OK = 0,
+    D4_EINVAL = -1,   /* bad argument / unsupported shape of problem      */
+    D4_ENOMEM = -2,   /* device or host allocation failed                 */
+    D4_EDEVICE = -3,  /* HIP runtime error (no device, launch failure ...) */
+    D4_ESTATE = -4,   /* call not valid in the handle's current state     */
+    D4_ERANGE = -5,   /* a capacity or counter limit was exceeded         */
+};
+
+enum {
+    D4_MODE_COUNT = 0, /* count tilings: frontier states carry u64 counts  */
+    D4_MODE_ENUM = 1,  /* also keep per-row back-pointers, so the full
+                          solution set and the first solution in the CPU
+                          searcher's DFS order can be reconstructed        */
+};
+
+enum {
+    D4_FLAG_NO_PRUNE = 1, /* disable region-connectivity pruning           */
+};
+
+typedef struct d4_library d4_library;
+typedef struct d4_searcher d4_searcher;
+typedef struct d4_batch d4_batch;
+
+typedef struct d4_stats {
+    uint64_t rows_done;        /* row steps executed                        */
+    uint64_t records_extended; /* parents expanded, summed over rows        */
+    uint64_t children;         /* children emitted (before dedup)           */
+    uint64_t unique;           /* unique states kept (after dedup)          */
+    uint64_t pruned;           /* children rejected by connectivity pruning */
+    uint64_t peak_frontier;    /* largest frontier (records)                */
+    uint64_t retries;          /* row steps re-run after a table overflow   */
+    double   gpu_ms;           /* device time of the row steps              */
+} d4_stats;
+
+/* Thread-local text of the last error. */
+const char *d4_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int d4_device_count(int *n);
+
+/* Free polyominoes with n cells (1 <= n <= 8), each as its canonical value
+ * (minimum over the 8 D4 transforms of the normalised 8-wide encoding),
+ * ascending: the reference's known_shapes[n] tables (src/known1.cpp:3-14,
+ * generated by src/enum.cpp:28-57).  Writes min(total, cap) values and
+ * returns the total.  Host-only; needs no device. */
+size_t d4_free_polyominoes(unsigned n, uint64_t *out, size_t cap);
+
+/* ---------------------------------------------------------------------- */
+/* piece library and placement tables                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Build the per-anchor placement tables for a library of n pieces
+ * (shapes[i] in the 8-wide encoding, normalised; counts[i] = multiplicity)
+ * under symmetry mask sym (0xff = all D4 transforms, src/Group.hpp:8-19) for
+ * boards of `width` columns, and upload them to the current device.
+ * Replaces compute_frow_on_cpu + transfer_frow_to_gpu + prepare_kernels
+ * (src/frow.cpp:39-166, src/frow.cu:7-35, src/kernel.cu:73-93); the tables
+ * are full-row (no L/R split) and carry no static island pruning.
+ * Limits: n <= 32, <= 64 unique (piece, transform) pairs, piece height <= 6,
+ * sum of per-piece counter widths + state bits <= 63. */
+int d4_library_create(const uint64_t *shapes, const uint32_t *counts,
+                      uint32_t n, uint32_t sym, uint32_t width,
+                      d4_library **out);
+void d4_library_destroy(d4_library *lib);
+/* n_cand = unique (piece, transform) pairs, max_height = tallest piece. */
+int d4_library_info(const d4_library *lib, uint32_t *n_cand,
+                    uint32_t *max_height);
+
+/* ---------------------------------------------------------------------- */
+/* single-instance searcher (CudaSearcher)                                 */
+/* ---------------------------------------------------------------------- */
+
+/* CudaSearcher::CudaSearcher(uint64_t empty_area), src/searcher_cuda.cu:12-21:
+ * an 8x8 board given as one u64 (bit = row*8+col set = open); height =
+ * ceil(bit_width(empty_area) / 8) as in the reference. */
+int d4_searcher_create_ea(const d4_library *lib, uint64_t empty_area,
+                          uint32_t mode, uint32_t flags, d4_searcher **out);
+/* General board: `height` rows of 8-bit masks. */
+int d4_searcher_create(const d4_library *lib, const uint8_t *rows,
+                       uint32_t height, uint32_t mode, uint32_t flags,
+                       d4_searcher **out);
+/* ~CudaSearcher, src/searcher_cuda.cu:23-30. */
+void d4_searcher_destroy(d4_searcher *s);
+/* get_height(), src/searcher_cuda.h:14: rows still to be filled. */
+uint32_t d4_searcher_height(const d4_searcher *s);
+/* size(), src/searcher_cuda.h:16-21: records in the current frontier. */
+uint64_t d4_searcher_size(const d4_searcher *s);
+/* next_size(), src/searcher_cuda.h:22-27, src/searcher_cuda.cu:84-90: an
+ * upper bound on the children the next step can emit (sum over records of
+ * the number of ways to fill the record's current row ignoring lower rows and
+ * piece counts). */
+uint64_t d4_searcher_next_size(const d4_searcher *s);
+/* search_GPU(), src/searcher_cuda.cu:32-82: advance the frontier by one row
+ * (extend + prune + dedup + compact on the device). */
+int d4_searcher_step(d4_searcher *s);
+/* Step until height == 0. */
+int d4_searcher_run(d4_searcher *s);
+/* Number of tilings (valid once height == 0): sum of terminal counts. */
+int d4_searcher_count(const d4_searcher *s, uint64_t *count);
+/* Solutions (D4_MODE_ENUM only, height == 0), in the CPU searcher's DFS
+ * order (solve(), src/Piece.cpp:66-102).  Each is written as
+ *   u8 nsteps, nsteps x {u8 piece_id, u8 trs_id, u8 y, u8 x}
+ * (Step<L>::piece_id, trs_id, y, x, src/Piece.hpp:31-36).  single != 0
+ * writes only the first solution (solve(..., single=true)).  *len gets the
+ * bytes needed, *nsol the solutions written (or that would be). */
+int d4_searcher_solutions(const d4_searcher *s, int single, uint8_t *out,
+                          size_t cap, size_t *len, uint64_t *nsol);
+int d4_searcher_stats(const d4_searcher *s, d4_stats *st);
+
+/* ---------------------------------------------------------------------- */
+/* batched instances (no reference counterpart: the reference runs one     */
+/* board config per CudaSearcher, src/searcher.cpp:68-107)                 */
+/* ---------------------------------------------------------------------- */
+
+/* n_inst boards sharing one library; boards[i*stride + r] = row r of board
+ * i, heights[i] its row count.  All instances advance row by row in one
+ * shared frontier (one launch per kernel per row for all of them). */
+int d4_batch_create(const d4_library *lib, uint32_t n_inst,
+                    const uint8_t *boards, uint32_t stride,
+                    const uint32_t *heights, uint32_t mode, uint32_t flags,
+                    d4_batch **out);
+void d4_batch_destroy(d4_batch *b);
+int d4_batch_run(d4_batch *b);
+/* counts[i] = number of tilings of board i (valid after d4_batch_run). */
+int d4_batch_counts(const d4_batch *b, uint64_t *counts);
+int d4_batch_stats(const d4_batch *b, d4_stats *st);
+/* Solutions of instance i (D4_MODE_ENUM), format as d4_searcher_solutions. */
+int d4_batch_solutions(const d4_batch *b, uint32_t i, int single,
+                       uint8_t *out, size_t cap, size_t *len, uint64_t *nsol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,153 @@
+"""GPU parity: the HIP row search (through the C ABI) against the oracle
+(restated src/Piece.cpp solve/solve_count) -- bit-exact counts and solution
+sets, plus known answers at sizes the oracle would take long on."""
+import random
+
+import pytest
+
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+CENTRE = [(3, 3), (3, 4), (4, 3), (4, 4)]
+
+
+def rect(w, h, blocked=()):
+    rows = [(1 << w) - 1] * h
+    for r, c in blocked:
+        rows[r] &= 0xFF ^ (1 << c)
+    return rows
+
+
+@pytest.fixture(scope="module")
+def p5(gpu):
+    return gpu.free_polyominoes(5)
+
+
+@pytest.fixture(scope="module")
+def lib6(gpu, p5):
+    return gpu.PieceLibrary(p5, [1] * 12, 0xFF, 6)
+
+
+@pytest.fixture(scope="module")
+def lib8(gpu, p5):
+    return gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+
+
+def gpu_count(d4, lib, rows, prune=True):
+    s = d4.CudaSearcher(lib, rows=rows, prune=prune)
+    s.run()
+    return s.count()
+
+
+def test_plain_6x10_full_set(gpu, lib6, oracle, p5):
+    """Config: 12 pentominoes x1 on 6x10, full solution set, bit-exact."""
+    rows = rect(6, 10)
+    s = gpu.CudaSearcher(lib6, rows=rows, mode=gpu.MODE_ENUM)
+    heights = []
+    while s.get_height():
+        heights.append(s.get_height())
+        s.search_GPU()
+    assert heights == list(range(10, 0, -1))
+    assert s.count() == 9356
+    sols = s.solutions()
+    want = oracle.solve(p5, [1] * 12, rows)
+    assert len(sols) == len(want) == 9356
+    assert sols == want
+    assert s.solutions(single=True) == oracle.solve(p5, [1] * 12, rows, single=True)
+
+
+@pytest.mark.parametrize("w,h,expect", [(5, 12, 4040), (4, 15, 1472), (3, 20, 8)])
+def test_pentomino_rectangles(gpu, p5, w, h, expect):
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, w)
+    assert gpu_count(gpu, lib, rect(w, h)) == expect
+
+
+def test_centre_hole_ea_constructor(gpu, lib8, oracle, p5):
+    """CudaSearcher(uint64_t empty_area) on the 8x8 board minus the centre."""
+    ea = 0
+    for r, v in enumerate(rect(8, 8, CENTRE)):
+        ea |= v << (8 * r)
+    s = gpu.CudaSearcher(lib8, empty_area=ea, mode=gpu.MODE_ENUM)
+    assert s.get_height() == 8
+    s.run()
+    assert s.count() == 520
+    rows = rect(8, 8, CENTRE)
+    assert s.solutions() == oracle.solve(p5, [1] * 12, rows)
+
+
+def test_seeded_8x8_batch(gpu, lib8, oracle, p5):
+    boards = workloads.seeded_8x8(12, seed=2)
+    b = gpu.Batch(lib8, boards)
+    b.run()
+    got = b.counts()
+    want = [oracle.count(p5, [1] * 12, r) for r in boards]
+    assert got == want
+    assert sum(want) > 0
+
+
+def test_batch_equals_alone(gpu, lib8):
+    boards = workloads.seeded_8x8(6, seed=5)
+    alone = [gpu_count(gpu, lib8, r) for r in boards]
+    b = gpu.Batch(lib8, boards)
+    b.run()
+    assert b.counts() == alone
+
+
+def test_prune_does_not_change_counts(gpu, lib8):
+    boards = workloads.seeded_8x8(4, seed=9)
+    for r in boards:
+        a = gpu.CudaSearcher(lib8, rows=r, prune=True).run()
+        b = gpu.CudaSearcher(lib8, rows=r, prune=False).run()
+        assert a.count() == b.count()
+        assert a.stats()["unique"] <= b.stats()["unique"]
+
+
+def test_dominoes(gpu):
+    fib = [1, 1]
+    for _ in range(30):
+        fib.append(fib[-1] + fib[-2])
+    lib = gpu.PieceLibrary([3], [63], 0xFF, 2)
+    for n in (1, 2, 7, 20, 40):
+        assert gpu_count(gpu, lib, rect(2, n)) == fib[n]
+    lib8d = gpu.PieceLibrary([3], [32], 0xFF, 8)
+    assert gpu_count(gpu, lib8d, rect(8, 8)) == 12988816
+
+
+def test_random_small_instances(gpu, oracle):
+    rng = random.Random(3)
+    pool = [1, 3, 7, 259] + gpu.free_polyominoes(4) + gpu.free_polyominoes(5)[:6]
+    for _ in range(30):
+        shapes = sorted(rng.sample(pool, rng.randint(2, 5)))
+        counts = [rng.randint(1, 3) for _ in shapes]
+        w, h = rng.randint(2, 6), rng.randint(2, 6)
+        blocked = [(rng.randrange(h), rng.randrange(w)) for _ in range(rng.randint(0, 3))]
+        rows = rect(w, h, blocked)
+        lib = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+        s = gpu.CudaSearcher(lib, rows=rows, mode=gpu.MODE_ENUM)
+        s.run()
+        want = oracle.solve(shapes, counts, rows)
+        assert s.count() == len(want), (shapes, counts, rows)
+        assert s.solutions() == want, (shapes, counts, rows)
+
+
+def test_edge_cases(gpu, lib8, p5):
+    # empty board: exactly one (empty) tiling
+    s = gpu.CudaSearcher(lib8, rows=[])
+    s.run()
+    assert s.count() == 1
+    # board with no tiling (area not a multiple of 5)
+    assert gpu_count(gpu, lib8, rect(3, 3)) == 0
+    # a fully blocked row inside the board
+    assert gpu_count(gpu, lib8, [0xFF, 0x00, 0xFF]) == 0
+    # one pentomino-shaped hole
+    assert gpu_count(gpu, lib8, [0x1F]) == 1
+
+
+def test_next_size_bounds_children(gpu, lib6):
+    s = gpu.CudaSearcher(lib6, rows=rect(6, 10))
+    while s.get_height():
+        bound = s.next_size()
+        before = s.stats()["children"]
+        s.search_GPU()
+        assert s.stats()["children"] - before <= bound

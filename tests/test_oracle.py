@@ -1,0 +1,230 @@
+"""Pins the oracle (CPU restatement of src/Piece.cpp solve/solve_count and the
+Shape geometry) against the reference's fixtures, closed forms and brute
+force -- before it is trusted as the GPU parity checker."""
+import itertools
+import json
+import os
+import random
+
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "known_shapes.json")
+
+# OEIS A000105 (free) / A001168 (fixed) polyominoes, n = 1..8
+A000105 = [1, 1, 2, 5, 12, 35, 108, 369]
+A001168 = [1, 2, 6, 19, 63, 216, 760, 2725]
+
+
+def rect(w, h, blocked=()):
+    rows = [(1 << w) - 1] * h
+    for r, c in blocked:
+        rows[r] &= 0xFF ^ (1 << c)
+    return rows
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return json.load(open(GOLDEN))
+
+
+@pytest.fixture(scope="module")
+def p5(oracle):
+    return oracle.enum_free(5)
+
+
+def test_enum_matches_reference_tables(oracle, golden):
+    assert golden["free_counts"][1:] == A000105
+    for n in range(1, 9):
+        got = oracle.enum_free(n)
+        assert len(got) == A000105[n - 1]
+        assert got == golden["free"][str(n)], n
+
+
+def test_transform_group(oracle):
+    # every transform of a normalised shape is normalised; involutions square
+    # to the identity and the two quarter turns are inverses (Group.cpp:3-32)
+    rng = random.Random(1)
+    shapes = oracle.enum_free(6) + oracle.enum_free(7)
+    inv = {0: 0, 1: 1, 2: 2, 3: 3, 4: 4, 7: 7, 5: 6, 6: 5}
+    for s in rng.sample(shapes, 40):
+        assert oracle.transform(s, 0) == s
+        for t in range(8):
+            assert oracle.transform(oracle.transform(s, t), inv[t]) == s
+        # closure: transforming any transform stays within the orbit
+        orbit = {oracle.transform(s, t) for t in range(8)}
+        for a in orbit:
+            assert {oracle.transform(a, t) for t in range(8)} == orbit
+
+
+def test_transform_concrete(oracle):
+    # L tetromino 0b1_00000111 (row0: ###, row1: #..)
+    L = 0b1_0000_0111 | 0
+    L = 7 | (1 << 8)
+    assert oracle.transform(L, 1) == 7 | (1 << 10)            # flip X: ###/..#
+    assert oracle.transform(L, 2) == (1 << 0) | (7 << 8)      # flip Y: #../###
+    assert oracle.transform(L, 4) == 0b11 | (1 << 8) | (1 << 16)  # transpose
+
+
+def test_fixed_orientation_counts(oracle):
+    for n in range(1, 8):
+        total = sum(len(oracle.orients(s)) for s in oracle.enum_free(n))
+        assert total == A001168[n - 1], n
+
+
+def test_pentomino_symmetry_classes(oracle, p5):
+    # X has 1 orientation, I 2, T/U/V/W/Z 4, F/L/N/P/Y 8 -> 63 in total
+    sizes = sorted(len(oracle.orients(s)) for s in p5)
+    assert sizes == [1, 2, 4, 4, 4, 4, 4, 8, 8, 8, 8, 8]
+
+
+@pytest.mark.parametrize("w,h,expect", [(6, 10, 9356), (5, 12, 4040), (4, 15, 1472),
+                                        (3, 20, 8)])
+def test_pentomino_rectangles(oracle, p5, w, h, expect):
+    assert oracle.count(p5, [1] * 12, rect(w, h)) == expect
+
+
+def test_scott_centre_hole(oracle, p5):
+    assert oracle.count(p5, [1] * 12, rect(8, 8, [(3, 3), (3, 4), (4, 3), (4, 4)])) == 520
+
+
+def test_domino_fibonacci(oracle):
+    fib = [1, 1]
+    for _ in range(20):
+        fib.append(fib[-1] + fib[-2])
+    for n in range(1, 14):
+        assert oracle.count([3], [n], rect(2, n)) == fib[n], n
+
+
+def test_domino_kasteleyn_8x8(oracle):
+    assert oracle.count([3], [32], rect(8, 8)) == 12988816
+
+
+def test_domino_kasteleyn_small(oracle):
+    import math
+    for m, n in [(3, 4), (4, 4), (4, 5), (5, 6), (6, 6)]:
+        prod = 1.0
+        for j in range(1, m + 1):
+            for k in range(1, n + 1):
+                prod *= 4 * math.cos(math.pi * j / (m + 1)) ** 2 + \
+                    4 * math.cos(math.pi * k / (n + 1)) ** 2
+        want = round(prod ** 0.25) if m * n % 2 == 0 else 0
+        assert oracle.count([3], [m * n // 2], rect(m, n)) == want, (m, n)
+
+
+def test_straight_tromino_recurrence(oracle):
+    a = [1, 1, 1]
+    for n in range(3, 16):
+        a.append(a[n - 1] + a[n - 3])
+    for n in range(1, 16):
+        assert oracle.count([7], [n], rect(3, n)) == a[n], n
+
+
+def _placements(shapes, counts, rows, oracle):
+    H = len(rows)
+    board = {(r, c) for r in range(H) for c in range(8) if (rows[r] >> c) & 1}
+    out = []
+    for i, s in enumerate(shapes):
+        for nm, _ in oracle.orients(s):
+            cells = [(b // 8, b % 8) for b in range(64) if (nm >> b) & 1]
+            for dy in range(H):
+                for dx in range(8):
+                    pc = frozenset((r + dy, c + dx) for r, c in cells)
+                    if pc <= board:
+                        out.append((i, pc))
+    return board, out
+
+
+def _brute(shapes, counts, rows, oracle):
+    """Exact covers as sets of placements (order-free), counts respected."""
+    board, pls = _placements(shapes, counts, rows, oracle)
+    total = 0
+
+    def rec(start, covered, used):
+        nonlocal total
+        if covered == board:
+            total += 1
+            return
+        for j in range(start, len(pls)):
+            i, pc = pls[j]
+            if used[i] < counts[i] and not (pc & covered):
+                used[i] += 1
+                rec(j + 1, covered | pc, used)
+                used[i] -= 1
+
+    rec(0, frozenset(), [0] * len(shapes))
+    return total
+
+
+def test_brute_force_tiny(oracle):
+    rng = random.Random(7)
+    pool = [1, 3, 7, 259] + oracle.enum_free(4)
+    checked = 0
+    for trial in range(60):
+        k = rng.randint(2, 4)
+        shapes = sorted(rng.sample(pool, k))
+        counts = [rng.randint(1, 3) for _ in shapes]
+        w, h = rng.randint(2, 4), rng.randint(2, 4)
+        blocked = [(rng.randrange(h), rng.randrange(w)) for _ in range(rng.randint(0, 2))]
+        rows = rect(w, h, blocked)
+        want = _brute(shapes, counts, rows, oracle)
+        assert oracle.count(shapes, counts, rows) == want, (shapes, counts, rows)
+        checked += want > 0
+    assert checked >= 10
+
+
+def _cover_ok(sol, shapes, counts, rows, oracle):
+    board = {(r, c) for r in range(len(rows)) for c in range(8) if (rows[r] >> c) & 1}
+    covered = set()
+    used = [0] * len(shapes)
+    for pid, trs, y, x in sol:
+        nm = dict((t, n) for n, t in oracle.orients(shapes[pid]))[trs]
+        cells = {(b // 8 + y, b % 8 + x) for b in range(64) if (nm >> b) & 1}
+        assert not (cells & covered)
+        covered |= cells
+        used[pid] += 1
+    return covered == board and all(u <= c for u, c in zip(used, counts))
+
+
+@pytest.mark.slow
+def test_solve_6x10_invariants(oracle, p5):
+    rows = rect(6, 10)
+    sols = oracle.solve(p5, [1] * 12, rows)
+    assert len(sols) == 9356 == len(set(map(tuple, sols)))
+    # DFS order: solutions come out sorted by their (piece, trs) choice sequence
+    keys = [[(p, t) for p, t, _, _ in s] for s in sols]
+    assert keys == sorted(keys)
+    first = oracle.solve(p5, [1] * 12, rows, single=True)
+    assert first == sols[:1]
+    for s in sols[::97]:
+        assert _cover_ok(s, p5, [1] * 12, rows, oracle)
+    # board symmetry: mirroring every solution left-right maps the set onto
+    # itself (as cell partitions labelled by piece)
+    def partition(sol, mirror):
+        out = []
+        for pid, trs, y, x in sol:
+            nm = dict((t, n) for n, t in oracle.orients(p5[pid]))[trs]
+            cells = frozenset((b // 8 + y, (5 - (b % 8 + x)) if mirror else b % 8 + x)
+                              for b in range(64) if (nm >> b) & 1)
+            out.append((pid, cells))
+        return frozenset(out)
+    plain = {partition(s, False) for s in sols}
+    assert {partition(s, True) for s in sols} == plain
+
+
+def test_solve_matches_count_small(oracle):
+    rng = random.Random(11)
+    pool = [3, 7, 259] + oracle.enum_free(4)
+    for _ in range(25):
+        shapes = sorted(rng.sample(pool, 3))
+        counts = [rng.randint(1, 3) for _ in shapes]
+        rows = rect(rng.randint(2, 5), rng.randint(2, 5))
+        sols = oracle.solve(shapes, counts, rows)
+        assert len(sols) == oracle.count(shapes, counts, rows)
+        for s in sols:
+            assert _cover_ok(s, shapes, counts, rows, oracle)
+
+
+def test_subset_tilings(oracle):
+    # pieces need not all be used (solve only prunes by area): a 2x2 board
+    # with a monomino x4 and a square tetromino x1 has 2 tilings
+    assert oracle.count([1, 15 & 0x3 | (0x3 << 8)], [4, 1], rect(2, 2)) == 2
