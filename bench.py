@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the row-by-row polyomino tiling search on one MI355X.
+
+Workload (the metric's configuration that fits one GPU): 256 seeded 8x8
+boards with 4 blocked cells (every open region's area a multiple of 5), the
+12 free pentominoes used once each, all tilings counted.  One step = one
+batched solve of all 256 boards (every row of every board, one shared device
+frontier).  Reported:
+
+  value            records extended/s (parents expanded, summed over rows)
+  solutions_per_s  tilings found per second
+  plain            the same boards one CudaSearcher at a time (a sample),
+                   records/s, solutions/s and per-board latency
+  roofline         the extend kernel's algorithmic HBM bytes / its average
+                   launch time (HIP events on the engine's stream), against
+                   MI355X's 8 TB/s
+  cpu_baseline     the oracle (CPU restatement of the reference backtracker)
+                   on a bounded sample of the same boards, P host threads
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "d4-pieces_amd"))
+
+METRIC = ("records extended/s + solutions/s on 1 MI355X, batched vs plain; "
+          "p99 latency")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def percentile(v, q):
+    v = sorted(v)
+    if not v:
+        return None
+    k = (len(v) - 1) * q
+    lo, hi = int(k), min(int(k) + 1, len(v) - 1)
+    return v[lo] + (v[hi] - v[lo]) * (k - lo)
+
+
+def cpu_baseline(boards, gpu_counts, shapes, seconds, threads):
+    """Oracle solve_count on the first boards of the workload until about
+    `seconds` of wall time; returns the baseline object and a parity flag."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import Oracle
+    path = os.path.join(ROOT, "oracle", "liboracle.so")
+    oracle = Oracle(path)
+    done, sols, ok = 0, 0, True
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        while done < len(boards) and time.perf_counter() - t0 < seconds:
+            chunk = boards[done:done + threads]
+            res = list(ex.map(lambda b: oracle.count(shapes, [1] * 12, b), chunk))
+            for j, c in enumerate(res):
+                ok &= c == gpu_counts[done + j]
+            sols += sum(res)
+            done += len(chunk)
+            log(f"cpu baseline {done} boards {time.perf_counter() - t0:.1f} s")
+    dt = time.perf_counter() - t0
+    return {
+        "value": sols / dt,
+        "unit": "solutions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {done} of {len(boards)} boards, one board per thread, "
+                  f"{dt:.1f} s wall, {done / dt:.3f} boards/s",
+    }, ok, done
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the "
+              "instances of this workload shard across GPUs with no exchange, so "
+              "N GPUs give N times the one-GPU value", file=sys.stderr)
+        sys.exit(2)
+
+    import d4pieces as d4
+    import workloads
+
+    if d4.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    # device-wide barrier through the library's own HIP runtime (torch ships
+    # a separate runtime copy, which is not needed here)
+    sync = d4.synchronize
+
+    shapes = workloads.pentominoes()
+    boards = workloads.seeded_8x8(args.instances, seed=2)
+    lib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
+
+    def step():
+        b = d4.Batch(lib, boards)
+        b.run()
+        return b
+
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        step()
+        log(f"warmup step {i} {time.perf_counter() - t:.2f} s")
+    sync()
+    t0 = time.perf_counter()
+    stats, counts = [], None
+    for i in range(args.steps):
+        t = time.perf_counter()
+        b = step()
+        stats.append(b.stats())
+        log(f"timed step {i} {time.perf_counter() - t:.2f} s")
+        c = b.counts()
+        if counts is not None and c != counts:
+            raise SystemExit("batched counts differ between steps")
+        counts = c
+    sync()
+    dt = time.perf_counter() - t0
+
+    rec = sum(s["records_extended"] for s in stats)
+    n_sol = sum(counts) * args.steps
+    ext_ms = sum(s["extend_ms"] for s in stats)
+    ext_launch = sum(s["extend_launches"] for s in stats)
+    ext_bytes = sum(s["extend_bytes"] for s in stats)
+    achieved = ext_bytes / (ext_ms / 1e3) / 1e9 if ext_ms else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": rec / dt,
+        "unit": "records extended/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic: seeded mt19937_64 boards (seed 2)",
+        "config": {
+            "workload": f"{args.instances} seeded 8x8 boards, 4 blocked cells, "
+                        "12 free pentominoes x1, all tilings, batched",
+            "instances": args.instances,
+            "board": "8x8",
+            "pieces": "12 free pentominoes x1, D4",
+            "mode": "count",
+            "parallelism": "instances batched in one frontier, 1 GPU",
+        },
+        "solutions_per_s": n_sol / dt,
+        "solutions_per_step": sum(counts),
+        "records_per_step": rec // args.steps,
+        "children_per_step": sum(s["children"] for s in stats) // args.steps,
+        "peak_frontier": max(s["peak_frontier"] for s in stats),
+        "latency_ms": {"batched_p50": dt / args.steps * 1e3,
+                       "batched_p99": dt / args.steps * 1e3},
+        "roofline": {
+            "kernel": "k_extend",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "avg_launch_ms": ext_ms / max(ext_launch, 1),
+            "bytes_per_launch": ext_bytes / max(ext_launch, 1),
+            "compact_GBs": (sum(s["compact_bytes"] for s in stats) /
+                            (sum(s["compact_ms"] for s in stats) / 1e3) / 1e9)
+            if sum(s["compact_ms"] for s in stats) else None,
+        },
+    }
+
+    if not args.no_plain:
+        n = min(args.plain_sample, len(boards))
+        lat, prec, psol = [], 0, 0
+        t1 = time.perf_counter()
+        for i in range(n):
+            s0 = time.perf_counter()
+            s = d4.CudaSearcher(lib, rows=boards[i])
+            s.run()
+            c = s.count()
+            sync()
+            lat.append((time.perf_counter() - s0) * 1e3)
+            if c != counts[i]:
+                raise SystemExit(f"plain count differs from batched on board {i}")
+            prec += s.stats()["records_extended"]
+            psol += c
+            log(f"plain board {i} {lat[-1]:.1f} ms")
+        pdt = time.perf_counter() - t1
+        out["plain"] = {
+            "boards": n,
+            "records_per_s": prec / pdt,
+            "solutions_per_s": psol / pdt,
+            "latency_ms_p50": percentile(lat, 0.5),
+            "latency_ms_p99": percentile(lat, 0.99),
+        }
+        out["batched_vs_plain"] = (rec / dt) / (prec / pdt) if prec else None
+        out["latency_ms"]["plain_p50"] = percentile(lat, 0.5)
+        out["latency_ms"]["plain_p99"] = percentile(lat, 0.99)
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        base, ok, n_checked = cpu_baseline(boards, counts, shapes, args.cpu_seconds, threads)
+        out["cpu_baseline"] = base
+        out["parity"] = {"boards_checked_against_oracle": n_checked, "equal": ok}
+        if not ok:
+            print(json.dumps(out))
+            raise SystemExit("GPU counts differ from the oracle")
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -284,7 +284,7 @@ struct Engine {
     uint32_t bw_stride = 0;
     // device state
     hipStream_t st = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evx0 = nullptr, evx1 = nullptr, evc0 = nullptr;
     DevBuf<uint64_t> bw;
     DevBuf<uint32_t> hgt;
     DevBuf<unsigned long long> result, result_save;
@@ -296,6 +296,7 @@ struct Engine {
     DevBuf<uint64_t> e_desc, e_slot;
     uint64_t n_front = 0;
     double grow_unique = 4.0, grow_children = 8.0;
+    static constexpr uint64_t kExactCountMax = 1u << 16;
     d4_stats stats{};
     std::vector<unsigned long long> host_result;
     std::vector<RowLog> logs;
@@ -331,6 +332,9 @@ struct Engine {
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
+        HIPCHK(hipEventCreate(&evx0));
+        HIPCHK(hipEventCreate(&evx1));
+        HIPCHK(hipEventCreate(&evc0));
         // board windows: bw[i][r] = rows r..r+7 of board i
         bw_stride = Hmax + 2;
         std::vector<uint64_t> hb((size_t)n * bw_stride, 0);
@@ -364,6 +368,8 @@ struct Engine {
     ~Engine() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : {evx0, evx1, evc0})
+            if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -375,6 +381,7 @@ struct Engine {
         a.in_key = fk.p;
         a.in_cnt = fc.p;
         a.n_in = (uint32_t)n_front;
+        a.split = n_front <= (1u << 16) ? 1u : 0u;
         a.row = row;
         a.fit = lib->d_fit;
         a.place = lib->d_place;
@@ -395,7 +402,7 @@ struct Engine {
         a.tab_key = tk.p;
         a.tab_cnt = tc.p;
         a.tab_mask = tab_m - 1;
-        a.max_probe = 4096;
+        a.max_probe = 256;
         a.c32 = c32.p;
         a.c64 = c64.p;
         a.e_parent = e_parent.p;
@@ -422,9 +429,19 @@ struct Engine {
         if (n_front > 0xffffffffull) throw Error(D4_ERANGE, "frontier exceeds 2^32 records");
         const bool en = mode == D4_MODE_ENUM;
         const bool prune = !(flags & D4_FLAG_NO_PRUNE);
-        uint64_t est_u = (uint64_t)(n_front * grow_unique * 1.5) + 1024;
-        uint64_t tab_m = next_pow2(2 * est_u);
-        uint64_t est_e = en ? (uint64_t)(n_front * grow_children * 1.5) + 1024 : 0;
+        // Table size: unique children <= children.  A small frontier gets an
+        // exact count from a search-only pass; a large one is estimated from
+        // the previous row's children per record (x2), with a rerun at 4x
+        // the size if the table still overflows.
+        uint64_t est_c;
+        if (n_front <= kExactCountMax) {
+            est_c = count_children();
+            stats.count_passes++;
+        } else {
+            est_c = (uint64_t)(n_front * std::min(std::max(grow_children, 1.0), 64.0) * 2.0) + 4096;
+        }
+        uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_c, 1024));
+        uint64_t est_e = en ? est_c + 64 : 0;
         if (en) {
             RowLog lg;
             lg.keys.resize(n_front);
@@ -452,11 +469,20 @@ struct Engine {
                                       hipMemcpyDeviceToDevice, st));
             launch_fill_table(tk.p, tc.p, tab_m, st);
             launch_reset_counters(c32.p, c64.p, st);
-            if (n_front) launch_extend(args(tab_m, (uint32_t)est_e), en, prune, lib->n_cu, st);
+            HIPCHK(hipEventRecord(evx0, st));
+            if (n_front) launch_extend(args(tab_m, (uint32_t)est_e), en, prune, false, lib->n_cu, st);
             HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(evx1, st));
             HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
+            if (n_front) {
+                float xms = 0;
+                HIPCHK(hipEventElapsedTime(&xms, evx0, evx1));
+                stats.extend_ms += xms;
+                stats.extend_launches++;
+                stats.extend_bytes += 16ull * n_front + 16ull * h64[C64_CHILDREN];
+            }
             bool ovf = h32[C32_OVERFLOW] != 0;
             bool eovf = en && h32[C32_EDGE_OVF] != 0;
             if (!ovf && !eovf) break;
@@ -481,6 +507,7 @@ struct Engine {
         nk.reserve(U);
         nc.reserve(U);
         if (en) slot_uid.reserve(tab_m);
+        HIPCHK(hipEventRecord(evc0, st));
         launch_compact(tk.p, tc.p, tab_m, nk.p, nc.p, c32.p, en ? slot_uid.p : nullptr,
                        lib->n_cu, st);
         uint32_t n_edges = en ? h32[C32_EDGES] : 0;
@@ -491,9 +518,13 @@ struct Engine {
         HIPCHK(hipEventRecord(ev1, st));
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
-        float ms = 0;
+        float ms = 0, cms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        HIPCHK(hipEventElapsedTime(&cms, evc0, ev1));
         stats.gpu_ms += ms;
+        stats.compact_ms += cms;
+        stats.compact_launches++;
+        stats.compact_bytes += 8ull * tab_m + 24ull * U;
         if (en) {
             RowLog &lg = logs.back();
             lg.e_parent.resize(n_edges);
@@ -524,6 +555,20 @@ struct Engine {
 
     void run() {
         while (!done) step();
+    }
+
+    // Children the current frontier will emit (after pruning): the extend
+    // search with inserts switched off.
+    uint64_t count_children() {
+        if (!n_front) return 0;
+        const bool prune = !(flags & D4_FLAG_NO_PRUNE);
+        launch_reset_counters(c32.p, c64.p, st);
+        launch_extend(args(1, 0), false, prune, true, lib->n_cu, st);
+        HIPCHK(hipGetLastError());
+        unsigned long long h64[C64_N];
+        HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return h64[C64_CHILDREN];
     }
 
     uint64_t next_size() {
@@ -661,6 +706,10 @@ int d4_device_count(int *n) {
         hipError_t e = hipGetDeviceCount(&c);
         *n = e == hipSuccess ? c : 0;
     });
+}
+
+int d4_device_synchronize(void) {
+    return guard([&] { HIPCHK(hipDeviceSynchronize()); });
 }
 
 size_t d4_free_polyominoes(unsigned n, uint64_t *out, size_t cap) {

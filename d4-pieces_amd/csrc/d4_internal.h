@@ -40,6 +40,7 @@ struct ExtendArgs {
     const uint64_t *in_cnt;
     uint32_t n_in;
     uint32_t row;
+    uint32_t split;          // 1: work item = (record, first placement)
     // library tables (global copies; staged into LDS)
     const uint64_t *fit;     // [8][fit_rows][256] candidate masks
     const uint64_t *place;   // [8][64] placed shape per anchor column
@@ -83,7 +84,7 @@ void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
-                   int n_cu, hipStream_t st);
+                   bool count_only, int n_cu, hipStream_t st);
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
                     uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,

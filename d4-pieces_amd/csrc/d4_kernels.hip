@@ -150,7 +150,9 @@ __device__ __forceinline__ uint64_t fitmask(const uint64_t *s_fit,
     return m;
 }
 
-template <bool ENUM, bool PRUNE>
+// COUNT_ONLY runs the same search but only counts the surviving children
+// (used to size the dedup table exactly when the frontier is small).
+template <bool ENUM, bool PRUNE, bool COUNT_ONLY>
 __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
     extern __shared__ uint64_t lds[];
     const uint32_t nfit = 8u * a.fit_rows * 256u;
@@ -177,130 +179,161 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
     const uint32_t rows = a.fit_rows;
     uint64_t n_children = 0, n_new = 0, n_pruned = 0;
 
+    // Lanes are refilled independently: a lane whose DFS is over takes the
+    // next work item (wave-aggregated fetch, one atomic per refill), so a
+    // record with a large subtree does not idle the rest of its wave.  In
+    // split mode a work item is (record, first placement) -- 64 items per
+    // record -- so tiny frontiers (the first rows) still fill the GPU.
+    const uint32_t n_items = a.split ? a.n_in * 64u : a.n_in;
+    bool exhausted = false;
+    bool searching = false, pass = false;
+    uint32_t rec = 0, inst = 0, used = 0, d = 0, c = 0, anc = 0;
+    uint64_t cnt = 0, BW = 0, BWn = 0, E = 0, avail = 0, desc = 0, ok = 0;
+
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.c32 + C32_WORK, 64u);
-        base = __shfl(base, 0);
-        if (base >= a.n_in) break;
-        const uint32_t idx = base + lane;
-        bool live = idx < a.n_in;
-        uint64_t key = live ? a.in_key[idx] : 0ull;
-        uint64_t cnt = live ? a.in_cnt[idx] : 0ull;
-        uint32_t inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
-        uint64_t occ = key & a.occ_mask;
-        uint32_t used = (uint32_t)(key >> a.occ_bits) & a.used_mask;
-        if (live && a.row >= a.hgt[inst]) {
-            // terminal state of a finished instance: every cell is covered
-            atomicAdd(a.result + inst, (unsigned long long)cnt);
-            live = false;
-        }
-        uint64_t BW = 0, BWn = 0;
-        if (live) {
-            BW = a.bw[(size_t)inst * a.bw_stride + a.row];
-            BWn = a.bw[(size_t)inst * a.bw_stride + a.row + 1];
-        }
-        uint64_t E = BW & ~occ;
-        uint64_t avail = 0;
-        if (live)
-            for (uint32_t p = 0; p < a.n_pieces; p++) {
-                uint32_t pi = s_pinfo[p];
-                uint32_t f = (used >> (pi & 0xffu)) & ((pi >> 8) & 0xffu);
-                if (f < (pi >> 16)) avail |= s_prange[p];
+        const bool idle = !(searching || pass);
+        const uint64_t im = __ballot(idle);
+        if (im && !exhausted) {
+            uint32_t leader = (uint32_t)__builtin_ctzll(im);
+            uint32_t base = 0;
+            if (lane == leader) {
+                if (__hip_atomic_load(a.c32 + C32_OVERFLOW, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT))
+                    base = n_items;  // table overflow: the step is rerun
+                else
+                    base = atomicAdd(a.c32 + C32_WORK, (uint32_t)__builtin_popcountll(im));
             }
-
-        // DFS over the completions of row r.  Stack: desc holds k+1 per
-        // level (one byte each), anc the anchor column (3 bits each).
-        uint32_t d = 0, c = 0, anc = 0;
-        uint64_t desc = 0, ok = 0;
-        bool searching = false, pass = false;
-        if (live) {
-            if (E & 0xffu) {
-                c = (uint32_t)__builtin_ctzll(E & 0xffu);
-                ok = fitmask(s_fit, rows, c, E) & avail;
-                searching = true;
-            } else {
-                pass = true;  // row already covered by pieces from above
-            }
-        }
-
-        while (__any(searching || pass)) {
-            bool emit = false;
-            uint64_t E2 = 0, desc2 = 0;
-            uint32_t used2 = 0;
-            if (pass) {
-                emit = true;
-                E2 = E;
-                used2 = used;
-                pass = false;
-            } else if (searching) {
-                if (ok) {
-                    uint32_t k = (uint32_t)__builtin_ctzll(ok);
-                    ok &= ok - 1;
-                    uint64_t pl = s_place[c * MAX_CAND + k];
-                    uint32_t ci = s_cinfo[k];
-                    E2 = E & ~pl;
-                    uint32_t inc = 1u << (ci & 0xffu);
-                    if (pl & ~E) {
-                        // fails in a window row the fit table does not cover
-                    } else if (!(E2 & 0xffu)) {
-                        emit = true;
-                        used2 = used + inc;
-                        desc2 = desc | ((uint64_t)(k + 1) << (8 * d));
-                    } else {
-                        E = E2;
-                        used += inc;
-                        if (((used >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
-                            avail &= ~s_range[k];
-                        desc |= (uint64_t)(k + 1) << (8 * d);
-                        anc |= c << (3 * d);
-                        d++;
+            base = __shfl(base, (int)leader);
+            if (base >= n_items) exhausted = true;
+            const uint32_t item = base + mbcnt(im);
+            if (idle && item < n_items) {
+                rec = a.split ? item >> 6 : item;
+                const uint32_t k0 = item & 63u;
+                const bool lead_item = !a.split || k0 == 0;
+                uint64_t key = a.in_key[rec];
+                cnt = a.in_cnt[rec];
+                inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
+                const uint64_t occ = key & a.occ_mask;
+                used = (uint32_t)(key >> a.occ_bits) & a.used_mask;
+                if (a.row >= a.hgt[inst]) {
+                    // terminal state of a finished instance: all cells covered
+                    if (lead_item && !COUNT_ONLY)
+                        atomicAdd(a.result + inst, (unsigned long long)cnt);
+                } else {
+                    BW = a.bw[(size_t)inst * a.bw_stride + a.row];
+                    BWn = a.bw[(size_t)inst * a.bw_stride + a.row + 1];
+                    E = BW & ~occ;
+                    d = 0;
+                    anc = 0;
+                    desc = 0;
+                    if (E & 0xffu) {
+                        avail = 0;
+                        for (uint32_t p = 0; p < a.n_pieces; p++) {
+                            uint32_t pi = s_pinfo[p];
+                            uint32_t f = (used >> (pi & 0xffu)) & ((pi >> 8) & 0xffu);
+                            if (f < (pi >> 16)) avail |= s_prange[p];
+                        }
                         c = (uint32_t)__builtin_ctzll(E & 0xffu);
                         ok = fitmask(s_fit, rows, c, E) & avail;
-                    }
-                } else if (d == 0) {
-                    searching = false;
-                } else {
-                    d--;
-                    uint32_t k = (uint32_t)((desc >> (8 * d)) & 0xffu) - 1u;
-                    c = (anc >> (3 * d)) & 7u;
-                    desc &= ~(0xffull << (8 * d));
-                    anc &= ~(7u << (3 * d));
-                    E |= s_place[c * MAX_CAND + k];
-                    used -= 1u << (s_cinfo[k] & 0xffu);
-                    avail |= s_range[k];
-                    ok = fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1);
-                }
-            }
-            bool keep = emit;
-            if (emit && PRUNE) {
-                uint64_t Ec = (E2 >> 8) | (BWn & 0xff00000000000000ull);
-                keep = viable(Ec, a.gcd, a.min_size);
-                n_pruned += keep ? 0 : 1;
-            }
-            uint64_t slot = 0;
-            if (keep) {
-                n_children++;
-                uint64_t occ2 = (BW & ~E2) >> 8;
-                uint64_t key2 = occ2 | ((uint64_t)used2 << a.occ_bits);
-                if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
-                int r = table_insert(a, key2, cnt, slot);
-                if (r < 0) {
-                    atomicOr(a.c32 + C32_OVERFLOW, 1u);
-                    keep = false;
-                } else {
-                    n_new += (uint64_t)r;
-                }
-            }
-            if (ENUM) {
-                uint32_t e = wave_alloc(a.c32 + C32_EDGES, keep);
-                if (keep) {
-                    if (e < a.e_cap) {
-                        a.e_parent[e] = idx;
-                        a.e_desc[e] = desc2;
-                        a.e_slot[e] = slot;
+                        if (a.split) ok &= 1ull << k0;
+                        searching = ok != 0;
                     } else {
-                        atomicOr(a.c32 + C32_EDGE_OVF, 1u);
+                        pass = lead_item;  // row already covered from above
                     }
+                }
+            }
+        }
+        if (!__any(searching || pass)) {
+            if (exhausted) break;
+            continue;
+        }
+
+        // one DFS step over the completions of row r.  Stack: desc holds k+1
+        // per level (one byte each), anc the anchor column (3 bits each).
+        bool emit = false;
+        uint64_t E2 = 0, desc2 = 0;
+        uint32_t used2 = 0;
+        if (pass) {
+            emit = true;
+            E2 = E;
+            used2 = used;
+            pass = false;
+        } else if (searching) {
+            if (ok) {
+                uint32_t k = (uint32_t)__builtin_ctzll(ok);
+                ok &= ok - 1;
+                uint64_t pl = s_place[c * MAX_CAND + k];
+                uint32_t ci = s_cinfo[k];
+                E2 = E & ~pl;
+                uint32_t inc = 1u << (ci & 0xffu);
+                if (pl & ~E) {
+                    // fails in a window row the fit table does not cover
+                } else if (!(E2 & 0xffu)) {
+                    emit = true;
+                    used2 = used + inc;
+                    desc2 = desc | ((uint64_t)(k + 1) << (8 * d));
+                } else {
+                    E = E2;
+                    used += inc;
+                    if (((used >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
+                        avail &= ~s_range[k];
+                    desc |= (uint64_t)(k + 1) << (8 * d);
+                    anc |= c << (3 * d);
+                    d++;
+                    c = (uint32_t)__builtin_ctzll(E & 0xffu);
+                    ok = fitmask(s_fit, rows, c, E) & avail;
+                }
+            } else if (d == 0) {
+                searching = false;
+            } else {
+                d--;
+                uint32_t k = (uint32_t)((desc >> (8 * d)) & 0xffu) - 1u;
+                c = (anc >> (3 * d)) & 7u;
+                desc &= ~(0xffull << (8 * d));
+                anc &= ~(7u << (3 * d));
+                E |= s_place[c * MAX_CAND + k];
+                used -= 1u << (s_cinfo[k] & 0xffu);
+                avail |= s_range[k];
+                ok = d || !a.split ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1) : 0;
+            }
+        }
+        bool keep = emit;
+        if (emit && PRUNE) {
+            uint64_t Ec = (E2 >> 8) | (BWn & 0xff00000000000000ull);
+            keep = viable(Ec, a.gcd, a.min_size);
+            n_pruned += keep ? 0 : 1;
+        }
+        uint64_t slot = 0;
+        if (COUNT_ONLY) {
+            n_children += keep ? 1 : 0;
+            continue;
+        }
+        if (keep) {
+            n_children++;
+            uint64_t occ2 = (BW & ~E2) >> 8;
+            uint64_t key2 = occ2 | ((uint64_t)used2 << a.occ_bits);
+            if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
+            int r = table_insert(a, key2, cnt, slot);
+            if (r < 0) {
+                // table too small: flag it, stop this search; the host
+                // reruns the step with a larger table
+                atomicOr(a.c32 + C32_OVERFLOW, 1u);
+                keep = false;
+                searching = false;
+                pass = false;
+            } else {
+                n_new += (uint64_t)r;
+            }
+        }
+        if (ENUM) {
+            uint32_t e = wave_alloc(a.c32 + C32_EDGES, keep);
+            if (keep) {
+                if (e < a.e_cap) {
+                    a.e_parent[e] = rec;
+                    a.e_desc[e] = desc2;
+                    a.e_slot[e] = slot;
+                } else {
+                    atomicOr(a.c32 + C32_EDGE_OVF, 1u);
                 }
             }
         }
@@ -396,21 +429,24 @@ void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t s
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c32, c64);
 }
 
-void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, int n_cu,
-                   hipStream_t st) {
+void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_only,
+                   int n_cu, hipStream_t st) {
     size_t lds = extend_lds_bytes(a.fit_rows);
-    uint64_t waves = ((uint64_t)a.n_in + 63) / 64;
+    uint64_t waves = ((uint64_t)a.n_in * (a.split ? 64u : 1u) + 63) / 64;
     uint64_t blocks = (waves + EXT_THREADS / 64 - 1) / (EXT_THREADS / 64);
     uint64_t cap = (uint64_t)n_cu * (lds <= 80 * 1024 ? 2 : 1);
     if (blocks > cap) blocks = cap;
     if (!blocks) blocks = 1;
     dim3 g((uint32_t)blocks), b(EXT_THREADS);
-    if (enum_mode) {
-        if (prune) hipLaunchKernelGGL((k_extend<true, true>), g, b, lds, st, a);
-        else hipLaunchKernelGGL((k_extend<true, false>), g, b, lds, st, a);
+    if (count_only) {
+        if (prune) hipLaunchKernelGGL((k_extend<false, true, true>), g, b, lds, st, a);
+        else hipLaunchKernelGGL((k_extend<false, false, true>), g, b, lds, st, a);
+    } else if (enum_mode) {
+        if (prune) hipLaunchKernelGGL((k_extend<true, true, false>), g, b, lds, st, a);
+        else hipLaunchKernelGGL((k_extend<true, false, false>), g, b, lds, st, a);
     } else {
-        if (prune) hipLaunchKernelGGL((k_extend<false, true>), g, b, lds, st, a);
-        else hipLaunchKernelGGL((k_extend<false, false>), g, b, lds, st, a);
+        if (prune) hipLaunchKernelGGL((k_extend<false, true, false>), g, b, lds, st, a);
+        else hipLaunchKernelGGL((k_extend<false, false, false>), g, b, lds, st, a);
     }
 }
 

@@ -45,6 +45,13 @@ class Stats(ctypes.Structure):
         ("peak_frontier", ctypes.c_uint64),
         ("retries", ctypes.c_uint64),
         ("gpu_ms", ctypes.c_double),
+        ("extend_ms", ctypes.c_double),
+        ("extend_launches", ctypes.c_uint64),
+        ("extend_bytes", ctypes.c_uint64),
+        ("compact_ms", ctypes.c_double),
+        ("compact_launches", ctypes.c_uint64),
+        ("compact_bytes", ctypes.c_uint64),
+        ("count_passes", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -62,6 +69,7 @@ _SZP = ctypes.POINTER(ctypes.c_size_t)
 SIGNATURES = [
     ("d4_last_error", ctypes.c_char_p, []),
     ("d4_device_count", ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    ("d4_device_synchronize", ctypes.c_int, []),
     ("d4_free_polyominoes", ctypes.c_size_t, [ctypes.c_uint, _U64P, ctypes.c_size_t]),
     ("d4_library_create", ctypes.c_int,
      [_U64P, _U32P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
@@ -121,6 +129,10 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     _check(load().d4_device_count(ctypes.byref(n)))
     return n.value
+
+
+def synchronize() -> None:
+    _check(load().d4_device_synchronize())
 
 
 def free_polyominoes(n: int) -> List[int]:

@@ -63,6 +63,15 @@ typedef struct d4_stats {
     uint64_t peak_frontier;    /* largest frontier (records)                */
     uint64_t retries;          /* row steps re-run after a table overflow   */
     double   gpu_ms;           /* device time of the row steps              */
+    double   extend_ms;        /* device time inside the extend kernel      */
+    uint64_t extend_launches;  /* extend launches (retries included)        */
+    uint64_t extend_bytes;     /* algorithmic HBM bytes of those launches:
+                                  16 per parent read + 16 per child merged  */
+    double   compact_ms;       /* device time inside the compaction kernel  */
+    uint64_t compact_launches;
+    uint64_t compact_bytes;    /* 8 per table slot scanned + 24 per unique
+                                  state gathered and written                */
+    uint64_t count_passes;     /* search-only passes sizing the dedup table */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -70,6 +79,9 @@ const char *d4_last_error(void);
 
 /* Number of visible HIP devices (0 when none). */
 int d4_device_count(int *n);
+
+/* Wait for all work on the current device (hipDeviceSynchronize). */
+int d4_device_synchronize(void);
 
 /* Free polyominoes with n cells (1 <= n <= 8), each as its canonical value
  * (minimum over the 8 D4 transforms of the normalised 8-wide encoding),

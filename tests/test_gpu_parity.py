@@ -77,7 +77,7 @@ def test_centre_hole_ea_constructor(gpu, lib8, oracle, p5):
 
 
 def test_seeded_8x8_batch(gpu, lib8, oracle, p5):
-    boards = workloads.seeded_8x8(12, seed=2)
+    boards = workloads.seeded_8x8(6, seed=2)
     b = gpu.Batch(lib8, boards)
     b.run()
     got = b.counts()
@@ -105,7 +105,7 @@ def test_prune_does_not_change_counts(gpu, lib8):
 
 def test_dominoes(gpu):
     fib = [1, 1]
-    for _ in range(30):
+    for _ in range(60):
         fib.append(fib[-1] + fib[-2])
     lib = gpu.PieceLibrary([3], [63], 0xFF, 2)
     for n in (1, 2, 7, 20, 40):
