@@ -9,6 +9,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -101,6 +102,72 @@ static uint64_t next_pow2(uint64_t v) {
     return p;
 }
 
+// Caching device allocator: freed blocks stay with the process and are
+// reused by later searches of similar size (hipMalloc / hipFree of multi-GB
+// tables per row would otherwise dominate short searches).  Blocks are
+// rounded to 2 MiB; a request takes the smallest cached block of at least its
+// size and at most twice it.  When the device runs out, the cache is
+// released and the allocation retried once.
+class Pool {
+    std::mutex mu_;
+    std::multimap<size_t, std::pair<int, void *>> free_;  // bytes -> (device, ptr)
+
+    void trim_locked() {
+        for (auto &kv : free_) {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(kv.second.first);
+            (void)hipFree(kv.second.second);
+            (void)hipSetDevice(cur);
+        }
+        free_.clear();
+    }
+
+  public:
+    static size_t round(size_t bytes) {
+        const size_t g = 2u << 20;
+        return (std::max<size_t>(bytes, 1) + g - 1) / g * g;
+    }
+    void *alloc(size_t bytes) {
+        bytes = round(bytes);
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (auto it = free_.lower_bound(bytes); it != free_.end() && it->first <= 2 * bytes; ++it)
+                if (it->second.first == dev) {
+                    void *p = it->second.second;
+                    free_.erase(it);
+                    return p;
+                }
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            std::lock_guard<std::mutex> lk(mu_);
+            trim_locked();
+            e = hipMalloc(&p, bytes);
+        }
+        if (e != hipSuccess)
+            throw Error(e == hipErrorOutOfMemory ? D4_ENOMEM : D4_EDEVICE,
+                        std::string("hipMalloc: ") + hipGetErrorString(e));
+        return p;
+    }
+    void release(void *p, size_t bytes) {
+        if (!p) return;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(round(bytes), std::make_pair(dev, p));
+    }
+};
+
+static Pool &pool() {
+    static Pool *p = new Pool();  // never destroyed: outlives every handle
+    return *p;
+}
+
 template <class T>
 struct DevBuf {
     T *p = nullptr;
@@ -110,16 +177,20 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) pool().release(p, cap * sizeof(T));
         p = nullptr;
         cap = 0;
     }
     void reserve(uint64_t n) {
         if (n <= cap) return;
         release();
-        uint64_t c = std::max<uint64_t>(n, 64);
-        HIPCHK(hipMalloc((void **)&p, c * sizeof(T)));
+        uint64_t c = Pool::round(std::max<uint64_t>(n, 64) * sizeof(T)) / sizeof(T);
+        p = static_cast<T *>(pool().alloc(c * sizeof(T)));
         cap = c;
+    }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
     }
 };
 
@@ -140,20 +211,20 @@ struct d4_library {
     uint64_t place[8][MAX_CAND] = {};
     uint64_t range[MAX_CAND] = {};
     uint32_t cinfo[MAX_CAND] = {};
-    uint32_t pinfo[MAX_PIECES] = {};
     uint64_t prange[MAX_PIECES] = {};
+    uint64_t xavail[4 * 256] = {};
     std::vector<uint64_t> fit;  // [8][maxh][256]
     unsigned long long comp[256] = {};
     uint32_t gcd = 1, min_size = 1;
     int device = 0;
     int n_cu = 1;
-    uint64_t *d_fit = nullptr, *d_place = nullptr, *d_range = nullptr, *d_prange = nullptr;
-    uint32_t *d_cinfo = nullptr, *d_pinfo = nullptr;
+    uint64_t *d_fit = nullptr, *d_place = nullptr, *d_range = nullptr, *d_xavail = nullptr;
+    uint32_t *d_cinfo = nullptr;
     unsigned long long *d_comp = nullptr;
 
     ~d4_library() {
-        for (void *p : {(void *)d_fit, (void *)d_place, (void *)d_range, (void *)d_prange,
-                        (void *)d_cinfo, (void *)d_pinfo, (void *)d_comp})
+        for (void *p : {(void *)d_fit, (void *)d_place, (void *)d_range, (void *)d_xavail,
+                        (void *)d_cinfo, (void *)d_comp})
             if (p) (void)hipFree(p);
     }
 };
@@ -167,6 +238,7 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
         L.counts[i] = counts[i];
         L.sizes[i] = (uint32_t)__builtin_popcountll(s);
         L.fbits[i] = bitw(counts[i]);
+        if ((off % 8) + L.fbits[i] > 8) off = (off + 7) & ~7u;  // no field straddles a byte
         L.off[i] = off;
         off += L.fbits[i];
         if (counts[i]) {
@@ -203,10 +275,21 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
         for (uint32_t j = first; j < k; j++) L.range[j] = r;
         uint32_t info = L.off[i] | (((1u << L.fbits[i]) - 1u) << 8) | (counts[i] << 16);
         for (uint32_t j = first; j < k; j++) L.cinfo[j] = info;
-        L.pinfo[i] = info;
-        L.prange[i] = counts[i] ? r : 0;
+        L.prange[i] = r;
     }
     if (off > 32) throw Error(D4_EINVAL, "piece counters need more than 32 bits");
+    // xavail[b][v]: candidates of the pieces whose counter lies in byte b and
+    // is exhausted when that byte reads v
+    for (uint32_t b = 0; b < 4; b++)
+        for (uint32_t v = 0; v < 256; v++) {
+            uint64_t m = 0;
+            for (uint32_t i = 0; i < L.n; i++) {
+                if (L.off[i] / 8 != b && !(L.fbits[i] == 0 && b == 0)) continue;
+                uint32_t f = L.fbits[i] ? (v >> (L.off[i] % 8)) & ((1u << L.fbits[i]) - 1u) : 0;
+                if (f >= L.counts[i]) m |= L.prange[i];
+            }
+            L.xavail[b * 256 + v] = m;
+        }
     if (L.maxh > (uint32_t)MAX_FIT_ROWS) throw Error(D4_EINVAL, "piece taller than 6 rows");
     L.used_bits = off;
     L.ncand = k;
@@ -292,7 +375,7 @@ struct Engine {
     DevBuf<unsigned long long> tk, tc;
     DevBuf<uint32_t> c32;
     DevBuf<unsigned long long> c64;
-    DevBuf<uint32_t> e_parent, e_child, slot_uid;
+    DevBuf<uint32_t> e_parent, e_child, slot_uid, blk;
     DevBuf<uint64_t> e_desc, e_slot;
     uint64_t n_front = 0;
     double grow_unique = 4.0, grow_children = 8.0;
@@ -381,16 +464,14 @@ struct Engine {
         a.in_key = fk.p;
         a.in_cnt = fc.p;
         a.n_in = (uint32_t)n_front;
-        a.split = n_front <= (1u << 16) ? 1u : 0u;
+        a.split = n_front <= 1024 ? 2u : n_front <= (1u << 16) ? 1u : 0u;
         a.row = row;
         a.fit = lib->d_fit;
         a.place = lib->d_place;
         a.range = lib->d_range;
         a.cinfo = lib->d_cinfo;
-        a.pinfo = lib->d_pinfo;
-        a.prange = lib->d_prange;
+        a.xavail = lib->d_xavail;
         a.fit_rows = lib->fit_rows;
-        a.n_pieces = lib->n;
         a.bw = bw.p;
         a.bw_stride = bw_stride;
         a.hgt = hgt.p;
@@ -409,6 +490,8 @@ struct Engine {
         a.e_desc = e_desc.p;
         a.e_slot = e_slot.p;
         a.e_cap = e_cap;
+        // rows (relative to the child's row) that no placed piece reaches
+        a.seed_mask = lib->maxh >= 2 ? ~((1ull << (8 * (lib->maxh - 1))) - 1ull) : ~0ull;
         a.gcd = lib->gcd;
         a.min_size = lib->min_size;
         return a;
@@ -433,14 +516,25 @@ struct Engine {
         // exact count from a search-only pass; a large one is estimated from
         // the previous row's children per record (x2), with a rerun at 4x
         // the size if the table still overflows.
-        uint64_t est_c;
-        if (n_front <= kExactCountMax) {
-            est_c = count_children();
-            stats.count_passes++;
-        } else {
-            est_c = (uint64_t)(n_front * std::min(std::max(grow_children, 1.0), 64.0) * 2.0) + 4096;
+        uint64_t est_c = 0, est_u = 0;
+        bool exact = false;
+        if (n_front <= (1u << 20)) {
+            // children <= sum over records of the row-0 completions (cheap)
+            const uint64_t bound = next_size();
+            if (bound <= (1ull << 26)) {
+                est_c = est_u = bound;
+                exact = true;
+            } else if (n_front <= kExactCountMax) {
+                est_c = est_u = count_children();
+                stats.count_passes++;
+                exact = true;
+            }
         }
-        uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_c, 1024));
+        if (!exact) {
+            est_c = (uint64_t)(n_front * std::min(std::max(grow_children, 1.0), 64.0) * 1.5) + 4096;
+            est_u = std::min<uint64_t>(est_c, (uint64_t)(n_front * std::max(grow_unique, 0.25) * 1.5) + 4096);
+        }
+        uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
         uint64_t est_e = en ? est_c + 64 : 0;
         if (en) {
             RowLog lg;
@@ -508,8 +602,9 @@ struct Engine {
         nc.reserve(U);
         if (en) slot_uid.reserve(tab_m);
         HIPCHK(hipEventRecord(evc0, st));
-        launch_compact(tk.p, tc.p, tab_m, nk.p, nc.p, c32.p, en ? slot_uid.p : nullptr,
-                       lib->n_cu, st);
+        blk.reserve(2 * COMPACT_MAX_BLOCKS);
+        launch_compact(tk.p, tc.p, tab_m, nk.p, nc.p, c32.p, blk.p, en ? slot_uid.p : nullptr,
+                       st);
         uint32_t n_edges = en ? h32[C32_EDGES] : 0;
         if (en && n_edges) {
             e_child.reserve(n_edges);
@@ -536,10 +631,8 @@ struct Engine {
                 HIPCHK(hipMemcpy(lg.e_child.data(), e_child.p, n_edges * 4, hipMemcpyDeviceToHost));
             }
         }
-        std::swap(fk.p, nk.p);
-        std::swap(fk.cap, nk.cap);
-        std::swap(fc.p, nc.p);
-        std::swap(fc.cap, nc.cap);
+        fk.swap(nk);
+        fc.swap(nc);
         n_front = U;
         if (!U) fk.reserve(1);
         stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, U);
@@ -769,8 +862,7 @@ int d4_library_create(const uint64_t *shapes, const uint32_t *counts, uint32_t n
         upload(L->d_place, &L->place[0][0], 8 * MAX_CAND);
         upload(L->d_range, L->range, MAX_CAND);
         upload(L->d_cinfo, L->cinfo, MAX_CAND);
-        upload(L->d_pinfo, L->pinfo, MAX_PIECES);
-        upload(L->d_prange, L->prange, MAX_PIECES);
+        upload(L->d_xavail, L->xavail, 4 * 256);
         upload(L->d_comp, L->comp, 256);
         *out = L.release();
     });

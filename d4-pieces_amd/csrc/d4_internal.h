@@ -19,7 +19,7 @@ constexpr int MAX_ROWS = 64;     // board height
 
 // Counter slots of the per-step device counter block.
 enum : int {
-    C32_WORK = 0,      // dynamic record fetch (extend / next_size)
+    C32_RESERVED = 0,
     C32_OVERFLOW = 1,  // hash probe limit hit
     C32_EDGES = 2,     // edges written (enum mode)
     C32_EDGE_OVF = 3,  // edge buffer overflow
@@ -31,7 +31,8 @@ enum : int {
     C64_NEW = 1,
     C64_PRUNED = 2,
     C64_NEXTSIZE = 3,
-    C64_N = 4,
+    C64_WORK = 4,      // dynamic work-item fetch (extend)
+    C64_N = 5,
 };
 
 // Arguments of the extend kernel (passed by value).
@@ -40,16 +41,14 @@ struct ExtendArgs {
     const uint64_t *in_cnt;
     uint32_t n_in;
     uint32_t row;
-    uint32_t split;          // 1: work item = (record, first placement)
+    uint32_t split;          // levels fixed per work item: 0, 1 or 2
     // library tables (global copies; staged into LDS)
     const uint64_t *fit;     // [8][fit_rows][256] candidate masks
     const uint64_t *place;   // [8][64] placed shape per anchor column
     const uint64_t *range;   // [64] mask of the candidate's piece
     const uint32_t *cinfo;   // [64] counter offset | field mask << 8 | count << 16
-    const uint32_t *pinfo;   // [n_pieces] same packing per piece
-    const uint64_t *prange;  // [n_pieces]
+    const uint64_t *xavail;  // [4][256] candidates ruled out per counter byte
     uint32_t fit_rows;
-    uint32_t n_pieces;
     // instances
     const uint64_t *bw;      // [n_inst][bw_stride] board window of row r
     uint32_t bw_stride;
@@ -74,6 +73,7 @@ struct ExtendArgs {
     uint64_t *e_slot;
     uint32_t e_cap;
     // pruning
+    uint64_t seed_mask;      // window rows no placed piece reaches yet
     uint32_t gcd;
     uint32_t min_size;
 };
@@ -85,10 +85,11 @@ void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
                    bool count_only, int n_cu, hipStream_t st);
+constexpr uint32_t COMPACT_MAX_BLOCKS = 65536;
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
                     uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
-                    uint32_t *slot_uid, int n_cu, hipStream_t st);
+                    uint32_t *blk, uint32_t *slot_uid, hipStream_t st);
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
                         uint32_t *e_child, uint32_t n, hipStream_t st);
 void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,

@@ -76,27 +76,19 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 
 // Insert (key, cnt) into the table: returns 1 when the key is new, 0 when it
 // merged into an existing slot, -1 when the probe limit was reached.  Keys
-// only ever change EMPTY -> key, so a stale plain read is harmless: EMPTY
-// leads to the CAS, which returns the true value.
+// only ever change EMPTY -> key, so the CAS's returned value decides: EMPTY
+// means this lane claimed the slot, the own key means a merge.  One round
+// trip per probe; the count update does not wait for its result.
 __device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
                                             uint64_t cnt, uint64_t &slot) {
     uint64_t h = mix64(key) & a.tab_mask;
     for (uint32_t p = 0; p < a.max_probe; p++) {
-        uint64_t cur = __hip_atomic_load(a.tab_key + h, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-        if (cur == EMPTY_KEY) {
-            cur = atomicCAS(a.tab_key + h, (unsigned long long)EMPTY_KEY,
-                            (unsigned long long)key);
-            if (cur == EMPTY_KEY) {
-                atomicAdd(a.tab_cnt + h, (unsigned long long)cnt);
-                slot = h;
-                return 1;
-            }
-        }
-        if (cur == key) {
+        uint64_t cur = atomicCAS(a.tab_key + h, (unsigned long long)EMPTY_KEY,
+                                 (unsigned long long)key);
+        if (cur == EMPTY_KEY || cur == key) {
             atomicAdd(a.tab_cnt + h, (unsigned long long)cnt);
             slot = h;
-            return 0;
+            return cur == EMPTY_KEY ? 1 : 0;
         }
         h = (h + 1) & a.tab_mask;
     }
@@ -110,16 +102,18 @@ __device__ __forceinline__ uint64_t grow4(uint64_t x) {
     return x | (x << 8) | (x >> 8) | ((x << 1) & ~COL0) | ((x >> 1) & ~COL7);
 }
 
-// Connectivity pruning of a child's open window E (8 rows, the last of which
-// continues into rows not in the window).  Every 4-connected open region
-// that does not reach the last row is closed: its area must be a multiple of
+// Connectivity pruning of a child's open window E (8 rows from the child's
+// row).  Rows selected by `seed` hold no piece yet (pieces reach at most
+// max_height-2 rows below the child's row), so every open region that meets
+// them is left alone; every other 4-connected open region is closed -- it
+// does not reach the window's last row -- and its area must be a multiple of
 // the gcd of the piece sizes and at least the smallest piece, or no tiling
 // can complete it.  (Shape::extend1 / connected / sml_island semantics,
 // src/Shape.cpp:81-123, applied to the live board at run time instead of the
 // reference's static table-time island test, src/frow.cpp:78-100.)
-__device__ __forceinline__ bool viable(uint64_t E, uint32_t g,
+__device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g,
                                        uint32_t min_size) {
-    uint64_t x = E & 0xff00000000000000ull;
+    uint64_t x = E & seed;
     for (;;) {
         uint64_t y = grow4(x) & E;
         if (y == x) break;
@@ -150,6 +144,20 @@ __device__ __forceinline__ uint64_t fitmask(const uint64_t *s_fit,
     return m;
 }
 
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
+    uint32_t lo = __shfl((uint32_t)v, (int)src);
+    uint32_t hi = __shfl((uint32_t)(v >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Candidates still available under the use counters `used`: counter fields
+// never straddle a byte, so each byte indexes a table of the candidates its
+// exhausted pieces rule out.
+__device__ __forceinline__ uint64_t avail_of(const uint64_t *s_xa, uint32_t used) {
+    return ~(s_xa[used & 0xffu] | s_xa[256 + ((used >> 8) & 0xffu)] |
+             s_xa[512 + ((used >> 16) & 0xffu)] | s_xa[768 + (used >> 24)]);
+}
+
 // COUNT_ONLY runs the same search but only counts the surviving children
 // (used to size the dedup table exactly when the frontier is small).
 template <bool ENUM, bool PRUNE, bool COUNT_ONLY>
@@ -159,19 +167,15 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
     uint64_t *s_fit = lds;
     uint64_t *s_place = s_fit + nfit;
     uint64_t *s_range = s_place + 8 * MAX_CAND;
-    uint64_t *s_prange = s_range + MAX_CAND;
-    uint32_t *s_cinfo = reinterpret_cast<uint32_t *>(s_prange + MAX_PIECES);
-    uint32_t *s_pinfo = s_cinfo + MAX_CAND;
+    uint64_t *s_xa = s_range + MAX_CAND;
+    uint32_t *s_cinfo = reinterpret_cast<uint32_t *>(s_xa + 4 * 256);
     for (uint32_t i = threadIdx.x; i < nfit; i += blockDim.x) s_fit[i] = a.fit[i];
     for (uint32_t i = threadIdx.x; i < 8 * MAX_CAND; i += blockDim.x)
         s_place[i] = a.place[i];
+    for (uint32_t i = threadIdx.x; i < 4 * 256; i += blockDim.x) s_xa[i] = a.xavail[i];
     if (threadIdx.x < MAX_CAND) {
         s_range[threadIdx.x] = a.range[threadIdx.x];
         s_cinfo[threadIdx.x] = a.cinfo[threadIdx.x];
-    }
-    if (threadIdx.x < MAX_PIECES) {
-        s_prange[threadIdx.x] = threadIdx.x < a.n_pieces ? a.prange[threadIdx.x] : 0;
-        s_pinfo[threadIdx.x] = threadIdx.x < a.n_pieces ? a.pinfo[threadIdx.x] : 0;
     }
     __syncthreads();
 
@@ -179,74 +183,132 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
     const uint32_t rows = a.fit_rows;
     uint64_t n_children = 0, n_new = 0, n_pruned = 0;
 
-    // Lanes are refilled independently: a lane whose DFS is over takes the
-    // next work item (wave-aggregated fetch, one atomic per refill), so a
-    // record with a large subtree does not idle the rest of its wave.  In
-    // split mode a work item is (record, first placement) -- 64 items per
-    // record -- so tiny frontiers (the first rows) still fill the GPU.
-    const uint32_t n_items = a.split ? a.n_in * 64u : a.n_in;
+    // Work distribution.  A work item is a record, or with split levels
+    // fix = 1 / 2 a (record, first placement[, second placement]) triple --
+    // 64 / 4096 items per record -- so tiny frontiers (the first rows) still
+    // fill the GPU.  A wave claims 64 consecutive items with one atomic and
+    // loads the records they cover with one coalesced access (lane j holds
+    // record j of the chunk with its board windows); lanes whose search is
+    // over take the next item of the chunk by shuffle, so a record with a
+    // large subtree does not idle the rest of its wave and no lane waits on
+    // memory per record.
+    const uint32_t fix = a.split;
+    const uint32_t per_rec = 1u << (6 * fix);
+    const uint64_t n_items = (uint64_t)a.n_in * per_rec;
     bool exhausted = false;
+    uint32_t buf_items = 0, buf_pos = 0, buf_rec0 = 0;
+    uint64_t buf_item0 = 0;
+    uint64_t bkey = 0, bcnt = 0, bBW = 0, bBWn = 0;
+    uint32_t bterm = 0;
+
     bool searching = false, pass = false;
     uint32_t rec = 0, inst = 0, used = 0, d = 0, c = 0, anc = 0;
     uint64_t cnt = 0, BW = 0, BWn = 0, E = 0, avail = 0, desc = 0, ok = 0;
 
     for (;;) {
-        const bool idle = !(searching || pass);
-        const uint64_t im = __ballot(idle);
-        if (im && !exhausted) {
-            uint32_t leader = (uint32_t)__builtin_ctzll(im);
-            uint32_t base = 0;
-            if (lane == leader) {
-                if (__hip_atomic_load(a.c32 + C32_OVERFLOW, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT))
-                    base = n_items;  // table overflow: the step is rerun
-                else
-                    base = atomicAdd(a.c32 + C32_WORK, (uint32_t)__builtin_popcountll(im));
+        uint64_t im = __ballot(!(searching || pass));
+        while (im) {
+            if (buf_pos >= buf_items) {
+                if (exhausted) break;
+                uint64_t base = 0;
+                if (lane == 0) {
+                    if (__hip_atomic_load(a.c32 + C32_OVERFLOW, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT))
+                        base = n_items;  // table overflow: the step is rerun
+                    else
+                        base = atomicAdd(a.c64 + C64_WORK, 64ull);
+                }
+                base = shfl64(base, 0);
+                if (base >= n_items) {
+                    exhausted = true;
+                    break;
+                }
+                buf_item0 = base;
+                buf_items = (uint32_t)min((uint64_t)64, n_items - base);
+                buf_pos = 0;
+                buf_rec0 = (uint32_t)(base >> (6 * fix));
+                const uint32_t nrec = fix ? 1u : buf_items;
+                if (lane < nrec) {
+                    bkey = a.in_key[buf_rec0 + lane];
+                    bcnt = a.in_cnt[buf_rec0 + lane];
+                    uint32_t bi = a.inst_shift < 64 ? (uint32_t)(bkey >> a.inst_shift) : 0u;
+                    bterm = a.row >= a.hgt[bi];
+                    const uint64_t *w = a.bw + (size_t)bi * a.bw_stride + a.row;
+                    bBW = bterm ? 0 : w[0];
+                    bBWn = bterm ? 0 : w[1];
+                }
             }
-            base = __shfl(base, (int)leader);
-            if (base >= n_items) exhausted = true;
-            const uint32_t item = base + mbcnt(im);
-            if (idle && item < n_items) {
-                rec = a.split ? item >> 6 : item;
-                const uint32_t k0 = item & 63u;
-                const bool lead_item = !a.split || k0 == 0;
-                uint64_t key = a.in_key[rec];
-                cnt = a.in_cnt[rec];
+            const bool idle = (im >> lane) & 1u;
+            const uint32_t q = buf_pos + mbcnt(im);
+            const uint32_t take = min((uint32_t)__builtin_popcountll(im), buf_items - buf_pos);
+            buf_pos += take;
+            // every lane joins the shuffles; idle lanes with q in range keep the result
+            const uint32_t src = fix ? 0u : min(q, 63u);
+            const uint64_t key = shfl64(bkey, src);
+            const uint64_t kc = shfl64(bcnt, src);
+            const uint64_t kBW = shfl64(bBW, src);
+            const uint64_t kBWn = shfl64(bBWn, src);
+            const uint32_t kterm = __shfl(bterm, (int)src);
+            if (idle && q < buf_items) {
+                const uint64_t item = buf_item0 + q;
+                const uint32_t sub = (uint32_t)(item & (per_rec - 1));  // k0 << 6 | k1
+                const bool lead_item = sub == 0;
+                rec = buf_rec0 + (fix ? 0u : q);
+                cnt = kc;
                 inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
-                const uint64_t occ = key & a.occ_mask;
                 used = (uint32_t)(key >> a.occ_bits) & a.used_mask;
-                if (a.row >= a.hgt[inst]) {
+                if (kterm) {
                     // terminal state of a finished instance: all cells covered
                     if (lead_item && !COUNT_ONLY)
                         atomicAdd(a.result + inst, (unsigned long long)cnt);
                 } else {
-                    BW = a.bw[(size_t)inst * a.bw_stride + a.row];
-                    BWn = a.bw[(size_t)inst * a.bw_stride + a.row + 1];
-                    E = BW & ~occ;
+                    BW = kBW;
+                    BWn = kBWn;
+                    E = BW & ~(key & a.occ_mask);
                     d = 0;
                     anc = 0;
                     desc = 0;
                     if (E & 0xffu) {
-                        avail = 0;
-                        for (uint32_t p = 0; p < a.n_pieces; p++) {
-                            uint32_t pi = s_pinfo[p];
-                            uint32_t f = (used >> (pi & 0xffu)) & ((pi >> 8) & 0xffu);
-                            if (f < (pi >> 16)) avail |= s_prange[p];
-                        }
+                        avail = avail_of(s_xa, used);
                         c = (uint32_t)__builtin_ctzll(E & 0xffu);
                         ok = fitmask(s_fit, rows, c, E) & avail;
-                        if (a.split) ok &= 1ull << k0;
+                        if (fix == 1) {
+                            ok &= 1ull << sub;
+                        } else if (fix == 2) {
+                            // commit the first placement k0, keep only k1 next
+                            const uint32_t k0 = sub >> 6, k1 = sub & 63u;
+                            ok &= 1ull << k0;
+                            const uint64_t pl = s_place[c * MAX_CAND + k0];
+                            if (ok && (pl & ~E)) ok = 0;
+                            if (ok) {
+                                const uint32_t ci = s_cinfo[k0];
+                                const uint64_t E1 = E & ~pl;
+                                if (!(E1 & 0xffu)) {
+                                    // k0 completes the row: the k1 == 0 item
+                                    // emits that child on its own
+                                    ok = k1 == 0 ? ok : 0;
+                                } else {
+                                    E = E1;
+                                    used += 1u << (ci & 0xffu);
+                                    if (((used >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
+                                        avail &= ~s_range[k0];
+                                    desc = (uint64_t)(k0 + 1);
+                                    anc = c;
+                                    d = 1;
+                                    c = (uint32_t)__builtin_ctzll(E & 0xffu);
+                                    ok = fitmask(s_fit, rows, c, E) & avail & (1ull << k1);
+                                }
+                            }
+                        }
                         searching = ok != 0;
                     } else {
                         pass = lead_item;  // row already covered from above
                     }
                 }
             }
+            im = __ballot(!(searching || pass));
         }
-        if (!__any(searching || pass)) {
-            if (exhausted) break;
-            continue;
-        }
+        if (!__any(searching || pass)) break;
 
         // one DFS step over the completions of row r.  Stack: desc holds k+1
         // per level (one byte each), anc the anchor column (3 bits each).
@@ -283,7 +345,8 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
                     c = (uint32_t)__builtin_ctzll(E & 0xffu);
                     ok = fitmask(s_fit, rows, c, E) & avail;
                 }
-            } else if (d == 0) {
+            } else if (d + 1 <= max(fix, 1u)) {
+                // levels below `fix` are fixed by the work item
                 searching = false;
             } else {
                 d--;
@@ -294,13 +357,13 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
                 E |= s_place[c * MAX_CAND + k];
                 used -= 1u << (s_cinfo[k] & 0xffu);
                 avail |= s_range[k];
-                ok = d || !a.split ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1) : 0;
+                ok = d >= fix ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1) : 0;
             }
         }
         bool keep = emit;
         if (emit && PRUNE) {
             uint64_t Ec = (E2 >> 8) | (BWn & 0xff00000000000000ull);
-            keep = viable(Ec, a.gcd, a.min_size);
+            keep = viable(Ec, a.seed_mask, a.gcd, a.min_size);
             n_pruned += keep ? 0 : 1;
         }
         uint64_t slot = 0;
@@ -364,22 +427,84 @@ __global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
     if (threadIdx.x < C64_N) c64[threadIdx.x] = 0;
 }
 
-__global__ __launch_bounds__(CMP_THREADS) void k_compact(
-    const unsigned long long *tab_key, const unsigned long long *tab_cnt,
-    uint64_t m, uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
+// Compaction of the dedup table into the next frontier, in slot order (so the
+// frontier is deterministic for a given table size): k_compact_count counts
+// the occupied slots of each block's contiguous range, k_scan_blocks turns
+// the counts into offsets, k_compact_write writes each range's states at its
+// offset, ordered within the range by wave ballot + mbcnt and an LDS prefix
+// over the block's waves.  No global atomics.
+__global__ __launch_bounds__(CMP_THREADS) void k_compact_count(
+    const unsigned long long *tab_key, uint64_t m, uint64_t per_block,
+    uint32_t *blk_cnt) {
+    __shared__ uint32_t s_w[CMP_THREADS / 64];
+    const uint64_t lo = blockIdx.x * per_block;
+    const uint64_t hi = min(m, lo + per_block);
+    uint32_t n = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += CMP_THREADS)
+        n += tab_key[i] != EMPTY_KEY;
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < CMP_THREADS / 64; w++) t += s_w[w];
+        blk_cnt[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of n block counts by one workgroup; total -> c32[C32_OUT].
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t *blk_cnt, uint32_t n,
+                                                      uint32_t *blk_off, uint32_t *c32) {
+    __shared__ uint32_t s_part[1024];
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t lo = threadIdx.x * per, hi = min(n, lo + per);
+    uint32_t t = 0;
+    for (uint32_t i = lo; i < hi; i++) t += blk_cnt[i];
+    s_part[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s_part[threadIdx.x] - t;  // exclusive
+    for (uint32_t i = lo; i < hi; i++) {
+        blk_off[i] = run;
+        run += blk_cnt[i];
+    }
+    if (threadIdx.x == 1023) c32[C32_OUT] = s_part[1023];
+}
+
+__global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
+    const unsigned long long *tab_key, const unsigned long long *tab_cnt, uint64_t m,
+    uint64_t per_block, const uint32_t *blk_off, uint64_t *out_key, uint64_t *out_cnt,
     uint32_t *slot_uid) {
-    const uint64_t waves = (uint64_t)gridDim.x * (CMP_THREADS / 64);
-    const uint64_t w = blockIdx.x * (uint64_t)(CMP_THREADS / 64) + (threadIdx.x >> 6);
-    for (uint64_t b = w * 64; b < m; b += waves * 64) {
-        uint64_t i = b + lane_id();
-        uint64_t k = i < m ? tab_key[i] : EMPTY_KEY;
-        bool valid = k != EMPTY_KEY;
-        uint32_t o = wave_alloc(c32 + C32_OUT, valid);
+    __shared__ uint32_t s_w[CMP_THREADS / 64];
+    const uint64_t lo = blockIdx.x * per_block;
+    const uint64_t hi = min(m, lo + per_block);
+    const uint32_t wv = threadIdx.x >> 6;
+    uint32_t base = blk_off[blockIdx.x];
+    for (uint64_t i0 = lo; i0 < hi; i0 += CMP_THREADS) {
+        const uint64_t i = i0 + threadIdx.x;
+        const uint64_t k = i < hi ? tab_key[i] : EMPTY_KEY;
+        const bool valid = k != EMPTY_KEY;
+        const uint64_t bm = __ballot(valid);
+        if (lane_id() == 0) s_w[wv] = (uint32_t)__builtin_popcountll(bm);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < CMP_THREADS / 64; w++) {
+            before += w < wv ? s_w[w] : 0;
+            total += s_w[w];
+        }
         if (valid) {
+            const uint32_t o = base + before + mbcnt(bm);
             out_key[o] = k;
             out_cnt[o] = tab_cnt[i];
             if (slot_uid) slot_uid[i] = o;
         }
+        base += total;
+        __syncthreads();
     }
 }
 
@@ -413,8 +538,8 @@ __global__ __launch_bounds__(256) void k_next_size(
 }  // namespace
 
 size_t extend_lds_bytes(uint32_t fit_rows) {
-    return sizeof(uint64_t) * (8u * fit_rows * 256u + 8u * MAX_CAND + MAX_CAND + MAX_PIECES) +
-           sizeof(uint32_t) * (MAX_CAND + MAX_PIECES);
+    return sizeof(uint64_t) * (8u * fit_rows * 256u + 8u * MAX_CAND + MAX_CAND + 4u * 256u) +
+           sizeof(uint32_t) * MAX_CAND;
 }
 
 void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
@@ -432,7 +557,7 @@ void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t s
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_only,
                    int n_cu, hipStream_t st) {
     size_t lds = extend_lds_bytes(a.fit_rows);
-    uint64_t waves = ((uint64_t)a.n_in * (a.split ? 64u : 1u) + 63) / 64;
+    uint64_t waves = (((uint64_t)a.n_in << (6 * a.split)) + 63) / 64;
     uint64_t blocks = (waves + EXT_THREADS / 64 - 1) / (EXT_THREADS / 64);
     uint64_t cap = (uint64_t)n_cu * (lds <= 80 * 1024 ? 2 : 1);
     if (blocks > cap) blocks = cap;
@@ -453,13 +578,17 @@ void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_o
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
                     uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
-                    uint32_t *slot_uid, int n_cu, hipStream_t st) {
-    uint64_t blocks = (m + CMP_THREADS - 1) / CMP_THREADS;
-    uint64_t cap = (uint64_t)n_cu * 8;
-    if (blocks > cap) blocks = cap;
-    if (!blocks) blocks = 1;
-    hipLaunchKernelGGL(k_compact, dim3((uint32_t)blocks), dim3(CMP_THREADS), 0, st,
-                       tab_key, tab_cnt, m, out_key, out_cnt, c32, slot_uid);
+                    uint32_t *blk, uint32_t *slot_uid, hipStream_t st) {
+    // blk holds 2 * COMPACT_MAX_BLOCKS words: counts, then offsets
+    uint64_t per_block = 4096;
+    while ((m + per_block - 1) / per_block > COMPACT_MAX_BLOCKS) per_block *= 2;
+    const uint32_t nb = (uint32_t)((m + per_block - 1) / per_block);
+    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, m,
+                       per_block, blk);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blk, nb,
+                       blk + COMPACT_MAX_BLOCKS, c32);
+    hipLaunchKernelGGL(k_compact_write, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, tab_cnt,
+                       m, per_block, blk + COMPACT_MAX_BLOCKS, out_key, out_cnt, slot_uid);
 }
 
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,

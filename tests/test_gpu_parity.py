@@ -151,3 +151,17 @@ def test_next_size_bounds_children(gpu, lib6):
         before = s.stats()["children"]
         s.search_GPU()
         assert s.stats()["children"] - before <= bound
+
+
+def test_benchmark_workload_all_boards(gpu, lib8):
+    """The 256 boards of the benchmark workload, batched, against the oracle's
+    counts (fixture written by tests/golden/make_seeded_counts.py)."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    boards = workloads.seeded_8x8(256, seed=2)
+    assert boards == fx["boards"]
+    b = gpu.Batch(lib8, boards)
+    b.run()
+    assert b.counts() == fx["counts"]
