@@ -856,7 +856,7 @@ int d4_library_create(const uint64_t *shapes, const uint32_t *counts, uint32_t n
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, dev));
         L->n_cu = prop.multiProcessorCount;
-        if (extend_lds_bytes(L->fit_rows) > (size_t)prop.sharedMemPerBlock)
+        if (extend_lds_bytes(L->fit_rows, true) > (size_t)prop.sharedMemPerBlock)
             throw Error(D4_EINVAL, "placement tables exceed LDS");
         upload(L->d_fit, L->fit.data(), L->fit.size());
         upload(L->d_place, &L->place[0][0], 8 * MAX_CAND);

@@ -98,7 +98,7 @@ void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
                       uint64_t occ_mask, const unsigned long long *comp,
                       uint32_t *c32, unsigned long long *c64, int n_cu,
                       hipStream_t st);
-size_t extend_lds_bytes(uint32_t fit_rows);
+size_t extend_lds_bytes(uint32_t fit_rows, bool staged);
 
 // Error plumbing (d4_api.cpp).
 void set_error(const std::string &msg);

@@ -34,6 +34,7 @@ namespace d4 {
 namespace {
 
 constexpr int EXT_THREADS = 1024;
+constexpr uint32_t STAGE = 128;  // staged children per wave
 constexpr int CMP_THREADS = 256;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
@@ -162,6 +163,7 @@ __device__ __forceinline__ uint64_t avail_of(const uint64_t *s_xa, uint32_t used
 // (used to size the dedup table exactly when the frontier is small).
 template <bool ENUM, bool PRUNE, bool COUNT_ONLY>
 __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
+    constexpr bool STAGED = !ENUM;
     extern __shared__ uint64_t lds[];
     const uint32_t nfit = 8u * a.fit_rows * 256u;
     uint64_t *s_fit = lds;
@@ -169,6 +171,7 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
     uint64_t *s_range = s_place + 8 * MAX_CAND;
     uint64_t *s_xa = s_range + MAX_CAND;
     uint32_t *s_cinfo = reinterpret_cast<uint32_t *>(s_xa + 4 * 256);
+    uint64_t *s_stage = s_xa + 4 * 256 + MAX_CAND / 2;  // after s_cinfo
     for (uint32_t i = threadIdx.x; i < nfit; i += blockDim.x) s_fit[i] = a.fit[i];
     for (uint32_t i = threadIdx.x; i < 8 * MAX_CAND; i += blockDim.x)
         s_place[i] = a.place[i];
@@ -204,6 +207,54 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
     bool searching = false, pass = false;
     uint32_t rec = 0, inst = 0, used = 0, d = 0, c = 0, anc = 0;
     uint64_t cnt = 0, BW = 0, BWn = 0, E = 0, avail = 0, desc = 0, ok = 0;
+
+    // Count-mode children are staged per wave in LDS (key, count, open
+    // window) and flushed 64 at a time: each lane prunes and inserts one.
+    uint64_t *st_key = s_stage + (threadIdx.x >> 6) * (3 * STAGE);
+    uint64_t *st_cnt = st_key + STAGE;
+    uint64_t *st_E = st_cnt + STAGE;
+    uint32_t sc = 0;  // staged entries (wave-uniform)
+    // Returns this lane's tallies: bit 0 child kept, bit 1 pruned, bit 2 new.
+    auto flush = [&](uint32_t n) __attribute__((always_inline)) -> uint32_t {
+        __builtin_amdgcn_wave_barrier();
+        const bool has = lane < n;
+        const uint32_t sl = sc - n + lane;
+        const uint64_t k2 = has ? st_key[sl] : 0ull;
+        const uint64_t c2 = has ? st_cnt[sl] : 0ull;
+        const uint64_t e2 = has ? st_E[sl] : 0ull;
+        __builtin_amdgcn_wave_barrier();
+        sc -= n;
+        bool keep = has;
+        uint32_t t = 0;
+        if (PRUNE && has) {
+            keep = viable(e2, a.seed_mask, a.gcd, a.min_size);
+            t |= keep ? 0u : 2u;
+        }
+        bool failed = false;
+        if (keep) {
+            t |= 1u;
+            if (!COUNT_ONLY) {
+                uint64_t slot = 0;
+                int r = table_insert(a, k2, c2, slot);
+                if (r < 0) failed = true;
+                else t |= r ? 4u : 0u;
+            }
+        }
+        if (__any(failed)) {
+            // table too small: flag it and stop; the host reruns the step
+            if (failed) atomicOr(a.c32 + C32_OVERFLOW, 1u);
+            searching = false;
+            pass = false;
+            exhausted = true;
+            buf_pos = buf_items;
+        }
+        return t;
+    };
+    auto tally = [&](uint32_t t) __attribute__((always_inline)) {
+        n_children += t & 1u;
+        n_pruned += (t >> 1) & 1u;
+        n_new += (t >> 2) & 1u;
+    };
 
     for (;;) {
         uint64_t im = __ballot(!(searching || pass));
@@ -360,6 +411,25 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
                 ok = d >= fix ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1) : 0;
             }
         }
+        if (STAGED) {
+            // queue the child in the wave's LDS stage: pruning and the table
+            // insert run in flush(), 64 children at a time with every lane
+            // active, instead of stalling this divergent loop on each CAS
+            const uint64_t em = __ballot(emit);
+            if (em) {
+                if (emit) {
+                    const uint32_t sl = sc + mbcnt(em);
+                    uint64_t key2 = ((BW & ~E2) >> 8) | ((uint64_t)used2 << a.occ_bits);
+                    if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
+                    st_key[sl] = key2;
+                    st_cnt[sl] = cnt;
+                    st_E[sl] = (E2 >> 8) | (BWn & 0xff00000000000000ull);
+                }
+                sc += (uint32_t)__builtin_popcountll(em);
+                if (sc >= 64) tally(flush(64));
+            }
+            continue;
+        }
         bool keep = emit;
         if (emit && PRUNE) {
             uint64_t Ec = (E2 >> 8) | (BWn & 0xff00000000000000ull);
@@ -401,6 +471,8 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
             }
         }
     }
+    if (STAGED)
+        while (sc) tally(flush(min(sc, 64u)));
 
     n_children = wave_sum_u64(n_children);
     n_new = wave_sum_u64(n_new);
@@ -537,9 +609,10 @@ __global__ __launch_bounds__(256) void k_next_size(
 
 }  // namespace
 
-size_t extend_lds_bytes(uint32_t fit_rows) {
+size_t extend_lds_bytes(uint32_t fit_rows, bool staged) {
     return sizeof(uint64_t) * (8u * fit_rows * 256u + 8u * MAX_CAND + MAX_CAND + 4u * 256u) +
-           sizeof(uint32_t) * MAX_CAND;
+           sizeof(uint32_t) * MAX_CAND +
+           (staged ? sizeof(uint64_t) * 3u * STAGE * (EXT_THREADS / 64) : 0);
 }
 
 void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
@@ -556,7 +629,7 @@ void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t s
 
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_only,
                    int n_cu, hipStream_t st) {
-    size_t lds = extend_lds_bytes(a.fit_rows);
+    size_t lds = extend_lds_bytes(a.fit_rows, !enum_mode || count_only);
     uint64_t waves = (((uint64_t)a.n_in << (6 * a.split)) + 63) / 64;
     uint64_t blocks = (waves + EXT_THREADS / 64 - 1) / (EXT_THREADS / 64);
     uint64_t cap = (uint64_t)n_cu * (lds <= 80 * 1024 ? 2 : 1);
