@@ -372,7 +372,7 @@ struct Engine {
     DevBuf<uint32_t> hgt;
     DevBuf<unsigned long long> result, result_save;
     DevBuf<uint64_t> fk, fc, nk, nc;
-    DevBuf<unsigned long long> tk, tc;
+    DevBuf<unsigned long long> tk;  // dedup table: {key, count} per slot
     DevBuf<uint32_t> c32;
     DevBuf<unsigned long long> c64;
     DevBuf<uint32_t> e_parent, e_child, slot_uid, blk;
@@ -481,7 +481,7 @@ struct Engine {
         a.occ_mask = occ_mask;
         a.used_mask = used_mask;
         a.tab_key = tk.p;
-        a.tab_cnt = tc.p;
+        a.tab_cnt = tk.p + 1;  // interleaved {key, count} slots
         a.tab_mask = tab_m - 1;
         a.max_probe = 256;
         a.c32 = c32.p;
@@ -549,8 +549,7 @@ struct Engine {
         unsigned long long h64[C64_N];
         for (;;) {
             if (tab_m > (1ull << 34)) throw Error(D4_ERANGE, "dedup table exceeds 2^34 slots");
-            tk.reserve(tab_m);
-            tc.reserve(tab_m);
+            tk.reserve(2 * tab_m);
             if (en) {
                 if (est_e > 0xffffffffull) throw Error(D4_ERANGE, "edge count exceeds 2^32");
                 e_parent.reserve(est_e);
@@ -561,7 +560,7 @@ struct Engine {
             if (absorbing)
                 HIPCHK(hipMemcpyAsync(result_save.p, result.p, n_inst * 8,
                                       hipMemcpyDeviceToDevice, st));
-            launch_fill_table(tk.p, tc.p, tab_m, st);
+            launch_fill_table(tk.p, tk.p + 1, tab_m, st);
             launch_reset_counters(c32.p, c64.p, st);
             HIPCHK(hipEventRecord(evx0, st));
             if (n_front) launch_extend(args(tab_m, (uint32_t)est_e), en, prune, false, lib->n_cu, st);
@@ -603,7 +602,7 @@ struct Engine {
         if (en) slot_uid.reserve(tab_m);
         HIPCHK(hipEventRecord(evc0, st));
         blk.reserve(2 * COMPACT_MAX_BLOCKS);
-        launch_compact(tk.p, tc.p, tab_m, nk.p, nc.p, c32.p, blk.p, en ? slot_uid.p : nullptr,
+        launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, c32.p, blk.p, en ? slot_uid.p : nullptr,
                        st);
         uint32_t n_edges = en ? h32[C32_EDGES] : 0;
         if (en && n_edges) {

@@ -75,6 +75,10 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     return x;
 }
 
+// The dedup table interleaves each slot's key and count (16 B per slot:
+// tab_key[2h], tab_cnt[2h] with tab_cnt = tab_key + 1), so an insert touches
+// one cache line.
+//
 // Insert (key, cnt) into the table: returns 1 when the key is new, 0 when it
 // merged into an existing slot, -1 when the probe limit was reached.  Keys
 // only ever change EMPTY -> key, so the CAS's returned value decides: EMPTY
@@ -84,10 +88,10 @@ __device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
                                             uint64_t cnt, uint64_t &slot) {
     uint64_t h = mix64(key) & a.tab_mask;
     for (uint32_t p = 0; p < a.max_probe; p++) {
-        uint64_t cur = atomicCAS(a.tab_key + h, (unsigned long long)EMPTY_KEY,
+        uint64_t cur = atomicCAS(a.tab_key + 2 * h, (unsigned long long)EMPTY_KEY,
                                  (unsigned long long)key);
         if (cur == EMPTY_KEY || cur == key) {
-            atomicAdd(a.tab_cnt + h, (unsigned long long)cnt);
+            atomicAdd(a.tab_cnt + 2 * h, (unsigned long long)cnt);
             slot = h;
             return cur == EMPTY_KEY ? 1 : 0;
         }
@@ -489,8 +493,8 @@ __global__ __launch_bounds__(256) void k_fill_table(unsigned long long *key,
                                                     uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        key[i] = EMPTY_KEY;
-        cnt[i] = 0;
+        key[2 * i] = EMPTY_KEY;
+        cnt[2 * i] = 0;
     }
 }
 
@@ -513,7 +517,7 @@ __global__ __launch_bounds__(CMP_THREADS) void k_compact_count(
     const uint64_t hi = min(m, lo + per_block);
     uint32_t n = 0;
     for (uint64_t i = lo + threadIdx.x; i < hi; i += CMP_THREADS)
-        n += tab_key[i] != EMPTY_KEY;
+        n += tab_key[2 * i] != EMPTY_KEY;
     for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
     if (lane_id() == 0) s_w[threadIdx.x >> 6] = n;
     __syncthreads();
@@ -559,7 +563,7 @@ __global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
     uint32_t base = blk_off[blockIdx.x];
     for (uint64_t i0 = lo; i0 < hi; i0 += CMP_THREADS) {
         const uint64_t i = i0 + threadIdx.x;
-        const uint64_t k = i < hi ? tab_key[i] : EMPTY_KEY;
+        const uint64_t k = i < hi ? tab_key[2 * i] : EMPTY_KEY;
         const bool valid = k != EMPTY_KEY;
         const uint64_t bm = __ballot(valid);
         if (lane_id() == 0) s_w[wv] = (uint32_t)__builtin_popcountll(bm);
@@ -572,7 +576,7 @@ __global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
         if (valid) {
             const uint32_t o = base + before + mbcnt(bm);
             out_key[o] = k;
-            out_cnt[o] = tab_cnt[i];
+            out_cnt[o] = tab_cnt[2 * i];
             if (slot_uid) slot_uid[i] = o;
         }
         base += total;
