@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune-sample", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the "
@@ -106,6 +108,13 @@ def main():
     shapes = workloads.pentominoes()
     boards = workloads.seeded_8x8(args.instances, seed=2)
     lib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
+    tuned = None
+    if not args.no_tune:
+        # measured autotune of the extend launch shape on a sample of the
+        # workload (outside the timed region)
+        t = time.perf_counter()
+        tuned = lib.tune(boards[:args.tune_sample])
+        log(f"autotune {tuned} in {time.perf_counter() - t:.1f} s")
 
     def step():
         b = d4.Batch(lib, boards)
@@ -163,6 +172,8 @@ def main():
             "pieces": "12 free pentominoes x1, D4",
             "mode": "count",
             "parallelism": "instances batched in one frontier, 1 GPU",
+            "launch": {"fit_rows": lib.get_launch()[0], "threads": lib.get_launch()[1],
+                       "autotuned": tuned is not None},
         },
         "solutions_per_s": n_sol / dt,
         "solutions_per_step": sum(counts),

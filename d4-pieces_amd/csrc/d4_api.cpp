@@ -209,6 +209,8 @@ struct d4_library {
     uint8_t cpiece[MAX_CAND] = {}, ctrs[MAX_CAND] = {}, cminc[MAX_CAND] = {};
     uint64_t cnormal[MAX_CAND] = {};
     uint64_t place[8][MAX_CAND] = {};
+    uint64_t placeinfo[8 * MAX_CAND * 2] = {};
+    uint32_t ext_threads = 1024;  // extend workgroup size (autotuned)
     uint64_t range[MAX_CAND] = {};
     uint32_t cinfo[MAX_CAND] = {};
     uint64_t prange[MAX_PIECES] = {};
@@ -216,18 +218,35 @@ struct d4_library {
     std::vector<uint64_t> fit;  // [8][maxh][256]
     unsigned long long comp[256] = {};
     uint32_t gcd = 1, min_size = 1;
+    uint32_t lds_limit = 65536;
     int device = 0;
     int n_cu = 1;
-    uint64_t *d_fit = nullptr, *d_place = nullptr, *d_range = nullptr, *d_xavail = nullptr;
-    uint32_t *d_cinfo = nullptr;
+    uint64_t *d_fit = nullptr, *d_placeinfo = nullptr, *d_xavail = nullptr;
     unsigned long long *d_comp = nullptr;
 
     ~d4_library() {
-        for (void *p : {(void *)d_fit, (void *)d_place, (void *)d_range, (void *)d_xavail,
-                        (void *)d_cinfo, (void *)d_comp})
+        for (void *p : {(void *)d_fit, (void *)d_placeinfo, (void *)d_xavail, (void *)d_comp})
             if (p) (void)hipFree(p);
     }
 };
+
+// fit[c][r][b]: candidates anchored at column c whose row-r cells lie in b.
+static void build_fit(d4_library &L, uint32_t fit_rows) {
+    L.fit_rows = fit_rows;
+    L.fit.assign((size_t)8 * fit_rows * 256, 0);
+    for (uint32_t c = 0; c < 8; c++)
+        for (uint32_t r = 0; r < fit_rows; r++)
+            for (uint32_t b = 0; b < 256; b++) {
+                uint64_t m = 0;
+                for (uint32_t j = 0; j < L.ncand; j++) {
+                    uint64_t p = L.place[c][j];
+                    if (!p) continue;
+                    uint64_t row = (p >> (8 * r)) & 0xffu;
+                    if (!(row & ~(uint64_t)b)) m |= 1ull << j;
+                }
+                L.fit[((size_t)c * fit_rows + r) * 256 + b] = m;
+            }
+}
 
 static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *counts) {
     uint32_t k = 0, off = 0, g = 0, mins = UINT32_MAX;
@@ -301,22 +320,18 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
             uint32_t mc = L.cminc[j];
             if (c >= mc && c - mc + width8(n) <= L.W) L.place[c][j] = n << (c - mc);
         }
+    // packed per-anchor entries: shape, counter info and the piece's
+    // candidate range [lo, hi)
+    for (uint32_t c = 0; c < 8; c++)
+        for (uint32_t j = 0; j < k; j++) {
+            uint64_t r = L.range[j];
+            uint64_t lo = (uint64_t)__builtin_ctzll(r), hi = 64 - (uint64_t)__builtin_clzll(r);
+            L.placeinfo[(c * MAX_CAND + j) * 2] = L.place[c][j];
+            L.placeinfo[(c * MAX_CAND + j) * 2 + 1] = (uint64_t)L.cinfo[j] | (lo << 32) | (hi << 40);
+        }
     // the fit table covers the first fit_rows window rows (the placement
     // itself is tested in full when chosen); 3 rows keep it at 48 KiB of LDS
-    L.fit_rows = std::max(1u, std::min(L.maxh, 3u));
-    L.fit.assign((size_t)8 * L.fit_rows * 256, 0);
-    for (uint32_t c = 0; c < 8; c++)
-        for (uint32_t r = 0; r < L.fit_rows; r++)
-            for (uint32_t b = 0; b < 256; b++) {
-                uint64_t m = 0;
-                for (uint32_t j = 0; j < k; j++) {
-                    uint64_t p = L.place[c][j];
-                    if (!p) continue;
-                    uint64_t row = (p >> (8 * r)) & 0xffu;
-                    if (!(row & ~(uint64_t)b)) m |= 1ull << j;
-                }
-                L.fit[((size_t)c * L.fit_rows + r) * 256 + b] = m;
-            }
+    build_fit(L, std::max(1u, std::min(L.maxh, 3u)));
     // comp[p]: ways to fill row pattern p with row-0 parts of placements,
     // ignoring lower rows and piece counts (bound used by next_size).
     L.comp[0] = 1;
@@ -464,12 +479,13 @@ struct Engine {
         a.in_key = fk.p;
         a.in_cnt = fc.p;
         a.n_in = (uint32_t)n_front;
+        // split levels: tiny frontiers and the heavy first rows spread each
+        // record over 64 / 4096 work items
         a.split = n_front <= 1024 ? 2u : n_front <= (1u << 16) ? 1u : 0u;
+
         a.row = row;
         a.fit = lib->d_fit;
-        a.place = lib->d_place;
-        a.range = lib->d_range;
-        a.cinfo = lib->d_cinfo;
+        a.placeinfo = lib->d_placeinfo;
         a.xavail = lib->d_xavail;
         a.fit_rows = lib->fit_rows;
         a.bw = bw.p;
@@ -563,7 +579,9 @@ struct Engine {
             launch_fill_table(tk.p, tk.p + 1, tab_m, st);
             launch_reset_counters(c32.p, c64.p, st);
             HIPCHK(hipEventRecord(evx0, st));
-            if (n_front) launch_extend(args(tab_m, (uint32_t)est_e), en, prune, false, lib->n_cu, st);
+            if (n_front)
+                launch_extend(args(tab_m, (uint32_t)est_e), en, prune, false, lib->ext_threads,
+                              lib->n_cu, st);
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(evx1, st));
             HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
@@ -655,7 +673,7 @@ struct Engine {
         if (!n_front) return 0;
         const bool prune = !(flags & D4_FLAG_NO_PRUNE);
         launch_reset_counters(c32.p, c64.p, st);
-        launch_extend(args(1, 0), false, prune, true, lib->n_cu, st);
+        launch_extend(args(1, 0), false, prune, true, lib->ext_threads, lib->n_cu, st);
         HIPCHK(hipGetLastError());
         unsigned long long h64[C64_N];
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
@@ -855,12 +873,11 @@ int d4_library_create(const uint64_t *shapes, const uint32_t *counts, uint32_t n
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, dev));
         L->n_cu = prop.multiProcessorCount;
-        if (extend_lds_bytes(L->fit_rows, true) > (size_t)prop.sharedMemPerBlock)
+        L->lds_limit = (uint32_t)prop.sharedMemPerBlock;
+        if (extend_lds_bytes(L->fit_rows, 1024, true) > L->lds_limit)
             throw Error(D4_EINVAL, "placement tables exceed LDS");
         upload(L->d_fit, L->fit.data(), L->fit.size());
-        upload(L->d_place, &L->place[0][0], 8 * MAX_CAND);
-        upload(L->d_range, L->range, MAX_CAND);
-        upload(L->d_cinfo, L->cinfo, MAX_CAND);
+        upload(L->d_placeinfo, L->placeinfo, 2 * 8 * MAX_CAND);
         upload(L->d_xavail, L->xavail, 4 * 256);
         upload(L->d_comp, L->comp, 256);
         *out = L.release();
@@ -874,6 +891,66 @@ int d4_library_info(const d4_library *lib, uint32_t *n_cand, uint32_t *max_heigh
         if (!lib) throw Error(D4_EINVAL, "null library");
         if (n_cand) *n_cand = lib->ncand;
         if (max_height) *max_height = lib->maxh;
+    });
+}
+
+int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads) {
+    return guard([&] {
+        if (!lib) throw Error(D4_EINVAL, "null library");
+        if (threads != 256 && threads != 512 && threads != 1024)
+            throw Error(D4_EINVAL, "threads must be 256, 512 or 1024");
+        if (fit_rows < 1 || fit_rows > lib->maxh) throw Error(D4_EINVAL, "fit_rows out of range");
+        if (extend_lds_bytes(fit_rows, threads, true) > lib->lds_limit)
+            throw Error(D4_EINVAL, "launch shape exceeds LDS");
+        if (fit_rows != lib->fit_rows) {
+            HIPCHK(hipSetDevice(lib->device));
+            build_fit(*lib, fit_rows);
+            uint64_t *old = lib->d_fit;
+            lib->d_fit = nullptr;
+            upload(lib->d_fit, lib->fit.data(), lib->fit.size());
+            if (old) (void)hipFree(old);
+        }
+        lib->ext_threads = threads;
+    });
+}
+
+int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows, uint32_t *threads) {
+    return guard([&] {
+        if (!lib) throw Error(D4_EINVAL, "null library");
+        if (fit_rows) *fit_rows = lib->fit_rows;
+        if (threads) *threads = lib->ext_threads;
+    });
+}
+
+int d4_library_tune(d4_library *lib, uint32_t n_inst, const uint8_t *boards, uint32_t stride,
+                    const uint32_t *heights, uint32_t *fit_rows, uint32_t *threads,
+                    double *best_ms) {
+    return guard([&] {
+        if (!lib || !boards || !heights || !n_inst) throw Error(D4_EINVAL, "null argument");
+        uint32_t best_f = lib->fit_rows, best_t = lib->ext_threads;
+        double best = -1;
+        {  // warm the allocator cache and the code objects
+            Engine e(lib, n_inst, boards, stride, heights, D4_MODE_COUNT, 0);
+            e.run();
+        }
+        for (uint32_t f = 1; f <= std::min(lib->maxh, 4u); f++)
+            for (uint32_t t : {256u, 512u, 1024u}) {
+                if (extend_lds_bytes(f, t, true) > lib->lds_limit) continue;
+                if (d4_library_set_launch(lib, f, t) != D4_OK) continue;
+                Engine e(lib, n_inst, boards, stride, heights, D4_MODE_COUNT, 0);
+                e.run();
+                double ms = e.stats.extend_ms;
+                if (best < 0 || ms < best) {
+                    best = ms;
+                    best_f = f;
+                    best_t = t;
+                }
+            }
+        if (d4_library_set_launch(lib, best_f, best_t) != D4_OK)
+            throw Error(D4_EDEVICE, g_err);
+        if (fit_rows) *fit_rows = best_f;
+        if (threads) *threads = best_t;
+        if (best_ms) *best_ms = best;
     });
 }
 

@@ -31,7 +31,7 @@ enum : int {
     C64_NEW = 1,
     C64_PRUNED = 2,
     C64_NEXTSIZE = 3,
-    C64_WORK = 4,      // dynamic work-item fetch (extend)
+    C64_WORK = 4,      // dynamic work-item claim (extend)
     C64_N = 5,
 };
 
@@ -44,9 +44,7 @@ struct ExtendArgs {
     uint32_t split;          // levels fixed per work item: 0, 1 or 2
     // library tables (global copies; staged into LDS)
     const uint64_t *fit;     // [8][fit_rows][256] candidate masks
-    const uint64_t *place;   // [8][64] placed shape per anchor column
-    const uint64_t *range;   // [64] mask of the candidate's piece
-    const uint32_t *cinfo;   // [64] counter offset | field mask << 8 | count << 16
+    const uint64_t *placeinfo;  // [8][64][2] placed shape, counter info + range
     const uint64_t *xavail;  // [4][256] candidates ruled out per counter byte
     uint32_t fit_rows;
     // instances
@@ -84,7 +82,7 @@ void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
-                   bool count_only, int n_cu, hipStream_t st);
+                   bool count_only, uint32_t threads, int n_cu, hipStream_t st);
 constexpr uint32_t COMPACT_MAX_BLOCKS = 65536;
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
@@ -98,7 +96,7 @@ void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
                       uint64_t occ_mask, const unsigned long long *comp,
                       uint32_t *c32, unsigned long long *c64, int n_cu,
                       hipStream_t st);
-size_t extend_lds_bytes(uint32_t fit_rows, bool staged);
+size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool staged);
 
 // Error plumbing (d4_api.cpp).
 void set_error(const std::string &msg);

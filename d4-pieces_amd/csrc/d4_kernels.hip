@@ -29,11 +29,12 @@
 // atomicAdd, and dedup runs on the host over PCIe.
 #include "d4_internal.h"
 
+#include <algorithm>
+
 namespace d4 {
 
 namespace {
 
-constexpr int EXT_THREADS = 1024;
 constexpr uint32_t STAGE = 128;  // staged children per wave
 constexpr int CMP_THREADS = 256;
 
@@ -165,25 +166,32 @@ __device__ __forceinline__ uint64_t avail_of(const uint64_t *s_xa, uint32_t used
 
 // COUNT_ONLY runs the same search but only counts the surviving children
 // (used to size the dedup table exactly when the frontier is small).
-template <bool ENUM, bool PRUNE, bool COUNT_ONLY>
-__global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
+// Packed placement entry: place[c][k] (shape) and the candidate's counter
+// info (offset | field mask << 8 | count << 16, piece's candidate range
+// [lo, hi) in bits 32..39 / 40..47), read with one 16-byte LDS load.
+struct __attribute__((aligned(16))) PlaceInfo {
+    uint64_t shape;
+    uint64_t info;
+};
+
+__device__ __forceinline__ uint64_t piece_range(uint64_t info) {
+    const uint32_t lo = (uint32_t)(info >> 32) & 0xffu, hi = (uint32_t)(info >> 40) & 0xffu;
+    return (hi >= 64 ? ~0ull : (1ull << hi) - 1ull) & (~0ull << lo);
+}
+
+template <bool ENUM, bool PRUNE, bool COUNT_ONLY, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     constexpr bool STAGED = !ENUM;
     extern __shared__ uint64_t lds[];
     const uint32_t nfit = 8u * a.fit_rows * 256u;
     uint64_t *s_fit = lds;
-    uint64_t *s_place = s_fit + nfit;
-    uint64_t *s_range = s_place + 8 * MAX_CAND;
-    uint64_t *s_xa = s_range + MAX_CAND;
-    uint32_t *s_cinfo = reinterpret_cast<uint32_t *>(s_xa + 4 * 256);
-    uint64_t *s_stage = s_xa + 4 * 256 + MAX_CAND / 2;  // after s_cinfo
-    for (uint32_t i = threadIdx.x; i < nfit; i += blockDim.x) s_fit[i] = a.fit[i];
-    for (uint32_t i = threadIdx.x; i < 8 * MAX_CAND; i += blockDim.x)
-        s_place[i] = a.place[i];
-    for (uint32_t i = threadIdx.x; i < 4 * 256; i += blockDim.x) s_xa[i] = a.xavail[i];
-    if (threadIdx.x < MAX_CAND) {
-        s_range[threadIdx.x] = a.range[threadIdx.x];
-        s_cinfo[threadIdx.x] = a.cinfo[threadIdx.x];
-    }
+    const PlaceInfo *s_pc = reinterpret_cast<const PlaceInfo *>(s_fit + nfit);
+    uint64_t *s_xa = s_fit + nfit + 2 * 8 * MAX_CAND;
+    uint64_t *s_stage = s_xa + 4 * 256;
+    for (uint32_t i = threadIdx.x; i < nfit; i += THREADS) s_fit[i] = a.fit[i];
+    for (uint32_t i = threadIdx.x; i < 2 * 8 * MAX_CAND; i += THREADS)
+        s_fit[nfit + i] = a.placeinfo[i];
+    for (uint32_t i = threadIdx.x; i < 4 * 256; i += THREADS) s_xa[i] = a.xavail[i];
     __syncthreads();
 
     const uint32_t lane = lane_id();
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
 
     // Count-mode children are staged per wave in LDS (key, count, open
     // window) and flushed 64 at a time: each lane prunes and inserts one.
-    uint64_t *st_key = s_stage + (threadIdx.x >> 6) * (3 * STAGE);
+    uint64_t *st_key = s_stage + (threadIdx.x >> 6) * (3 * STAGE);  // per-wave stage
     uint64_t *st_cnt = st_key + STAGE;
     uint64_t *st_E = st_cnt + STAGE;
     uint32_t sc = 0;  // staged entries (wave-uniform)
@@ -333,10 +341,11 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
                             // commit the first placement k0, keep only k1 next
                             const uint32_t k0 = sub >> 6, k1 = sub & 63u;
                             ok &= 1ull << k0;
-                            const uint64_t pl = s_place[c * MAX_CAND + k0];
+                            const PlaceInfo pi0 = s_pc[c * MAX_CAND + k0];
+                            const uint64_t pl = pi0.shape;
                             if (ok && (pl & ~E)) ok = 0;
                             if (ok) {
-                                const uint32_t ci = s_cinfo[k0];
+                                const uint32_t ci = (uint32_t)pi0.info;
                                 const uint64_t E1 = E & ~pl;
                                 if (!(E1 & 0xffu)) {
                                     // k0 completes the row: the k1 == 0 item
@@ -346,7 +355,7 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
                                     E = E1;
                                     used += 1u << (ci & 0xffu);
                                     if (((used >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
-                                        avail &= ~s_range[k0];
+                                        avail &= ~piece_range(pi0.info);
                                     desc = (uint64_t)(k0 + 1);
                                     anc = c;
                                     d = 1;
@@ -379,8 +388,9 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
             if (ok) {
                 uint32_t k = (uint32_t)__builtin_ctzll(ok);
                 ok &= ok - 1;
-                uint64_t pl = s_place[c * MAX_CAND + k];
-                uint32_t ci = s_cinfo[k];
+                const PlaceInfo pk = s_pc[c * MAX_CAND + k];
+                const uint64_t pl = pk.shape;
+                const uint32_t ci = (uint32_t)pk.info;
                 E2 = E & ~pl;
                 uint32_t inc = 1u << (ci & 0xffu);
                 if (pl & ~E) {
@@ -393,7 +403,7 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
                     E = E2;
                     used += inc;
                     if (((used >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
-                        avail &= ~s_range[k];
+                        avail &= ~piece_range(pk.info);
                     desc |= (uint64_t)(k + 1) << (8 * d);
                     anc |= c << (3 * d);
                     d++;
@@ -409,9 +419,10 @@ __global__ __launch_bounds__(EXT_THREADS) void k_extend(ExtendArgs a) {
                 c = (anc >> (3 * d)) & 7u;
                 desc &= ~(0xffull << (8 * d));
                 anc &= ~(7u << (3 * d));
-                E |= s_place[c * MAX_CAND + k];
-                used -= 1u << (s_cinfo[k] & 0xffu);
-                avail |= s_range[k];
+                const PlaceInfo pk = s_pc[c * MAX_CAND + k];
+                E |= pk.shape;
+                used -= 1u << (pk.info & 0xffu);
+                avail |= piece_range(pk.info);
                 ok = d >= fix ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1) : 0;
             }
         }
@@ -613,10 +624,9 @@ __global__ __launch_bounds__(256) void k_next_size(
 
 }  // namespace
 
-size_t extend_lds_bytes(uint32_t fit_rows, bool staged) {
-    return sizeof(uint64_t) * (8u * fit_rows * 256u + 8u * MAX_CAND + MAX_CAND + 4u * 256u) +
-           sizeof(uint32_t) * MAX_CAND +
-           (staged ? sizeof(uint64_t) * 3u * STAGE * (EXT_THREADS / 64) : 0);
+size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool staged) {
+    return sizeof(uint64_t) * (8u * fit_rows * 256u + 2u * 8u * MAX_CAND + 4u * 256u) +
+           (staged ? sizeof(uint64_t) * 3u * STAGE * (threads / 64) : 0);
 }
 
 void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
@@ -631,24 +641,44 @@ void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t s
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c32, c64);
 }
 
+template <bool ENUM, bool PRUNE, bool COUNT_ONLY>
+static void launch_extend_t(const ExtendArgs &a, uint32_t threads, dim3 g, size_t lds,
+                            hipStream_t st) {
+    switch (threads) {
+        case 256:
+            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 256>), g, dim3(256), lds, st, a);
+            break;
+        case 512:
+            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 512>), g, dim3(512), lds, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 1024>), g, dim3(1024), lds, st, a);
+            break;
+    }
+}
+
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_only,
-                   int n_cu, hipStream_t st) {
-    size_t lds = extend_lds_bytes(a.fit_rows, !enum_mode || count_only);
-    uint64_t waves = (((uint64_t)a.n_in << (6 * a.split)) + 63) / 64;
-    uint64_t blocks = (waves + EXT_THREADS / 64 - 1) / (EXT_THREADS / 64);
-    uint64_t cap = (uint64_t)n_cu * (lds <= 80 * 1024 ? 2 : 1);
-    if (blocks > cap) blocks = cap;
+                   uint32_t threads, int n_cu, hipStream_t st) {
+    if (enum_mode && !count_only) threads = 1024;
+    if (threads != 256 && threads != 512) threads = 1024;
+    const size_t lds = extend_lds_bytes(a.fit_rows, threads, !enum_mode || count_only);
+    // resident workgroups per CU: 160 KiB of LDS and 32 waves
+    uint64_t per_cu = std::min<uint64_t>(160u * 1024u / lds, 32u / (threads / 64));
+    if (!per_cu) per_cu = 1;
+    const uint64_t waves = (((uint64_t)a.n_in << (6 * a.split)) + 63) / 64;
+    uint64_t blocks = (waves + threads / 64 - 1) / (threads / 64);
+    blocks = std::min<uint64_t>(blocks, (uint64_t)n_cu * per_cu);
     if (!blocks) blocks = 1;
-    dim3 g((uint32_t)blocks), b(EXT_THREADS);
+    const dim3 g((uint32_t)blocks);
     if (count_only) {
-        if (prune) hipLaunchKernelGGL((k_extend<false, true, true>), g, b, lds, st, a);
-        else hipLaunchKernelGGL((k_extend<false, false, true>), g, b, lds, st, a);
+        if (prune) launch_extend_t<false, true, true>(a, threads, g, lds, st);
+        else launch_extend_t<false, false, true>(a, threads, g, lds, st);
     } else if (enum_mode) {
-        if (prune) hipLaunchKernelGGL((k_extend<true, true, false>), g, b, lds, st, a);
-        else hipLaunchKernelGGL((k_extend<true, false, false>), g, b, lds, st, a);
+        if (prune) launch_extend_t<true, true, false>(a, 1024, g, lds, st);
+        else launch_extend_t<true, false, false>(a, 1024, g, lds, st);
     } else {
-        if (prune) hipLaunchKernelGGL((k_extend<false, true, false>), g, b, lds, st, a);
-        else hipLaunchKernelGGL((k_extend<false, false, false>), g, b, lds, st, a);
+        if (prune) launch_extend_t<false, true, false>(a, threads, g, lds, st);
+        else launch_extend_t<false, false, false>(a, threads, g, lds, st);
     }
 }
 

@@ -75,6 +75,11 @@ SIGNATURES = [
      [_U64P, _U32P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
     ("d4_library_destroy", None, [_P]),
     ("d4_library_info", ctypes.c_int, [_P, _U32P, _U32P]),
+    ("d4_library_set_launch", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    ("d4_library_get_launch", ctypes.c_int, [_P, _U32P, _U32P]),
+    ("d4_library_tune", ctypes.c_int,
+     [_P, ctypes.c_uint32, _U8P, ctypes.c_uint32, _U32P, _U32P, _U32P,
+      ctypes.POINTER(ctypes.c_double)]),
     ("d4_searcher_create_ea", ctypes.c_int,
      [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
     ("d4_searcher_create", ctypes.c_int,
@@ -185,10 +190,35 @@ class PieceLibrary:
         _check(lib.d4_library_info(h, ctypes.byref(nc), ctypes.byref(mh)))
         self.n_cand, self.max_height = nc.value, mh.value
 
+    def set_launch(self, fit_rows: int, threads: int) -> None:
+        _check(load().d4_library_set_launch(self._h, fit_rows, threads))
+
+    def get_launch(self) -> Tuple[int, int]:
+        f, t = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(load().d4_library_get_launch(self._h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
+    def tune(self, boards: Sequence[Sequence[int]]) -> dict:
+        """Measured autotune of the extend launch shape on these boards."""
+        flat, stride, heights = _flatten(boards)
+        f, t, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        _check(load().d4_library_tune(self._h, len(boards), _u8arr(flat), stride,
+                                      _u32arr(heights), ctypes.byref(f), ctypes.byref(t),
+                                      ctypes.byref(ms)))
+        return {"fit_rows": f.value, "threads": t.value, "extend_ms": ms.value}
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.d4_library_destroy(self._h)
             self._h = None
+
+
+def _flatten(boards):
+    stride = max([len(b) for b in boards] + [1])
+    flat = []
+    for b in boards:
+        flat.extend(list(b) + [0] * (stride - len(b)))
+    return flat, stride, [len(b) for b in boards]
 
 
 def _rows_arg(rows: Sequence[int]):
@@ -264,11 +294,7 @@ class Batch:
         lib = load()
         self.library = library
         self.n = len(boards)
-        stride = max([len(b) for b in boards] + [1])
-        flat = []
-        for b in boards:
-            flat.extend(list(b) + [0] * (stride - len(b)))
-        heights = [len(b) for b in boards]
+        flat, stride, heights = _flatten(boards)
         h = _P()
         _check(lib.d4_batch_create(library._h, self.n, _u8arr(flat), stride, _u32arr(heights),
                                    mode, 0 if prune else FLAG_NO_PRUNE, ctypes.byref(h)))

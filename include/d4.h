@@ -111,6 +111,23 @@ void d4_library_destroy(d4_library *lib);
 int d4_library_info(const d4_library *lib, uint32_t *n_cand,
                     uint32_t *max_height);
 
+/* Launch shape of the extend kernel: fit_rows = window rows covered by the
+ * LDS fit table (1..max_height; the chosen placement is always tested in
+ * full), threads = workgroup size (256, 512 or 1024).  Not to be changed
+ * while a search on this library is running.  Replaces the reference's
+ * fixed KParams choice (src/kernel.h:44-67). */
+int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads);
+int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows,
+                          uint32_t *threads);
+/* Measured autotuner, replacing the analytic cost model of
+ * KSizing::optimize / KParams::fom (src/kernel.cu:95-236): solves the given
+ * boards (count mode, as d4_batch_*) once per candidate launch shape, keeps
+ * the shape with the least extend-kernel time and sets it on the library.
+ * Outputs the chosen shape and its extend time in ms (any may be NULL). */
+int d4_library_tune(d4_library *lib, uint32_t n_inst, const uint8_t *boards,
+                    uint32_t stride, const uint32_t *heights,
+                    uint32_t *fit_rows, uint32_t *threads, double *best_ms);
+
 /* ---------------------------------------------------------------------- */
 /* single-instance searcher (CudaSearcher)                                 */
 /* ---------------------------------------------------------------------- */
