@@ -172,8 +172,8 @@ def main():
             "pieces": "12 free pentominoes x1, D4",
             "mode": "count",
             "parallelism": "instances batched in one frontier, 1 GPU",
-            "launch": {"fit_rows": lib.get_launch()[0], "threads": lib.get_launch()[1],
-                       "autotuned": tuned is not None},
+            "launch": dict(zip(("fit_rows", "threads", "refill"), lib.get_launch()),
+                           autotuned=tuned is not None),
         },
         "solutions_per_s": n_sol / dt,
         "solutions_per_step": sum(counts),

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -211,6 +212,7 @@ struct d4_library {
     uint64_t place[8][MAX_CAND] = {};
     uint64_t placeinfo[8 * MAX_CAND * 2] = {};
     uint32_t ext_threads = 1024;  // extend workgroup size (autotuned)
+    uint32_t ext_refill = 16;     // idle lanes per refill round (autotuned)
     uint64_t range[MAX_CAND] = {};
     uint32_t cinfo[MAX_CAND] = {};
     uint64_t prange[MAX_PIECES] = {};
@@ -361,20 +363,30 @@ static void upload(T *&dst, const T *src, size_t n) {
 
 namespace {
 
-struct RowLog {  // enum mode: what reconstruction needs from one row step
-    std::vector<uint64_t> keys;       // frontier of this row (before the step)
-    std::vector<uint32_t> e_parent;   // edges to the next row's frontier
+struct RowLog {  // enum mode: what reconstruction needs from one global step
+    std::vector<uint64_t> keys;       // frontier before the step
+    std::vector<uint32_t> e_parent;   // edges to the next step's frontier
     std::vector<uint64_t> e_desc;
     std::vector<uint32_t> e_child;
 };
 
+// The frontier engine: S instance slots sharing one library and one device
+// frontier.  An instance joins at a global step (its "start") and advances
+// one of its rows per step; it finishes at step start + height, when its
+// terminal states are absorbed into its count.  A batch admits every
+// instance at step 0; a server admits requests between steps (continuous
+// batching).
 struct Engine {
     const d4_library *lib;
-    uint32_t n_inst, mode, flags;
-    std::vector<uint32_t> heights;
-    std::vector<uint8_t> boards;  // [n_inst][64 + 8]
-    uint32_t Hmax = 0, row = 0;
-    bool done = false;
+    uint32_t S, mode, flags, max_h;
+    bool keep_logs;
+    // per slot (host side)
+    std::vector<uint32_t> heights, start;
+    std::vector<uint8_t> boards;  // [S][MAX_ROWS + 8]
+    std::vector<uint8_t> active;
+    std::vector<unsigned long long> host_result;
+    std::vector<uint32_t> pending, completed;
+    uint32_t g = 0, n_active = 0;
     // key layout
     uint32_t occ_bits = 0, inst_shift = 64;
     uint64_t occ_mask = 0;
@@ -384,7 +396,7 @@ struct Engine {
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evx0 = nullptr, evx1 = nullptr, evc0 = nullptr;
     DevBuf<uint64_t> bw;
-    DevBuf<uint32_t> hgt;
+    DevBuf<uint32_t> hgt, dstart;
     DevBuf<unsigned long long> result, result_save;
     DevBuf<uint64_t> fk, fc, nk, nc;
     DevBuf<unsigned long long> tk;  // dedup table: {key, count} per slot
@@ -396,30 +408,24 @@ struct Engine {
     double grow_unique = 4.0, grow_children = 8.0;
     static constexpr uint64_t kExactCountMax = 1u << 16;
     d4_stats stats{};
-    std::vector<unsigned long long> host_result;
-    std::vector<RowLog> logs;
-    std::vector<uint64_t> final_keys;
+    std::deque<RowLog> logs;  // logs[j] = global step log0 + j
+    uint32_t log0 = 0;
 
-    Engine(const d4_library *l, uint32_t n, const uint8_t *b, uint32_t stride,
-           const uint32_t *h, uint32_t m, uint32_t f)
-        : lib(l), n_inst(n), mode(m), flags(f) {
-        if (!n) throw Error(D4_EINVAL, "no instances");
+    Engine(const d4_library *l, uint32_t slots, uint32_t max_height, uint32_t m, uint32_t f,
+           bool keep)
+        : lib(l), S(slots), mode(m), flags(f), max_h(max_height), keep_logs(keep) {
+        if (!S) throw Error(D4_EINVAL, "no instances");
         if (mode > D4_MODE_ENUM) throw Error(D4_EINVAL, "bad mode");
-        heights.assign(h, h + n);
-        boards.assign((size_t)n * (MAX_ROWS + 8), 0);
-        for (uint32_t i = 0; i < n; i++) {
-            if (heights[i] > (uint32_t)MAX_ROWS) throw Error(D4_EINVAL, "board taller than 64 rows");
-            for (uint32_t r = 0; r < heights[i]; r++) {
-                uint8_t v = b[(size_t)i * stride + r];
-                if (lib->W < 8 && (v >> lib->W)) throw Error(D4_EINVAL, "board wider than the library's width");
-                boards[(size_t)i * (MAX_ROWS + 8) + r] = v;
-            }
-            Hmax = std::max(Hmax, heights[i]);
-        }
+        if (max_h > (uint32_t)MAX_ROWS) throw Error(D4_EINVAL, "board taller than 64 rows");
+        heights.assign(S, 0);
+        start.assign(S, 0);
+        active.assign(S, 0);
+        host_result.assign(S, 0);
+        boards.assign((size_t)S * (MAX_ROWS + 8), 0);
         occ_bits = 8 * (lib->maxh > 1 ? lib->maxh - 1 : 0);
         occ_mask = occ_bits ? (occ_bits >= 64 ? ~0ull : (1ull << occ_bits) - 1) : 0;
         used_mask = lib->used_bits >= 32 ? ~0u : (1u << lib->used_bits) - 1;
-        uint32_t ib = n_inst > 1 ? bitw(n_inst - 1) : 0;
+        uint32_t ib = S > 1 ? bitw(S - 1) : 0;
         if (occ_bits + lib->used_bits + ib > 63)
             throw Error(D4_EINVAL, "state does not fit a 63-bit key (occupancy " +
                                        std::to_string(occ_bits) + " + counters " +
@@ -428,51 +434,65 @@ struct Engine {
         inst_shift = ib ? occ_bits + lib->used_bits : 64;
         HIPCHK(hipSetDevice(lib->device));
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        HIPCHK(hipEventCreate(&ev0));
-        HIPCHK(hipEventCreate(&ev1));
-        HIPCHK(hipEventCreate(&evx0));
-        HIPCHK(hipEventCreate(&evx1));
-        HIPCHK(hipEventCreate(&evc0));
-        // board windows: bw[i][r] = rows r..r+7 of board i
-        bw_stride = Hmax + 2;
-        std::vector<uint64_t> hb((size_t)n * bw_stride, 0);
-        for (uint32_t i = 0; i < n; i++)
-            for (uint32_t r = 0; r < bw_stride; r++) {
-                uint64_t w = 0;
-                for (uint32_t j = 0; j < 8 && r + j < (uint32_t)MAX_ROWS + 8; j++)
-                    w |= (uint64_t)boards[(size_t)i * (MAX_ROWS + 8) + r + j] << (8 * j);
-                hb[(size_t)i * bw_stride + r] = w;
-            }
-        bw.reserve(hb.size());
-        HIPCHK(hipMemcpy(bw.p, hb.data(), hb.size() * 8, hipMemcpyHostToDevice));
-        hgt.reserve(n);
-        HIPCHK(hipMemcpy(hgt.p, heights.data(), n * 4, hipMemcpyHostToDevice));
-        result.reserve(n);
-        HIPCHK(hipMemset(result.p, 0, n * 8));
+        for (hipEvent_t *e : {&ev0, &ev1, &evx0, &evx1, &evc0}) HIPCHK(hipEventCreate(e));
+        bw_stride = max_h + 2;
+        bw.reserve((uint64_t)S * bw_stride);
+        hgt.reserve(S);
+        dstart.reserve(S);
+        result.reserve(S);
+        HIPCHK(hipMemsetAsync(bw.p, 0, (uint64_t)S * bw_stride * 8, st));
+        HIPCHK(hipMemsetAsync(hgt.p, 0, (uint64_t)S * 4, st));
+        HIPCHK(hipMemsetAsync(dstart.p, 0, (uint64_t)S * 4, st));
+        HIPCHK(hipMemsetAsync(result.p, 0, (uint64_t)S * 8, st));
         c32.reserve(C32_N);
         c64.reserve(C64_N);
-        // root frontier: one empty-occupancy state per instance, count 1
-        std::vector<uint64_t> k0(n), c0(n, 1);
-        for (uint32_t i = 0; i < n; i++) k0[i] = inst_shift < 64 ? (uint64_t)i << inst_shift : 0;
-        fk.reserve(n);
-        fc.reserve(n);
-        HIPCHK(hipMemcpy(fk.p, k0.data(), n * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(fc.p, c0.data(), n * 8, hipMemcpyHostToDevice));
-        n_front = n;
-        stats.peak_frontier = n;
-        if (Hmax == 0) row_step();
+        fk.reserve(S);
+        fc.reserve(S);
+        HIPCHK(hipStreamSynchronize(st));
     }
 
     ~Engine() {
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        for (hipEvent_t e : {evx0, evx1, evc0})
+        (void)hipStreamSynchronize(st);
+        for (hipEvent_t e : {ev0, ev1, evx0, evx1, evc0})
             if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
 
-    // rows still to be filled (CudaSearcher::get_height)
-    uint32_t height() const { return done ? 0 : Hmax - row; }
+    // Queue an instance in a free slot; it joins the frontier at the next
+    // step.
+    void admit(uint32_t slot, const uint8_t *rows, uint32_t h) {
+        if (slot >= S || active[slot]) throw Error(D4_EINVAL, "slot not free");
+        if (h > max_h) throw Error(D4_EINVAL, "board taller than the engine's maximum");
+        uint8_t *bd = &boards[(size_t)slot * (MAX_ROWS + 8)];
+        std::memset(bd, 0, MAX_ROWS + 8);
+        for (uint32_t r = 0; r < h; r++) {
+            if (lib->W < 8 && (rows[r] >> lib->W))
+                throw Error(D4_EINVAL, "board wider than the library's width");
+            bd[r] = rows[r];
+        }
+        heights[slot] = h;
+        active[slot] = 1;
+        host_result[slot] = 0;
+        pending.push_back(slot);
+        n_active++;
+    }
+
+    bool idle() const { return n_active == 0; }
+
+    // Device bytes held by this engine's buffers (high-water marks).
+    uint64_t device_bytes() const {
+        return 8 * (bw.cap + fk.cap + fc.cap + nk.cap + nc.cap + tk.cap + result.cap +
+                    result_save.cap + c64.cap + e_desc.cap + e_slot.cap) +
+               4 * (hgt.cap + dstart.cap + c32.cap + e_parent.cap + e_child.cap + slot_uid.cap +
+                    blk.cap);
+    }
+
+    // rows still to be filled by a slot's instance (CudaSearcher::get_height)
+    uint32_t height_of(uint32_t slot) const {
+        if (!active[slot]) return 0;
+        bool queued = std::find(pending.begin(), pending.end(), slot) != pending.end();
+        return queued ? heights[slot] : heights[slot] - std::min(heights[slot], g - start[slot]);
+    }
 
     ExtendArgs args(uint64_t tab_m, uint32_t e_cap) {
         ExtendArgs a{};
@@ -482,8 +502,8 @@ struct Engine {
         // split levels: tiny frontiers and the heavy first rows spread each
         // record over 64 / 4096 work items
         a.split = n_front <= 1024 ? 2u : n_front <= (1u << 16) ? 1u : 0u;
-
-        a.row = row;
+        a.refill = lib->ext_refill;
+        a.row = g;
         a.fit = lib->d_fit;
         a.placeinfo = lib->d_placeinfo;
         a.xavail = lib->d_xavail;
@@ -491,6 +511,7 @@ struct Engine {
         a.bw = bw.p;
         a.bw_stride = bw_stride;
         a.hgt = hgt.p;
+        a.start = dstart.p;
         a.result = result.p;
         a.occ_bits = occ_bits;
         a.inst_shift = inst_shift;
@@ -513,29 +534,88 @@ struct Engine {
         return a;
     }
 
-    // One row step (search_GPU).  When the last row is done the terminal
-    // states are absorbed into the per-instance counts in the same call.
-    void step() {
-        if (done) throw Error(D4_ESTATE, "search already finished");
-        if (row < Hmax) row_step();
-        if (!done && row >= Hmax) row_step();
+    // Move queued instances into the frontier: their board windows, height,
+    // start step and a zero count go to the device, and a root state (empty
+    // occupancy, count 1) is appended to the frontier.
+    void upload_pending() {
+        if (pending.empty()) return;
+        std::vector<uint64_t> win(bw_stride), k0, c0;
+        for (uint32_t slot : pending) {
+            const uint8_t *bd = &boards[(size_t)slot * (MAX_ROWS + 8)];
+            for (uint32_t r = 0; r < bw_stride; r++) {
+                uint64_t w = 0;
+                for (uint32_t j = 0; j < 8 && r + j < (uint32_t)MAX_ROWS + 8; j++)
+                    w |= (uint64_t)bd[r + j] << (8 * j);
+                win[r] = w;
+            }
+            HIPCHK(hipMemcpy(bw.p + (size_t)slot * bw_stride, win.data(), bw_stride * 8,
+                             hipMemcpyHostToDevice));
+            start[slot] = g;
+            k0.push_back(inst_shift < 64 ? (uint64_t)slot << inst_shift : 0);
+            c0.push_back(1);
+        }
+        HIPCHK(hipMemcpy(hgt.p, heights.data(), (size_t)S * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dstart.p, start.data(), (size_t)S * 4, hipMemcpyHostToDevice));
+        std::vector<unsigned long long> zero(1, 0);
+        for (uint32_t slot : pending)
+            HIPCHK(hipMemcpy(result.p + slot, zero.data(), 8, hipMemcpyHostToDevice));
+        const uint64_t n = n_front + k0.size();
+        if (n > fk.cap || n > fc.cap) {  // grow, keeping the current frontier
+            DevBuf<uint64_t> k2, c2;
+            k2.reserve(n);
+            c2.reserve(n);
+            if (n_front) {
+                HIPCHK(hipMemcpy(k2.p, fk.p, n_front * 8, hipMemcpyDeviceToDevice));
+                HIPCHK(hipMemcpy(c2.p, fc.p, n_front * 8, hipMemcpyDeviceToDevice));
+            }
+            fk.swap(k2);
+            fc.swap(c2);
+        }
+        HIPCHK(hipMemcpy(fk.p + n_front, k0.data(), k0.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(fc.p + n_front, c0.data(), c0.size() * 8, hipMemcpyHostToDevice));
+        n_front = n;
+        stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, n_front);
+        pending.clear();
     }
 
-    // Rows 0..Hmax-1 extend the frontier; the step at row Hmax (and, for an
-    // instance of height h, every step at row >= h) only absorbs terminal
-    // states into the per-instance counts.
+    // One global step: admitted instances join, every instance advances one
+    // row (search_GPU), and those that reach their height finish.
+    void step() {
+        completed.clear();
+        upload_pending();
+        if (!n_active) throw Error(D4_ESTATE, "no instance left to search");
+        row_step();
+        bool any = false;
+        for (uint32_t i = 0; i < S; i++)
+            if (active[i] && start[i] + heights[i] == g) any = true;
+        if (any) {
+            HIPCHK(hipMemcpy(host_result.data(), result.p, (size_t)S * 8, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < S; i++)
+                if (active[i] && start[i] + heights[i] == g) {
+                    completed.push_back(i);
+                    active[i] = 0;
+                    n_active--;
+                }
+        }
+        g++;
+    }
+
+    void run() {
+        while (n_active) step();
+    }
+
     void row_step() {
         if (n_front > 0xffffffffull) throw Error(D4_ERANGE, "frontier exceeds 2^32 records");
         const bool en = mode == D4_MODE_ENUM;
         const bool prune = !(flags & D4_FLAG_NO_PRUNE);
         // Table size: unique children <= children.  A small frontier gets an
+        // exact bound (row-0 completions per record) or, failing that, an
         // exact count from a search-only pass; a large one is estimated from
-        // the previous row's children per record (x2), with a rerun at 4x
-        // the size if the table still overflows.
+        // the previous step's children per record, with a rerun at 4x the
+        // size if the table still overflows.
         uint64_t est_c = 0, est_u = 0;
         bool exact = false;
         if (n_front <= (1u << 20)) {
-            // children <= sum over records of the row-0 completions (cheap)
             const uint64_t bound = next_size();
             if (bound <= (1ull << 26)) {
                 est_c = est_u = bound;
@@ -553,14 +633,14 @@ struct Engine {
         uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
         uint64_t est_e = en ? est_c + 64 : 0;
         if (en) {
+            if (logs.empty()) log0 = g;
             RowLog lg;
             lg.keys.resize(n_front);
             if (n_front)
                 HIPCHK(hipMemcpy(lg.keys.data(), fk.p, n_front * 8, hipMemcpyDeviceToHost));
             logs.push_back(std::move(lg));
         }
-        const bool absorbing = row >= *std::min_element(heights.begin(), heights.end());
-        if (absorbing) result_save.reserve(n_inst);
+        result_save.reserve(S);
         uint32_t h32[C32_N];
         unsigned long long h64[C64_N];
         for (;;) {
@@ -573,9 +653,10 @@ struct Engine {
                 e_slot.reserve(est_e);
             }
             HIPCHK(hipEventRecord(ev0, st));
-            if (absorbing)
-                HIPCHK(hipMemcpyAsync(result_save.p, result.p, n_inst * 8,
-                                      hipMemcpyDeviceToDevice, st));
+            // terminal counts are added during the step: keep a copy so a
+            // rerun after an overflow starts from the same counts
+            HIPCHK(hipMemcpyAsync(result_save.p, result.p, (size_t)S * 8, hipMemcpyDeviceToDevice,
+                                  st));
             launch_fill_table(tk.p, tk.p + 1, tab_m, st);
             launch_reset_counters(c32.p, c64.p, st);
             HIPCHK(hipEventRecord(evx0, st));
@@ -597,12 +678,9 @@ struct Engine {
             bool ovf = h32[C32_OVERFLOW] != 0;
             bool eovf = en && h32[C32_EDGE_OVF] != 0;
             if (!ovf && !eovf) break;
+            // the step is a pure function of the frontier: rerun it bigger
             stats.retries++;
-            // the step is a pure function of the frontier: rerun it bigger,
-            // after undoing the terminal counts it already added
-            if (absorbing)
-                HIPCHK(hipMemcpy(result.p, result_save.p, n_inst * 8,
-                                 hipMemcpyDeviceToDevice));
+            HIPCHK(hipMemcpy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice));
             if (ovf) tab_m *= 4;
             if (eovf) est_e = (uint64_t)h32[C32_EDGES] + 1024;
         }
@@ -651,20 +729,12 @@ struct Engine {
         fk.swap(nk);
         fc.swap(nc);
         n_front = U;
-        if (!U) fk.reserve(1);
+        if (!U) {
+            fk.reserve(1);
+            fc.reserve(1);
+        }
         stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, U);
         stats.rows_done++;
-        if (row >= Hmax) {
-            done = true;
-            host_result.resize(n_inst);
-            HIPCHK(hipMemcpy(host_result.data(), result.p, n_inst * 8, hipMemcpyDeviceToHost));
-        } else {
-            row++;
-        }
-    }
-
-    void run() {
-        while (!done) step();
     }
 
     // Children the current frontier will emit (after pruning): the extend
@@ -682,9 +752,10 @@ struct Engine {
     }
 
     uint64_t next_size() {
-        if (done || !n_front) return 0;
+        upload_pending();
+        if (!n_front) return 0;
         launch_reset_counters(c32.p, c64.p, st);
-        launch_next_size(fk.p, (uint32_t)n_front, row, bw.p, bw_stride, hgt.p, inst_shift,
+        launch_next_size(fk.p, (uint32_t)n_front, g, dstart.p, bw.p, bw_stride, hgt.p, inst_shift,
                          occ_mask, lib->d_comp, c32.p, c64.p, lib->n_cu, st);
         unsigned long long h64[C64_N];
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
@@ -692,31 +763,46 @@ struct Engine {
         return h64[C64_NEXTSIZE];
     }
 
+    // Drop enum logs no unfinished instance can need any more.
+    void trim_logs() {
+        if (keep_logs || logs.empty()) return;
+        uint32_t keep_from = g;
+        for (uint32_t i = 0; i < S; i++)
+            if (active[i]) keep_from = std::min(keep_from, start[i]);
+        while (!logs.empty() && log0 < keep_from) {
+            logs.pop_front();
+            log0++;
+        }
+    }
+
     // ---- reconstruction (enum mode) --------------------------------------
 
-    // Each solution as its sequence of candidate ids in placement order.
-    // Walks the back-pointers from instance i's terminal states (the
-    // frontier at row heights[i]) down to the root.
-    std::vector<std::vector<uint8_t>> solutions_of(uint32_t inst, bool single) const {
+    // Each solution of the instance in `slot` (finished at step
+    // start + height) as its sequence of candidate ids in placement order:
+    // the back-pointers are walked from its terminal states down to its root.
+    std::vector<std::vector<uint8_t>> solutions_of(uint32_t slot, bool single) const {
         if (mode != D4_MODE_ENUM) throw Error(D4_ESTATE, "solutions need D4_MODE_ENUM");
-        if (!done) throw Error(D4_ESTATE, "search not finished");
-        const uint32_t H = heights[inst];
-        std::vector<std::vector<uint8_t>> out;
+        if (slot >= S) throw Error(D4_EINVAL, "instance out of range");
+        if (active[slot]) throw Error(D4_ESTATE, "search not finished");
+        const uint32_t H = heights[slot], s0 = start[slot];
+        if (s0 < log0 || s0 + H >= log0 + logs.size())
+            throw Error(D4_ESTATE, "solutions of this instance are no longer held");
+        auto L = [&](uint32_t r) -> const RowLog & { return logs[s0 + r - log0]; };
         auto inst_of = [&](uint64_t key) -> uint32_t {
             return inst_shift < 64 ? (uint32_t)(key >> inst_shift) : 0u;
         };
         // reverse adjacency per row: child -> edges
-        std::vector<std::vector<uint32_t>> start(H), order(H);
+        std::vector<std::vector<uint32_t>> first(H), order(H);
         for (uint32_t r = 0; r < H; r++) {
-            const RowLog &lg = logs[r];
-            size_t nchild = r + 1 < logs.size() ? logs[r + 1].keys.size() : 0;
+            const RowLog &lg = L(r);
+            size_t nchild = L(r + 1).keys.size();
             std::vector<uint32_t> cnt(nchild + 1, 0);
             for (uint32_t c : lg.e_child) cnt[c + 1]++;
             for (size_t j = 1; j <= nchild; j++) cnt[j] += cnt[j - 1];
             std::vector<uint32_t> ord(lg.e_child.size());
             std::vector<uint32_t> pos(cnt.begin(), cnt.end() - 1);
             for (uint32_t e = 0; e < lg.e_child.size(); e++) ord[pos[lg.e_child[e]]++] = e;
-            start[r] = std::move(cnt);
+            first[r] = std::move(cnt);
             order[r] = std::move(ord);
         }
         std::vector<uint64_t> path(H);
@@ -727,20 +813,18 @@ struct Engine {
                 paths.push_back(path);
                 return;
             }
-            const RowLog &lg = logs[r - 1];
-            for (uint32_t j = start[r - 1][u]; j < start[r - 1][u + 1]; j++) {
+            const RowLog &lg = L(r - 1);
+            for (uint32_t j = first[r - 1][u]; j < first[r - 1][u + 1]; j++) {
                 uint32_t e = order[r - 1][j];
+                if (inst_of(L(r - 1).keys[lg.e_parent[e]]) != slot) continue;
                 path[r - 1] = lg.e_desc[e];
                 walk(r - 1, lg.e_parent[e]);
             }
         };
-        if (H == 0) {
-            paths.push_back({});
-        } else if (H < logs.size()) {
-            const std::vector<uint64_t> &term = logs[H].keys;
-            for (uint32_t u = 0; u < term.size(); u++)
-                if (inst_of(term[u]) == inst) walk(H, u);
-        }
+        const std::vector<uint64_t> &term = L(H).keys;
+        for (uint32_t u = 0; u < term.size(); u++)
+            if (inst_of(term[u]) == slot) walk(H, u);
+        std::vector<std::vector<uint8_t>> out;
         for (auto &p : paths) {
             std::vector<uint8_t> seq;
             for (uint64_t d : p)
@@ -757,14 +841,14 @@ struct Engine {
 
     // Steps {piece_id, trs_id, y, x} of a candidate sequence, replayed on the
     // board (anchor = lowest open cell, Piece.inl:5-24).
-    void encode(uint32_t inst, const std::vector<uint8_t> &seq, std::vector<uint8_t> &buf) const {
+    void encode(uint32_t slot, const std::vector<uint8_t> &seq, std::vector<uint8_t> &buf) const {
         uint8_t open[MAX_ROWS + 8];
-        std::memcpy(open, &boards[(size_t)inst * (MAX_ROWS + 8)], sizeof open);
+        std::memcpy(open, &boards[(size_t)slot * (MAX_ROWS + 8)], sizeof open);
         buf.push_back((uint8_t)seq.size());
         uint32_t r = 0;
         for (uint8_t k : seq) {
-            while (r < heights[inst] && !open[r]) r++;
-            if (r >= heights[inst]) throw Error(D4_EDEVICE, "inconsistent back-pointers");
+            while (r < heights[slot] && !open[r]) r++;
+            if (r >= heights[slot]) throw Error(D4_EDEVICE, "inconsistent back-pointers");
             uint32_t c = (uint32_t)__builtin_ctz(open[r]);
             uint64_t pl = lib->place[c][k];
             for (uint32_t j = 0; j < 8 && r + j < (uint32_t)MAX_ROWS + 8; j++) {
@@ -779,23 +863,108 @@ struct Engine {
         }
     }
 
-    void solutions(uint32_t inst, int single, uint8_t *out, size_t cap, size_t *len,
-                   uint64_t *nsol) const {
-        if (inst >= n_inst) throw Error(D4_EINVAL, "instance out of range");
-        auto seqs = solutions_of(inst, single != 0);
+    std::vector<uint8_t> solution_bytes(uint32_t slot, bool single, uint64_t *nsol) const {
+        auto seqs = solutions_of(slot, single);
         std::vector<uint8_t> buf;
-        for (auto &s : seqs) encode(inst, s, buf);
-        if (len) *len = buf.size();
+        for (auto &sq : seqs) encode(slot, sq, buf);
         if (nsol) *nsol = seqs.size();
+        return buf;
+    }
+
+    void solutions(uint32_t slot, int single, uint8_t *out, size_t cap, size_t *len,
+                   uint64_t *nsol) const {
+        auto buf = solution_bytes(slot, single != 0, nsol);
+        if (len) *len = buf.size();
         if (out && buf.size() <= cap) std::memcpy(out, buf.data(), buf.size());
         else if (out && cap) throw Error(D4_ERANGE, "solution buffer too small");
     }
 };
 
+// Engine with every instance admitted at step 0 (batch and single search).
+static std::unique_ptr<Engine> make_batch(const d4_library *lib, uint32_t n, const uint8_t *b,
+                                          uint32_t stride, const uint32_t *h, uint32_t mode,
+                                          uint32_t flags) {
+    if (!n) throw Error(D4_EINVAL, "no instances");
+    uint32_t hm = 0;
+    for (uint32_t i = 0; i < n; i++) hm = std::max(hm, h[i]);
+    auto e = std::make_unique<Engine>(lib, n, hm, mode, flags, true);
+    for (uint32_t i = 0; i < n; i++) e->admit(i, b + (size_t)i * stride, h[i]);
+    e->upload_pending();
+    return e;
+}
+
 }  // namespace
 
 struct d4_searcher {
     std::unique_ptr<Engine> e;
+};
+
+// Request loop with continuous batching: "solve all" requests run in a
+// count-mode engine, "solve one" requests in an enum-mode engine; each step
+// admits waiting requests into free slots (while slots and the device-memory
+// budget allow), advances every in-flight request by one row and retires the
+// ones that finished.  Requests that arrive while others are on the device
+// join the same frontier at the next row.
+struct d4_server {
+    const d4_library *lib = nullptr;
+    uint32_t slots = 0, max_h = 0;
+    uint64_t budget = 0, est_per_req = 16ull << 20;
+    std::unique_ptr<Engine> eng[2];  // [0] count, [1] enum
+    std::vector<uint32_t> free_slots[2];
+    std::vector<uint64_t> slot_req[2];
+    struct Req {
+        uint64_t id;
+        std::vector<uint8_t> rows;
+        bool single;
+    };
+    std::deque<Req> waiting;
+    struct Done {
+        uint64_t id, count;
+        std::vector<uint8_t> sol;
+    };
+    std::deque<Done> done;
+    uint64_t next_id = 1, admitted = 0, rejected_mem = 0;
+
+    uint32_t in_flight() const { return eng[0]->n_active + eng[1]->n_active; }
+
+    void admit() {
+        while (!waiting.empty()) {
+            Req &r = waiting.front();
+            const int k = r.single ? 1 : 0;
+            if (free_slots[k].empty()) break;
+            // admission by device memory: what the engines hold now, plus
+            // the larger of a default and the observed bytes per request
+            const uint64_t used = eng[0]->device_bytes() + eng[1]->device_bytes();
+            const uint32_t fl = in_flight();
+            if (fl) est_per_req = std::max(est_per_req, used / fl);
+            if (fl && used + est_per_req > budget) {
+                rejected_mem++;
+                break;
+            }
+            const uint32_t slot = free_slots[k].back();
+            free_slots[k].pop_back();
+            eng[k]->admit(slot, r.rows.data(), (uint32_t)r.rows.size());
+            slot_req[k][slot] = r.id;
+            admitted++;
+            waiting.pop_front();
+        }
+    }
+
+    void step() {
+        admit();
+        for (int k = 0; k < 2; k++) {
+            Engine &e = *eng[k];
+            if (!e.n_active) continue;
+            e.step();
+            for (uint32_t slot : e.completed) {
+                Done d{slot_req[k][slot], e.host_result[slot], {}};
+                if (k == 1) d.sol = e.solution_bytes(slot, true, nullptr);
+                done.push_back(std::move(d));
+                free_slots[k].push_back(slot);
+            }
+            e.trim_logs();
+        }
+    }
 };
 struct d4_batch {
     std::unique_ptr<Engine> e;
@@ -894,9 +1063,11 @@ int d4_library_info(const d4_library *lib, uint32_t *n_cand, uint32_t *max_heigh
     });
 }
 
-int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads) {
+int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads,
+                          uint32_t refill) {
     return guard([&] {
         if (!lib) throw Error(D4_EINVAL, "null library");
+        if (refill < 1 || refill > 64) throw Error(D4_EINVAL, "refill must be 1..64");
         if (threads != 256 && threads != 512 && threads != 1024)
             throw Error(D4_EINVAL, "threads must be 256, 512 or 1024");
         if (fit_rows < 1 || fit_rows > lib->maxh) throw Error(D4_EINVAL, "fit_rows out of range");
@@ -911,45 +1082,59 @@ int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads) 
             if (old) (void)hipFree(old);
         }
         lib->ext_threads = threads;
+        lib->ext_refill = refill;
     });
 }
 
-int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows, uint32_t *threads) {
+int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows, uint32_t *threads,
+                          uint32_t *refill) {
     return guard([&] {
         if (!lib) throw Error(D4_EINVAL, "null library");
         if (fit_rows) *fit_rows = lib->fit_rows;
         if (threads) *threads = lib->ext_threads;
+        if (refill) *refill = lib->ext_refill;
     });
 }
 
 int d4_library_tune(d4_library *lib, uint32_t n_inst, const uint8_t *boards, uint32_t stride,
                     const uint32_t *heights, uint32_t *fit_rows, uint32_t *threads,
-                    double *best_ms) {
+                    uint32_t *refill, double *best_ms) {
     return guard([&] {
         if (!lib || !boards || !heights || !n_inst) throw Error(D4_EINVAL, "null argument");
-        uint32_t best_f = lib->fit_rows, best_t = lib->ext_threads;
+        uint32_t best_f = lib->fit_rows, best_t = lib->ext_threads, best_r = lib->ext_refill;
         double best = -1;
-        {  // warm the allocator cache and the code objects
-            Engine e(lib, n_inst, boards, stride, heights, D4_MODE_COUNT, 0);
-            e.run();
-        }
+        auto measure = [&]() {
+            auto e = make_batch(lib, n_inst, boards, stride, heights, D4_MODE_COUNT, 0);
+            e->run();
+            return e->stats.extend_ms;
+        };
+        measure();  // warm the allocator cache and the code objects
+        // coordinate search: table rows x workgroup size first, then the
+        // refill threshold at the best of those
         for (uint32_t f = 1; f <= std::min(lib->maxh, 4u); f++)
             for (uint32_t t : {256u, 512u, 1024u}) {
                 if (extend_lds_bytes(f, t, true) > lib->lds_limit) continue;
-                if (d4_library_set_launch(lib, f, t) != D4_OK) continue;
-                Engine e(lib, n_inst, boards, stride, heights, D4_MODE_COUNT, 0);
-                e.run();
-                double ms = e.stats.extend_ms;
+                if (d4_library_set_launch(lib, f, t, best_r) != D4_OK) continue;
+                double ms = measure();
                 if (best < 0 || ms < best) {
                     best = ms;
                     best_f = f;
                     best_t = t;
                 }
             }
-        if (d4_library_set_launch(lib, best_f, best_t) != D4_OK)
+        for (uint32_t r : {1u, 4u, 8u, 16u, 32u, 48u}) {
+            if (d4_library_set_launch(lib, best_f, best_t, r) != D4_OK) continue;
+            double ms = measure();
+            if (ms < best) {
+                best = ms;
+                best_r = r;
+            }
+        }
+        if (d4_library_set_launch(lib, best_f, best_t, best_r) != D4_OK)
             throw Error(D4_EDEVICE, g_err);
         if (fit_rows) *fit_rows = best_f;
         if (threads) *threads = best_t;
+        if (refill) *refill = best_r;
         if (best_ms) *best_ms = best;
     });
 }
@@ -960,7 +1145,8 @@ int d4_searcher_create(const d4_library *lib, const uint8_t *rows, uint32_t heig
         if (!lib || !out || (!rows && height)) throw Error(D4_EINVAL, "null argument");
         *out = nullptr;
         auto s = std::make_unique<d4_searcher>();
-        s->e = std::make_unique<Engine>(lib, 1, rows, height, &height, mode, flags);
+        s->e = make_batch(lib, 1, rows, height, &height, mode, flags);
+        if (!height) s->e->run();  // an empty board has its one (empty) tiling
         *out = s.release();
     });
 }
@@ -976,7 +1162,7 @@ int d4_searcher_create_ea(const d4_library *lib, uint64_t empty_area, uint32_t m
 
 void d4_searcher_destroy(d4_searcher *s) { delete s; }
 
-uint32_t d4_searcher_height(const d4_searcher *s) { return s ? s->e->height() : 0; }
+uint32_t d4_searcher_height(const d4_searcher *s) { return s ? s->e->height_of(0) : 0; }
 
 uint64_t d4_searcher_size(const d4_searcher *s) { return s ? s->e->n_front : 0; }
 
@@ -990,7 +1176,10 @@ uint64_t d4_searcher_next_size(const d4_searcher *s) {
 int d4_searcher_step(d4_searcher *s) {
     return guard([&] {
         if (!s) throw Error(D4_EINVAL, "null searcher");
+        if (!s->e->n_active) throw Error(D4_ESTATE, "search already finished");
         s->e->step();
+        // after the last row the terminal states are absorbed in the same call
+        if (s->e->n_active && s->e->height_of(0) == 0) s->e->step();
     });
 }
 
@@ -1004,7 +1193,7 @@ int d4_searcher_run(d4_searcher *s) {
 int d4_searcher_count(const d4_searcher *s, uint64_t *count) {
     return guard([&] {
         if (!s || !count) throw Error(D4_EINVAL, "null argument");
-        if (!s->e->done) throw Error(D4_ESTATE, "search not finished");
+        if (s->e->n_active) throw Error(D4_ESTATE, "search not finished");
         *count = s->e->host_result[0];
     });
 }
@@ -1024,6 +1213,88 @@ int d4_searcher_stats(const d4_searcher *s, d4_stats *st) {
     });
 }
 
+int d4_server_create(const d4_library *lib, uint32_t slots, uint32_t max_height,
+                     uint64_t mem_budget, d4_server **out) {
+    return guard([&] {
+        if (!lib || !out) throw Error(D4_EINVAL, "null argument");
+        *out = nullptr;
+        if (!slots) throw Error(D4_EINVAL, "need at least one slot");
+        auto sv = std::make_unique<d4_server>();
+        sv->lib = lib;
+        sv->slots = slots;
+        sv->max_h = max_height;
+        HIPCHK(hipSetDevice(lib->device));
+        if (!mem_budget) {
+            size_t fr = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fr, &tot));
+            mem_budget = fr / 2;
+        }
+        sv->budget = mem_budget;
+        for (int k = 0; k < 2; k++) {
+            sv->eng[k] = std::make_unique<Engine>(lib, slots, max_height,
+                                                  k ? D4_MODE_ENUM : D4_MODE_COUNT, 0, false);
+            sv->slot_req[k].assign(slots, 0);
+            for (uint32_t i = slots; i-- > 0;) sv->free_slots[k].push_back(i);
+        }
+        *out = sv.release();
+    });
+}
+
+void d4_server_destroy(d4_server *s) { delete s; }
+
+int d4_server_submit(d4_server *s, const uint8_t *rows, uint32_t height, int single,
+                     uint64_t *req_id) {
+    return guard([&] {
+        if (!s || (!rows && height)) throw Error(D4_EINVAL, "null argument");
+        if (height > s->max_h) throw Error(D4_EINVAL, "board taller than the server's maximum");
+        for (uint32_t r = 0; r < height; r++)
+            if (s->lib->W < 8 && (rows[r] >> s->lib->W))
+                throw Error(D4_EINVAL, "board wider than the library's width");
+        d4_server::Req r{s->next_id++, std::vector<uint8_t>(rows, rows + height), single != 0};
+        if (req_id) *req_id = r.id;
+        s->waiting.push_back(std::move(r));
+    });
+}
+
+int d4_server_step(d4_server *s, uint32_t *in_flight, uint32_t *waiting) {
+    return guard([&] {
+        if (!s) throw Error(D4_EINVAL, "null server");
+        s->step();
+        if (in_flight) *in_flight = s->in_flight();
+        if (waiting) *waiting = (uint32_t)s->waiting.size();
+    });
+}
+
+int d4_server_poll(d4_server *s, uint64_t *req_id, uint64_t *count, uint8_t *sol, size_t cap,
+                   size_t *len) {
+    int got = 0;
+    int rc = guard([&] {
+        if (!s) throw Error(D4_EINVAL, "null server");
+        if (s->done.empty()) return;
+        d4_server::Done &d = s->done.front();
+        if (len) *len = d.sol.size();
+        if (sol && d.sol.size() > cap) throw Error(D4_ERANGE, "solution buffer too small");
+        if (sol && !d.sol.empty()) std::memcpy(sol, d.sol.data(), d.sol.size());
+        if (req_id) *req_id = d.id;
+        if (count) *count = d.count;
+        s->done.pop_front();
+        got = 1;
+    });
+    return rc == D4_OK ? got : rc;
+}
+
+int d4_server_stats(const d4_server *s, d4_stats *all, d4_stats *one, uint64_t *admitted,
+                    uint64_t *mem_waits, uint64_t *device_bytes) {
+    return guard([&] {
+        if (!s) throw Error(D4_EINVAL, "null server");
+        if (all) *all = s->eng[0]->stats;
+        if (one) *one = s->eng[1]->stats;
+        if (admitted) *admitted = s->admitted;
+        if (mem_waits) *mem_waits = s->rejected_mem;
+        if (device_bytes) *device_bytes = s->eng[0]->device_bytes() + s->eng[1]->device_bytes();
+    });
+}
+
 int d4_batch_create(const d4_library *lib, uint32_t n_inst, const uint8_t *boards,
                     uint32_t stride, const uint32_t *heights, uint32_t mode, uint32_t flags,
                     d4_batch **out) {
@@ -1031,7 +1302,7 @@ int d4_batch_create(const d4_library *lib, uint32_t n_inst, const uint8_t *board
         if (!lib || !out || !boards || !heights) throw Error(D4_EINVAL, "null argument");
         *out = nullptr;
         auto b = std::make_unique<d4_batch>();
-        b->e = std::make_unique<Engine>(lib, n_inst, boards, stride, heights, mode, flags);
+        b->e = make_batch(lib, n_inst, boards, stride, heights, mode, flags);
         *out = b.release();
     });
 }
@@ -1048,8 +1319,8 @@ int d4_batch_run(d4_batch *b) {
 int d4_batch_counts(const d4_batch *b, uint64_t *counts) {
     return guard([&] {
         if (!b || !counts) throw Error(D4_EINVAL, "null argument");
-        if (!b->e->done) throw Error(D4_ESTATE, "batch not finished");
-        for (uint32_t i = 0; i < b->e->n_inst; i++) counts[i] = b->e->host_result[i];
+        if (b->e->n_active) throw Error(D4_ESTATE, "batch not finished");
+        for (uint32_t i = 0; i < b->e->S; i++) counts[i] = b->e->host_result[i];
     });
 }
 

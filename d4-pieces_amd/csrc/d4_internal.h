@@ -40,8 +40,9 @@ struct ExtendArgs {
     const uint64_t *in_key;
     const uint64_t *in_cnt;
     uint32_t n_in;
-    uint32_t row;
+    uint32_t row;            // global step
     uint32_t split;          // levels fixed per work item: 0, 1 or 2
+    uint32_t refill;         // idle lanes that trigger a refill round (1..64)
     // library tables (global copies; staged into LDS)
     const uint64_t *fit;     // [8][fit_rows][256] candidate masks
     const uint64_t *placeinfo;  // [8][64][2] placed shape, counter info + range
@@ -50,7 +51,9 @@ struct ExtendArgs {
     // instances
     const uint64_t *bw;      // [n_inst][bw_stride] board window of row r
     uint32_t bw_stride;
-    const uint32_t *hgt;     // [n_inst]
+    const uint32_t *hgt;     // [n_inst] board heights
+    const uint32_t *start;   // [n_inst] step at which the instance joined:
+                             // its current row is row - start[inst]
     unsigned long long *result;  // [n_inst] terminal counts
     // key layout
     uint32_t occ_bits;
@@ -91,7 +94,7 @@ void launch_compact(const unsigned long long *tab_key,
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
                         uint32_t *e_child, uint32_t n, hipStream_t st);
 void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
-                      const uint64_t *bw, uint32_t bw_stride,
+                      const uint32_t *start, const uint64_t *bw, uint32_t bw_stride,
                       const uint32_t *hgt, uint32_t inst_shift,
                       uint64_t occ_mask, const unsigned long long *comp,
                       uint32_t *c32, unsigned long long *c64, int n_cu,

@@ -269,7 +269,10 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     };
 
     for (;;) {
+        // refill idle lanes once at least a.refill of them are idle (or the
+        // whole wave is): each refill round costs the wave a full handout
         uint64_t im = __ballot(!(searching || pass));
+        if ((uint32_t)__builtin_popcountll(im) < a.refill && ~im) im = 0;
         while (im) {
             if (buf_pos >= buf_items) {
                 if (exhausted) break;
@@ -294,9 +297,11 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 if (lane < nrec) {
                     bkey = a.in_key[buf_rec0 + lane];
                     bcnt = a.in_cnt[buf_rec0 + lane];
+                    // the instance's own row: instances join at their start step
                     uint32_t bi = a.inst_shift < 64 ? (uint32_t)(bkey >> a.inst_shift) : 0u;
-                    bterm = a.row >= a.hgt[bi];
-                    const uint64_t *w = a.bw + (size_t)bi * a.bw_stride + a.row;
+                    const uint32_t rr = a.row - a.start[bi];
+                    bterm = rr >= a.hgt[bi];
+                    const uint64_t *w = a.bw + (size_t)bi * a.bw_stride + rr;
                     bBW = bterm ? 0 : w[0];
                     bBWn = bterm ? 0 : w[1];
                 }
@@ -605,7 +610,7 @@ __global__ __launch_bounds__(256) void k_remap_edges(const uint64_t *e_slot,
 }
 
 __global__ __launch_bounds__(256) void k_next_size(
-    const uint64_t *key, uint32_t n, uint32_t row, const uint64_t *bw,
+    const uint64_t *key, uint32_t n, uint32_t row, const uint32_t *start, const uint64_t *bw,
     uint32_t bw_stride, const uint32_t *hgt, uint32_t inst_shift,
     uint64_t occ_mask, const unsigned long long *comp, uint32_t *c32,
     unsigned long long *c64) {
@@ -614,8 +619,9 @@ __global__ __launch_bounds__(256) void k_next_size(
          i += gridDim.x * blockDim.x) {
         uint64_t k = key[i];
         uint32_t inst = inst_shift < 64 ? (uint32_t)(k >> inst_shift) : 0u;
-        if (row >= hgt[inst]) continue;
-        uint64_t E = bw[(size_t)inst * bw_stride + row] & ~(k & occ_mask);
+        const uint32_t rr = row - start[inst];
+        if (rr >= hgt[inst]) continue;
+        uint64_t E = bw[(size_t)inst * bw_stride + rr] & ~(k & occ_mask);
         acc += comp[E & 0xffu];
     }
     acc = wave_sum_u64(acc);
@@ -708,7 +714,7 @@ void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
 }
 
 void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
-                      const uint64_t *bw, uint32_t bw_stride,
+                      const uint32_t *start, const uint64_t *bw, uint32_t bw_stride,
                       const uint32_t *hgt, uint32_t inst_shift,
                       uint64_t occ_mask, const unsigned long long *comp,
                       uint32_t *c32, unsigned long long *c64, int n_cu,
@@ -716,7 +722,7 @@ void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
     uint32_t blocks = (n + 255) / 256;
     if (blocks > (uint32_t)n_cu * 4) blocks = (uint32_t)n_cu * 4;
     if (!blocks) blocks = 1;
-    hipLaunchKernelGGL(k_next_size, dim3(blocks), dim3(256), 0, st, key, n, row, bw,
+    hipLaunchKernelGGL(k_next_size, dim3(blocks), dim3(256), 0, st, key, n, row, start, bw,
                        bw_stride, hgt, inst_shift, occ_mask, comp, c32, c64);
 }
 

@@ -75,10 +75,11 @@ SIGNATURES = [
      [_U64P, _U32P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
     ("d4_library_destroy", None, [_P]),
     ("d4_library_info", ctypes.c_int, [_P, _U32P, _U32P]),
-    ("d4_library_set_launch", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
-    ("d4_library_get_launch", ctypes.c_int, [_P, _U32P, _U32P]),
+    ("d4_library_set_launch", ctypes.c_int,
+     [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    ("d4_library_get_launch", ctypes.c_int, [_P, _U32P, _U32P, _U32P]),
     ("d4_library_tune", ctypes.c_int,
-     [_P, ctypes.c_uint32, _U8P, ctypes.c_uint32, _U32P, _U32P, _U32P,
+     [_P, ctypes.c_uint32, _U8P, ctypes.c_uint32, _U32P, _U32P, _U32P, _U32P,
       ctypes.POINTER(ctypes.c_double)]),
     ("d4_searcher_create_ea", ctypes.c_int,
      [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
@@ -103,6 +104,15 @@ SIGNATURES = [
     ("d4_batch_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("d4_batch_solutions", ctypes.c_int,
      [_P, ctypes.c_uint32, ctypes.c_int, _U8P, ctypes.c_size_t, _SZP, _U64P]),
+    ("d4_server_create", ctypes.c_int,
+     [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    ("d4_server_destroy", None, [_P]),
+    ("d4_server_submit", ctypes.c_int,
+     [_P, _U8P, ctypes.c_uint32, ctypes.c_int, _U64P]),
+    ("d4_server_step", ctypes.c_int, [_P, _U32P, _U32P]),
+    ("d4_server_poll", ctypes.c_int, [_P, _U64P, _U64P, _U8P, ctypes.c_size_t, _SZP]),
+    ("d4_server_stats", ctypes.c_int,
+     [_P, ctypes.POINTER(Stats), ctypes.POINTER(Stats), _U64P, _U64P, _U64P]),
 ]
 
 
@@ -190,22 +200,24 @@ class PieceLibrary:
         _check(lib.d4_library_info(h, ctypes.byref(nc), ctypes.byref(mh)))
         self.n_cand, self.max_height = nc.value, mh.value
 
-    def set_launch(self, fit_rows: int, threads: int) -> None:
-        _check(load().d4_library_set_launch(self._h, fit_rows, threads))
+    def set_launch(self, fit_rows: int, threads: int, refill: int = 16) -> None:
+        _check(load().d4_library_set_launch(self._h, fit_rows, threads, refill))
 
-    def get_launch(self) -> Tuple[int, int]:
-        f, t = ctypes.c_uint32(), ctypes.c_uint32()
-        _check(load().d4_library_get_launch(self._h, ctypes.byref(f), ctypes.byref(t)))
-        return f.value, t.value
+    def get_launch(self) -> Tuple[int, int, int]:
+        f, t, r = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(load().d4_library_get_launch(self._h, ctypes.byref(f), ctypes.byref(t),
+                                            ctypes.byref(r)))
+        return f.value, t.value, r.value
 
     def tune(self, boards: Sequence[Sequence[int]]) -> dict:
         """Measured autotune of the extend launch shape on these boards."""
         flat, stride, heights = _flatten(boards)
-        f, t, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        f, t, r, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
         _check(load().d4_library_tune(self._h, len(boards), _u8arr(flat), stride,
                                       _u32arr(heights), ctypes.byref(f), ctypes.byref(t),
-                                      ctypes.byref(ms)))
-        return {"fit_rows": f.value, "threads": t.value, "extend_ms": ms.value}
+                                      ctypes.byref(r), ctypes.byref(ms)))
+        return {"fit_rows": f.value, "threads": t.value, "refill": r.value,
+                "extend_ms": ms.value}
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -328,3 +340,56 @@ class Batch:
         st = Stats()
         _check(load().d4_batch_stats(self._h, ctypes.byref(st)))
         return st.as_dict()
+
+
+class Server:
+    """Continuous-batching request loop (d4_server_*)."""
+
+    def __init__(self, library: PieceLibrary, slots: int = 256, max_height: int = 8,
+                 mem_budget: int = 0):
+        self.library = library
+        h = _P()
+        _check(load().d4_server_create(library._h, slots, max_height, mem_budget,
+                                       ctypes.byref(h)))
+        self._h = h
+        self._buf = (ctypes.c_uint8 * 4096)()
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.d4_server_destroy(self._h)
+            self._h = None
+
+    def submit(self, rows: Sequence[int], single: bool) -> int:
+        rid = ctypes.c_uint64()
+        rows = list(rows)
+        _check(load().d4_server_submit(self._h, _u8arr(rows), len(rows), int(single),
+                                       ctypes.byref(rid)))
+        return rid.value
+
+    def step(self) -> Tuple[int, int]:
+        fl, wt = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(load().d4_server_step(self._h, ctypes.byref(fl), ctypes.byref(wt)))
+        return fl.value, wt.value
+
+    def poll(self):
+        """One finished request as (req_id, count, solution or None), or None."""
+        rid, cnt, ln = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_size_t()
+        rc = load().d4_server_poll(self._h, ctypes.byref(rid), ctypes.byref(cnt), self._buf,
+                                   len(self._buf), ctypes.byref(ln))
+        if rc == D4_ERANGE:
+            self._buf = (ctypes.c_uint8 * (2 * len(self._buf)))()
+            return self.poll()
+        if rc < 0:
+            _check(rc)
+        if rc == 0:
+            return None
+        sols = decode_solutions(bytes(self._buf[:ln.value]))
+        return rid.value, cnt.value, (sols[0] if sols else None)
+
+    def stats(self) -> dict:
+        a, o = Stats(), Stats()
+        adm, mw, db = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(load().d4_server_stats(self._h, ctypes.byref(a), ctypes.byref(o),
+                                      ctypes.byref(adm), ctypes.byref(mw), ctypes.byref(db)))
+        return {"all": a.as_dict(), "one": o.as_dict(), "admitted": adm.value,
+                "mem_waits": mw.value, "device_bytes": db.value}

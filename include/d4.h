@@ -113,12 +113,14 @@ int d4_library_info(const d4_library *lib, uint32_t *n_cand,
 
 /* Launch shape of the extend kernel: fit_rows = window rows covered by the
  * LDS fit table (1..max_height; the chosen placement is always tested in
- * full), threads = workgroup size (256, 512 or 1024).  Not to be changed
- * while a search on this library is running.  Replaces the reference's
- * fixed KParams choice (src/kernel.h:44-67). */
-int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads);
+ * full), threads = workgroup size (256, 512 or 1024), refill = idle lanes
+ * of a wave that trigger a refill round (1..64).  Not to be changed while a
+ * search on this library is running.  Replaces the reference's KParams
+ * choice (src/kernel.h:44-67). */
+int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads,
+                          uint32_t refill);
 int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows,
-                          uint32_t *threads);
+                          uint32_t *threads, uint32_t *refill);
 /* Measured autotuner, replacing the analytic cost model of
  * KSizing::optimize / KParams::fom (src/kernel.cu:95-236): solves the given
  * boards (count mode, as d4_batch_*) once per candidate launch shape, keeps
@@ -126,7 +128,8 @@ int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows,
  * Outputs the chosen shape and its extend time in ms (any may be NULL). */
 int d4_library_tune(d4_library *lib, uint32_t n_inst, const uint8_t *boards,
                     uint32_t stride, const uint32_t *heights,
-                    uint32_t *fit_rows, uint32_t *threads, double *best_ms);
+                    uint32_t *fit_rows, uint32_t *threads, uint32_t *refill,
+                    double *best_ms);
 
 /* ---------------------------------------------------------------------- */
 /* single-instance searcher (CudaSearcher)                                 */
@@ -189,6 +192,43 @@ int d4_batch_stats(const d4_batch *b, d4_stats *st);
 /* Solutions of instance i (D4_MODE_ENUM), format as d4_searcher_solutions. */
 int d4_batch_solutions(const d4_batch *b, uint32_t i, int single,
                        uint8_t *out, size_t cap, size_t *len, uint64_t *nsol);
+
+/* ---------------------------------------------------------------------- */
+/* serving: continuous batching of requests (no reference counterpart)     */
+/* ---------------------------------------------------------------------- */
+
+typedef struct d4_server d4_server;
+
+/* A request loop over one library.  `slots` requests of each kind can be in
+ * flight at once ("solve all" requests run in a count-mode frontier, "solve
+ * one" requests in an enum-mode frontier); boards up to max_height rows.
+ * Requests are admitted between steps while a slot is free and the device
+ * memory the server holds plus the observed per-request footprint stays
+ * within mem_budget bytes (0 = half of the free device memory at creation).
+ * Requests admitted while others are in flight join the same frontier and
+ * overlap with them on the device. */
+int d4_server_create(const d4_library *lib, uint32_t slots, uint32_t max_height,
+                     uint64_t mem_budget, d4_server **out);
+void d4_server_destroy(d4_server *s);
+/* Queue a request: single != 0 asks for the first solution in the CPU
+ * searcher's DFS order (solve(..., true), src/Piece.cpp:66-102), else for
+ * all tilings (their count, solve_count, src/Piece.cpp:105-139). */
+int d4_server_submit(d4_server *s, const uint8_t *rows, uint32_t height,
+                     int single, uint64_t *req_id);
+/* Admit waiting requests and advance every in-flight request by one row. */
+int d4_server_step(d4_server *s, uint32_t *in_flight, uint32_t *waiting);
+/* Take one finished request: returns 1 and fills req_id, count (number of
+ * tilings) and, for a single request, its solution in the d4_searcher_
+ * solutions format (*len bytes; empty when there is no tiling); returns 0
+ * when nothing has finished.  A too-small buffer gives D4_ERANGE and keeps
+ * the result queued. */
+int d4_server_poll(d4_server *s, uint64_t *req_id, uint64_t *count,
+                   uint8_t *sol, size_t cap, size_t *len);
+/* Engine statistics of the two frontiers, requests admitted so far, steps
+ * at which admission waited for memory, device bytes held. */
+int d4_server_stats(const d4_server *s, d4_stats *all, d4_stats *one,
+                    uint64_t *admitted, uint64_t *mem_waits,
+                    uint64_t *device_bytes);
 
 #ifdef __cplusplus
 }

@@ -168,20 +168,20 @@ def test_benchmark_workload_all_boards(gpu, lib8):
 
 
 def test_launch_shapes_agree(gpu, p5):
-    """Every extend launch shape (fit-table rows x workgroup size) gives the
-    same counts; the autotuner picks one of them."""
+    """Every extend launch shape (fit-table rows x workgroup size x refill
+    threshold) gives the same counts; the autotuner picks one of them."""
     lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
     boards = workloads.seeded_8x8(8, seed=13)
     want = None
-    for f in (1, 2, 3, 4, 5):
-        for t in (256, 512, 1024):
-            lib.set_launch(f, t)
-            assert lib.get_launch() == (f, t)
-            b = gpu.Batch(lib, boards)
-            b.run()
-            if want is None:
-                want = b.counts()
-            assert b.counts() == want, (f, t)
+    for f, t, r in [(f, t, 16) for f in (1, 2, 3, 4, 5) for t in (256, 512, 1024)] + \
+            [(3, 1024, r) for r in (1, 7, 32, 64)]:
+        lib.set_launch(f, t, r)
+        assert lib.get_launch() == (f, t, r)
+        b = gpu.Batch(lib, boards)
+        b.run()
+        if want is None:
+            want = b.counts()
+        assert b.counts() == want, (f, t, r)
     choice = lib.tune(boards)
-    assert lib.get_launch() == (choice["fit_rows"], choice["threads"])
+    assert lib.get_launch() == (choice["fit_rows"], choice["threads"], choice["refill"])
     assert choice["extend_ms"] > 0
