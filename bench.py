@@ -76,6 +76,73 @@ def cpu_baseline(boards, gpu_counts, shapes, seconds, threads):
     }, ok, done
 
 
+def serve_run(d4, lib, reqs, slots):
+    """Open-loop replay of a request stream (arrival times in seconds from
+    the start) through one Server; returns latencies (ms), wall time and the
+    results."""
+    srv = d4.Server(lib, slots=slots, max_height=8)
+    n = len(reqs)
+    t_arr, sub, res, lat, last_log = {}, 0, {}, [], -1
+    t0 = time.perf_counter()
+    while len(res) < n:
+        now = time.perf_counter() - t0
+        while sub < n and reqs[sub][0] <= now:
+            rid = srv.submit(reqs[sub][1], reqs[sub][2])
+            t_arr[rid] = reqs[sub][0]
+            sub += 1
+        fl, wt = srv.step() if (sub > len(res)) else (0, 0)
+        while True:
+            r = srv.poll()
+            if r is None:
+                break
+            res[r[0]] = (r[1], r[2])
+            lat.append((time.perf_counter() - t0 - t_arr[r[0]]) * 1e3)
+        if len(res) and len(res) % 200 == 0 and len(res) != last_log:
+            last_log = len(res)
+            log(f"  served {len(res)}/{n}")
+        if sub == len(res) and sub < n:
+            # idle until the next arrival
+            time.sleep(max(0.0, reqs[sub][0] - (time.perf_counter() - t0)))
+    return lat, time.perf_counter() - t0, res, srv.stats()
+
+
+def serve_section(d4, lib, workloads, rates, n_req, slots):
+    """Served requests (Poisson arrivals, 70% first-solution) against the
+    same requests solved one per CudaSearcher, one after another."""
+    out = {"rates": []}
+    base_reqs = workloads.request_stream(n_req, rate=1.0, seed=4)
+    # one-request-per-launch baseline on a sample
+    ns = min(64, n_req)
+    log(f"serving: one-request-per-launch baseline on {ns} requests")
+    t = time.perf_counter()
+    for _, rows, single in base_reqs[:ns]:
+        s = d4.CudaSearcher(lib, rows=rows, mode=d4.MODE_ENUM if single else d4.MODE_COUNT)
+        s.run()
+        if single:
+            s.solutions(single=True)
+        else:
+            s.count()
+    plain_rps = ns / (time.perf_counter() - t)
+    out["plain_requests_per_s"] = plain_rps
+    for rate in rates:
+        reqs = workloads.request_stream(n_req, rate=rate, seed=4)
+        lat, wall, res, st = serve_run(d4, lib, reqs, slots)
+        tilings = sum(c for c, _ in res.values())
+        found = sum(1 for c, sol in res.values() if c)
+        row = {"rate": rate, "requests": n_req, "wall_s": wall,
+               "requests_per_s": n_req / wall,
+               "solutions_per_s": tilings / wall,
+               "latency_ms_p50": percentile(lat, 0.5),
+               "latency_ms_p99": percentile(lat, 0.99),
+               "solvable": found,
+               "served_vs_plain": (n_req / wall) / plain_rps,
+               "mem_waits": st["mem_waits"]}
+        log(f"serve rate {rate}/s: {row['requests_per_s']:.0f} req/s, p50 "
+            f"{row['latency_ms_p50']:.1f} ms, p99 {row['latency_ms_p99']:.1f} ms")
+        out["rates"].append(row)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +155,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--tune-sample", type=int, default=64)
+    ap.add_argument("--tune-sample", type=int, default=256)
+    ap.add_argument("--serve-rates", default="1000,20000",
+                    help="comma-separated arrival rates (requests/s); empty to skip")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-slots", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the "
@@ -225,6 +296,15 @@ def main():
         out["batched_vs_plain"] = (rec / dt) / (prec / pdt) if prec else None
         out["latency_ms"]["plain_p50"] = percentile(lat, 0.5)
         out["latency_ms"]["plain_p99"] = percentile(lat, 0.99)
+
+    if args.serve_rates:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        out["served"] = serve_section(d4, lib, workloads, rates, args.serve_requests,
+                                      args.serve_slots)
+        best = max(out["served"]["rates"], key=lambda r: r["requests_per_s"])
+        out["latency_ms"]["served_p50"] = best["latency_ms_p50"]
+        out["latency_ms"]["served_p99"] = best["latency_ms_p99"]
+        out["served_vs_plain"] = best["served_vs_plain"]
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -363,11 +363,19 @@ static void upload(T *&dst, const T *src, size_t n) {
 
 namespace {
 
-struct RowLog {  // enum mode: what reconstruction needs from one global step
+struct RowLog {  // enum mode: one global step, copied to the host
     std::vector<uint64_t> keys;       // frontier before the step
     std::vector<uint32_t> e_parent;   // edges to the next step's frontier
     std::vector<uint64_t> e_desc;
     std::vector<uint32_t> e_child;
+};
+
+struct DevLog {  // enum mode: one global step, kept on the device
+    DevBuf<uint64_t> keys;  // frontier before the step
+    uint64_t n_keys = 0;
+    DevBuf<uint32_t> par, chi;  // edges to the next step's frontier
+    DevBuf<uint64_t> desc;
+    uint64_t n_edges = 0;
 };
 
 // The frontier engine: S instance slots sharing one library and one device
@@ -408,7 +416,7 @@ struct Engine {
     double grow_unique = 4.0, grow_children = 8.0;
     static constexpr uint64_t kExactCountMax = 1u << 16;
     d4_stats stats{};
-    std::deque<RowLog> logs;  // logs[j] = global step log0 + j
+    std::deque<DevLog> logs;  // logs[j] = global step log0 + j
     uint32_t log0 = 0;
 
     Engine(const d4_library *l, uint32_t slots, uint32_t max_height, uint32_t m, uint32_t f,
@@ -481,10 +489,13 @@ struct Engine {
 
     // Device bytes held by this engine's buffers (high-water marks).
     uint64_t device_bytes() const {
-        return 8 * (bw.cap + fk.cap + fc.cap + nk.cap + nc.cap + tk.cap + result.cap +
-                    result_save.cap + c64.cap + e_desc.cap + e_slot.cap) +
-               4 * (hgt.cap + dstart.cap + c32.cap + e_parent.cap + e_child.cap + slot_uid.cap +
-                    blk.cap);
+        uint64_t b = 8 * (bw.cap + fk.cap + fc.cap + nk.cap + nc.cap + tk.cap + result.cap +
+                          result_save.cap + c64.cap + e_desc.cap + e_slot.cap) +
+                     4 * (hgt.cap + dstart.cap + c32.cap + e_parent.cap + e_child.cap +
+                          slot_uid.cap + blk.cap);
+        for (const DevLog &lg : logs)
+            b += 8 * (lg.keys.cap + lg.desc.cap) + 4 * (lg.par.cap + lg.chi.cap);
+        return b;
     }
 
     // rows still to be filled by a slot's instance (CudaSearcher::get_height)
@@ -632,14 +643,7 @@ struct Engine {
         }
         uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
         uint64_t est_e = en ? est_c + 64 : 0;
-        if (en) {
-            if (logs.empty()) log0 = g;
-            RowLog lg;
-            lg.keys.resize(n_front);
-            if (n_front)
-                HIPCHK(hipMemcpy(lg.keys.data(), fk.p, n_front * 8, hipMemcpyDeviceToHost));
-            logs.push_back(std::move(lg));
-        }
+        if (en && logs.empty()) log0 = g;
         result_save.reserve(S);
         uint32_t h32[C32_N];
         unsigned long long h64[C64_N];
@@ -716,15 +720,16 @@ struct Engine {
         stats.compact_launches++;
         stats.compact_bytes += 8ull * tab_m + 24ull * U;
         if (en) {
-            RowLog &lg = logs.back();
-            lg.e_parent.resize(n_edges);
-            lg.e_desc.resize(n_edges);
-            lg.e_child.resize(n_edges);
-            if (n_edges) {
-                HIPCHK(hipMemcpy(lg.e_parent.data(), e_parent.p, n_edges * 4, hipMemcpyDeviceToHost));
-                HIPCHK(hipMemcpy(lg.e_desc.data(), e_desc.p, n_edges * 8, hipMemcpyDeviceToHost));
-                HIPCHK(hipMemcpy(lg.e_child.data(), e_child.p, n_edges * 4, hipMemcpyDeviceToHost));
-            }
+            // the step's frontier and edges stay on the device as its log;
+            // the next step gets fresh buffers from the allocator cache
+            logs.emplace_back();
+            DevLog &lg = logs.back();
+            lg.keys.swap(fk);
+            lg.n_keys = n_front;
+            lg.par.swap(e_parent);
+            lg.chi.swap(e_child);
+            lg.desc.swap(e_desc);
+            lg.n_edges = n_edges;
         }
         fk.swap(nk);
         fc.swap(nc);
@@ -787,7 +792,9 @@ struct Engine {
         const uint32_t H = heights[slot], s0 = start[slot];
         if (s0 < log0 || s0 + H >= log0 + logs.size())
             throw Error(D4_ESTATE, "solutions of this instance are no longer held");
-        auto L = [&](uint32_t r) -> const RowLog & { return logs[s0 + r - log0]; };
+        std::vector<RowLog> hl(H + 1);
+        for (uint32_t r = 0; r <= H; r++) host_log(s0 + r - log0, hl[r], r < H);
+        auto L = [&](uint32_t r) -> const RowLog & { return hl[r]; };
         auto inst_of = [&](uint64_t key) -> uint32_t {
             return inst_shift < 64 ? (uint32_t)(key >> inst_shift) : 0u;
         };
@@ -839,6 +846,119 @@ struct Engine {
         return out;
     }
 
+    void host_log(size_t j, RowLog &out, bool edges) const {
+        const DevLog &lg = logs[j];
+        out.keys.resize(lg.n_keys);
+        if (lg.n_keys)
+            HIPCHK(hipMemcpy(out.keys.data(), lg.keys.p, lg.n_keys * 8, hipMemcpyDeviceToHost));
+        if (!edges) return;
+        out.e_parent.resize(lg.n_edges);
+        out.e_child.resize(lg.n_edges);
+        out.e_desc.resize(lg.n_edges);
+        if (lg.n_edges) {
+            HIPCHK(hipMemcpy(out.e_parent.data(), lg.par.p, lg.n_edges * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(out.e_child.data(), lg.chi.p, lg.n_edges * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(out.e_desc.data(), lg.desc.p, lg.n_edges * 8, hipMemcpyDeviceToHost));
+        }
+    }
+
+    // First solutions in the CPU searcher's order (solve(..., true)) of
+    // finished instances, without listing the others, on the device.  Per
+    // group of instances sharing a start step: states that reach one of the
+    // group's terminal states are marked backwards; in the same pass every
+    // marked state keeps its smallest completion (candidate ids compared in
+    // placement order) to a marked child; then one thread per instance walks
+    // from its root along those.  Two completions of one state never have
+    // one as a prefix of the other, and smaller completions order the whole
+    // sequences, so the walk gives the lexicographic minimum.  An instance
+    // with no tiling gets an empty list.  Cost: one pass over the group's
+    // edges, whatever the number of instances or solutions.
+    std::map<uint32_t, std::vector<std::vector<uint8_t>>> first_solutions(
+        const std::vector<uint32_t> &slots) {
+        if (mode != D4_MODE_ENUM) throw Error(D4_ESTATE, "solutions need D4_MODE_ENUM");
+        std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> groups;
+        for (uint32_t slot : slots) {
+            if (slot >= S) throw Error(D4_EINVAL, "instance out of range");
+            if (active[slot]) throw Error(D4_ESTATE, "search not finished");
+            if (start[slot] < log0 || start[slot] + heights[slot] >= log0 + logs.size())
+                throw Error(D4_ESTATE, "solutions of this instance are no longer held");
+            groups[{start[slot], heights[slot]}].push_back(slot);
+        }
+        std::map<uint32_t, std::vector<std::vector<uint8_t>>> out;
+        for (auto &gr : groups) {
+            const uint32_t s0 = gr.first.first, H = gr.first.second;
+            const std::vector<uint32_t> &mem = gr.second;
+            std::vector<uint8_t> member(S, 0);
+            for (uint32_t slot : mem) member[slot] = 1;
+            DevBuf<uint8_t> d_member;
+            DevBuf<uint32_t> d_root, d_slots;
+            d_member.reserve(S);
+            d_root.reserve(S);
+            d_slots.reserve(mem.size());
+            HIPCHK(hipMemcpyAsync(d_member.p, member.data(), S, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemsetAsync(d_root.p, 0xff, (size_t)S * 4, st));
+            HIPCHK(hipMemcpyAsync(d_slots.p, mem.data(), mem.size() * 4, hipMemcpyHostToDevice, st));
+            std::vector<DevBuf<uint8_t>> live(H + 1);
+            std::vector<DevBuf<unsigned long long>> best(H);
+            std::vector<DevBuf<uint32_t>> next(H);
+            auto LG = [&](uint32_t r) -> const DevLog & { return logs[s0 + r - log0]; };
+            for (uint32_t r = 0; r <= H; r++) {
+                live[r].reserve(std::max<uint64_t>(LG(r).n_keys, 1));
+                HIPCHK(hipMemsetAsync(live[r].p, 0, std::max<uint64_t>(LG(r).n_keys, 1), st));
+            }
+            launch_first_solutions_ends(LG(H).keys.p, LG(H).n_keys, LG(0).keys.p, LG(0).n_keys,
+                                        inst_shift, d_member.p, live[H].p, d_root.p, st);
+            for (uint32_t r = H; r-- > 0;) {
+                const DevLog &lg = LG(r);
+                const uint64_t nk = std::max<uint64_t>(lg.n_keys, 1);
+                best[r].reserve(nk);
+                next[r].reserve(nk);
+                HIPCHK(hipMemsetAsync(best[r].p, 0xff, nk * 8, st));
+                HIPCHK(hipMemsetAsync(next[r].p, 0xff, nk * 4, st));
+                if (lg.n_edges)
+                    launch_first_solutions_row(lg.par.p, lg.chi.p, lg.desc.p, lg.n_edges,
+                                               live[r + 1].p, live[r].p, best[r].p, next[r].p, st);
+            }
+            std::vector<const unsigned long long *> hb(std::max(H, 1u), nullptr);
+            std::vector<const uint32_t *> hn(std::max(H, 1u), nullptr);
+            for (uint32_t r = 0; r < H; r++) {
+                hb[r] = best[r].p;
+                hn[r] = next[r].p;
+            }
+            DevBuf<uint64_t> d_ptr, d_out;
+            d_ptr.reserve(2 * hb.size());
+            d_out.reserve(mem.size() * (H + 1));
+            HIPCHK(hipMemcpyAsync(d_ptr.p, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d_ptr.p + hb.size(), hn.data(), hn.size() * 8,
+                                  hipMemcpyHostToDevice, st));
+            launch_first_solutions_walk(d_slots.p, (uint32_t)mem.size(), d_root.p, live[0].p,
+                                        reinterpret_cast<const unsigned long long *const *>(d_ptr.p),
+                                        reinterpret_cast<const uint32_t *const *>(d_ptr.p + hb.size()),
+                                        H, d_out.p, st);
+            HIPCHK(hipGetLastError());
+            std::vector<uint64_t> ho(mem.size() * (H + 1));
+            HIPCHK(hipMemcpyAsync(ho.data(), d_out.p, ho.size() * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (size_t j = 0; j < mem.size(); j++) {
+                const uint64_t *o = &ho[j * (H + 1)];
+                if (!o[H]) {
+                    out[mem[j]] = {};
+                    continue;
+                }
+                std::vector<uint8_t> seq;
+                for (uint32_t r = 0; r < H; r++)
+                    for (int b = 0; b < 8 && ((o[r] >> (8 * b)) & 0xffu); b++)
+                        seq.push_back((uint8_t)(((o[r] >> (8 * b)) & 0xffu) - 1));
+                out[mem[j]] = {seq};
+            }
+        }
+        return out;
+    }
+
+    std::vector<std::vector<uint8_t>> first_solution_of(uint32_t slot) {
+        return first_solutions({slot})[slot];
+    }
+
     // Steps {piece_id, trs_id, y, x} of a candidate sequence, replayed on the
     // board (anchor = lowest open cell, Piece.inl:5-24).
     void encode(uint32_t slot, const std::vector<uint8_t> &seq, std::vector<uint8_t> &buf) const {
@@ -863,8 +983,8 @@ struct Engine {
         }
     }
 
-    std::vector<uint8_t> solution_bytes(uint32_t slot, bool single, uint64_t *nsol) const {
-        auto seqs = solutions_of(slot, single);
+    std::vector<uint8_t> solution_bytes(uint32_t slot, bool single, uint64_t *nsol) {
+        auto seqs = single ? first_solution_of(slot) : solutions_of(slot, false);
         std::vector<uint8_t> buf;
         for (auto &sq : seqs) encode(slot, sq, buf);
         if (nsol) *nsol = seqs.size();
@@ -872,7 +992,7 @@ struct Engine {
     }
 
     void solutions(uint32_t slot, int single, uint8_t *out, size_t cap, size_t *len,
-                   uint64_t *nsol) const {
+                   uint64_t *nsol) {
         auto buf = solution_bytes(slot, single != 0, nsol);
         if (len) *len = buf.size();
         if (out && buf.size() <= cap) std::memcpy(out, buf.data(), buf.size());
@@ -956,9 +1076,12 @@ struct d4_server {
             Engine &e = *eng[k];
             if (!e.n_active) continue;
             e.step();
+            std::map<uint32_t, std::vector<std::vector<uint8_t>>> firsts;
+            if (k == 1 && !e.completed.empty()) firsts = e.first_solutions(e.completed);
             for (uint32_t slot : e.completed) {
                 Done d{slot_req[k][slot], e.host_result[slot], {}};
-                if (k == 1) d.sol = e.solution_bytes(slot, true, nullptr);
+                if (k == 1)
+                    for (auto &sq : firsts[slot]) e.encode(slot, sq, d.sol);
                 done.push_back(std::move(d));
                 free_slots[k].push_back(slot);
             }
@@ -1043,7 +1166,7 @@ int d4_library_create(const uint64_t *shapes, const uint32_t *counts, uint32_t n
         HIPCHK(hipGetDeviceProperties(&prop, dev));
         L->n_cu = prop.multiProcessorCount;
         L->lds_limit = (uint32_t)prop.sharedMemPerBlock;
-        if (extend_lds_bytes(L->fit_rows, 1024, true) > L->lds_limit)
+        if (extend_lds_bytes(L->fit_rows, 256, true) > L->lds_limit)
             throw Error(D4_EINVAL, "placement tables exceed LDS");
         upload(L->d_fit, L->fit.data(), L->fit.size());
         upload(L->d_placeinfo, L->placeinfo, 2 * 8 * MAX_CAND);
@@ -1071,7 +1194,7 @@ int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads,
         if (threads != 256 && threads != 512 && threads != 1024)
             throw Error(D4_EINVAL, "threads must be 256, 512 or 1024");
         if (fit_rows < 1 || fit_rows > lib->maxh) throw Error(D4_EINVAL, "fit_rows out of range");
-        if (extend_lds_bytes(fit_rows, threads, true) > lib->lds_limit)
+        if (extend_lds_bytes(fit_rows, threads, false) > lib->lds_limit)
             throw Error(D4_EINVAL, "launch shape exceeds LDS");
         if (fit_rows != lib->fit_rows) {
             HIPCHK(hipSetDevice(lib->device));
@@ -1113,7 +1236,7 @@ int d4_library_tune(d4_library *lib, uint32_t n_inst, const uint8_t *boards, uin
         // refill threshold at the best of those
         for (uint32_t f = 1; f <= std::min(lib->maxh, 4u); f++)
             for (uint32_t t : {256u, 512u, 1024u}) {
-                if (extend_lds_bytes(f, t, true) > lib->lds_limit) continue;
+                if (extend_lds_bytes(f, t, false) > lib->lds_limit) continue;
                 if (d4_library_set_launch(lib, f, t, best_r) != D4_OK) continue;
                 double ms = measure();
                 if (best < 0 || ms < best) {

@@ -99,7 +99,19 @@ void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
                       uint64_t occ_mask, const unsigned long long *comp,
                       uint32_t *c32, unsigned long long *c64, int n_cu,
                       hipStream_t st);
-size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool staged);
+// first solutions of finished instances (enum mode), see d4_kernels.hip
+void launch_first_solutions_row(const uint32_t *par, const uint32_t *chi, const uint64_t *desc,
+                                uint64_t n, const uint8_t *live_child, uint8_t *live_par,
+                                unsigned long long *best, uint32_t *next, hipStream_t st);
+void launch_first_solutions_ends(const uint64_t *keys_term, uint64_t n_term,
+                                 const uint64_t *keys_root, uint64_t n_root, uint32_t inst_shift,
+                                 const uint8_t *member, uint8_t *live_term, uint32_t *root,
+                                 hipStream_t st);
+void launch_first_solutions_walk(const uint32_t *slots, uint32_t n_slots, const uint32_t *root,
+                                 const uint8_t *live0, const unsigned long long *const *best,
+                                 const uint32_t *const *next, uint32_t H, uint64_t *out,
+                                 hipStream_t st);
+size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool enum_mode);
 
 // Error plumbing (d4_api.cpp).
 void set_error(const std::string &msg);

@@ -181,7 +181,7 @@ __device__ __forceinline__ uint64_t piece_range(uint64_t info) {
 
 template <bool ENUM, bool PRUNE, bool COUNT_ONLY, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
-    constexpr bool STAGED = !ENUM;
+    constexpr uint32_t SW = ENUM ? 5 : 3;  // staged u64 words per child
     extern __shared__ uint64_t lds[];
     const uint32_t nfit = 8u * a.fit_rows * 256u;
     uint64_t *s_fit = lds;
@@ -220,11 +220,15 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     uint32_t rec = 0, inst = 0, used = 0, d = 0, c = 0, anc = 0;
     uint64_t cnt = 0, BW = 0, BWn = 0, E = 0, avail = 0, desc = 0, ok = 0;
 
-    // Count-mode children are staged per wave in LDS (key, count, open
-    // window) and flushed 64 at a time: each lane prunes and inserts one.
-    uint64_t *st_key = s_stage + (threadIdx.x >> 6) * (3 * STAGE);  // per-wave stage
+    // Children are staged per wave in LDS (key, count, open window; in enum
+    // mode also the row descriptor and the parent) and flushed 64 at a time:
+    // each lane prunes and inserts one, and enum mode writes the kept
+    // children's edges with one slot allocation per flush.
+    uint64_t *st_key = s_stage + (threadIdx.x >> 6) * (SW * STAGE);  // per-wave stage
     uint64_t *st_cnt = st_key + STAGE;
     uint64_t *st_E = st_cnt + STAGE;
+    uint64_t *st_desc = st_E + STAGE;     // enum mode: row descriptor
+    uint64_t *st_rec = st_desc + STAGE;   // enum mode: parent record
     uint32_t sc = 0;  // staged entries (wave-uniform)
     // Returns this lane's tallies: bit 0 child kept, bit 1 pruned, bit 2 new.
     auto flush = [&](uint32_t n) __attribute__((always_inline)) -> uint32_t {
@@ -234,6 +238,8 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
         const uint64_t k2 = has ? st_key[sl] : 0ull;
         const uint64_t c2 = has ? st_cnt[sl] : 0ull;
         const uint64_t e2 = has ? st_E[sl] : 0ull;
+        const uint64_t d2 = ENUM && has ? st_desc[sl] : 0ull;
+        const uint32_t r2 = ENUM && has ? (uint32_t)st_rec[sl] : 0u;
         __builtin_amdgcn_wave_barrier();
         sc -= n;
         bool keep = has;
@@ -243,13 +249,26 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
             t |= keep ? 0u : 2u;
         }
         bool failed = false;
+        uint64_t slot = 0;
         if (keep) {
             t |= 1u;
             if (!COUNT_ONLY) {
-                uint64_t slot = 0;
                 int r = table_insert(a, k2, c2, slot);
                 if (r < 0) failed = true;
                 else t |= r ? 4u : 0u;
+            }
+        }
+        if (ENUM && !COUNT_ONLY) {
+            const bool edge = keep && !failed;
+            const uint32_t e = wave_alloc(a.c32 + C32_EDGES, edge);
+            if (edge) {
+                if (e < a.e_cap) {
+                    a.e_parent[e] = r2;
+                    a.e_desc[e] = d2;
+                    a.e_slot[e] = slot;
+                } else {
+                    atomicOr(a.c32 + C32_EDGE_OVF, 1u);
+                }
             }
         }
         if (__any(failed)) {
@@ -431,68 +450,29 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 ok = d >= fix ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1) : 0;
             }
         }
-        if (STAGED) {
-            // queue the child in the wave's LDS stage: pruning and the table
-            // insert run in flush(), 64 children at a time with every lane
-            // active, instead of stalling this divergent loop on each CAS
-            const uint64_t em = __ballot(emit);
-            if (em) {
-                if (emit) {
-                    const uint32_t sl = sc + mbcnt(em);
-                    uint64_t key2 = ((BW & ~E2) >> 8) | ((uint64_t)used2 << a.occ_bits);
-                    if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
-                    st_key[sl] = key2;
-                    st_cnt[sl] = cnt;
-                    st_E[sl] = (E2 >> 8) | (BWn & 0xff00000000000000ull);
-                }
-                sc += (uint32_t)__builtin_popcountll(em);
-                if (sc >= 64) tally(flush(64));
-            }
-            continue;
-        }
-        bool keep = emit;
-        if (emit && PRUNE) {
-            uint64_t Ec = (E2 >> 8) | (BWn & 0xff00000000000000ull);
-            keep = viable(Ec, a.seed_mask, a.gcd, a.min_size);
-            n_pruned += keep ? 0 : 1;
-        }
-        uint64_t slot = 0;
-        if (COUNT_ONLY) {
-            n_children += keep ? 1 : 0;
-            continue;
-        }
-        if (keep) {
-            n_children++;
-            uint64_t occ2 = (BW & ~E2) >> 8;
-            uint64_t key2 = occ2 | ((uint64_t)used2 << a.occ_bits);
-            if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
-            int r = table_insert(a, key2, cnt, slot);
-            if (r < 0) {
-                // table too small: flag it, stop this search; the host
-                // reruns the step with a larger table
-                atomicOr(a.c32 + C32_OVERFLOW, 1u);
-                keep = false;
-                searching = false;
-                pass = false;
-            } else {
-                n_new += (uint64_t)r;
-            }
-        }
-        if (ENUM) {
-            uint32_t e = wave_alloc(a.c32 + C32_EDGES, keep);
-            if (keep) {
-                if (e < a.e_cap) {
-                    a.e_parent[e] = rec;
-                    a.e_desc[e] = desc2;
-                    a.e_slot[e] = slot;
-                } else {
-                    atomicOr(a.c32 + C32_EDGE_OVF, 1u);
+        // queue the child in the wave's LDS stage: pruning, the table insert
+        // and the edge write run in flush(), 64 children at a time with
+        // every lane active, instead of stalling this divergent loop on each
+        // CAS
+        const uint64_t em = __ballot(emit);
+        if (em) {
+            if (emit) {
+                const uint32_t sl = sc + mbcnt(em);
+                uint64_t key2 = ((BW & ~E2) >> 8) | ((uint64_t)used2 << a.occ_bits);
+                if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
+                st_key[sl] = key2;
+                st_cnt[sl] = cnt;
+                st_E[sl] = (E2 >> 8) | (BWn & 0xff00000000000000ull);
+                if (ENUM) {
+                    st_desc[sl] = desc2;
+                    st_rec[sl] = rec;
                 }
             }
+            sc += (uint32_t)__builtin_popcountll(em);
+            if (sc >= 64) tally(flush(64));
         }
     }
-    if (STAGED)
-        while (sc) tally(flush(min(sc, 64u)));
+    while (sc) tally(flush(min(sc, 64u)));
 
     n_children = wave_sum_u64(n_children);
     n_new = wave_sum_u64(n_new);
@@ -628,11 +608,113 @@ __global__ __launch_bounds__(256) void k_next_size(
     if (lane_id() == 0 && acc) atomicAdd(c64 + C64_NEXTSIZE, (unsigned long long)acc);
 }
 
+// ---- first solutions on the device (enum mode) ---------------------------
+// live[u] = the state's instance is in the group (terminal states).
+__global__ __launch_bounds__(256) void k_fs_terminal(const uint64_t *keys, uint64_t n,
+                                                     uint32_t inst_shift, const uint8_t *member,
+                                                     uint8_t *live) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = inst_shift < 64 ? (uint32_t)(keys[u] >> inst_shift) : 0u;
+        live[u] = member[i];
+    }
+}
+
+// Backward pass over one row's edges: a parent with an edge to a live child
+// is live, and best[parent] keeps the smallest descriptor of such edges in
+// byte order (candidate ids in placement order).
+__global__ __launch_bounds__(256) void k_fs_back(const uint32_t *par, const uint32_t *chi,
+                                                 const uint64_t *desc, uint64_t n,
+                                                 const uint8_t *live_child, uint8_t *live_par,
+                                                 unsigned long long *best) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n;
+         e += (uint64_t)gridDim.x * blockDim.x) {
+        if (!live_child[chi[e]]) continue;
+        live_par[par[e]] = 1;
+        atomicMin(best + par[e], (unsigned long long)__builtin_bswap64(desc[e]));
+    }
+}
+
+// The child reached by each parent's best edge (equal descriptors from one
+// parent are one completion, so the child is unique).
+__global__ __launch_bounds__(256) void k_fs_pick(const uint32_t *par, const uint32_t *chi,
+                                                 const uint64_t *desc, uint64_t n,
+                                                 const uint8_t *live_child,
+                                                 const unsigned long long *best, uint32_t *next) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n;
+         e += (uint64_t)gridDim.x * blockDim.x)
+        if (live_child[chi[e]] && __builtin_bswap64(desc[e]) == best[par[e]])
+            next[par[e]] = chi[e];
+}
+
+// root[i] = index of instance i's state in the frontier of its start step.
+__global__ __launch_bounds__(256) void k_fs_roots(const uint64_t *keys, uint64_t n,
+                                                  uint32_t inst_shift, const uint8_t *member,
+                                                  uint32_t *root) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = inst_shift < 64 ? (uint32_t)(keys[u] >> inst_shift) : 0u;
+        if (member[i]) root[i] = (uint32_t)u;
+    }
+}
+
+// One thread per instance: follow the best edges from the root; out[j][r]
+// is row r's descriptor, out[j][H] = 1 when the instance has a tiling.
+__global__ __launch_bounds__(64) void k_fs_walk(const uint32_t *slots, uint32_t n_slots,
+                                                const uint32_t *root, const uint8_t *live0,
+                                                const unsigned long long *const *best,
+                                                const uint32_t *const *next, uint32_t H,
+                                                uint64_t *out) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_slots) return;
+    uint64_t *o = out + (size_t)j * (H + 1);
+    uint32_t u = root[slots[j]];
+    if (u == 0xffffffffu || !live0[u]) {
+        o[H] = 0;
+        return;
+    }
+    for (uint32_t r = 0; r < H; r++) {
+        o[r] = __builtin_bswap64(best[r][u]);
+        u = next[r][u];
+    }
+    o[H] = 1;
+}
+
 }  // namespace
 
-size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool staged) {
+void launch_first_solutions_row(const uint32_t *par, const uint32_t *chi, const uint64_t *desc,
+                                uint64_t n, const uint8_t *live_child, uint8_t *live_par,
+                                unsigned long long *best, uint32_t *next, hipStream_t st) {
+    const uint32_t b = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), 65536);
+    hipLaunchKernelGGL(k_fs_back, dim3(b), dim3(256), 0, st, par, chi, desc, n, live_child,
+                       live_par, best);
+    hipLaunchKernelGGL(k_fs_pick, dim3(b), dim3(256), 0, st, par, chi, desc, n, live_child,
+                       (const unsigned long long *)best, next);
+}
+
+void launch_first_solutions_ends(const uint64_t *keys_term, uint64_t n_term,
+                                 const uint64_t *keys_root, uint64_t n_root, uint32_t inst_shift,
+                                 const uint8_t *member, uint8_t *live_term, uint32_t *root,
+                                 hipStream_t st) {
+    uint32_t b = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n_term + 255) / 256, 1), 65536);
+    hipLaunchKernelGGL(k_fs_terminal, dim3(b), dim3(256), 0, st, keys_term, n_term, inst_shift,
+                       member, live_term);
+    b = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n_root + 255) / 256, 1), 65536);
+    hipLaunchKernelGGL(k_fs_roots, dim3(b), dim3(256), 0, st, keys_root, n_root, inst_shift,
+                       member, root);
+}
+
+void launch_first_solutions_walk(const uint32_t *slots, uint32_t n_slots, const uint32_t *root,
+                                 const uint8_t *live0, const unsigned long long *const *best,
+                                 const uint32_t *const *next, uint32_t H, uint64_t *out,
+                                 hipStream_t st) {
+    hipLaunchKernelGGL(k_fs_walk, dim3((n_slots + 63) / 64), dim3(64), 0, st, slots, n_slots,
+                       root, live0, best, next, H, out);
+}
+
+size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool enum_mode) {
     return sizeof(uint64_t) * (8u * fit_rows * 256u + 2u * 8u * MAX_CAND + 4u * 256u) +
-           (staged ? sizeof(uint64_t) * 3u * STAGE * (threads / 64) : 0);
+           sizeof(uint64_t) * (enum_mode ? 5u : 3u) * STAGE * (threads / 64);
 }
 
 void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
@@ -665,9 +747,11 @@ static void launch_extend_t(const ExtendArgs &a, uint32_t threads, dim3 g, size_
 
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_only,
                    uint32_t threads, int n_cu, hipStream_t st) {
-    if (enum_mode && !count_only) threads = 1024;
     if (threads != 256 && threads != 512) threads = 1024;
-    const size_t lds = extend_lds_bytes(a.fit_rows, threads, !enum_mode || count_only);
+    const bool en = enum_mode && !count_only;
+    // enum mode stages more per child: fewer waves per workgroup if needed
+    while (threads > 256 && extend_lds_bytes(a.fit_rows, threads, en) > 160u * 1024u) threads /= 2;
+    const size_t lds = extend_lds_bytes(a.fit_rows, threads, en);
     // resident workgroups per CU: 160 KiB of LDS and 32 waves
     uint64_t per_cu = std::min<uint64_t>(160u * 1024u / lds, 32u / (threads / 64));
     if (!per_cu) per_cu = 1;
@@ -680,8 +764,8 @@ void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_o
         if (prune) launch_extend_t<false, true, true>(a, threads, g, lds, st);
         else launch_extend_t<false, false, true>(a, threads, g, lds, st);
     } else if (enum_mode) {
-        if (prune) launch_extend_t<true, true, false>(a, 1024, g, lds, st);
-        else launch_extend_t<true, false, false>(a, 1024, g, lds, st);
+        if (prune) launch_extend_t<true, true, false>(a, threads, g, lds, st);
+        else launch_extend_t<true, false, false>(a, threads, g, lds, st);
     } else {
         if (prune) launch_extend_t<false, true, false>(a, threads, g, lds, st);
         else launch_extend_t<false, false, false>(a, threads, g, lds, st);

@@ -11,8 +11,8 @@ Workloads (boards are lists of 8-bit row masks, bit c = column c open):
   seeded_8x8(n, seed) 8x8 with 4 blocked cells, every open region's area a
                       multiple of 5 (first n accepted draws)
   large_8x15()        12 free pentominoes x2 on 8 wide x 15 rows
-  request_stream(...) Poisson arrivals of 8x8 boards with 30-64 open cells
-                      (same region filter), 70% first-solution
+  request_stream(...) Poisson arrivals of 8x8 boards with 30-60 open cells in
+                      one region (area a multiple of 5), 70% first-solution
 """
 from __future__ import annotations
 
@@ -126,11 +126,46 @@ def centre_hole_8x8() -> List[int]:
     return [0xFF, 0xFF, 0xFF, 0xE7, 0xE7, 0xFF, 0xFF, 0xFF]
 
 
+def grown_region_8x8(rng: MT19937_64, free: int) -> List[int]:
+    """An 8x8 board whose open cells are one 4-connected region of `free`
+    cells, grown from a uniform seed cell by adding uniform frontier cells."""
+    start = rng.randint(0, 63)
+    cells = {start}
+    frontier = []
+
+    def nbrs(v):
+        r, c = divmod(v, 8)
+        if r > 0:
+            yield v - 8
+        if r < 7:
+            yield v + 8
+        if c > 0:
+            yield v - 1
+        if c < 7:
+            yield v + 1
+
+    frontier.extend(x for x in nbrs(start))
+    while len(cells) < free:
+        v = frontier[rng.randint(0, len(frontier) - 1)]
+        if v in cells:
+            frontier.remove(v)
+            continue
+        cells.add(v)
+        frontier.extend(x for x in nbrs(v) if x not in cells)
+    rows = [0] * 8
+    for v in cells:
+        rows[v // 8] |= 1 << (v % 8)
+    return rows
+
+
 def request_stream(n: int, rate: float, seed: int = 4):
     """n requests with Poisson(rate) arrivals: list of (t_arrival_s, rows,
-    single).  Boards are 8x8 with U{30..64} open cells at uniform positions,
-    redrawn until every open region's area is a multiple of 5; 70% of the
-    requests ask for the first solution only."""
+    single).  Each board is 8x8 with F ~ U{30..64} open cells rounded down to
+    a multiple of 5 (so every open region's area is a multiple of 5), the
+    open cells forming one connected region grown from a uniform seed cell.
+    (Uniform blocked cells with a region-area filter almost never pass when
+    20 or more cells are blocked.)  70% of the requests ask for the first
+    solution only."""
     import math
     rng = MT19937_64(seed)
     t = 0.0
@@ -138,11 +173,8 @@ def request_stream(n: int, rate: float, seed: int = 4):
     for _ in range(n):
         u = rng.uniform()
         t += -math.log(1.0 - u) / rate
-        while True:
-            free = rng.randint(30, 64)
-            rows = _blocked_board(rng, 64 - free)
-            if all(a % 5 == 0 for a in regions(rows)):
-                break
+        free = rng.randint(30, 64) // 5 * 5
+        rows = grown_region_8x8(rng, free)
         single = rng.uniform() < 0.7
         out.append((t, rows, single))
     return out

@@ -185,3 +185,26 @@ def test_launch_shapes_agree(gpu, p5):
     choice = lib.tune(boards)
     assert lib.get_launch() == (choice["fit_rows"], choice["threads"], choice["refill"])
     assert choice["extend_ms"] > 0
+
+
+def test_first_solution_walk(gpu, oracle):
+    """single=True takes the greedy walk over the back-pointers; it must equal
+    the minimum of the full list and the CPU's solve(single=true)."""
+    import random
+    rng = random.Random(21)
+    pool = [1, 3, 7, 259] + gpu.free_polyominoes(4) + gpu.free_polyominoes(5)
+    checked = 0
+    for _ in range(25):
+        shapes = sorted(rng.sample(pool, rng.randint(3, 6)))
+        counts = [rng.randint(1, 2) for _ in shapes]
+        w, h = rng.randint(3, 7), rng.randint(3, 7)
+        rows = rect(w, h, [(rng.randrange(h), rng.randrange(w)) for _ in range(2)])
+        lib = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+        s = gpu.CudaSearcher(lib, rows=rows, mode=gpu.MODE_ENUM)
+        s.run()
+        first = s.solutions(single=True)
+        full = s.solutions()
+        assert first == full[:1]
+        assert first == oracle.solve(shapes, counts, rows, single=True)
+        checked += bool(first)
+    assert checked >= 5
