@@ -412,7 +412,7 @@ struct Engine {
     DevBuf<unsigned long long> c64;
     DevBuf<uint32_t> e_parent, e_child, slot_uid, blk;
     DevBuf<uint64_t> e_desc, e_slot;
-    uint64_t n_front = 0;
+    uint64_t n_front = 0, new_roots = 0;
     double grow_unique = 4.0, grow_children = 8.0;
     static constexpr uint64_t kExactCountMax = 1u << 16;
     d4_stats stats{};
@@ -505,14 +505,14 @@ struct Engine {
         return queued ? heights[slot] : heights[slot] - std::min(heights[slot], g - start[slot]);
     }
 
-    ExtendArgs args(uint64_t tab_m, uint32_t e_cap) {
+    ExtendArgs args(uint64_t tab_m, uint32_t e_cap, uint64_t begin, uint64_t count,
+                    uint32_t split) {
         ExtendArgs a{};
-        a.in_key = fk.p;
-        a.in_cnt = fc.p;
-        a.n_in = (uint32_t)n_front;
-        // split levels: tiny frontiers and the heavy first rows spread each
-        // record over 64 / 4096 work items
-        a.split = n_front <= 1024 ? 2u : n_front <= (1u << 16) ? 1u : 0u;
+        a.in_key = fk.p + begin;
+        a.in_cnt = fc.p + begin;
+        a.n_in = (uint32_t)count;
+        a.rec_base = (uint32_t)begin;
+        a.split = split;
         a.refill = lib->ext_refill;
         a.row = g;
         a.fit = lib->d_fit;
@@ -585,6 +585,7 @@ struct Engine {
         HIPCHK(hipMemcpy(fk.p + n_front, k0.data(), k0.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(fc.p + n_front, c0.data(), c0.size() * 8, hipMemcpyHostToDevice));
         n_front = n;
+        new_roots = k0.size();
         stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, n_front);
         pending.clear();
     }
@@ -664,9 +665,7 @@ struct Engine {
             launch_fill_table(tk.p, tk.p + 1, tab_m, st);
             launch_reset_counters(c32.p, c64.p, st);
             HIPCHK(hipEventRecord(evx0, st));
-            if (n_front)
-                launch_extend(args(tab_m, (uint32_t)est_e), en, prune, false, lib->ext_threads,
-                              lib->n_cu, st);
+            launch_ranges(tab_m, (uint32_t)est_e, en, prune, false);
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(evx1, st));
             HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
@@ -740,6 +739,31 @@ struct Engine {
         }
         stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, U);
         stats.rows_done++;
+        new_roots = 0;
+    }
+
+    // Extend the whole frontier.  Split levels spread each record over 64 /
+    // 4096 work items for tiny frontiers; the roots admitted this step (a
+    // contiguous range at the end) always get their own launch with two
+    // split levels, because their first row has the largest subtrees and a
+    // mixed frontier would otherwise leave each of them to a single lane.
+    void launch_ranges(uint64_t tab_m, uint32_t e_cap, bool en, bool prune, bool count_only) {
+        if (!n_front) return;
+        auto split_of = [](uint64_t n) {
+            return n <= 1024 ? 2u : n <= (1u << 16) ? 1u : 0u;
+        };
+        const uint64_t roots = std::min<uint64_t>(new_roots, n_front);
+        const uint64_t old = n_front - roots;
+        if (old && roots) {
+            launch_extend(args(tab_m, e_cap, 0, old, split_of(old)), en, prune, count_only,
+                          lib->ext_threads, lib->n_cu, st);
+            launch_reset_work(c64.p, st);
+            launch_extend(args(tab_m, e_cap, old, roots, 2), en, prune, count_only,
+                          lib->ext_threads, lib->n_cu, st);
+        } else {
+            launch_extend(args(tab_m, e_cap, 0, n_front, split_of(n_front)), en, prune,
+                          count_only, lib->ext_threads, lib->n_cu, st);
+        }
     }
 
     // Children the current frontier will emit (after pruning): the extend
@@ -748,7 +772,7 @@ struct Engine {
         if (!n_front) return 0;
         const bool prune = !(flags & D4_FLAG_NO_PRUNE);
         launch_reset_counters(c32.p, c64.p, st);
-        launch_extend(args(1, 0), false, prune, true, lib->ext_threads, lib->n_cu, st);
+        launch_ranges(1, 0, false, prune, true);
         HIPCHK(hipGetLastError());
         unsigned long long h64[C64_N];
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));

@@ -37,9 +37,10 @@ enum : int {
 
 // Arguments of the extend kernel (passed by value).
 struct ExtendArgs {
-    const uint64_t *in_key;
+    const uint64_t *in_key;  // records rec_base .. rec_base + n_in - 1
     const uint64_t *in_cnt;
     uint32_t n_in;
+    uint32_t rec_base;       // frontier index of in_key[0] (edge parents)
     uint32_t row;            // global step
     uint32_t split;          // levels fixed per work item: 0, 1 or 2
     uint32_t refill;         // idle lanes that trigger a refill round (1..64)
@@ -84,6 +85,7 @@ void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
                        uint64_t n, hipStream_t st);
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
+void launch_reset_work(unsigned long long *c64, hipStream_t st);
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
                    bool count_only, uint32_t threads, int n_cu, hipStream_t st);
 constexpr uint32_t COMPACT_MAX_BLOCKS = 65536;

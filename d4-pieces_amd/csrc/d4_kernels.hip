@@ -263,7 +263,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
             const uint32_t e = wave_alloc(a.c32 + C32_EDGES, edge);
             if (edge) {
                 if (e < a.e_cap) {
-                    a.e_parent[e] = r2;
+                    a.e_parent[e] = a.rec_base + r2;
                     a.e_desc[e] = d2;
                     a.e_slot[e] = slot;
                 } else {
@@ -499,6 +499,8 @@ __global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
     if (threadIdx.x < C64_N) c64[threadIdx.x] = 0;
 }
 
+__global__ void k_reset_work(unsigned long long *c64) { c64[C64_WORK] = 0; }
+
 // Compaction of the dedup table into the next frontier, in slot order (so the
 // frontier is deterministic for a given table size): k_compact_count counts
 // the occupied slots of each block's contiguous range, k_scan_blocks turns
@@ -727,6 +729,10 @@ void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
 
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c32, c64);
+}
+
+void launch_reset_work(unsigned long long *c64, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, st, c64);
 }
 
 template <bool ENUM, bool PRUNE, bool COUNT_ONLY>
