@@ -1353,6 +1353,88 @@ int d4_searcher_solutions(const d4_searcher *s, int single, uint8_t *out, size_t
     });
 }
 
+int d4_searcher_export(const d4_searcher *s, void *rx, size_t cap, size_t *n,
+                       uint64_t *counts) {
+    return guard([&] {
+        if (!s || !n) throw Error(D4_EINVAL, "null argument");
+        Engine &e = *s->e;
+        const d4_library &L = *e.lib;
+        if (!e.active[0]) throw Error(D4_ESTATE, "search finished: no frontier left");
+        e.upload_pending();
+        const uint32_t H = e.height_of(0);
+        if (H > 8 || H == 0) throw Error(D4_ERANGE, "the record format holds 1..8 rows");
+        if (L.n > 16) throw Error(D4_ERANGE, "the record format holds at most 16 pieces");
+        for (uint32_t i = 0; i < L.n; i++)
+            if (L.counts[i] > 1) throw Error(D4_ERANGE, "the record format has no piece counts");
+        std::vector<uint64_t> keys(e.n_front), cnts(e.n_front);
+        if (e.n_front) {
+            HIPCHK(hipMemcpy(keys.data(), e.fk.p, e.n_front * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(cnts.data(), e.fc.p, e.n_front * 8, hipMemcpyDeviceToHost));
+        }
+        const uint32_t row = e.g - e.start[0];
+        uint64_t BW = 0;
+        for (uint32_t j = 0; j < 8; j++)
+            BW |= (uint64_t)e.boards[row + j] << (8 * j);
+        struct Rec {
+            uint32_t w[6];  // R (5 words) + ea in the low byte of w[5]
+            uint64_t cnt;
+        };
+        std::vector<Rec> out(e.n_front);
+        for (size_t i = 0; i < keys.size(); i++) {
+            const uint64_t E = BW & ~(keys[i] & e.occ_mask);
+            const uint32_t used = (uint32_t)(keys[i] >> e.occ_bits) & e.used_mask;
+            uint8_t nm[16];
+            std::memset(nm, 0xff, sizeof nm);
+            uint32_t cnt = 0;
+            for (uint32_t p = 0; p < L.n; p++)  // ascending ids = sorted list
+                if ((used >> L.off[p]) & 1u) nm[cnt++] = (uint8_t)p;
+            const uint32_t cap_ids = H == 8 ? 0 : H == 7 ? 8 : H == 6 ? 14 : H == 5 ? 15 : 16;
+            if (cnt > cap_ids) throw Error(D4_ERANGE, "more pieces than the record holds");
+            uint32_t ex[4];
+            for (int w = 0; w < 4; w++)
+                ex[w] = nm[4 * w] | nm[4 * w + 1] << 8 | nm[4 * w + 2] << 16 |
+                        (uint32_t)nm[4 * w + 3] << 24;
+            const uint64_t area = E >> 8;  // rows 1.. of the window
+            Rec &r = out[i];
+            r.w[2] = ex[0];
+            r.w[3] = ex[1];
+            r.w[4] = ex[2];
+            if (H == 8) {  // the root record: CudaSearcher's ctor leaves ex0..2 zero
+                r.w[0] = (uint32_t)area;
+                r.w[1] = (uint32_t)(area >> 32);
+                r.w[2] = r.w[3] = r.w[4] = 0;
+            } else if (H == 7) {  // ex2 is not part of a height-7 record
+                r.w[0] = (uint32_t)area;
+                r.w[1] = (uint32_t)(area >> 32 & 0xffffu) | cnt << 24;
+                r.w[4] = ~0u;
+            } else if (H == 6) {
+                r.w[0] = (uint32_t)area;
+                r.w[1] = (ex[3] & 0xffffu) | (uint32_t)(area >> 32 & 0xffu) << 16 | cnt << 24;
+            } else if (H == 5) {
+                r.w[0] = (uint32_t)area;
+                r.w[1] = (ex[3] & 0xffffffu) | cnt << 24;
+            } else {
+                r.w[0] = (uint32_t)(area & 0xffffffu) | cnt << 24;
+                r.w[1] = ex[3];
+            }
+            r.w[5] = (uint32_t)(E & 0xffu);
+            r.cnt = cnts[i];
+        }
+        std::sort(out.begin(), out.end(), [](const Rec &a, const Rec &b) {
+            if (a.w[5] != b.w[5]) return a.w[5] < b.w[5];
+            return std::lexicographical_compare(a.w, a.w + 5, b.w, b.w + 5);
+        });
+        *n = out.size();
+        if (!rx || !cap) return;
+        if (cap < out.size()) throw Error(D4_ERANGE, "record buffer too small");
+        uint8_t *dst = static_cast<uint8_t *>(rx);
+        for (size_t i = 0; i < out.size(); i++) {
+            std::memcpy(dst + 24 * i, out[i].w, 24);
+            if (counts) counts[i] = out[i].cnt;
+        }
+    });
+}
+
 int d4_searcher_stats(const d4_searcher *s, d4_stats *st) {
     return guard([&] {
         if (!s || !st) throw Error(D4_EINVAL, "null argument");

@@ -95,6 +95,7 @@ SIGNATURES = [
     ("d4_searcher_solutions", ctypes.c_int,
      [_P, ctypes.c_int, _U8P, ctypes.c_size_t, _SZP, _U64P]),
     ("d4_searcher_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    ("d4_searcher_export", ctypes.c_int, [_P, _P, ctypes.c_size_t, _SZP, _U64P]),
     ("d4_batch_create", ctypes.c_int,
      [_P, ctypes.c_uint32, _U8P, ctypes.c_uint32, _U32P, ctypes.c_uint32, ctypes.c_uint32,
       ctypes.POINTER(_P)]),
@@ -296,6 +297,18 @@ class CudaSearcher:
         st = Stats()
         _check(load().d4_searcher_stats(self._h, ctypes.byref(st)))
         return st.as_dict()
+
+    def export(self):
+        """The frontier as the reference's RX records: list of (ea, R words
+        (5 ints), count), ordered by (ea, R)."""
+        lib = load()
+        n = ctypes.c_size_t()
+        _check(lib.d4_searcher_export(self._h, None, 0, ctypes.byref(n), None))
+        buf = (ctypes.c_uint32 * (6 * max(n.value, 1)))()
+        cnt = (ctypes.c_uint64 * max(n.value, 1))()
+        _check(lib.d4_searcher_export(self._h, buf, n.value, ctypes.byref(n), cnt))
+        return [(buf[6 * i + 5] & 0xFF, tuple(buf[6 * i:6 * i + 5]), cnt[i])
+                for i in range(n.value)]
 
 
 class Batch:

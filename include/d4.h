@@ -171,6 +171,16 @@ int d4_searcher_count(const d4_searcher *s, uint64_t *count);
 int d4_searcher_solutions(const d4_searcher *s, int single, uint8_t *out,
                           size_t cap, size_t *len, uint64_t *nsol);
 int d4_searcher_stats(const d4_searcher *s, d4_stats *st);
+/* The current frontier in the reference's record format (src/record.h:5-23):
+ * one 24-byte RX per state -- R {xaL, xaH, ex0, ex1, ex2} as parse_R<H>
+ * reads it (src/record.cuh:5-41; H = rows left including the current one)
+ * and ea = the current row's open cells -- with piece ids = library indices,
+ * sorted, 0xff padded.  Ordered by (ea, R).  counts (optional) receives each
+ * state's tiling count (the reference keeps none).  Only the reference's
+ * domain can be written: 8x8 boards (H <= 8), at most 16 pieces, counts 1,
+ * else D4_ERANGE.  *n gets the number of records (call with cap 0 first). */
+int d4_searcher_export(const d4_searcher *s, void *rx, size_t cap, size_t *n,
+                       uint64_t *counts);
 
 /* ---------------------------------------------------------------------- */
 /* batched instances (no reference counterpart: the reference runs one     */

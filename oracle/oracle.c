@@ -303,3 +303,98 @@ uint64_t or_solve(const uint64_t *shapes, const uint32_t *counts, uint32_t n,
     free(c);
     return r;
 }
+
+/* ------------------------------------------------------------------ */
+/* record codec (src/record.cuh:5-71)                                   */
+/* ------------------------------------------------------------------ */
+
+/* CUDA __byte_perm (PTX prmt, default mode): byte n of the result is byte
+ * s[4n+2:4n] of {y:x} (x = bytes 0..3, y = bytes 4..7); selector bit 4n+3
+ * replaces it by its sign bit replicated. */
+static uint32_t byte_perm(uint32_t x, uint32_t y, uint32_t s) {
+    uint64_t v = ((uint64_t)y << 32) | x;
+    uint32_t r = 0;
+    for (int n = 0; n < 4; n++) {
+        uint32_t sel = (s >> (4 * n)) & 0xfu;
+        uint32_t b = (uint32_t)(v >> (8 * (sel & 7u))) & 0xffu;
+        if (sel & 8u) b = (b & 0x80u) ? 0xffu : 0u;
+        r |= b << (8 * n);
+    }
+    return r;
+}
+
+static void ids_to_ex(const uint8_t *nm, uint32_t *ex) {
+    for (int w = 0; w < 4; w++)
+        ex[w] = (uint32_t)nm[4 * w] | (uint32_t)nm[4 * w + 1] << 8 |
+                (uint32_t)nm[4 * w + 2] << 16 | (uint32_t)nm[4 * w + 3] << 24;
+}
+
+void or_parse_R(unsigned H, const uint32_t *r, uint8_t ea, uint64_t *empty_area,
+                uint32_t *nm_cnt, uint8_t *nm) {
+    const uint32_t xaL = r[0], xaH = r[1], ex0 = r[2], ex1 = r[3], ex2 = r[4];
+    uint64_t e = 0;
+    uint32_t cnt = 0, ex[4] = {~0u, ~0u, ~0u, ~0u};
+    if (H == 8) {
+        e = ((uint64_t)xaL | (uint64_t)xaH << 32) << 8 | ea;
+        *empty_area = e;
+        *nm_cnt = 0;
+        memset(nm, 0xff, 16);
+        return;
+    }
+    if (H == 7) {
+        e = xaL | (uint64_t)(xaH & 0x0000ffffu) << 32;
+        cnt = xaH >> 24;
+    }
+    if (H == 6) {
+        e = xaL | (uint64_t)(xaH >> 16 & 0xffu) << 32;
+        cnt = xaH >> 24;
+        ex[3] = xaH | 0xffff0000u;
+    }
+    if (H == 5) {
+        e = xaL;
+        cnt = xaH >> 24;
+        ex[3] = xaH | 0xff000000u;
+    }
+    if (H <= 4) {
+        e = xaL & 0x00ffffffu;
+        cnt = xaL >> 24;
+        ex[3] = xaH;
+    }
+    if (H <= 6) ex[2] = ex2;
+    ex[1] = ex1;
+    ex[0] = ex0;
+    *empty_area = e << 8 | ea;
+    *nm_cnt = cnt;
+    for (int w = 0; w < 4; w++)
+        for (int b = 0; b < 4; b++) nm[4 * w + b] = (uint8_t)(ex[w] >> (8 * b));
+}
+
+void or_assemble_R(unsigned H, uint64_t empty_area, uint32_t nm_cnt, const uint8_t *nm,
+                   uint32_t *r, uint8_t *ea) {
+    uint32_t ex[4];
+    ids_to_ex(nm, ex);
+    uint32_t xaL = 0, xaH = 0, ex2 = ~0u;  /* ex2 unset at H == 7 (record.cuh:47) */
+    *ea = (uint8_t)(empty_area >> 8 & 0xffu);
+    if (H == 7) {
+        xaL = (uint32_t)(empty_area >> 16);
+        xaH = (uint32_t)(empty_area >> 48) | nm_cnt << 24;
+    }
+    if (H == 6) {
+        xaL = (uint32_t)(empty_area >> 16);
+        xaH = byte_perm(byte_perm(ex[3], nm_cnt, 0x4f10u), (uint32_t)(empty_area >> 32), 0x3610u);
+    }
+    if (H == 5) {
+        xaL = (uint32_t)(empty_area >> 16);
+        xaH = byte_perm(ex[3], nm_cnt, 0x4210u);
+    }
+    if (H <= 4) {
+        xaL = byte_perm((uint32_t)(empty_area >> 16), nm_cnt, 0x4210u);
+        xaH = ex[3];
+    }
+    if (H <= 6) ex2 = ex[2];
+    r[0] = xaL;
+    r[1] = xaH;
+    r[2] = ex[0];
+    r[3] = ex[1];
+    r[4] = ex2;
+}

@@ -33,6 +33,9 @@
  *                        multiplicity invariants, D2 symmetry closure of the
  *                        6x10 solution set, first-solution = lexicographic
  *                        minimum of the full set
+ *   or_parse_R /      -> round trip parse(assemble(x)) == x for random records
+ *   or_assemble_R        at every height (the reference's bmark/record.cu
+ *                        check), and hand-computed byte layouts of record.h
  */
 #ifndef D4_ORACLE_H
 #define D4_ORACLE_H
@@ -87,6 +90,21 @@ uint64_t or_solve(const uint64_t *shapes, const uint32_t *counts, uint32_t n,
 
 /* make_step's 2x2 matrix for transform trs (Piece.cpp:41-54): a, b, c, d. */
 void or_step_matrix(unsigned trs, int *abcd);
+
+/* The reference's frontier record codec (src/record.h:5-32,
+ * src/record.cuh:5-71).  r[0..4] = R {xaL, xaH, ex0, ex1, ex2}.
+ *   or_parse_R:    parse_R<H>(R, ea) -> empty_area (row 0 = ea), nm_cnt,
+ *                  nm[16] (ids, 0xff padded)
+ *   or_assemble_R: the R part of assemble_R<H>(RCfg) for an RCfg whose
+ *                  empty_area has the just-filled row at bits 0..7 (so the
+ *                  result's ea is empty_area >> 8 & 0xff, returned in *ea).
+ * H is the height of the record produced / parsed (1..8; assemble 1..7).
+ * __byte_perm is evaluated as CUDA's prmt default mode (selector bit 3
+ * replicates the selected byte's sign bit). */
+void or_parse_R(unsigned H, const uint32_t *r, uint8_t ea, uint64_t *empty_area,
+                uint32_t *nm_cnt, uint8_t *nm);
+void or_assemble_R(unsigned H, uint64_t empty_area, uint32_t nm_cnt, const uint8_t *nm,
+                   uint32_t *r, uint8_t *ea);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,12 @@ class Oracle:
                                ctypes.c_void_p, ctypes.c_size_t,
                                ctypes.POINTER(ctypes.c_size_t)]
         L.or_step_matrix.argtypes = [ctypes.c_uint, ctypes.c_void_p]
+        L.or_parse_R.argtypes = [ctypes.c_uint, ctypes.c_void_p, ctypes.c_uint8,
+                                 ctypes.POINTER(ctypes.c_uint64),
+                                 ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p]
+        L.or_assemble_R.argtypes = [ctypes.c_uint, ctypes.c_uint64, ctypes.c_uint32,
+                                    ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.POINTER(ctypes.c_uint8)]
         self.L = L
 
     def transform(self, s, t):
@@ -60,6 +66,20 @@ class Oracle:
         tr = (ctypes.c_uint * 8)()
         k = self.L.or_piece_orients(s, sym, nm, tr)
         return list(zip(nm[:k], tr[:k]))
+
+    def parse_R(self, H, r, ea):
+        rr = (ctypes.c_uint32 * 5)(*r)
+        e, c = ctypes.c_uint64(), ctypes.c_uint32()
+        nm = (ctypes.c_uint8 * 16)()
+        self.L.or_parse_R(H, rr, ea, ctypes.byref(e), ctypes.byref(c), nm)
+        return e.value, c.value, list(nm)
+
+    def assemble_R(self, H, empty_area, cnt, nm):
+        n = (ctypes.c_uint8 * 16)(*(list(nm) + [0xFF] * (16 - len(nm))))
+        r = (ctypes.c_uint32 * 5)()
+        ea = ctypes.c_uint8()
+        self.L.or_assemble_R(H, empty_area, cnt, n, r, ctypes.byref(ea))
+        return tuple(r), ea.value
 
     @staticmethod
     def _args(shapes, counts, rows):

@@ -228,3 +228,40 @@ def test_subset_tilings(oracle):
     # pieces need not all be used (solve only prunes by area): a 2x2 board
     # with a monomino x4 and a square tetromino x1 has 2 tilings
     assert oracle.count([1, 15 & 0x3 | (0x3 << 8)], [4, 1], rect(2, 2)) == 2
+
+
+def test_record_codec_round_trip(oracle):
+    """parse_R<H>(assemble_R<H>(cfg)) restores cfg (the reference's
+    bmark/record.cu property) for every height and id count it can hold."""
+    rng = random.Random(5)
+    cap = {7: 8, 6: 14, 5: 15, 4: 16, 3: 16, 2: 16, 1: 16}
+    for H in range(1, 8):
+        for _ in range(300):
+            cnt = rng.randint(0, cap[H])
+            ids = sorted(rng.sample(range(40), cnt))
+            e = rng.getrandbits(8 * (H + 1))  # rows 0..H; row 0 just filled
+            r, ea = oracle.assemble_R(H, e, cnt, ids)
+            e2, c2, nm = oracle.parse_R(H, r, ea)
+            assert e2 == e >> 8, H
+            assert c2 == cnt
+            assert nm[:cnt] == ids
+            # positions the record holds beyond the ids read as padding
+            assert all(v == 0xFF for v in nm[cnt:cap[H]])
+
+
+def test_record_codec_layout(oracle):
+    # record.h: R = {xaL, xaH, ex0, ex1, ex2}; ids fill ex0, ex1, ex2 then
+    # the spare bytes of xaH (H 5, 6) or all of xaH (H <= 4)
+    ids = list(range(1, 15))
+    e = 0x77_66_55_44_33_22_11_00  # byte k = row k; row 0 just filled
+    r, ea = oracle.assemble_R(6, e, 14, ids)
+    assert ea == 0x11                                   # new current row
+    assert r[0] == 0x55443322                           # new rows 1..4
+    assert (r[1] >> 16) & 0xFF == 0x66                  # new row 5
+    assert r[2] == 0x04030201 and r[3] == 0x08070605 and r[4] == 0x0C0B0A09
+    assert r[1] & 0xFFFF == 0x0E0D and r[1] >> 24 == 14
+    r, ea = oracle.assemble_R(7, e, 2, [3, 9])
+    assert r[0] == 0x55443322 and r[1] == 0x7766 | (2 << 24)
+    r, ea = oracle.assemble_R(4, 0x00_112233_44 << 8, 3, [7, 8, 9])
+    assert r[0] == (0x112233 & 0xFFFFFF) | (3 << 24)
+    assert r[2] == 0xFF090807 and r[1] == 0xFFFFFFFF
