@@ -143,6 +143,66 @@ def serve_section(d4, lib, workloads, rates, n_req, slots):
     return out
 
 
+def multi_libraries(d4, insts):
+    cache, libs = {}, []
+    for shapes, counts, _, w in insts:
+        key = (tuple(shapes), tuple(counts), w)
+        if key not in cache:
+            cache[key] = d4.PieceLibrary(shapes, counts, 0xFF, w)
+        libs.append(cache[key])
+    return libs
+
+
+def uneven_section(d4, workloads, n, budget, plain_sample):
+    """The uneven puzzles (own piece set and width per instance) in one
+    multi-library batch with a per-instance state budget, against the same
+    instances one per batch; parity against the oracle's counts of the
+    instances it finished (tests/golden/uneven_4096_seed3.json)."""
+    insts = workloads.uneven_instances(4096, seed=3)[:n]
+    libs = multi_libraries(d4, insts)
+    log(f"uneven: {n} instances, state budget {budget} per instance")
+    t = time.perf_counter()
+    b = d4.MultiBatch(libs, [x[2] for x in insts])
+    b.set_budget(budget)
+    b.run()
+    d4.synchronize()
+    wall = time.perf_counter() - t
+    counts = b.counts()
+    peak, fin = b.instances()
+    st = b.stats()
+    out = {"instances": n, "state_budget": budget, "finished": sum(fin), "wall_s": wall,
+           "records_per_s": st["records_extended"] / wall,
+           "solutions_per_s": sum(counts) / wall,
+           "records_extended": st["records_extended"], "peak_frontier": st["peak_frontier"],
+           "splits": st["splits"],
+           "extend_GBs": st["extend_bytes"] / (st["extend_ms"] / 1e3) / 1e9
+           if st["extend_ms"] else None}
+    log(f"uneven batched: {wall:.1f} s, {sum(fin)}/{n} finished, "
+        f"{out['records_per_s'] / 1e6:.0f} M records/s")
+    fx_path = os.path.join(ROOT, "tests", "golden", "uneven_4096_seed3.json")
+    if os.path.exists(fx_path):
+        pinned = {int(k): v for k, v in json.load(open(fx_path))["counts"].items()
+                  if int(k) < n}
+        chk = [i for i in pinned if fin[i]]
+        out["parity"] = {"checked_against_oracle": len(chk),
+                         "equal": all(counts[i] == pinned[i] for i in chk)}
+    if plain_sample:
+        m = min(plain_sample, n)
+        t = time.perf_counter()
+        prec = 0
+        for i in range(m):
+            pb = d4.MultiBatch([libs[i]], [insts[i][2]])
+            pb.set_budget(budget)
+            pb.run()
+            prec += pb.stats()["records_extended"]
+        d4.synchronize()
+        pdt = time.perf_counter() - t
+        out["plain"] = {"instances": m, "records_per_s": prec / pdt, "wall_s": pdt}
+        out["batched_vs_plain"] = out["records_per_s"] / (prec / pdt) if prec else None
+        log(f"uneven plain: {m} instances one per batch, {prec / pdt / 1e6:.0f} M records/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +220,10 @@ def main():
                     help="comma-separated arrival rates (requests/s); empty to skip")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-slots", type=int, default=1024)
+    ap.add_argument("--uneven", type=int, default=1024,
+                    help="uneven puzzles in the multi-library section (0: skip)")
+    ap.add_argument("--uneven-budget", type=int, default=1 << 20)
+    ap.add_argument("--uneven-plain", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the "
@@ -305,6 +369,10 @@ def main():
         out["latency_ms"]["served_p50"] = best["latency_ms_p50"]
         out["latency_ms"]["served_p99"] = best["latency_ms_p99"]
         out["served_vs_plain"] = best["served_vs_plain"]
+
+    if args.uneven:
+        out["uneven"] = uneven_section(d4, workloads, args.uneven, args.uneven_budget,
+                                       args.uneven_plain)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

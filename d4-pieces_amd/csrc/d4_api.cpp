@@ -15,185 +15,14 @@
 #include <string>
 #include <vector>
 
-#include "d4_internal.h"
+#include "d4_host.h"
+#include "d4_library.h"
 
 namespace d4 {
 
 static thread_local std::string g_err;
 
 void set_error(const std::string &msg) { g_err = msg; }
-
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(x)                                                              \
-    do {                                                                       \
-        hipError_t e_ = (x);                                                   \
-        if (e_ != hipSuccess)                                                  \
-            throw Error(e_ == hipErrorOutOfMemory ? D4_ENOMEM : D4_EDEVICE,     \
-                        std::string(#x) + ": " + hipGetErrorString(e_));        \
-    } while (0)
-
-template <class F>
-static int guard(F &&f) {
-    try {
-        f();
-        return D4_OK;
-    } catch (const Error &e) {
-        set_error(e.what());
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        set_error("host allocation failed");
-        return D4_ENOMEM;
-    } catch (const std::exception &e) {
-        set_error(e.what());
-        return D4_EINVAL;
-    }
-}
-
-// --------------------------------------------------------------------------
-// geometry (8-wide bitboards, bit = row*8 + col)
-// --------------------------------------------------------------------------
-
-static uint64_t normalize8(uint64_t v) {
-    if (!v) return 0;
-    while (!(v & 0xffull)) v >>= 8;
-    while (!(v & 0x0101010101010101ull)) v >>= 1;
-    return v;
-}
-
-// D4 transform t in the reference's order (Shape.cpp:28-40: identity, flip
-// X, flip Y, both, then the same three after a transpose).  Written as the
-// image of each input cell: flips act on the input's coordinates, then the
-// transpose swaps them.
-static uint64_t transform8(uint64_t s, unsigned t) {
-    const bool sw = t & 4, fx = t & 1, fy = t & 2;
-    uint64_t out = 0;
-    for (uint64_t b = s; b; b &= b - 1) {
-        unsigned i = (unsigned)__builtin_ctzll(b);
-        unsigned y = i / 8, x = i % 8;
-        unsigned yy = fy ? 7 - y : y, xx = fx ? 7 - x : x;
-        unsigned r = sw ? xx : yy, c = sw ? yy : xx;
-        out |= 1ull << (r * 8 + c);
-    }
-    return normalize8(out);
-}
-
-static unsigned width8(uint64_t s) {
-    unsigned w = 0;
-    for (unsigned c = 0; c < 8; c++)
-        if (s & (0x0101010101010101ull << c)) w = c + 1;
-    return w;
-}
-
-static unsigned height8(uint64_t s) {
-    unsigned h = 0;
-    for (unsigned r = 0; r < 8; r++)
-        if (s & (0xffull << (8 * r))) h = r + 1;
-    return h;
-}
-
-static unsigned bitw(uint32_t v) { return v ? 32u - (unsigned)__builtin_clz(v) : 0u; }
-
-static uint64_t next_pow2(uint64_t v) {
-    uint64_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
-
-// Caching device allocator: freed blocks stay with the process and are
-// reused by later searches of similar size (hipMalloc / hipFree of multi-GB
-// tables per row would otherwise dominate short searches).  Blocks are
-// rounded to 2 MiB; a request takes the smallest cached block of at least its
-// size and at most twice it.  When the device runs out, the cache is
-// released and the allocation retried once.
-class Pool {
-    std::mutex mu_;
-    std::multimap<size_t, std::pair<int, void *>> free_;  // bytes -> (device, ptr)
-
-    void trim_locked() {
-        for (auto &kv : free_) {
-            int cur = 0;
-            (void)hipGetDevice(&cur);
-            (void)hipSetDevice(kv.second.first);
-            (void)hipFree(kv.second.second);
-            (void)hipSetDevice(cur);
-        }
-        free_.clear();
-    }
-
-  public:
-    static size_t round(size_t bytes) {
-        const size_t g = 2u << 20;
-        return (std::max<size_t>(bytes, 1) + g - 1) / g * g;
-    }
-    void *alloc(size_t bytes) {
-        bytes = round(bytes);
-        int dev = 0;
-        HIPCHK(hipGetDevice(&dev));
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            for (auto it = free_.lower_bound(bytes); it != free_.end() && it->first <= 2 * bytes; ++it)
-                if (it->second.first == dev) {
-                    void *p = it->second.second;
-                    free_.erase(it);
-                    return p;
-                }
-        }
-        void *p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e == hipErrorOutOfMemory) {
-            (void)hipGetLastError();
-            std::lock_guard<std::mutex> lk(mu_);
-            trim_locked();
-            e = hipMalloc(&p, bytes);
-        }
-        if (e != hipSuccess)
-            throw Error(e == hipErrorOutOfMemory ? D4_ENOMEM : D4_EDEVICE,
-                        std::string("hipMalloc: ") + hipGetErrorString(e));
-        return p;
-    }
-    void release(void *p, size_t bytes) {
-        if (!p) return;
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        std::lock_guard<std::mutex> lk(mu_);
-        free_.emplace(round(bytes), std::make_pair(dev, p));
-    }
-};
-
-static Pool &pool() {
-    static Pool *p = new Pool();  // never destroyed: outlives every handle
-    return *p;
-}
-
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    uint64_t cap = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) pool().release(p, cap * sizeof(T));
-        p = nullptr;
-        cap = 0;
-    }
-    void reserve(uint64_t n) {
-        if (n <= cap) return;
-        release();
-        uint64_t c = Pool::round(std::max<uint64_t>(n, 64) * sizeof(T)) / sizeof(T);
-        p = static_cast<T *>(pool().alloc(c * sizeof(T)));
-        cap = c;
-    }
-    void swap(DevBuf &o) {
-        std::swap(p, o.p);
-        std::swap(cap, o.cap);
-    }
-};
 
 }  // namespace d4
 
@@ -203,34 +32,6 @@ using namespace d4;
 // piece library and placement tables
 // --------------------------------------------------------------------------
 
-struct d4_library {
-    uint32_t n = 0, sym = 0, W = 0, maxh = 0, ncand = 0, used_bits = 0, fit_rows = 1;
-    uint32_t counts[MAX_PIECES] = {}, sizes[MAX_PIECES] = {};
-    uint32_t off[MAX_PIECES] = {}, fbits[MAX_PIECES] = {};
-    uint8_t cpiece[MAX_CAND] = {}, ctrs[MAX_CAND] = {}, cminc[MAX_CAND] = {};
-    uint64_t cnormal[MAX_CAND] = {};
-    uint64_t place[8][MAX_CAND] = {};
-    uint64_t placeinfo[8 * MAX_CAND * 2] = {};
-    uint32_t ext_threads = 1024;  // extend workgroup size (autotuned)
-    uint32_t ext_refill = 16;     // idle lanes per refill round (autotuned)
-    uint64_t range[MAX_CAND] = {};
-    uint32_t cinfo[MAX_CAND] = {};
-    uint64_t prange[MAX_PIECES] = {};
-    uint64_t xavail[4 * 256] = {};
-    std::vector<uint64_t> fit;  // [8][maxh][256]
-    unsigned long long comp[256] = {};
-    uint32_t gcd = 1, min_size = 1;
-    uint32_t lds_limit = 65536;
-    int device = 0;
-    int n_cu = 1;
-    uint64_t *d_fit = nullptr, *d_placeinfo = nullptr, *d_xavail = nullptr;
-    unsigned long long *d_comp = nullptr;
-
-    ~d4_library() {
-        for (void *p : {(void *)d_fit, (void *)d_placeinfo, (void *)d_xavail, (void *)d_comp})
-            if (p) (void)hipFree(p);
-    }
-};
 
 // fit[c][r][b]: candidates anchored at column c whose row-r cells lie in b.
 static void build_fit(d4_library &L, uint32_t fit_rows) {
@@ -250,6 +51,8 @@ static void build_fit(d4_library &L, uint32_t fit_rows) {
             }
 }
 
+// Piece counters, candidates and the pruning constants (any library of up
+// to MAX_LIB_CAND candidates).
 static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *counts) {
     uint32_t k = 0, off = 0, g = 0, mins = UINT32_MAX;
     for (uint32_t i = 0; i < L.n; i++) {
@@ -280,8 +83,8 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
             bool dup = std::find(seen.begin(), seen.end(), sh) != seen.end();
             seen.push_back(sh);
             if (!dup) {
-                if (k >= (uint32_t)MAX_CAND)
-                    throw Error(D4_EINVAL, "more than 64 (piece, transform) pairs");
+                if (k >= (uint32_t)MAX_LIB_CAND)
+                    throw Error(D4_EINVAL, "more than 128 (piece, transform) pairs");
                 L.cpiece[k] = (uint8_t)i;
                 L.ctrs[k] = (uint8_t)nall;
                 L.cnormal[k] = sh;
@@ -291,14 +94,35 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
             }
             nall++;
         }
-        uint64_t r = 0;
-        for (uint32_t j = first; j < k; j++) r |= 1ull << j;
-        for (uint32_t j = first; j < k; j++) L.range[j] = r;
         uint32_t info = L.off[i] | (((1u << L.fbits[i]) - 1u) << 8) | (counts[i] << 16);
-        for (uint32_t j = first; j < k; j++) L.cinfo[j] = info;
-        L.prange[i] = r;
+        for (uint32_t j = first; j < k; j++) {
+            L.cinfo[j] = info;
+            L.clo[j] = (uint8_t)first;
+            L.chi[j] = (uint8_t)k;
+        }
     }
     if (off > 32) throw Error(D4_EINVAL, "piece counters need more than 32 bits");
+    if (L.maxh > (uint32_t)MAX_FIT_ROWS) throw Error(D4_EINVAL, "piece taller than 6 rows");
+    L.used_bits = off;
+    L.ncand = k;
+    L.gcd = g ? g : 1;
+    L.min_size = mins == UINT32_MAX ? 1 : mins;
+    // the fit table covers the first fit_rows window rows (the placement
+    // itself is tested in full when chosen); 3 rows keep it at 48 KiB of LDS
+    L.fit_rows = std::max(1u, std::min(L.maxh, 3u));
+}
+
+// Single-library tables of the LDS-staged extend kernel: u64 candidate masks
+// per anchor column and window row, counter-byte availability tables, packed
+// placement entries and the row-0 completion counts used by next_size.
+static void build_fast(d4_library &L) {
+    const uint32_t k = L.ncand;
+    for (uint32_t j = 0; j < k; j++) {
+        uint64_t r = 0;
+        for (uint32_t q = L.clo[j]; q < L.chi[j]; q++) r |= 1ull << q;
+        L.range[j] = r;
+        L.prange[L.cpiece[j]] = r;
+    }
     // xavail[b][v]: candidates of the pieces whose counter lies in byte b and
     // is exhausted when that byte reads v
     for (uint32_t b = 0; b < 4; b++)
@@ -311,11 +135,6 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
             }
             L.xavail[b * 256 + v] = m;
         }
-    if (L.maxh > (uint32_t)MAX_FIT_ROWS) throw Error(D4_EINVAL, "piece taller than 6 rows");
-    L.used_bits = off;
-    L.ncand = k;
-    L.gcd = g ? g : 1;
-    L.min_size = mins == UINT32_MAX ? 1 : mins;
     for (uint32_t c = 0; c < 8; c++)
         for (uint32_t j = 0; j < k; j++) {
             uint64_t n = L.cnormal[j];
@@ -326,14 +145,11 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
     // candidate range [lo, hi)
     for (uint32_t c = 0; c < 8; c++)
         for (uint32_t j = 0; j < k; j++) {
-            uint64_t r = L.range[j];
-            uint64_t lo = (uint64_t)__builtin_ctzll(r), hi = 64 - (uint64_t)__builtin_clzll(r);
             L.placeinfo[(c * MAX_CAND + j) * 2] = L.place[c][j];
-            L.placeinfo[(c * MAX_CAND + j) * 2 + 1] = (uint64_t)L.cinfo[j] | (lo << 32) | (hi << 40);
+            L.placeinfo[(c * MAX_CAND + j) * 2 + 1] =
+                (uint64_t)L.cinfo[j] | ((uint64_t)L.clo[j] << 32) | ((uint64_t)L.chi[j] << 40);
         }
-    // the fit table covers the first fit_rows window rows (the placement
-    // itself is tested in full when chosen); 3 rows keep it at 48 KiB of LDS
-    build_fit(L, std::max(1u, std::min(L.maxh, 3u)));
+    build_fit(L, L.fit_rows);
     // comp[p]: ways to fill row pattern p with row-0 parts of placements,
     // ignoring lower rows and piece counts (bound used by next_size).
     L.comp[0] = 1;
@@ -355,6 +171,25 @@ template <class T>
 static void upload(T *&dst, const T *src, size_t n) {
     HIPCHK(hipMalloc((void **)&dst, std::max<size_t>(n, 1) * sizeof(T)));
     if (n) HIPCHK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// Build and upload the single-library tables once (engines call this).
+static void ensure_fast(const d4_library *cl) {
+    d4_library *L = const_cast<d4_library *>(cl);
+    std::lock_guard<std::mutex> lk(L->fast_mu);
+    if (L->fast_ready) return;
+    if (L->ncand > (uint32_t)MAX_CAND)
+        throw Error(D4_EINVAL, "more than 64 (piece, transform) pairs: this library runs "
+                               "only in a multi-library batch (d4_mbatch_create)");
+    build_fast(*L);
+    if (extend_lds_bytes(L->fit_rows, 256, true) > L->lds_limit)
+        throw Error(D4_EINVAL, "placement tables exceed LDS");
+    HIPCHK(hipSetDevice(L->device));
+    upload(L->d_fit, L->fit.data(), L->fit.size());
+    upload(L->d_placeinfo, L->placeinfo, 2 * 8 * MAX_CAND);
+    upload(L->d_xavail, L->xavail, 4 * 256);
+    upload(L->d_comp, L->comp, 256);
+    L->fast_ready = true;
 }
 
 // --------------------------------------------------------------------------
@@ -424,6 +259,7 @@ struct Engine {
         : lib(l), S(slots), mode(m), flags(f), max_h(max_height), keep_logs(keep) {
         if (!S) throw Error(D4_EINVAL, "no instances");
         if (mode > D4_MODE_ENUM) throw Error(D4_EINVAL, "bad mode");
+        ensure_fast(lib);
         if (max_h > (uint32_t)MAX_ROWS) throw Error(D4_EINVAL, "board taller than 64 rows");
         heights.assign(S, 0);
         start.assign(S, 0);
@@ -1190,12 +1026,6 @@ int d4_library_create(const uint64_t *shapes, const uint32_t *counts, uint32_t n
         HIPCHK(hipGetDeviceProperties(&prop, dev));
         L->n_cu = prop.multiProcessorCount;
         L->lds_limit = (uint32_t)prop.sharedMemPerBlock;
-        if (extend_lds_bytes(L->fit_rows, 256, true) > L->lds_limit)
-            throw Error(D4_EINVAL, "placement tables exceed LDS");
-        upload(L->d_fit, L->fit.data(), L->fit.size());
-        upload(L->d_placeinfo, L->placeinfo, 2 * 8 * MAX_CAND);
-        upload(L->d_xavail, L->xavail, 4 * 256);
-        upload(L->d_comp, L->comp, 256);
         *out = L.release();
     });
 }
@@ -1220,6 +1050,7 @@ int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads,
         if (fit_rows < 1 || fit_rows > lib->maxh) throw Error(D4_EINVAL, "fit_rows out of range");
         if (extend_lds_bytes(fit_rows, threads, false) > lib->lds_limit)
             throw Error(D4_EINVAL, "launch shape exceeds LDS");
+        ensure_fast(lib);
         if (fit_rows != lib->fit_rows) {
             HIPCHK(hipSetDevice(lib->device));
             build_fit(*lib, fit_rows);

@@ -1,0 +1,190 @@
+// d4_host.h -- host-side helpers shared by the engines: error plumbing,
+// bitboard geometry and the caching device allocator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "d4_internal.h"
+
+namespace d4 {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                              \
+    do {                                                                       \
+        hipError_t e_ = (x);                                                   \
+        if (e_ != hipSuccess)                                                  \
+            throw Error(e_ == hipErrorOutOfMemory ? D4_ENOMEM : D4_EDEVICE,     \
+                        std::string(#x) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+template <class F>
+inline int guard(F &&f) {
+    try {
+        f();
+        return D4_OK;
+    } catch (const Error &e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return D4_ENOMEM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return D4_EINVAL;
+    }
+}
+
+// --------------------------------------------------------------------------
+// geometry (8-wide bitboards, bit = row*8 + col)
+// --------------------------------------------------------------------------
+
+inline uint64_t normalize8(uint64_t v) {
+    if (!v) return 0;
+    while (!(v & 0xffull)) v >>= 8;
+    while (!(v & 0x0101010101010101ull)) v >>= 1;
+    return v;
+}
+
+// D4 transform t in the reference's order (Shape.cpp:28-40: identity, flip
+// X, flip Y, both, then the same three after a transpose).  Written as the
+// image of each input cell: flips act on the input's coordinates, then the
+// transpose swaps them.
+inline uint64_t transform8(uint64_t s, unsigned t) {
+    const bool sw = t & 4, fx = t & 1, fy = t & 2;
+    uint64_t out = 0;
+    for (uint64_t b = s; b; b &= b - 1) {
+        unsigned i = (unsigned)__builtin_ctzll(b);
+        unsigned y = i / 8, x = i % 8;
+        unsigned yy = fy ? 7 - y : y, xx = fx ? 7 - x : x;
+        unsigned r = sw ? xx : yy, c = sw ? yy : xx;
+        out |= 1ull << (r * 8 + c);
+    }
+    return normalize8(out);
+}
+
+inline unsigned width8(uint64_t s) {
+    unsigned w = 0;
+    for (unsigned c = 0; c < 8; c++)
+        if (s & (0x0101010101010101ull << c)) w = c + 1;
+    return w;
+}
+
+inline unsigned height8(uint64_t s) {
+    unsigned h = 0;
+    for (unsigned r = 0; r < 8; r++)
+        if (s & (0xffull << (8 * r))) h = r + 1;
+    return h;
+}
+
+inline unsigned bitw(uint32_t v) { return v ? 32u - (unsigned)__builtin_clz(v) : 0u; }
+
+inline uint64_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Caching device allocator: freed blocks stay with the process and are
+// reused by later searches of similar size (hipMalloc / hipFree of multi-GB
+// tables per row would otherwise dominate short searches).  Blocks are
+// rounded to 2 MiB; a request takes the smallest cached block of at least its
+// size and at most twice it.  When the device runs out, the cache is
+// released and the allocation retried once.
+class Pool {
+    std::mutex mu_;
+    std::multimap<size_t, std::pair<int, void *>> free_;  // bytes -> (device, ptr)
+
+    void trim_locked() {
+        for (auto &kv : free_) {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(kv.second.first);
+            (void)hipFree(kv.second.second);
+            (void)hipSetDevice(cur);
+        }
+        free_.clear();
+    }
+
+  public:
+    static size_t round(size_t bytes) {
+        const size_t g = 2u << 20;
+        return (std::max<size_t>(bytes, 1) + g - 1) / g * g;
+    }
+    void *alloc(size_t bytes) {
+        bytes = round(bytes);
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (auto it = free_.lower_bound(bytes); it != free_.end() && it->first <= 2 * bytes; ++it)
+                if (it->second.first == dev) {
+                    void *p = it->second.second;
+                    free_.erase(it);
+                    return p;
+                }
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            std::lock_guard<std::mutex> lk(mu_);
+            trim_locked();
+            e = hipMalloc(&p, bytes);
+        }
+        if (e != hipSuccess)
+            throw Error(e == hipErrorOutOfMemory ? D4_ENOMEM : D4_EDEVICE,
+                        std::string("hipMalloc: ") + hipGetErrorString(e));
+        return p;
+    }
+    void release(void *p, size_t bytes) {
+        if (!p) return;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(round(bytes), std::make_pair(dev, p));
+    }
+};
+
+inline Pool &pool() {
+    static Pool *p = new Pool();  // never destroyed: outlives every handle
+    return *p;
+}
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    uint64_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) pool().release(p, cap * sizeof(T));
+        p = nullptr;
+        cap = 0;
+    }
+    void reserve(uint64_t n) {
+        if (n <= cap) return;
+        release();
+        uint64_t c = Pool::round(std::max<uint64_t>(n, 64) * sizeof(T)) / sizeof(T);
+        p = static_cast<T *>(pool().alloc(c * sizeof(T)));
+        cap = c;
+    }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+    }
+};
+
+}  // namespace d4

@@ -12,7 +12,8 @@
 namespace d4 {
 
 constexpr uint64_t EMPTY_KEY = ~0ull;
-constexpr int MAX_CAND = 64;     // (piece, transform) pairs: one u64 mask
+constexpr int MAX_CAND = 64;       // (piece, transform) pairs of the single-library kernel
+constexpr int MAX_LIB_CAND = 128;  // (piece, transform) pairs of any library (two u64 masks)
 constexpr int MAX_PIECES = 32;
 constexpr int MAX_FIT_ROWS = 6;  // tallest piece
 constexpr int MAX_ROWS = 64;     // board height
@@ -80,7 +81,49 @@ struct ExtendArgs {
     uint32_t min_size;
 };
 
-// Launchers (d4_kernels.hip).
+// Tables of one library in a multi-library batch (device layout, 16-byte
+// aligned).  Candidate masks are 128 bits: word 0 = candidates 0..63.
+struct MLib {
+    // fit[512 - (512 >> c) + (b >> c)]: candidates anchored at column c whose
+    // row-0 cells lie inside row pattern b (b's bits below c are clear)
+    uint64_t fit[512][2];
+    // per candidate: normalised shape, and info = counter offset |
+    // field mask << 8 | count << 16 | anchor column of the shape << 32 |
+    // piece's candidate range lo << 40 | hi << 48
+    uint64_t cand[MAX_LIB_CAND][2];
+    // per piece: offset | field mask << 8 | count << 16 | lo << 32 | hi << 40
+    uint64_t piece[MAX_PIECES];
+    uint64_t occ_mask;   // occupancy bits of a state
+    uint64_t seed_mask;  // child window rows no placed piece reaches
+    uint32_t gcd, min_size, n_pieces, n_cand;
+};
+
+// Multi-library state key: word 0 = occupancy | instance << MKEY_INST_SHIFT,
+// word 1 = use counters.
+constexpr uint32_t MKEY_INST_SHIFT = 48;
+constexpr uint32_t MAX_MULTI_INST = 1u << 15;
+
+struct MultiArgs {
+    const uint64_t *in_k0, *in_k1, *in_cnt;
+    uint32_t n_in;
+    uint32_t row;            // step = current row of every instance
+    uint32_t split;          // levels fixed per work item: 0, 1 or 2
+    uint32_t refill;
+    const MLib *libs;
+    const uint32_t *inst_lib;  // [n_inst] library index
+    const uint64_t *bw;        // [n_inst][bw_stride] board window of row r
+    uint32_t bw_stride;
+    const uint32_t *hgt;
+    const uint8_t *dropped;    // [n_inst] nonzero: instance left the search
+    unsigned long long *result;
+    unsigned long long *tab;   // 4 words per slot: k0, k1, count, unused
+    uint64_t tab_mask;
+    uint32_t max_probe;
+    uint32_t *c32;
+    unsigned long long *c64;
+};
+
+// Launchers (d4_kernels.hip, d4_multi.hip).
 void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
                        uint64_t n, hipStream_t st);
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
@@ -93,6 +136,15 @@ void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
                     uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
                     uint32_t *blk, uint32_t *slot_uid, hipStream_t st);
+// multi-library batch: 4-word table slots, two-word keys
+void launch_fill_table4(unsigned long long *tab, uint64_t n, hipStream_t st);
+void launch_compact4(const unsigned long long *tab, uint64_t m, uint64_t *out_k0,
+                     uint64_t *out_k1, uint64_t *out_cnt, uint32_t *c32, uint32_t *blk,
+                     hipStream_t st);
+void launch_extend_multi(const MultiArgs &a, bool prune, bool count_only, uint32_t threads,
+                         int n_cu, hipStream_t st);
+void launch_inst_hist(const uint64_t *k0, uint64_t n, uint32_t n_inst, unsigned long long *hist,
+                      int n_cu, hipStream_t st);
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
                         uint32_t *e_child, uint32_t n, hipStream_t st);
 void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,

@@ -28,6 +28,7 @@
 // half-row cross product is tested per record, every child takes one
 // atomicAdd, and dedup runs on the host over PCIe.
 #include "d4_internal.h"
+#include "d4_device.h"
 
 #include <algorithm>
 
@@ -35,46 +36,10 @@ namespace d4 {
 
 namespace {
 
+using namespace dev;
+
 constexpr uint32_t STAGE = 128;  // staged children per wave
 constexpr int CMP_THREADS = 256;
-
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-
-__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Wave-aggregated slot allocation: every active lane calls it; lanes with
-// pred get consecutive indices from *ctr with one atomic for the wave.
-__device__ __forceinline__ uint32_t wave_alloc(uint32_t *ctr, bool pred) {
-    uint64_t m = __ballot(pred);
-    uint64_t act = __ballot(1);
-    uint32_t leader = (uint32_t)__builtin_ctzll(act);
-    uint32_t base = 0;
-    if (lane_id() == leader && m)
-        base = atomicAdd(ctr, (uint32_t)__builtin_popcountll(m));
-    base = __shfl(base, (int)leader);
-    return base + mbcnt(m);
-}
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t lo = __shfl_xor((uint32_t)v, o);
-        uint32_t hi = __shfl_xor((uint32_t)(v >> 32), o);
-        v += ((uint64_t)hi << 32) | lo;
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-    x ^= x >> 31;
-    x *= 0x7fb5d329728ea185ull;
-    x ^= x >> 27;
-    x *= 0x81dadef4bc2dd44dull;
-    x ^= x >> 33;
-    return x;
-}
 
 // The dedup table interleaves each slot's key and count (16 B per slot:
 // tab_key[2h], tab_cnt[2h] with tab_cnt = tab_key + 1), so an insert touches
@@ -101,45 +66,6 @@ __device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
     return -1;
 }
 
-constexpr uint64_t COL0 = 0x0101010101010101ull;
-constexpr uint64_t COL7 = 0x8080808080808080ull;
-
-__device__ __forceinline__ uint64_t grow4(uint64_t x) {
-    return x | (x << 8) | (x >> 8) | ((x << 1) & ~COL0) | ((x >> 1) & ~COL7);
-}
-
-// Connectivity pruning of a child's open window E (8 rows from the child's
-// row).  Rows selected by `seed` hold no piece yet (pieces reach at most
-// max_height-2 rows below the child's row), so every open region that meets
-// them is left alone; every other 4-connected open region is closed -- it
-// does not reach the window's last row -- and its area must be a multiple of
-// the gcd of the piece sizes and at least the smallest piece, or no tiling
-// can complete it.  (Shape::extend1 / connected / sml_island semantics,
-// src/Shape.cpp:81-123, applied to the live board at run time instead of the
-// reference's static table-time island test, src/frow.cpp:78-100.)
-__device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g,
-                                       uint32_t min_size) {
-    uint64_t x = E & seed;
-    for (;;) {
-        uint64_t y = grow4(x) & E;
-        if (y == x) break;
-        x = y;
-    }
-    uint64_t closed = E & ~x;
-    while (closed) {
-        uint64_t comp = closed & (~closed + 1ull);
-        for (;;) {
-            uint64_t y = grow4(comp) & closed;
-            if (y == comp) break;
-            comp = y;
-        }
-        uint32_t area = (uint32_t)__builtin_popcountll(comp);
-        if (area < min_size || area % g) return false;
-        closed &= ~comp;
-    }
-    return true;
-}
-
 __device__ __forceinline__ uint64_t fitmask(const uint64_t *s_fit,
                                             uint32_t rows, uint32_t c,
                                             uint64_t E) {
@@ -148,12 +74,6 @@ __device__ __forceinline__ uint64_t fitmask(const uint64_t *s_fit,
     for (uint32_t j = 1; j < rows; j++)
         m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
     return m;
-}
-
-__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
-    uint32_t lo = __shfl((uint32_t)v, (int)src);
-    uint32_t hi = __shfl((uint32_t)(v >> 32), (int)src);
-    return ((uint64_t)hi << 32) | lo;
 }
 
 // Candidates still available under the use counters `used`: counter fields
@@ -494,6 +414,16 @@ __global__ __launch_bounds__(256) void k_fill_table(unsigned long long *key,
     }
 }
 
+// 4-word slots {k0, k1, count, unused} of the multi-library batch.
+__global__ __launch_bounds__(256) void k_fill_table4(unsigned long long *tab, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        tab[4 * i] = EMPTY_KEY;
+        tab[4 * i + 1] = EMPTY_KEY;
+        tab[4 * i + 2] = 0;
+    }
+}
+
 __global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
     if (threadIdx.x < C32_N) c32[threadIdx.x] = 0;
     if (threadIdx.x < C64_N) c64[threadIdx.x] = 0;
@@ -507,6 +437,7 @@ __global__ void k_reset_work(unsigned long long *c64) { c64[C64_WORK] = 0; }
 // the counts into offsets, k_compact_write writes each range's states at its
 // offset, ordered within the range by wave ballot + mbcnt and an LDS prefix
 // over the block's waves.  No global atomics.
+template <int SW>  // table words per slot; the key's first word leads the slot
 __global__ __launch_bounds__(CMP_THREADS) void k_compact_count(
     const unsigned long long *tab_key, uint64_t m, uint64_t per_block,
     uint32_t *blk_cnt) {
@@ -515,7 +446,7 @@ __global__ __launch_bounds__(CMP_THREADS) void k_compact_count(
     const uint64_t hi = min(m, lo + per_block);
     uint32_t n = 0;
     for (uint64_t i = lo + threadIdx.x; i < hi; i += CMP_THREADS)
-        n += tab_key[2 * i] != EMPTY_KEY;
+        n += tab_key[SW * i] != EMPTY_KEY;
     for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
     if (lane_id() == 0) s_w[threadIdx.x >> 6] = n;
     __syncthreads();
@@ -550,10 +481,13 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t *blk_cnt, u
     if (threadIdx.x == 1023) c32[C32_OUT] = s_part[1023];
 }
 
+// SW = 2: {key, count} slots (tab_cnt = tab_key + 1); SW = 4: {k0, k1,
+// count, unused} slots (tab_cnt = tab_key + 2), k1 goes to out_k1.
+template <int SW>
 __global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
     const unsigned long long *tab_key, const unsigned long long *tab_cnt, uint64_t m,
-    uint64_t per_block, const uint32_t *blk_off, uint64_t *out_key, uint64_t *out_cnt,
-    uint32_t *slot_uid) {
+    uint64_t per_block, const uint32_t *blk_off, uint64_t *out_key, uint64_t *out_k1,
+    uint64_t *out_cnt, uint32_t *slot_uid) {
     __shared__ uint32_t s_w[CMP_THREADS / 64];
     const uint64_t lo = blockIdx.x * per_block;
     const uint64_t hi = min(m, lo + per_block);
@@ -561,7 +495,7 @@ __global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
     uint32_t base = blk_off[blockIdx.x];
     for (uint64_t i0 = lo; i0 < hi; i0 += CMP_THREADS) {
         const uint64_t i = i0 + threadIdx.x;
-        const uint64_t k = i < hi ? tab_key[2 * i] : EMPTY_KEY;
+        const uint64_t k = i < hi ? tab_key[SW * i] : EMPTY_KEY;
         const bool valid = k != EMPTY_KEY;
         const uint64_t bm = __ballot(valid);
         if (lane_id() == 0) s_w[wv] = (uint32_t)__builtin_popcountll(bm);
@@ -574,7 +508,8 @@ __global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
         if (valid) {
             const uint32_t o = base + before + mbcnt(bm);
             out_key[o] = k;
-            out_cnt[o] = tab_cnt[2 * i];
+            if (SW == 4) out_k1[o] = tab_key[SW * i + 1];
+            out_cnt[o] = tab_cnt[SW * i];
             if (slot_uid) slot_uid[i] = o;
         }
         base += total;
@@ -786,12 +721,32 @@ void launch_compact(const unsigned long long *tab_key,
     uint64_t per_block = 4096;
     while ((m + per_block - 1) / per_block > COMPACT_MAX_BLOCKS) per_block *= 2;
     const uint32_t nb = (uint32_t)((m + per_block - 1) / per_block);
-    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, m,
+    hipLaunchKernelGGL(k_compact_count<2>, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, m,
                        per_block, blk);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blk, nb,
                        blk + COMPACT_MAX_BLOCKS, c32);
-    hipLaunchKernelGGL(k_compact_write, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, tab_cnt,
-                       m, per_block, blk + COMPACT_MAX_BLOCKS, out_key, out_cnt, slot_uid);
+    hipLaunchKernelGGL(k_compact_write<2>, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, tab_cnt,
+                       m, per_block, blk + COMPACT_MAX_BLOCKS, out_key, nullptr, out_cnt,
+                       slot_uid);
+}
+
+void launch_fill_table4(unsigned long long *tab, uint64_t n, hipStream_t st) {
+    uint64_t blocks = std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), 65536);
+    hipLaunchKernelGGL(k_fill_table4, dim3((uint32_t)blocks), dim3(256), 0, st, tab, n);
+}
+
+void launch_compact4(const unsigned long long *tab, uint64_t m, uint64_t *out_k0,
+                     uint64_t *out_k1, uint64_t *out_cnt, uint32_t *c32, uint32_t *blk,
+                     hipStream_t st) {
+    uint64_t per_block = 2048;
+    while ((m + per_block - 1) / per_block > COMPACT_MAX_BLOCKS) per_block *= 2;
+    const uint32_t nb = (uint32_t)((m + per_block - 1) / per_block);
+    hipLaunchKernelGGL(k_compact_count<4>, dim3(nb), dim3(CMP_THREADS), 0, st, tab, m,
+                       per_block, blk);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blk, nb,
+                       blk + COMPACT_MAX_BLOCKS, c32);
+    hipLaunchKernelGGL(k_compact_write<4>, dim3(nb), dim3(CMP_THREADS), 0, st, tab, tab + 2, m,
+                       per_block, blk + COMPACT_MAX_BLOCKS, out_k0, out_k1, out_cnt, nullptr);
 }
 
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,

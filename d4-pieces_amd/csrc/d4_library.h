@@ -1,0 +1,48 @@
+// d4_library.h -- a piece library: counters, (piece, transform) candidates
+// and, for the single-library kernel, its LDS tables.
+#pragma once
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "d4_internal.h"
+
+using d4::MAX_CAND;
+using d4::MAX_LIB_CAND;
+using d4::MAX_PIECES;
+
+struct d4_library {
+    uint32_t n = 0, sym = 0, W = 0, maxh = 0, ncand = 0, used_bits = 0, fit_rows = 1;
+    uint32_t counts[MAX_PIECES] = {}, sizes[MAX_PIECES] = {};
+    uint32_t off[MAX_PIECES] = {}, fbits[MAX_PIECES] = {};
+    // (piece, transform) candidates in search order, up to MAX_LIB_CAND
+    uint8_t cpiece[MAX_LIB_CAND] = {}, ctrs[MAX_LIB_CAND] = {}, cminc[MAX_LIB_CAND] = {};
+    uint8_t clo[MAX_LIB_CAND] = {}, chi[MAX_LIB_CAND] = {};  // the piece's candidate range
+    uint64_t cnormal[MAX_LIB_CAND] = {};
+    uint32_t cinfo[MAX_LIB_CAND] = {};
+    uint32_t gcd = 1, min_size = 1;
+    uint32_t lds_limit = 65536;
+    int device = 0;
+    int n_cu = 1;
+    uint32_t ext_threads = 1024;  // extend workgroup size (autotuned)
+    uint32_t ext_refill = 16;     // idle lanes per refill round (autotuned)
+    // single-library tables (at most MAX_CAND candidates: one u64 mask),
+    // built and uploaded on first use by an engine
+    std::mutex fast_mu;
+    bool fast_ready = false;
+    uint64_t place[8][MAX_CAND] = {};
+    uint64_t placeinfo[8 * MAX_CAND * 2] = {};
+    uint64_t range[MAX_CAND] = {};
+    uint64_t prange[MAX_PIECES] = {};
+    uint64_t xavail[4 * 256] = {};
+    std::vector<uint64_t> fit;  // [8][fit_rows][256]
+    unsigned long long comp[256] = {};
+    uint64_t *d_fit = nullptr, *d_placeinfo = nullptr, *d_xavail = nullptr;
+    unsigned long long *d_comp = nullptr;
+
+    ~d4_library() {
+        for (void *p : {(void *)d_fit, (void *)d_placeinfo, (void *)d_xavail, (void *)d_comp})
+            if (p) (void)hipFree(p);
+    }
+};

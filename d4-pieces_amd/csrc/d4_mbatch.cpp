@@ -1,0 +1,532 @@
+// d4_mbatch.cpp -- batches of instances with their own piece libraries and
+// board widths: one frontier holds every instance's states (two-word keys
+// naming the instance), each row step is one launch of k_extend_multi over
+// all of them, and the per-library tables are packed into one device array.
+//
+// The reference runs one configuration per process: its piece library is a
+// process-wide global (g_nme / g_sym, src/frow.h:8-9) and
+// SearcherFactory::run iterates board configurations of that one library
+// (src/searcher.cpp:86-107).  A batch here mixes libraries and widths.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "d4_host.h"
+#include "d4_library.h"
+
+namespace d4 {
+namespace {
+
+// Device tables of one library (layout in d4_internal.h).
+void build_mlib(const d4_library &L, MLib &M) {
+    std::memset(&M, 0, sizeof M);
+    const uint32_t k = L.ncand;
+    auto placed = [&](uint32_t c, uint32_t j) -> uint64_t {
+        const uint64_t n = L.cnormal[j];
+        const uint32_t mc = L.cminc[j];
+        if (c < mc || c - mc + width8(n) > L.W) return 0;
+        return n << (c - mc);
+    };
+    for (uint32_t c = 0; c < 8; c++) {
+        const uint32_t base = 512u - (512u >> c);
+        for (uint32_t v = 0; v < (256u >> c); v++) {
+            const uint32_t b = v << c;
+            uint64_t m[2] = {0, 0};
+            for (uint32_t j = 0; j < k; j++) {
+                const uint64_t pl = placed(c, j);
+                if (!pl || ((uint32_t)(pl & 0xffu) & ~b)) continue;
+                m[j >> 6] |= 1ull << (j & 63u);
+            }
+            M.fit[base + v][0] = m[0];
+            M.fit[base + v][1] = m[1];
+        }
+    }
+    for (uint32_t j = 0; j < k; j++) {
+        M.cand[j][0] = L.cnormal[j];
+        M.cand[j][1] = (uint64_t)(L.cinfo[j] & 0xffffu) | ((uint64_t)(L.cinfo[j] >> 16) << 16) |
+                       ((uint64_t)L.cminc[j] << 32) | ((uint64_t)L.clo[j] << 40) |
+                       ((uint64_t)L.chi[j] << 48);
+    }
+    for (uint32_t i = 0; i < L.n; i++) {
+        uint64_t lo = 0, hi = 0;
+        for (uint32_t j = 0; j < k; j++)
+            if (L.cpiece[j] == i) {
+                lo = L.clo[j];
+                hi = L.chi[j];
+                break;
+            }
+        M.piece[i] = (uint64_t)L.off[i] | ((uint64_t)((1u << L.fbits[i]) - 1u) << 8) |
+                     ((uint64_t)L.counts[i] << 16) | (lo << 32) | (hi << 40);
+    }
+    const uint32_t occ_bits = 8 * (L.maxh > 1 ? L.maxh - 1 : 0);
+    M.occ_mask = occ_bits ? (1ull << occ_bits) - 1ull : 0;
+    M.seed_mask = L.maxh >= 2 ? ~((1ull << (8 * (L.maxh - 1))) - 1ull) : ~0ull;
+    M.gcd = L.gcd;
+    M.min_size = L.min_size;
+    M.n_pieces = L.n;
+    M.n_cand = k;
+}
+
+}  // namespace
+}  // namespace d4
+
+using namespace d4;
+
+struct d4_mbatch {
+    uint32_t N = 0, flags = 0, max_h = 0, bw_stride = 0;
+    uint32_t threads = 256, refill = 16;
+    int device = 0, n_cu = 1;
+    std::vector<uint32_t> heights;
+    std::vector<unsigned long long> counts;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evx0 = nullptr, evx1 = nullptr, evc0 = nullptr;
+    DevBuf<MLib> libs;
+    DevBuf<uint32_t> inst_lib, hgt;
+    DevBuf<uint64_t> bw;
+    DevBuf<unsigned long long> result, result_save;
+    DevBuf<uint64_t> f0, f1, fc, n0, n1, nc;
+    DevBuf<unsigned long long> tab;
+    DevBuf<uint32_t> c32, blk;
+    DevBuf<unsigned long long> c64;
+    uint64_t n_front = 0;
+    double grow_unique = 1.0, grow_children = 1.0;
+    // bounded memory: frontier halves parked for later (depth first)
+    struct Part {
+        uint32_t row = 0;
+        uint64_t n = 0;
+        double grow_unique = 1.0, grow_children = 1.0;
+        DevBuf<uint64_t> k0, k1, c;
+    };
+    std::vector<std::unique_ptr<Part>> parked;
+    uint64_t parked_bytes = 0, mem_cap = 0;
+    // per instance: largest frontier share seen, and the state budget beyond
+    // which an instance leaves the search unfinished (0 = no budget)
+    uint64_t budget = 0;
+    std::vector<uint64_t> peak_states;
+    std::vector<uint8_t> h_dropped;
+    std::vector<unsigned long long> h_hist;
+    DevBuf<uint8_t> dropped;
+    DevBuf<unsigned long long> hist;
+    static constexpr uint64_t kExactCountMax = 1u << 16;
+    bool done = false;
+    bool trace = getenv("D4_MBATCH_TRACE") != nullptr;  // per-row lines on stderr
+    d4_stats stats{};
+
+    ~d4_mbatch() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (hipEvent_t e : {ev0, ev1, evx0, evx1, evc0})
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void roots() {
+        // one root state per instance: nothing placed, counters zero
+        std::vector<uint64_t> k0(N), k1(N, 0), c(N, 1);
+        for (uint32_t i = 0; i < N; i++) k0[i] = (uint64_t)i << MKEY_INST_SHIFT;
+        f0.reserve(N);
+        f1.reserve(N);
+        fc.reserve(N);
+        HIPCHK(hipMemcpyAsync(f0.p, k0.data(), 8ull * N, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(f1.p, k1.data(), 8ull * N, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(fc.p, c.data(), 8ull * N, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemsetAsync(result.p, 0, 8ull * N, st));
+        HIPCHK(hipStreamSynchronize(st));
+        n_front = N;
+        grow_unique = grow_children = 1.0;
+    }
+
+    MultiArgs args(uint32_t row, uint64_t tab_m) const {
+        MultiArgs a{};
+        a.in_k0 = f0.p;
+        a.in_k1 = f1.p;
+        a.in_cnt = fc.p;
+        a.n_in = (uint32_t)n_front;
+        a.row = row;
+        a.split = n_front <= 1024 ? 2u : n_front <= (1u << 16) ? 1u : 0u;
+        a.refill = refill;
+        a.dropped = budget ? dropped.p : nullptr;
+        a.libs = libs.p;
+        a.inst_lib = inst_lib.p;
+        a.bw = bw.p;
+        a.bw_stride = bw_stride;
+        a.hgt = hgt.p;
+        a.result = result.p;
+        a.tab = tab.p;
+        a.tab_mask = tab_m - 1;
+        a.max_probe = 256;
+        a.c32 = c32.p;
+        a.c64 = c64.p;
+        return a;
+    }
+
+    // Children the frontier will emit after pruning (search only).
+    uint64_t count_children(uint32_t row) {
+        launch_reset_counters(c32.p, c64.p, st);
+        launch_extend_multi(args(row, 1), !(flags & D4_FLAG_NO_PRUNE), true, threads, n_cu, st);
+        HIPCHK(hipGetLastError());
+        unsigned long long h64[C64_N];
+        HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        stats.count_passes++;
+        return h64[C64_CHILDREN];
+    }
+
+    // Device bytes of a row step with a table of tab_m slots and u children.
+    uint64_t need(uint64_t tab_m, uint64_t u) const {
+        return 32ull * tab_m + 24ull * u + 24ull * n_front + parked_bytes;
+    }
+
+    // Park the upper half of the frontier (resumed at this row later).
+    void split(uint32_t row) {
+        const uint64_t keep = n_front / 2, m = n_front - keep;
+        auto p = std::make_unique<Part>();
+        p->row = row;
+        p->n = m;
+        p->grow_unique = grow_unique;
+        p->grow_children = grow_children;
+        p->k0.reserve(m);
+        p->k1.reserve(m);
+        p->c.reserve(m);
+        HIPCHK(hipMemcpyAsync(p->k0.p, f0.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(p->k1.p, f1.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(p->c.p, fc.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        parked_bytes += 24 * m;
+        parked.push_back(std::move(p));
+        n_front = keep;
+        stats.splits++;
+        if (trace)
+            fprintf(stderr, "mbatch row %u: split, %llu records parked\n", row,
+                    (unsigned long long)m);
+    }
+
+    // Resume the most recently parked part; returns its row.
+    uint32_t resume() {
+        std::unique_ptr<Part> p = std::move(parked.back());
+        parked.pop_back();
+        parked_bytes -= 24 * p->n;
+        f0.swap(p->k0);
+        f1.swap(p->k1);
+        fc.swap(p->c);
+        n_front = p->n;
+        grow_unique = p->grow_unique;
+        grow_children = p->grow_children;
+        return p->row;
+    }
+
+    // One row step over the whole frontier.  Returns false, with nothing
+    // changed, when it would not fit in the memory cap.
+    bool row_step(uint32_t row) {
+        if (n_front > 0xffffffffull) throw Error(D4_ERANGE, "frontier exceeds 2^32 records");
+        const bool prune = !(flags & D4_FLAG_NO_PRUNE);
+        // Table size: unique children <= children.  A small frontier gets an
+        // exact children count from a search-only pass; a large one an
+        // estimate from the previous step's ratios (first rows grow by
+        // thousands per record, later ones by a few), with a rerun at 4x the
+        // size if the table still overflows.
+        uint64_t est_u;
+        if (n_front <= kExactCountMax) {
+            est_u = count_children(row);
+        } else {
+            const uint64_t est_c =
+                (uint64_t)((double)n_front * std::min(std::max(grow_children, 1.0), 64.0) * 1.5) +
+                4096;
+            est_u = std::min<uint64_t>(
+                est_c, (uint64_t)((double)n_front * std::max(grow_unique, 0.25) * 1.5) + 4096);
+        }
+        uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
+        uint32_t h32[C32_N];
+        unsigned long long h64[C64_N];
+        for (;;) {
+            if (n_front >= 2 && (tab_m > (1ull << 33) || need(tab_m, std::min(est_u, tab_m)) > mem_cap))
+                return false;
+            if (tab_m > (1ull << 33)) throw Error(D4_ERANGE, "dedup table exceeds 2^33 slots");
+            try {
+                tab.reserve(4 * tab_m);
+            } catch (const Error &e) {
+                if (e.code != D4_ENOMEM || n_front < 2) throw;
+                return false;
+            }
+            HIPCHK(hipEventRecord(ev0, st));
+            HIPCHK(hipMemcpyAsync(result_save.p, result.p, 8ull * N, hipMemcpyDeviceToDevice, st));
+            launch_fill_table4(tab.p, tab_m, st);
+            launch_reset_counters(c32.p, c64.p, st);
+            HIPCHK(hipEventRecord(evx0, st));
+            if (n_front) {
+                launch_extend_multi(args(row, tab_m), prune, false, threads, n_cu, st);
+                HIPCHK(hipGetLastError());
+            }
+            HIPCHK(hipEventRecord(evx1, st));
+            HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            float xms = 0;
+            if (n_front) {
+                HIPCHK(hipEventElapsedTime(&xms, evx0, evx1));
+                stats.extend_ms += xms;
+                stats.extend_launches++;
+                stats.extend_bytes += 24ull * n_front + 24ull * h64[C64_CHILDREN];
+            }
+            if (trace)
+                fprintf(stderr,
+                        "mbatch row %u: n_front %llu table %llu children %llu new %llu "
+                        "extend %.3f ms%s\n",
+                        row, (unsigned long long)n_front, (unsigned long long)tab_m,
+                        (unsigned long long)h64[C64_CHILDREN], (unsigned long long)h64[C64_NEW],
+                        xms, h32[C32_OVERFLOW] ? " overflow" : "");
+            if (!h32[C32_OVERFLOW]) break;
+            stats.retries++;
+            HIPCHK(hipMemcpy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice));
+            tab_m *= 4;
+        }
+        const uint64_t U = h64[C64_NEW];
+        try {
+            n0.reserve(U);
+            n1.reserve(U);
+            nc.reserve(U);
+        } catch (const Error &e) {
+            if (e.code != D4_ENOMEM || n_front < 2) throw;
+            HIPCHK(hipMemcpy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice));
+            return false;
+        }
+        stats.children += h64[C64_CHILDREN];
+        stats.pruned += h64[C64_PRUNED];
+        stats.records_extended += n_front;
+        stats.unique += U;
+        if (n_front) {
+            grow_unique = std::max(1.0, (double)U / (double)n_front);
+            grow_children = std::max(1.0, (double)h64[C64_CHILDREN] / (double)n_front);
+        }
+        blk.reserve(2 * COMPACT_MAX_BLOCKS);
+        HIPCHK(hipEventRecord(evc0, st));
+        launch_compact4(tab.p, tab_m, n0.p, n1.p, nc.p, c32.p, blk.p, st);
+        HIPCHK(hipEventRecord(ev1, st));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        float ms = 0, cms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        HIPCHK(hipEventElapsedTime(&cms, evc0, ev1));
+        stats.gpu_ms += ms;
+        stats.compact_ms += cms;
+        stats.compact_launches++;
+        stats.compact_bytes += 8ull * tab_m + 32ull * U;
+        f0.swap(n0);
+        f1.swap(n1);
+        fc.swap(nc);
+        n_front = U;
+        if (!U) {
+            f0.reserve(1);
+            f1.reserve(1);
+            fc.reserve(1);
+        }
+        stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, U);
+        stats.rows_done++;
+        if (U) account(row);
+        return true;
+    }
+
+    // Per-instance frontier shares of the new frontier; instances over the
+    // budget are dropped from the next step on.
+    void account(uint32_t row) {
+        HIPCHK(hipMemsetAsync(hist.p, 0, 8ull * N, st));
+        launch_inst_hist(f0.p, n_front, N, hist.p, n_cu, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h_hist.data(), hist.p, 8ull * N, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        bool changed = false;
+        for (uint32_t i = 0; i < N; i++) {
+            peak_states[i] = std::max<uint64_t>(peak_states[i], h_hist[i]);
+            if (budget && h_hist[i] > budget && !h_dropped[i]) {
+                h_dropped[i] = 1;
+                changed = true;
+                if (trace)
+                    fprintf(stderr, "mbatch row %u: instance %u dropped (%llu states)\n", row, i,
+                            (unsigned long long)h_hist[i]);
+            }
+        }
+        if (changed)
+            HIPCHK(hipMemcpy(dropped.p, h_dropped.data(), N, hipMemcpyHostToDevice));
+    }
+
+    void run() {
+        stats = d4_stats{};
+        parked.clear();
+        parked_bytes = 0;
+        peak_states.assign(N, 0);
+        h_dropped.assign(N, 0);
+        h_hist.assign(N, 0);
+        dropped.reserve(N);
+        hist.reserve(N);
+        HIPCHK(hipMemcpy(dropped.p, h_dropped.data(), N, hipMemcpyHostToDevice));
+        roots();
+        // step r extends every instance's row r; an instance of height h is
+        // absorbed into its count at step h.  Parts of a split frontier run
+        // to the end one after the other; their counts add up per instance.
+        uint32_t r = 0;
+        for (;;) {
+            while (n_front && r <= max_h) {
+                if (row_step(r)) {
+                    r++;
+                    continue;
+                }
+                if (n_front < 2) throw Error(D4_ENOMEM, "one state's row step exceeds the memory cap");
+                split(r);
+            }
+            if (parked.empty()) break;
+            r = resume();
+        }
+        counts.assign(N, 0);
+        HIPCHK(hipMemcpy(counts.data(), result.p, 8ull * N, hipMemcpyDeviceToHost));
+        done = true;
+    }
+};
+
+extern "C" {
+
+int d4_mbatch_create(uint32_t n_inst, const d4_library *const *libs, const uint8_t *boards,
+                     uint32_t stride, const uint32_t *heights, uint32_t flags, d4_mbatch **out) {
+    return guard([&] {
+        if (!out || !libs || !boards || !heights) throw Error(D4_EINVAL, "null argument");
+        *out = nullptr;
+        if (!n_inst || n_inst > MAX_MULTI_INST)
+            throw Error(D4_EINVAL, "need 1..32768 instances");
+        if (flags & ~(uint32_t)D4_FLAG_NO_PRUNE) throw Error(D4_EINVAL, "unknown flags");
+        auto B = std::make_unique<d4_mbatch>();
+        B->N = n_inst;
+        B->flags = flags;
+        B->heights.assign(heights, heights + n_inst);
+        std::map<const d4_library *, uint32_t> index;
+        std::vector<const d4_library *> uniq;
+        std::vector<uint32_t> il(n_inst);
+        for (uint32_t i = 0; i < n_inst; i++) {
+            const d4_library *L = libs[i];
+            if (!L) throw Error(D4_EINVAL, "null library");
+            if (heights[i] > (uint32_t)MAX_ROWS) throw Error(D4_EINVAL, "board taller than 64 rows");
+            if (stride < heights[i]) throw Error(D4_EINVAL, "stride shorter than a board");
+            const uint32_t occ_bits = 8 * (L->maxh > 1 ? L->maxh - 1 : 0);
+            if (occ_bits > MKEY_INST_SHIFT) throw Error(D4_EINVAL, "occupancy exceeds 48 bits");
+            const uint8_t wmask = (uint8_t)((1u << L->W) - 1u);
+            for (uint32_t r = 0; r < heights[i]; r++)
+                if (boards[(size_t)i * stride + r] & ~wmask)
+                    throw Error(D4_EINVAL, "board cell outside the library's width");
+            auto it = index.find(L);
+            if (it == index.end()) {
+                if (!uniq.empty() && L->device != uniq[0]->device)
+                    throw Error(D4_EINVAL, "libraries on different devices");
+                it = index.emplace(L, (uint32_t)uniq.size()).first;
+                uniq.push_back(L);
+            }
+            il[i] = it->second;
+            B->max_h = std::max(B->max_h, heights[i]);
+        }
+        B->device = uniq[0]->device;
+        B->n_cu = uniq[0]->n_cu;
+        HIPCHK(hipSetDevice(B->device));
+        HIPCHK(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking));
+        for (hipEvent_t *e : {&B->ev0, &B->ev1, &B->evx0, &B->evx1, &B->evc0})
+            HIPCHK(hipEventCreate(e));
+        std::vector<MLib> ml(uniq.size());
+        for (size_t j = 0; j < uniq.size(); j++) build_mlib(*uniq[j], ml[j]);
+        B->libs.reserve(ml.size());
+        HIPCHK(hipMemcpyAsync(B->libs.p, ml.data(), sizeof(MLib) * ml.size(),
+                              hipMemcpyHostToDevice, B->st));
+        B->inst_lib.reserve(n_inst);
+        HIPCHK(hipMemcpyAsync(B->inst_lib.p, il.data(), 4ull * n_inst, hipMemcpyHostToDevice,
+                              B->st));
+        B->hgt.reserve(n_inst);
+        HIPCHK(hipMemcpyAsync(B->hgt.p, heights, 4ull * n_inst, hipMemcpyHostToDevice, B->st));
+        // board windows: bw[i][r] = rows r .. r+7 of instance i (0 past the
+        // bottom), for r = 0 .. max_h + 1
+        B->bw_stride = B->max_h + 2;
+        std::vector<uint64_t> w((size_t)n_inst * B->bw_stride, 0);
+        for (uint32_t i = 0; i < n_inst; i++)
+            for (uint32_t r = 0; r < B->bw_stride; r++) {
+                uint64_t v = 0;
+                for (uint32_t j = 0; j < 8; j++)
+                    if (r + j < heights[i])
+                        v |= (uint64_t)boards[(size_t)i * stride + r + j] << (8 * j);
+                w[(size_t)i * B->bw_stride + r] = v;
+            }
+        B->bw.reserve(w.size());
+        HIPCHK(hipMemcpyAsync(B->bw.p, w.data(), 8 * w.size(), hipMemcpyHostToDevice, B->st));
+        B->result.reserve(n_inst);
+        B->result_save.reserve(n_inst);
+        B->c32.reserve(C32_N);
+        B->c64.reserve(C64_N);
+        HIPCHK(hipStreamSynchronize(B->st));
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        B->mem_cap = (uint64_t)(0.8 * (double)free_b);
+        *out = B.release();
+    });
+}
+
+int d4_mbatch_set_launch(d4_mbatch *b, uint32_t threads, uint32_t refill) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        if (threads != 256 && threads != 512 && threads != 1024)
+            throw Error(D4_EINVAL, "threads must be 256, 512 or 1024");
+        if (refill < 1 || refill > 64) throw Error(D4_EINVAL, "refill must be 1..64");
+        b->threads = threads;
+        b->refill = refill;
+    });
+}
+
+int d4_mbatch_set_memory(d4_mbatch *b, uint64_t bytes) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        if (bytes < (1u << 20)) throw Error(D4_EINVAL, "memory cap below 1 MiB");
+        b->mem_cap = bytes;
+    });
+}
+
+int d4_mbatch_set_budget(d4_mbatch *b, uint64_t max_states) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        b->budget = max_states;
+    });
+}
+
+int d4_mbatch_instances(const d4_mbatch *b, uint64_t *peak_states, uint8_t *finished) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        if (!b->done) throw Error(D4_ESTATE, "batch not run");
+        for (uint32_t i = 0; i < b->N; i++) {
+            if (peak_states) peak_states[i] = b->peak_states[i];
+            if (finished) finished[i] = b->h_dropped[i] ? 0 : 1;
+        }
+    });
+}
+
+int d4_mbatch_run(d4_mbatch *b) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        HIPCHK(hipSetDevice(b->device));
+        b->run();
+    });
+}
+
+int d4_mbatch_counts(const d4_mbatch *b, uint64_t *counts) {
+    return guard([&] {
+        if (!b || !counts) throw Error(D4_EINVAL, "null argument");
+        if (!b->done) throw Error(D4_ESTATE, "batch not run");
+        for (uint32_t i = 0; i < b->N; i++) counts[i] = b->counts[i];
+    });
+}
+
+int d4_mbatch_stats(const d4_mbatch *b, d4_stats *st) {
+    return guard([&] {
+        if (!b || !st) throw Error(D4_EINVAL, "null argument");
+        *st = b->stats;
+    });
+}
+
+void d4_mbatch_destroy(d4_mbatch *b) { delete b; }
+
+}  // extern "C"

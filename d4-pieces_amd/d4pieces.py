@@ -11,6 +11,8 @@ benchmarks read like the reference's own driver code:
   free_polyominoes <- known_shapes[n]           src/known1.cpp:3-14
   Batch            <- SearcherFactory::run over many board configs,
                       src/searcher.cpp:86-107, as one shared device frontier
+  MultiBatch       <- the same with a piece library per instance (the
+                      reference's library is a process global, src/frow.h:8-9)
 
 There is no fallback: if libd4.so is missing or no HIP device is visible the
 calls raise.
@@ -52,6 +54,7 @@ class Stats(ctypes.Structure):
         ("compact_launches", ctypes.c_uint64),
         ("compact_bytes", ctypes.c_uint64),
         ("count_passes", ctypes.c_uint64),
+        ("splits", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -105,6 +108,17 @@ SIGNATURES = [
     ("d4_batch_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("d4_batch_solutions", ctypes.c_int,
      [_P, ctypes.c_uint32, ctypes.c_int, _U8P, ctypes.c_size_t, _SZP, _U64P]),
+    ("d4_mbatch_create", ctypes.c_int,
+     [ctypes.c_uint32, ctypes.POINTER(_P), _U8P, ctypes.c_uint32, _U32P, ctypes.c_uint32,
+      ctypes.POINTER(_P)]),
+    ("d4_mbatch_destroy", None, [_P]),
+    ("d4_mbatch_set_launch", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    ("d4_mbatch_set_memory", ctypes.c_int, [_P, ctypes.c_uint64]),
+    ("d4_mbatch_set_budget", ctypes.c_int, [_P, ctypes.c_uint64]),
+    ("d4_mbatch_run", ctypes.c_int, [_P]),
+    ("d4_mbatch_instances", ctypes.c_int, [_P, _U64P, _U8P]),
+    ("d4_mbatch_counts", ctypes.c_int, [_P, _U64P]),
+    ("d4_mbatch_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("d4_server_create", ctypes.c_int,
      [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_P)]),
     ("d4_server_destroy", None, [_P]),
@@ -352,6 +366,62 @@ class Batch:
     def stats(self) -> dict:
         st = Stats()
         _check(load().d4_batch_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+
+class MultiBatch:
+    """Instances with their own piece libraries and board widths, advanced
+    together in one frontier (d4_mbatch_*); count mode."""
+
+    def __init__(self, libraries: Sequence[PieceLibrary], boards: Sequence[Sequence[int]],
+                 prune: bool = True):
+        lib = load()
+        if len(libraries) != len(boards):
+            raise ValueError("one library per board")
+        self.libraries = list(libraries)  # keep the handles alive
+        self.n = len(boards)
+        flat, stride, heights = _flatten(boards)
+        handles = (_P * self.n)(*[L._h for L in libraries])
+        h = _P()
+        _check(lib.d4_mbatch_create(self.n, handles, _u8arr(flat), stride, _u32arr(heights),
+                                    0 if prune else FLAG_NO_PRUNE, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.d4_mbatch_destroy(self._h)
+            self._h = None
+
+    def set_launch(self, threads: int, refill: int = 16) -> None:
+        _check(load().d4_mbatch_set_launch(self._h, threads, refill))
+
+    def set_memory(self, nbytes: int) -> None:
+        _check(load().d4_mbatch_set_memory(self._h, nbytes))
+
+    def set_budget(self, max_states: int) -> None:
+        """States one instance may hold in a frontier before it is dropped
+        (reported unfinished); 0 = no limit."""
+        _check(load().d4_mbatch_set_budget(self._h, max_states))
+
+    def instances(self) -> Tuple[List[int], List[bool]]:
+        """(peak states per instance, finished per instance)."""
+        peak = (ctypes.c_uint64 * self.n)()
+        fin = (ctypes.c_uint8 * self.n)()
+        _check(load().d4_mbatch_instances(self._h, peak, fin))
+        return list(peak), [bool(x) for x in fin]
+
+    def run(self) -> "MultiBatch":
+        _check(load().d4_mbatch_run(self._h))
+        return self
+
+    def counts(self) -> List[int]:
+        buf = (ctypes.c_uint64 * self.n)()
+        _check(load().d4_mbatch_counts(self._h, buf))
+        return list(buf)
+
+    def stats(self) -> dict:
+        st = Stats()
+        _check(load().d4_mbatch_stats(self._h, ctypes.byref(st)))
         return st.as_dict()
 
 

@@ -11,6 +11,8 @@ Workloads (boards are lists of 8-bit row masks, bit c = column c open):
   seeded_8x8(n, seed) 8x8 with 4 blocked cells, every open region's area a
                       multiple of 5 (first n accepted draws)
   large_8x15()        12 free pentominoes x2 on 8 wide x 15 rows
+  uneven_instances()  4096 puzzles, each with its own 6-14 polyominoes of
+                      sizes 3-6 (multiplicity 1-2) and a 4-8 wide board
   request_stream(...) Poisson arrivals of 8x8 boards with 30-60 open cells in
                       one region (area a multiple of 5), 70% first-solution
 """
@@ -156,6 +158,45 @@ def grown_region_8x8(rng: MT19937_64, free: int) -> List[int]:
     for v in cells:
         rows[v // 8] |= 1 << (v % 8)
     return rows
+
+
+def uneven_instances(n: int = 4096, seed: int = 3):
+    """n puzzles of uneven size: list of (shapes, counts, rows, width).
+
+    Piece set: k ~ U{6..14} distinct shapes drawn uniformly (rejecting
+    repeats) from the 54 free polyominoes of sizes 3..6 (in size, then
+    canonical-value order), kept in that order; multiplicity U{1, 2} each.
+    Board: width W ~ U{4..8}, H = ceil(A / W) rows for the total piece area
+    A; the W*H - A (< W) surplus cells, drawn uniformly and distinct over the
+    row-major cell index r*W + c, are blocked, so the open area equals A and a
+    tiling uses every piece its full multiplicity."""
+    from d4pieces import free_polyominoes
+    pool = [s for k in (3, 4, 5, 6) for s in free_polyominoes(k)]
+    rng = MT19937_64(seed)
+    out = []
+    for _ in range(n):
+        k = rng.randint(6, 14)
+        idx: List[int] = []
+        while len(idx) < k:
+            v = rng.randint(0, len(pool) - 1)
+            if v not in idx:
+                idx.append(v)
+        idx.sort()
+        shapes = [pool[i] for i in idx]
+        counts = [rng.randint(1, 2) for _ in shapes]
+        area = sum(c * bin(s).count("1") for s, c in zip(shapes, counts))
+        W = rng.randint(4, 8)
+        H = -(-area // W)
+        blocked: List[int] = []
+        while len(blocked) < W * H - area:
+            v = rng.randint(0, W * H - 1)
+            if v not in blocked:
+                blocked.append(v)
+        rows = [(1 << W) - 1] * H
+        for v in blocked:
+            rows[v // W] &= ~(1 << (v % W)) & 0xFF
+        out.append((shapes, counts, rows, W))
+    return out
 
 
 def request_stream(n: int, rate: float, seed: int = 4):

@@ -72,6 +72,7 @@ typedef struct d4_stats {
     uint64_t compact_bytes;    /* 8 per table slot scanned + 24 per unique
                                   state gathered and written                */
     uint64_t count_passes;     /* search-only passes sizing the dedup table */
+    uint64_t splits;           /* frontier halves parked to stay in the memory cap */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -202,6 +203,43 @@ int d4_batch_stats(const d4_batch *b, d4_stats *st);
 /* Solutions of instance i (D4_MODE_ENUM), format as d4_searcher_solutions. */
 int d4_batch_solutions(const d4_batch *b, uint32_t i, int single,
                        uint8_t *out, size_t cap, size_t *len, uint64_t *nsol);
+
+/* ---------------------------------------------------------------------- */
+/* multi-library batches: every instance has its own piece set and board   */
+/* width (no reference counterpart: the reference's library is a process   */
+/* global, g_nme / g_sym, src/frow.h:8-9, set once per run of              */
+/* SearcherFactory::run, src/searcher.cpp:86-107)                          */
+/* ---------------------------------------------------------------------- */
+
+typedef struct d4_mbatch d4_mbatch;
+
+/* n_inst instances; libs[i] = instance i's library (libraries may repeat and
+ * may hold up to 128 (piece, transform) pairs; rows are within libs[i]'s
+ * width), boards[i*stride + r] = row r, heights[i] = row count.  Count mode
+ * (all tilings).  At most 32768 instances. */
+int d4_mbatch_create(uint32_t n_inst, const d4_library *const *libs,
+                     const uint8_t *boards, uint32_t stride,
+                     const uint32_t *heights, uint32_t flags, d4_mbatch **out);
+void d4_mbatch_destroy(d4_mbatch *b);
+/* Extend workgroup size (256, 512, 1024) and refill threshold (1..64). */
+int d4_mbatch_set_launch(d4_mbatch *b, uint32_t threads, uint32_t refill);
+/* Device bytes the frontier, its dedup table and parked frontier parts may
+ * take (default: 80% of the free device memory when the batch is created).
+ * A row step that would exceed it parks half of the frontier and finishes
+ * the other half first; counts are unchanged. */
+int d4_mbatch_set_memory(d4_mbatch *b, uint64_t bytes);
+/* States one instance may hold in a frontier (0, the default: no limit).
+ * An instance that exceeds it leaves the search unfinished: its count is 0
+ * and d4_mbatch_instances reports it. */
+int d4_mbatch_set_budget(d4_mbatch *b, uint64_t max_states);
+int d4_mbatch_run(d4_mbatch *b);
+/* counts[i] = number of tilings of instance i (valid after d4_mbatch_run). */
+int d4_mbatch_counts(const d4_mbatch *b, uint64_t *counts);
+/* Per instance: the largest number of its states in one frontier, and
+ * whether its count is complete (1) or it exceeded the budget (0). */
+int d4_mbatch_instances(const d4_mbatch *b, uint64_t *peak_states,
+                        uint8_t *finished);
+int d4_mbatch_stats(const d4_mbatch *b, d4_stats *st);
 
 /* ---------------------------------------------------------------------- */
 /* serving: continuous batching of requests (no reference counterpart)     */

@@ -158,6 +158,9 @@ typedef struct {
     size_t cap, len;
     uint8_t hist[256][4];
     unsigned nhist;
+    /* node budget (0 = none) */
+    uint64_t nodes, max_nodes;
+    int aborted;
 } ctx_t;
 
 static uint64_t get_win(const ctx_t *c, unsigned row) {
@@ -202,6 +205,10 @@ static void emit_solution(ctx_t *c) {
 /* one DFS level of solve / solve_count (Piece.cpp:66-139); returns nonzero
  * when a single-mode search has found its solution. */
 static int dfs(ctx_t *c, unsigned first_row) {
+    if (c->max_nodes && ++c->nodes > c->max_nodes) {
+        c->aborted = 1;
+        return 1;
+    }
     if (c->open_cnt > c->max_tiles) return 0;
     if (!c->open_cnt) {
         c->nsol++;
@@ -282,6 +289,22 @@ uint64_t or_solve_count(const uint64_t *shapes, const uint32_t *counts,
     if (!setup(c, shapes, counts, n, sym, rows, H)) {
         dfs(c, 0);
         r = c->nsol;
+    }
+    free(c);
+    return r;
+}
+
+uint64_t or_solve_count_budget(const uint64_t *shapes, const uint32_t *counts,
+                               uint32_t n, uint32_t sym, const uint8_t *rows,
+                               uint32_t H, uint64_t max_nodes, int *complete) {
+    ctx_t *c = malloc(sizeof *c);
+    uint64_t r = 0;
+    *complete = 0;
+    if (!setup(c, shapes, counts, n, sym, rows, H)) {
+        c->max_nodes = max_nodes;
+        dfs(c, 0);
+        r = c->nsol;
+        *complete = !c->aborted;
     }
     free(c);
     return r;

@@ -77,6 +77,12 @@ uint64_t or_solve_count(const uint64_t *shapes, const uint32_t *counts,
                         uint32_t n, uint32_t sym, const uint8_t *rows,
                         uint32_t H);
 
+/* or_solve_count with a budget of DFS nodes (calls of the recursion):
+ * *complete = 0 when the budget ran out (the count is then partial). */
+uint64_t or_solve_count_budget(const uint64_t *shapes, const uint32_t *counts,
+                               uint32_t n, uint32_t sym, const uint8_t *rows,
+                               uint32_t H, uint64_t max_nodes, int *complete);
+
 /* solve (Piece.cpp:66-102).  Each solution is written to `out` as
  *   u8 nsteps, then nsteps x {u8 piece_id, u8 trs_id, u8 y, u8 x}
  * in the DFS order (= anchor order), where (y, x) is the translation of the

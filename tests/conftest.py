@@ -41,6 +41,10 @@ class Oracle:
                                ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
                                ctypes.c_void_p, ctypes.c_size_t,
                                ctypes.POINTER(ctypes.c_size_t)]
+        L.or_solve_count_budget.restype = ctypes.c_uint64
+        L.or_solve_count_budget.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                            ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                                            ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
         L.or_step_matrix.argtypes = [ctypes.c_uint, ctypes.c_void_p]
         L.or_parse_R.argtypes = [ctypes.c_uint, ctypes.c_void_p, ctypes.c_uint8,
                                  ctypes.POINTER(ctypes.c_uint64),
@@ -91,6 +95,14 @@ class Oracle:
     def count(self, shapes, counts, rows, sym=0xFF):
         sh, cn, rw = self._args(shapes, counts, rows)
         return self.L.or_solve_count(sh, cn, len(shapes), sym, rw, len(rows))
+
+    def count_budget(self, shapes, counts, rows, max_nodes, sym=0xFF):
+        """(count, complete): complete is False when max_nodes ran out."""
+        sh, cn, rw = self._args(shapes, counts, rows)
+        done = ctypes.c_int()
+        n = self.L.or_solve_count_budget(sh, cn, len(shapes), sym, rw, len(rows), max_nodes,
+                                         ctypes.byref(done))
+        return n, bool(done.value)
 
     def solve(self, shapes, counts, rows, single=False, sym=0xFF):
         sh, cn, rw = self._args(shapes, counts, rows)

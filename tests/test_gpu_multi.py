@@ -1,0 +1,198 @@
+"""Multi-library batches (d4_mbatch_*): every instance with its own piece set
+and board width in one frontier, against the oracle (restated
+src/Piece.cpp solve_count), the single-library path, and board symmetries."""
+import json
+import os
+import random
+
+import pytest
+
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rect(w, h, blocked=()):
+    rows = [(1 << w) - 1] * h
+    for r, c in blocked:
+        rows[r] &= 0xFF ^ (1 << c)
+    return rows
+
+
+def mirror(rows, w):
+    """Left-right reflection within width w."""
+    return [int(format(r, f"0{w}b")[::-1], 2) for r in rows]
+
+
+@pytest.fixture(scope="module")
+def uneven(gpu):
+    return workloads.uneven_instances(4096, seed=3)
+
+
+@pytest.fixture(scope="module")
+def uneven_fixture():
+    return json.load(open(os.path.join(GOLDEN, "uneven_4096_seed3.json")))
+
+
+def run_multi(gpu, insts, prune=True, cache=None):
+    cache = {} if cache is None else cache
+    libs = []
+    for shapes, counts, rows, w in insts:
+        key = (tuple(shapes), tuple(counts), w)
+        if key not in cache:
+            cache[key] = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+        libs.append(cache[key])
+    b = gpu.MultiBatch(libs, [x[2] for x in insts], prune=prune)
+    b.run()
+    return b.counts(), b.stats()
+
+
+def test_random_small_mixed(gpu, oracle):
+    """Distinct piece sets, multiplicities and widths in one batch."""
+    rng = random.Random(31)
+    pool = [1, 3, 7, 259] + gpu.free_polyominoes(4) + gpu.free_polyominoes(5)
+    insts = []
+    for _ in range(120):
+        shapes = sorted(rng.sample(pool, rng.randint(2, 6)))
+        counts = [rng.randint(1, 3) for _ in shapes]
+        w, h = rng.randint(2, 8), rng.randint(1, 6)
+        blocked = [(rng.randrange(h), rng.randrange(w)) for _ in range(rng.randint(0, 3))]
+        insts.append((shapes, counts, rect(w, h, blocked), w))
+    got, st = run_multi(gpu, insts)
+    want = [oracle.count(s, c, r) for s, c, r, _ in insts]
+    assert got == want
+    assert sum(want) > 0 and st["retries"] >= 0
+    # pruning changes the work, never the counts
+    got2, _ = run_multi(gpu, insts, prune=False)
+    assert got2 == want
+
+
+def test_more_than_64_candidates(gpu, oracle):
+    """Libraries whose (piece, transform) pairs need a 128-bit mask: 14 free
+    hexominoes (subset tilings of small boards)."""
+    p6 = gpu.free_polyominoes(6)
+    rng = random.Random(5)
+    boards = [(6, 2), (6, 3), (4, 3), (8, 3), (6, 4), (4, 6), (6, 5), (3, 8), (6, 6)]
+    insts = []
+    for w, h in boards:
+        shapes = sorted(rng.sample(p6, 14))
+        insts.append((shapes, [1] * 14, rect(w, h), w))
+    libs = [gpu.PieceLibrary(s, c, 0xFF, w) for s, c, _, w in insts]
+    assert max(L.n_cand for L in libs) > 64
+    got, _ = run_multi(gpu, insts)
+    want = [oracle.count(s, c, r) for s, c, r, _ in insts]
+    assert got == want
+    assert sum(1 for x in want if x) >= 3
+    # the single-library searcher refuses such a library with a clear error
+    big = next(L for L in libs if L.n_cand > 64)
+    with pytest.raises(gpu.D4Error):
+        gpu.CudaSearcher(big, rows=rect(big.width, 2))
+
+
+def test_single_library_batch_agrees(gpu):
+    """The 256 seeded 8x8 boards through the multi-library kernel equal the
+    oracle's counts (fixture written by tests/golden/make_seeded_counts.py)."""
+    fx = json.load(open(os.path.join(GOLDEN, "seeded_8x8_seed2.json")))
+    p5 = gpu.free_polyominoes(5)
+    insts = [(p5, [1] * 12, rows, 8) for rows in fx["boards"]]
+    got, st = run_multi(gpu, insts)
+    assert got == fx["counts"]
+    assert st["rows_done"] == 9
+
+
+def test_memory_cap_splits_frontier(gpu):
+    """A memory cap far below the frontier's size parks frontier halves and
+    finishes them one after the other: counts are unchanged."""
+    fx = json.load(open(os.path.join(GOLDEN, "seeded_8x8_seed2.json")))
+    p5 = gpu.free_polyominoes(5)
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    n = 16
+    b = gpu.MultiBatch([lib] * n, fx["boards"][:n])
+    b.set_memory(24 << 20)
+    b.run()
+    assert b.counts() == fx["counts"][:n]
+    assert b.stats()["splits"] > 0
+
+
+def test_budget_drops_instances(gpu):
+    """Instances whose frontier share exceeds the state budget leave the
+    search and are reported unfinished; the others keep exact counts."""
+    fx = json.load(open(os.path.join(GOLDEN, "seeded_8x8_seed2.json")))
+    p5 = gpu.free_polyominoes(5)
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    n = 32
+    b = gpu.MultiBatch([lib] * n, fx["boards"][:n])
+    b.run()
+    peak, fin = b.instances()
+    assert all(fin) and b.counts() == fx["counts"][:n]
+    budget = sorted(peak)[n // 2]
+    b2 = gpu.MultiBatch([lib] * n, fx["boards"][:n])
+    b2.set_budget(budget)
+    b2.run()
+    peak2, fin2 = b2.instances()
+    got = b2.counts()
+    assert 0 < sum(fin2) < n
+    for i in range(n):
+        if fin2[i]:
+            assert got[i] == fx["counts"][i] and peak2[i] <= budget
+        else:
+            assert got[i] == 0 and peak2[i] > budget
+
+
+def test_edge_cases(gpu, oracle):
+    p4 = gpu.free_polyominoes(4)
+    insts = [
+        ([3], [1], [], 2),                 # empty board: one empty tiling
+        ([3], [1], [0, 0], 2),             # all cells blocked
+        ([3], [0], [3], 2),                # piece with multiplicity 0
+        ([7], [4], rect(3, 4), 3),         # one piece kind, several copies
+        (p4, [1] * 5, rect(4, 5), 4),      # 5 tetrominoes on 4x5
+        ([1], [64], rect(8, 8), 8),        # monominoes, 7-bit counter
+    ]
+    got, _ = run_multi(gpu, insts)
+    want = [oracle.count(s, c, r) for s, c, r, _ in insts]
+    assert want[0] == 1 and want[1] == 1 and want[5] == 1
+    assert got == want
+
+
+def test_uneven_instances(gpu, uneven, uneven_fixture):
+    """The uneven puzzles (first 1024 of the 4096) in one batch with a state
+    budget of 2^20 per instance: every instance that finishes and that the
+    oracle finished within its node budget (fixture written by
+    tests/golden/make_uneven_counts.py) has the oracle's count; the count is
+    invariant under the board's left-right reflection (same frontier sizes,
+    so the same instances finish) and top-bottom reflection (compared where
+    both finish); the piece sets are closed under D4."""
+    from golden.make_uneven_counts import digest
+    assert digest(uneven) == uneven_fixture["sha256"]
+    n, budget = 1024, 1 << 20
+    insts = uneven[:n]
+    cache = {}
+
+    def run(batch):
+        libs = []
+        for shapes, counts, _, w in batch:
+            key = (tuple(shapes), tuple(counts), w)
+            if key not in cache:
+                cache[key] = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+            libs.append(cache[key])
+        b = gpu.MultiBatch(libs, [x[2] for x in batch])
+        b.set_budget(budget)
+        b.run()
+        return b.counts(), b.instances()[1]
+
+    got, fin = run(insts)
+    pinned = {int(k): v for k, v in uneven_fixture["counts"].items() if int(k) < n}
+    checked = [i for i in pinned if fin[i]]
+    assert len(checked) >= 300
+    bad = [(i, got[i], pinned[i]) for i in checked if got[i] != pinned[i]]
+    assert not bad, bad[:10]
+    assert sum(fin) >= n // 2
+    got_m, fin_m = run([(s, c, mirror(r, w), w) for s, c, r, w in insts])
+    assert fin_m == fin and got_m == got
+    got_f, fin_f = run([(s, c, r[::-1], w) for s, c, r, w in insts])
+    both = [i for i in range(n) if fin[i] and fin_f[i]]
+    assert len(both) >= n // 2
+    assert all(got_f[i] == got[i] for i in both)
