@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -249,6 +250,17 @@ struct Engine {
     DevBuf<uint64_t> e_desc, e_slot;
     uint64_t n_front = 0, new_roots = 0;
     double grow_unique = 4.0, grow_children = 8.0;
+    // bounded memory (count mode, run()): frontier halves parked for later
+    uint64_t mem_cap = 0;  // 0: unbounded
+    struct Part {
+        uint32_t g = 0;
+        uint64_t n = 0;
+        double grow_unique = 1.0, grow_children = 1.0;
+        DevBuf<uint64_t> k, c;
+    };
+    std::vector<std::unique_ptr<Part>> parked;
+    uint64_t parked_bytes = 0;
+    bool trace = getenv("D4_TRACE") != nullptr;  // per-step lines of run() on stderr
     static constexpr uint64_t kExactCountMax = 1u << 16;
     d4_stats stats{};
     std::deque<DevLog> logs;  // logs[j] = global step log0 + j
@@ -449,12 +461,17 @@ struct Engine {
     }
 
     void run() {
-        while (n_active) step();
+        if (mem_cap && mode == D4_MODE_COUNT) run_bounded();
+        else
+            while (n_active) step();
     }
 
-    void row_step() {
+    // One row step over the whole frontier.  With a memory cap (count mode)
+    // it returns false, with nothing changed, when it would not fit.
+    bool row_step(bool may_split = false) {
         if (n_front > 0xffffffffull) throw Error(D4_ERANGE, "frontier exceeds 2^32 records");
         const bool en = mode == D4_MODE_ENUM;
+        const bool bounded = may_split && mem_cap && !en && n_front >= 2;
         const bool prune = !(flags & D4_FLAG_NO_PRUNE);
         // Table size: unique children <= children.  A small frontier gets an
         // exact bound (row-0 completions per record) or, failing that, an
@@ -485,8 +502,17 @@ struct Engine {
         uint32_t h32[C32_N];
         unsigned long long h64[C64_N];
         for (;;) {
+            if (bounded && (tab_m > (1ull << 34) ||
+                            16ull * tab_m + 16ull * std::min(est_u, tab_m) + 16ull * n_front +
+                                    parked_bytes > mem_cap))
+                return false;
             if (tab_m > (1ull << 34)) throw Error(D4_ERANGE, "dedup table exceeds 2^34 slots");
-            tk.reserve(2 * tab_m);
+            try {
+                tk.reserve(2 * tab_m);
+            } catch (const Error &e) {
+                if (e.code != D4_ENOMEM || !bounded) throw;
+                return false;
+            }
             if (en) {
                 if (est_e > 0xffffffffull) throw Error(D4_ERANGE, "edge count exceeds 2^32");
                 e_parent.reserve(est_e);
@@ -524,6 +550,14 @@ struct Engine {
             if (eovf) est_e = (uint64_t)h32[C32_EDGES] + 1024;
         }
         const uint64_t U = h64[C64_NEW];
+        try {
+            nk.reserve(U);
+            nc.reserve(U);
+        } catch (const Error &e) {
+            if (e.code != D4_ENOMEM || !bounded) throw;
+            HIPCHK(hipMemcpy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice));
+            return false;
+        }
         stats.children += h64[C64_CHILDREN];
         stats.pruned += h64[C64_PRUNED];
         stats.records_extended += n_front;
@@ -532,8 +566,6 @@ struct Engine {
             grow_unique = std::max(1.0, (double)U / (double)n_front);
             grow_children = std::max(1.0, (double)h64[C64_CHILDREN] / (double)n_front);
         }
-        nk.reserve(U);
-        nc.reserve(U);
         if (en) slot_uid.reserve(tab_m);
         HIPCHK(hipEventRecord(evc0, st));
         blk.reserve(2 * COMPACT_MAX_BLOCKS);
@@ -576,6 +608,75 @@ struct Engine {
         stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, U);
         stats.rows_done++;
         new_roots = 0;
+        return true;
+    }
+
+    // Park the upper half of the frontier, to resume at this step later.
+    void split() {
+        const uint64_t keep = n_front / 2, m = n_front - keep;
+        auto p = std::make_unique<Part>();
+        p->g = g;
+        p->n = m;
+        p->grow_unique = grow_unique;
+        p->grow_children = grow_children;
+        p->k.reserve(m);
+        p->c.reserve(m);
+        HIPCHK(hipMemcpyAsync(p->k.p, fk.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(p->c.p, fc.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        parked_bytes += 16 * m;
+        parked.push_back(std::move(p));
+        new_roots = std::min<uint64_t>(new_roots, keep);
+        n_front = keep;
+        stats.splits++;
+    }
+
+    void resume() {
+        std::unique_ptr<Part> p = std::move(parked.back());
+        parked.pop_back();
+        parked_bytes -= 16 * p->n;
+        fk.swap(p->k);
+        fc.swap(p->c);
+        n_front = p->n;
+        g = p->g;
+        grow_unique = p->grow_unique;
+        grow_children = p->grow_children;
+        new_roots = 0;
+    }
+
+    // Count mode under a memory cap: every row of every instance, parking
+    // frontier halves that do not fit and finishing them one after the
+    // other (depth first); tiling counts add up per instance.
+    void run_bounded() {
+        completed.clear();
+        upload_pending();
+        uint32_t gend = g;
+        for (uint32_t i = 0; i < S; i++)
+            if (active[i]) gend = std::max(gend, start[i] + heights[i]);
+        for (;;) {
+            while (n_front && g <= gend) {
+                if (row_step(true)) {
+                    if (trace)
+                        fprintf(stderr, "run step %u: frontier %llu, %zu parts parked\n", g,
+                                (unsigned long long)n_front, parked.size());
+                    g++;
+                    continue;
+                }
+                if (n_front < 2) throw Error(D4_ENOMEM, "one state's row step exceeds the memory cap");
+                split();
+            }
+            if (parked.empty()) break;
+            resume();
+        }
+        g = gend + 1;
+        n_front = 0;
+        HIPCHK(hipMemcpy(host_result.data(), result.p, (size_t)S * 8, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < S; i++)
+            if (active[i]) {
+                completed.push_back(i);
+                active[i] = 0;
+                n_active--;
+            }
     }
 
     // Extend the whole frontier.  Split levels spread each record over 64 /
@@ -870,6 +971,11 @@ static std::unique_ptr<Engine> make_batch(const d4_library *lib, uint32_t n, con
     auto e = std::make_unique<Engine>(lib, n, hm, mode, flags, true);
     for (uint32_t i = 0; i < n; i++) e->admit(i, b + (size_t)i * stride, h[i]);
     e->upload_pending();
+    if (mode == D4_MODE_COUNT) {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        e->mem_cap = (uint64_t)(0.8 * (double)free_b);
+    }
     return e;
 }
 
@@ -1138,6 +1244,15 @@ int d4_searcher_create_ea(const d4_library *lib, uint64_t empty_area, uint32_t m
     return d4_searcher_create(lib, rows, h, mode, flags, out);
 }
 
+int d4_searcher_set_memory(d4_searcher *s, uint64_t bytes) {
+    return guard([&] {
+        if (!s) throw Error(D4_EINVAL, "null searcher");
+        if (s->e->mode != D4_MODE_COUNT) throw Error(D4_ESTATE, "memory cap needs count mode");
+        if (bytes < (1u << 20)) throw Error(D4_EINVAL, "memory cap below 1 MiB");
+        s->e->mem_cap = bytes;
+    });
+}
+
 void d4_searcher_destroy(d4_searcher *s) { delete s; }
 
 uint32_t d4_searcher_height(const d4_searcher *s) { return s ? s->e->height_of(0) : 0; }
@@ -1368,6 +1483,15 @@ int d4_batch_create(const d4_library *lib, uint32_t n_inst, const uint8_t *board
 }
 
 void d4_batch_destroy(d4_batch *b) { delete b; }
+
+int d4_batch_set_memory(d4_batch *b, uint64_t bytes) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        if (b->e->mode != D4_MODE_COUNT) throw Error(D4_ESTATE, "memory cap needs count mode");
+        if (bytes < (1u << 20)) throw Error(D4_EINVAL, "memory cap below 1 MiB");
+        b->e->mem_cap = bytes;
+    });
+}
 
 int d4_batch_run(d4_batch *b) {
     return guard([&] {

@@ -115,7 +115,7 @@ struct d4_mbatch {
     DevBuf<unsigned long long> hist;
     static constexpr uint64_t kExactCountMax = 1u << 16;
     bool done = false;
-    bool trace = getenv("D4_MBATCH_TRACE") != nullptr;  // per-row lines on stderr
+    bool trace = getenv("D4_TRACE") != nullptr;  // per-row lines on stderr
     d4_stats stats{};
 
     ~d4_mbatch() {
@@ -383,6 +383,10 @@ struct d4_mbatch {
         }
         counts.assign(N, 0);
         HIPCHK(hipMemcpy(counts.data(), result.p, 8ull * N, hipMemcpyDeviceToHost));
+        // a dropped instance may have absorbed part of its count before
+        // another frontier part crossed the budget: report none of it
+        for (uint32_t i = 0; i < N; i++)
+            if (h_dropped[i]) counts[i] = 0;
         done = true;
     }
 };

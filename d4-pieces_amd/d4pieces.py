@@ -94,6 +94,7 @@ SIGNATURES = [
     ("d4_searcher_next_size", ctypes.c_uint64, [_P]),
     ("d4_searcher_step", ctypes.c_int, [_P]),
     ("d4_searcher_run", ctypes.c_int, [_P]),
+    ("d4_searcher_set_memory", ctypes.c_int, [_P, ctypes.c_uint64]),
     ("d4_searcher_count", ctypes.c_int, [_P, _U64P]),
     ("d4_searcher_solutions", ctypes.c_int,
      [_P, ctypes.c_int, _U8P, ctypes.c_size_t, _SZP, _U64P]),
@@ -104,6 +105,7 @@ SIGNATURES = [
       ctypes.POINTER(_P)]),
     ("d4_batch_destroy", None, [_P]),
     ("d4_batch_run", ctypes.c_int, [_P]),
+    ("d4_batch_set_memory", ctypes.c_int, [_P, ctypes.c_uint64]),
     ("d4_batch_counts", ctypes.c_int, [_P, _U64P]),
     ("d4_batch_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
     ("d4_batch_solutions", ctypes.c_int,
@@ -285,6 +287,11 @@ class CudaSearcher:
     def next_size(self) -> int:
         return load().d4_searcher_next_size(self._h)
 
+    def set_memory(self, nbytes: int) -> None:
+        """Count mode: device bytes run() may use (frontier parts beyond it
+        are parked and finished one after the other)."""
+        _check(load().d4_searcher_set_memory(self._h, nbytes))
+
     def search_GPU(self) -> None:
         _check(load().d4_searcher_step(self._h))
 
@@ -343,6 +350,9 @@ class Batch:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.d4_batch_destroy(self._h)
             self._h = None
+
+    def set_memory(self, nbytes: int) -> None:
+        _check(load().d4_batch_set_memory(self._h, nbytes))
 
     def run(self) -> "Batch":
         _check(load().d4_batch_run(self._h))

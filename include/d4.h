@@ -161,6 +161,12 @@ uint64_t d4_searcher_next_size(const d4_searcher *s);
 int d4_searcher_step(d4_searcher *s);
 /* Step until height == 0. */
 int d4_searcher_run(d4_searcher *s);
+/* Count mode: device bytes the frontier and its dedup table may take in
+ * d4_searcher_run (default: 80% of the free device memory at creation).  A
+ * row that would exceed it parks half of the frontier and finishes the
+ * other half first (depth first), streaming a frontier larger than memory
+ * through it; the count is unchanged.  d4_searcher_step is not bounded. */
+int d4_searcher_set_memory(d4_searcher *s, uint64_t bytes);
 /* Number of tilings (valid once height == 0): sum of terminal counts. */
 int d4_searcher_count(const d4_searcher *s, uint64_t *count);
 /* Solutions (D4_MODE_ENUM only, height == 0), in the CPU searcher's DFS
@@ -197,6 +203,8 @@ int d4_batch_create(const d4_library *lib, uint32_t n_inst,
                     d4_batch **out);
 void d4_batch_destroy(d4_batch *b);
 int d4_batch_run(d4_batch *b);
+/* Count mode: memory cap of d4_batch_run, as d4_searcher_set_memory. */
+int d4_batch_set_memory(d4_batch *b, uint64_t bytes);
 /* counts[i] = number of tilings of board i (valid after d4_batch_run). */
 int d4_batch_counts(const d4_batch *b, uint64_t *counts);
 int d4_batch_stats(const d4_batch *b, d4_stats *st);

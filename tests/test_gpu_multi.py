@@ -163,8 +163,10 @@ def test_uneven_instances(gpu, uneven, uneven_fixture):
     oracle finished within its node budget (fixture written by
     tests/golden/make_uneven_counts.py) has the oracle's count; the count is
     invariant under the board's left-right reflection (same frontier sizes,
-    so the same instances finish) and top-bottom reflection (compared where
-    both finish); the piece sets are closed under D4."""
+    and top-bottom reflection; the piece sets are closed under D4.  (Which
+    instances finish can differ between the runs when the frontier is split
+    to fit in memory, since the budget applies per frontier part: counts are
+    compared where both runs finish.)"""
     from golden.make_uneven_counts import digest
     assert digest(uneven) == uneven_fixture["sha256"]
     n, budget = 1024, 1 << 20
@@ -190,9 +192,10 @@ def test_uneven_instances(gpu, uneven, uneven_fixture):
     bad = [(i, got[i], pinned[i]) for i in checked if got[i] != pinned[i]]
     assert not bad, bad[:10]
     assert sum(fin) >= n // 2
-    got_m, fin_m = run([(s, c, mirror(r, w), w) for s, c, r, w in insts])
-    assert fin_m == fin and got_m == got
-    got_f, fin_f = run([(s, c, r[::-1], w) for s, c, r, w in insts])
-    both = [i for i in range(n) if fin[i] and fin_f[i]]
-    assert len(both) >= n // 2
-    assert all(got_f[i] == got[i] for i in both)
+    assert all(got[i] == 0 for i in range(n) if not fin[i])
+    for reflect in (lambda r, w: mirror(r, w), lambda r, w: r[::-1]):
+        got_r, fin_r = run([(s, c, reflect(r, w), w) for s, c, r, w in insts])
+        both = [i for i in range(n) if fin[i] and fin_r[i]]
+        assert len(both) >= n // 2
+        bad = [(i, got[i], got_r[i]) for i in both if got_r[i] != got[i]]
+        assert not bad, bad[:10]

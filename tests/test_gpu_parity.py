@@ -208,3 +208,38 @@ def test_first_solution_walk(gpu, oracle):
         assert first == oracle.solve(shapes, counts, rows, single=True)
         checked += bool(first)
     assert checked >= 5
+
+
+def test_memory_cap_streams_frontier(gpu, lib6):
+    """A memory cap far below the frontier parks frontier halves and finishes
+    them one after the other: counts are unchanged (batch and searcher)."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    lib = gpu.PieceLibrary(gpu.free_polyominoes(5), [1] * 12, 0xFF, 8)
+    b = gpu.Batch(lib, fx["boards"][:32])
+    b.set_memory(8 << 20)
+    b.run()
+    assert b.counts() == fx["counts"][:32]
+    assert b.stats()["splits"] > 0
+    s = gpu.CudaSearcher(lib6, rows=rect(6, 10))
+    s.set_memory(2 << 20)
+    s.run()
+    assert s.count() == 9356 and s.stats()["splits"] > 0
+
+
+def test_doubled_pentominoes(gpu, oracle):
+    """Each pentomino available twice (the large single enumeration's
+    library) on 8x5 and its transpose 5x8, streamed under a small memory cap,
+    against the oracle (on 5x8; the two boards have the same count)."""
+    p5 = gpu.free_polyominoes(5)
+    want = oracle.count(p5, [2] * 12, rect(5, 8))
+    assert want == 695086
+    for w, h in ((8, 5), (5, 8)):
+        lib = gpu.PieceLibrary(p5, [2] * 12, 0xFF, w)
+        s = gpu.CudaSearcher(lib, rows=rect(w, h))
+        s.set_memory(4 << 20)
+        s.run()
+        assert s.count() == want, (w, h)
+
