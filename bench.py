@@ -216,7 +216,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-sample", type=int, default=256)
-    ap.add_argument("--serve-rates", default="1000,20000",
+    ap.add_argument("--serve-rates", default="100,20000",
                     help="comma-separated arrival rates (requests/s); empty to skip")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-slots", type=int, default=1024)
