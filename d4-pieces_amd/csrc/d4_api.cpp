@@ -248,6 +248,7 @@ struct Engine {
     DevBuf<unsigned long long> c64;
     DevBuf<uint32_t> e_parent, e_child, slot_uid, blk;
     DevBuf<uint64_t> e_desc, e_slot;
+    DevBuf<uint64_t> don_a, don_b;  // donated work items (two words each)
     uint64_t n_front = 0, new_roots = 0;
     double grow_unique = 4.0, grow_children = 8.0;
     // bounded memory (count mode, run()): frontier halves parked for later
@@ -390,7 +391,41 @@ struct Engine {
         a.seed_mask = lib->maxh >= 2 ? ~((1ull << (8 * (lib->maxh - 1))) - 1ull) : ~0ull;
         a.gcd = lib->gcd;
         a.min_size = lib->min_size;
+        a.budget = lib->ext_budget;
+        a.don_in = nullptr;
+        a.n_don = 0;
+        a.don_out = don_a.p;
+        a.don_cap = (uint32_t)(don_a.cap / 2);
         return a;
+    }
+
+    // Run the work items that lanes over their step budget handed over,
+    // launch after launch, until none are left (after a few rounds the
+    // budget is lifted so the chain ends).
+    void run_donations(uint64_t tab_m, uint32_t e_cap, bool en, bool prune, bool count_only) {
+        if (!lib->ext_budget) return;
+        for (uint32_t round = 0;; round++) {
+            uint32_t h[C32_N];
+            HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (h[C32_OVERFLOW]) return;  // the step is rerun
+            const uint32_t cap = (uint32_t)(don_a.cap / 2);
+            const uint32_t n =
+                h[C32_DON] <= cap ? h[C32_DON] : (h[C32_DON_END] ? h[C32_DON_END] - 1 : cap);
+            if (!n) return;
+            stats.donated += n;
+            launch_reset_donations(c32.p, c64.p, st);
+            ExtendArgs a = args(tab_m, e_cap, 0, n_front, 0);
+            a.don_in = don_a.p;
+            a.n_don = n;
+            a.don_out = don_b.p;
+            a.don_cap = (uint32_t)(don_b.cap / 2);
+            // follow-up items are smaller: split them sooner
+            a.budget = round < 24 ? std::max(lib->ext_budget / 4, 32u) : 0;
+            launch_extend(a, en, prune, count_only, lib->ext_threads, lib->n_cu, st);
+            HIPCHK(hipGetLastError());
+            don_a.swap(don_b);
+        }
     }
 
     // Move queued instances into the frontier: their board windows, height,
@@ -686,6 +721,8 @@ struct Engine {
     // mixed frontier would otherwise leave each of them to a single lane.
     void launch_ranges(uint64_t tab_m, uint32_t e_cap, bool en, bool prune, bool count_only) {
         if (!n_front) return;
+        don_a.reserve(2u << 20);  // 1 Mi donated items per launch
+        don_b.reserve(2u << 20);
         auto split_of = [](uint64_t n) {
             return n <= 1024 ? 2u : n <= (1u << 16) ? 1u : 0u;
         };
@@ -701,6 +738,7 @@ struct Engine {
             launch_extend(args(tab_m, e_cap, 0, n_front, split_of(n_front)), en, prune,
                           count_only, lib->ext_threads, lib->n_cu, st);
         }
+        run_donations(tab_m, e_cap, en, prune, count_only);
     }
 
     // Children the current frontier will emit (after pruning): the extend
@@ -1170,6 +1208,20 @@ int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads,
     });
 }
 
+int d4_library_set_budget(d4_library *lib, uint32_t steps) {
+    return guard([&] {
+        if (!lib) throw Error(D4_EINVAL, "null library");
+        lib->ext_budget = steps;
+    });
+}
+
+int d4_library_get_budget(const d4_library *lib, uint32_t *steps) {
+    return guard([&] {
+        if (!lib || !steps) throw Error(D4_EINVAL, "null argument");
+        *steps = lib->ext_budget;
+    });
+}
+
 int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows, uint32_t *threads,
                           uint32_t *refill) {
     return guard([&] {
@@ -1216,6 +1268,17 @@ int d4_library_tune(d4_library *lib, uint32_t n_inst, const uint8_t *boards, uin
         }
         if (d4_library_set_launch(lib, best_f, best_t, best_r) != D4_OK)
             throw Error(D4_EDEVICE, g_err);
+        // then the donation budget (work split for the launch's tail)
+        uint32_t best_b = lib->ext_budget;
+        for (uint32_t b : {0u, 256u, 384u, 512u, 768u, 1024u, 2048u}) {
+            lib->ext_budget = b;
+            double ms = measure();
+            if (ms < best) {
+                best = ms;
+                best_b = b;
+            }
+        }
+        lib->ext_budget = best_b;
         if (fit_rows) *fit_rows = best_f;
         if (threads) *threads = best_t;
         if (refill) *refill = best_r;

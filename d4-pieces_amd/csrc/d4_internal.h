@@ -25,6 +25,8 @@ enum : int {
     C32_EDGES = 2,     // edges written (enum mode)
     C32_EDGE_OVF = 3,  // edge buffer overflow
     C32_OUT = 4,       // compaction output cursor
+    C32_DON = 5,       // donated work items written (extend)
+    C32_DON_END = 6,   // 1 + end of the valid donated items when C32_DON passed the capacity
     C32_N = 8,
 };
 enum : int {
@@ -79,6 +81,16 @@ struct ExtendArgs {
     uint64_t seed_mask;      // window rows no placed piece reaches yet
     uint32_t gcd;
     uint32_t min_size;
+    // load balance: a lane whose DFS has run `budget` steps on one work item
+    // hands the rest of its search tree to the next launch as donated items
+    // {frontier index | first candidate << 32 | level << 40 | only-first <<
+    // 48, placements so far}; a launch with don_in set runs such items
+    // instead of records
+    uint32_t budget;         // 0: never donate
+    const uint64_t *don_in;  // [n_don][2] or null
+    uint32_t n_don;
+    uint64_t *don_out;       // [don_cap][2]
+    uint32_t don_cap;
 };
 
 // Tables of one library in a multi-library batch (device layout, 16-byte
@@ -129,6 +141,7 @@ void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
 void launch_reset_work(unsigned long long *c64, hipStream_t st);
+void launch_reset_donations(uint32_t *c32, unsigned long long *c64, hipStream_t st);
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
                    bool count_only, uint32_t threads, int n_cu, hipStream_t st);
 constexpr uint32_t COMPACT_MAX_BLOCKS = 65536;

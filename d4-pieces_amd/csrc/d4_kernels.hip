@@ -127,17 +127,23 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     // over take the next item of the chunk by shuffle, so a record with a
     // large subtree does not idle the rest of its wave and no lane waits on
     // memory per record.
-    const uint32_t fix = a.split;
+    const bool don_mode = a.don_in != nullptr;
+    const uint32_t fix = don_mode ? 0u : a.split;
     const uint32_t per_rec = 1u << (6 * fix);
-    const uint64_t n_items = (uint64_t)a.n_in * per_rec;
+    const uint64_t n_items = don_mode ? (uint64_t)a.n_don : (uint64_t)a.n_in * per_rec;
     bool exhausted = false;
     uint32_t buf_items = 0, buf_pos = 0, buf_rec0 = 0;
     uint64_t buf_item0 = 0;
     uint64_t bkey = 0, bcnt = 0, bBW = 0, bBWn = 0;
     uint32_t bterm = 0;
+    uint64_t bit0 = 0, bit1 = 0;  // donated item words (don_mode)
 
     bool searching = false, pass = false;
     uint32_t rec = 0, inst = 0, used = 0, d = 0, c = 0, anc = 0;
+    // the lane's work item owns levels >= base_d of the DFS; sib: whether
+    // siblings at level base_d belong to it; steps: DFS steps on the item
+    uint32_t base_d = 0, steps = 0;
+    bool sib = true;
     uint64_t cnt = 0, BW = 0, BWn = 0, E = 0, avail = 0, desc = 0, ok = 0;
 
     // Children are staged per wave in LDS (key, count, open window; in enum
@@ -206,6 +212,41 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
         n_pruned += (t >> 1) & 1u;
         n_new += (t >> 2) & 1u;
     };
+    // Hand the rest of this lane's search tree to the next launch: the
+    // untried candidates at the current level and, at each level below it
+    // that the item owns, the candidates after the one in progress.  The
+    // items cover exactly the work not yet done.  Returns false (and the
+    // lane goes on) when there is nothing to hand over or no room.
+    auto donate = [&]() __attribute__((always_inline)) -> bool {
+        // the current level's untried candidates go out one item each (the
+        // widest fan-out), the levels below as "candidates after k" items
+        uint32_t n = (uint32_t)__builtin_popcountll(ok);
+        for (uint32_t l = base_d; l < d; l++) {
+            const uint32_t kl = (uint32_t)((desc >> (8 * l)) & 0xffu) - 1u;
+            n += (l > base_d || sib) && kl + 1 < (uint32_t)MAX_CAND;
+        }
+        if (!n) return false;
+        const uint32_t b = atomicAdd(a.c32 + C32_DON, n);
+        if (b + n > a.don_cap) {
+            if (b < a.don_cap) a.c32[C32_DON_END] = b + 1;  // the one range crossing the end
+            return false;
+        }
+        const uint64_t r = (uint64_t)(a.rec_base + rec);
+        uint32_t o = b;
+        auto put = [&](uint32_t lvl, uint32_t first, bool single) __attribute__((always_inline)) {
+            const uint64_t pm = lvl >= 8 ? ~0ull : (1ull << (8 * lvl)) - 1ull;
+            a.don_out[2 * o] = r | ((uint64_t)first << 32) | ((uint64_t)lvl << 40) |
+                               ((uint64_t)single << 48);
+            a.don_out[2 * o + 1] = desc & pm;
+            o++;
+        };
+        for (uint64_t m = ok; m; m &= m - 1) put(d, (uint32_t)__builtin_ctzll(m), true);
+        for (uint32_t l = base_d; l < d; l++) {
+            const uint32_t kl = (uint32_t)((desc >> (8 * l)) & 0xffu) - 1u;
+            if ((l > base_d || sib) && kl + 1 < (uint32_t)MAX_CAND) put(l, kl + 1, false);
+        }
+        return true;
+    };
 
     for (;;) {
         // refill idle lanes once at least a.refill of them are idle (or the
@@ -234,8 +275,14 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 buf_rec0 = (uint32_t)(base >> (6 * fix));
                 const uint32_t nrec = fix ? 1u : buf_items;
                 if (lane < nrec) {
-                    bkey = a.in_key[buf_rec0 + lane];
-                    bcnt = a.in_cnt[buf_rec0 + lane];
+                    uint32_t r0 = buf_rec0 + lane;
+                    if (don_mode) {
+                        bit0 = a.don_in[2 * (buf_item0 + lane)];
+                        bit1 = a.don_in[2 * (buf_item0 + lane) + 1];
+                        r0 = (uint32_t)bit0;
+                    }
+                    bkey = a.in_key[r0];
+                    bcnt = a.in_cnt[r0];
                     // the instance's own row: instances join at their start step
                     uint32_t bi = a.inst_shift < 64 ? (uint32_t)(bkey >> a.inst_shift) : 0u;
                     const uint32_t rr = a.row - a.start[bi];
@@ -256,11 +303,19 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
             const uint64_t kBW = shfl64(bBW, src);
             const uint64_t kBWn = shfl64(bBWn, src);
             const uint32_t kterm = __shfl(bterm, (int)src);
+            uint64_t it0 = 0, it1 = 0;
+            if (don_mode) {
+                it0 = shfl64(bit0, src);
+                it1 = shfl64(bit1, src);
+            }
             if (idle && q < buf_items) {
                 const uint64_t item = buf_item0 + q;
                 const uint32_t sub = (uint32_t)(item & (per_rec - 1));  // k0 << 6 | k1
-                const bool lead_item = sub == 0;
-                rec = buf_rec0 + (fix ? 0u : q);
+                const bool lead_item = sub == 0 && !don_mode;
+                rec = don_mode ? (uint32_t)it0 : buf_rec0 + (fix ? 0u : q);
+                steps = 0;
+                base_d = 0;
+                sib = fix == 0;
                 cnt = kc;
                 inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
                 used = (uint32_t)(key >> a.occ_bits) & a.used_mask;
@@ -303,10 +358,34 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                                     desc = (uint64_t)(k0 + 1);
                                     anc = c;
                                     d = 1;
+                                    base_d = 1;
                                     c = (uint32_t)__builtin_ctzll(E & 0xffu);
                                     ok = fitmask(s_fit, rows, c, E) & avail & (1ull << k1);
                                 }
                             }
+                        } else if (don_mode) {
+                            // replay the item's placements, then take the
+                            // candidates from `first` on at its level
+                            const uint32_t lvl = (uint32_t)(it0 >> 40) & 0xffu;
+                            const uint32_t first = (uint32_t)(it0 >> 32) & 0xffu;
+                            const bool single = (it0 >> 48) & 1u;
+                            for (uint32_t j = 0; j < lvl; j++) {
+                                const uint32_t kj = (uint32_t)((it1 >> (8 * j)) & 0xffu) - 1u;
+                                const PlaceInfo pj = s_pc[c * MAX_CAND + kj];
+                                const uint32_t cj = (uint32_t)pj.info;
+                                E &= ~pj.shape;
+                                used += 1u << (cj & 0xffu);
+                                if (((used >> (cj & 0xffu)) & ((cj >> 8) & 0xffu)) == (cj >> 16))
+                                    avail &= ~piece_range(pj.info);
+                                anc |= c << (3 * j);
+                                c = (uint32_t)__builtin_ctzll(E & 0xffu);
+                            }
+                            desc = it1;
+                            d = lvl;
+                            base_d = lvl;
+                            sib = !single;
+                            ok = fitmask(s_fit, rows, c, E) & avail &
+                                 (single ? 1ull << first : ~0ull << first);
                         }
                         searching = ok != 0;
                     } else {
@@ -329,7 +408,9 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
             used2 = used;
             pass = false;
         } else if (searching) {
-            if (ok) {
+            if (a.budget && ++steps > a.budget && donate()) {
+                searching = false;
+            } else if (ok) {
                 uint32_t k = (uint32_t)__builtin_ctzll(ok);
                 ok &= ok - 1;
                 const PlaceInfo pk = s_pc[c * MAX_CAND + k];
@@ -354,8 +435,8 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                     c = (uint32_t)__builtin_ctzll(E & 0xffu);
                     ok = fitmask(s_fit, rows, c, E) & avail;
                 }
-            } else if (d + 1 <= max(fix, 1u)) {
-                // levels below `fix` are fixed by the work item
+            } else if (d <= base_d) {
+                // the levels below base_d are fixed by the work item
                 searching = false;
             } else {
                 d--;
@@ -367,7 +448,8 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 E |= pk.shape;
                 used -= 1u << (pk.info & 0xffu);
                 avail |= piece_range(pk.info);
-                ok = d >= fix ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1) : 0;
+                ok = (d > base_d || sib) ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1)
+                                         : 0;
             }
         }
         // queue the child in the wave's LDS stage: pruning, the table insert
@@ -430,6 +512,12 @@ __global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
 }
 
 __global__ void k_reset_work(unsigned long long *c64) { c64[C64_WORK] = 0; }
+
+__global__ void k_reset_donations(uint32_t *c32, unsigned long long *c64) {
+    c32[C32_DON] = 0;
+    c32[C32_DON_END] = 0;
+    c64[C64_WORK] = 0;
+}
 
 // Compaction of the dedup table into the next frontier, in slot order (so the
 // frontier is deterministic for a given table size): k_compact_count counts
@@ -670,6 +758,10 @@ void launch_reset_work(unsigned long long *c64, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, st, c64);
 }
 
+void launch_reset_donations(uint32_t *c32, unsigned long long *c64, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_donations, dim3(1), dim3(1), 0, st, c32, c64);
+}
+
 template <bool ENUM, bool PRUNE, bool COUNT_ONLY>
 static void launch_extend_t(const ExtendArgs &a, uint32_t threads, dim3 g, size_t lds,
                             hipStream_t st) {
@@ -696,7 +788,8 @@ void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_o
     // resident workgroups per CU: 160 KiB of LDS and 32 waves
     uint64_t per_cu = std::min<uint64_t>(160u * 1024u / lds, 32u / (threads / 64));
     if (!per_cu) per_cu = 1;
-    const uint64_t waves = (((uint64_t)a.n_in << (6 * a.split)) + 63) / 64;
+    const uint64_t waves =
+        ((a.don_in ? (uint64_t)a.n_don : ((uint64_t)a.n_in << (6 * a.split))) + 63) / 64;
     uint64_t blocks = (waves + threads / 64 - 1) / (threads / 64);
     blocks = std::min<uint64_t>(blocks, (uint64_t)n_cu * per_cu);
     if (!blocks) blocks = 1;

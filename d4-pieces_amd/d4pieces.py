@@ -55,6 +55,7 @@ class Stats(ctypes.Structure):
         ("compact_bytes", ctypes.c_uint64),
         ("count_passes", ctypes.c_uint64),
         ("splits", ctypes.c_uint64),
+        ("donated", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -81,6 +82,8 @@ SIGNATURES = [
     ("d4_library_set_launch", ctypes.c_int,
      [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("d4_library_get_launch", ctypes.c_int, [_P, _U32P, _U32P, _U32P]),
+    ("d4_library_set_budget", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_library_get_budget", ctypes.c_int, [_P, _U32P]),
     ("d4_library_tune", ctypes.c_int,
      [_P, ctypes.c_uint32, _U8P, ctypes.c_uint32, _U32P, _U32P, _U32P, _U32P,
       ctypes.POINTER(ctypes.c_double)]),
@@ -220,6 +223,15 @@ class PieceLibrary:
     def set_launch(self, fit_rows: int, threads: int, refill: int = 16) -> None:
         _check(load().d4_library_set_launch(self._h, fit_rows, threads, refill))
 
+    def set_budget(self, steps: int) -> None:
+        """DFS steps per work item before a lane donates the rest (0: never)."""
+        _check(load().d4_library_set_budget(self._h, steps))
+
+    def get_budget(self) -> int:
+        v = ctypes.c_uint32()
+        _check(load().d4_library_get_budget(self._h, ctypes.byref(v)))
+        return v.value
+
     def get_launch(self) -> Tuple[int, int, int]:
         f, t, r = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(load().d4_library_get_launch(self._h, ctypes.byref(f), ctypes.byref(t),
@@ -234,7 +246,7 @@ class PieceLibrary:
                                       _u32arr(heights), ctypes.byref(f), ctypes.byref(t),
                                       ctypes.byref(r), ctypes.byref(ms)))
         return {"fit_rows": f.value, "threads": t.value, "refill": r.value,
-                "extend_ms": ms.value}
+                "budget": self.get_budget(), "extend_ms": ms.value}
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -73,6 +73,7 @@ typedef struct d4_stats {
                                   state gathered and written                */
     uint64_t count_passes;     /* search-only passes sizing the dedup table */
     uint64_t splits;           /* frontier halves parked to stay in the memory cap */
+    uint64_t donated;          /* work items handed between extend launches */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -120,6 +121,12 @@ int d4_library_info(const d4_library *lib, uint32_t *n_cand,
  * choice (src/kernel.h:44-67). */
 int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads,
                           uint32_t refill);
+/* Load balance of the extend kernel: a lane that has run `steps` DFS steps
+ * on one work item hands the rest of its search tree to a follow-up launch
+ * (0: never).  d4_library_tune sets it too.  (KSizing::optimize has no
+ * counterpart: the reference's tiles are fixed per launch, src/kernel.cu.) */
+int d4_library_set_budget(d4_library *lib, uint32_t steps);
+int d4_library_get_budget(const d4_library *lib, uint32_t *steps);
 int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows,
                           uint32_t *threads, uint32_t *refill);
 /* Measured autotuner, replacing the analytic cost model of
