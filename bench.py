@@ -214,7 +214,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the measured autotuner instead of --launch")
+    ap.add_argument("--no-tune", action="store_true", help="library defaults, no --launch")
+    ap.add_argument("--launch", default="3,256,16,768",
+                    help="fit_rows,threads,refill,budget of the extend kernel (the "
+                         "autotuner's choice for this workload on MI355X)")
     ap.add_argument("--tune-sample", type=int, default=256)
     ap.add_argument("--serve-rates", default="100,20000",
                     help="comma-separated arrival rates (requests/s); empty to skip")
@@ -244,7 +249,11 @@ def main():
     boards = workloads.seeded_8x8(args.instances, seed=2)
     lib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
     tuned = None
-    if not args.no_tune:
+    if args.launch and not args.tune and not args.no_tune:
+        f, t, r, b = (int(x) for x in args.launch.split(","))
+        lib.set_launch(f, t, r)
+        lib.set_budget(b)
+    elif args.tune:
         # measured autotune of the extend launch shape on a sample of the
         # workload (outside the timed region)
         t = time.perf_counter()
@@ -284,7 +293,7 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "extend_traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_row_step")
 
     out = {
         "metric": METRIC,
@@ -308,7 +317,7 @@ def main():
             "mode": "count",
             "parallelism": "instances batched in one frontier, 1 GPU",
             "launch": dict(zip(("fit_rows", "threads", "refill"), lib.get_launch()),
-                           autotuned=tuned is not None),
+                           budget=lib.get_budget(), autotuned=tuned is not None),
         },
         "solutions_per_s": n_sol / dt,
         "solutions_per_step": sum(counts),
@@ -325,8 +334,12 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            # one unit = one row step's extend: the main launch plus the
+            # load-balance follow-up launches (HIP events around them)
+            "per": "row step",
             "avg_launch_ms": ext_ms / max(ext_launch, 1),
             "bytes_per_launch": ext_bytes / max(ext_launch, 1),
+            "dispatches_per_row_step": None,
             "compact_GBs": (sum(s["compact_bytes"] for s in stats) /
                             (sum(s["compact_ms"] for s in stats) / 1e3) / 1e9)
             if sum(s["compact_ms"] for s in stats) else None,

@@ -408,6 +408,7 @@ struct Engine {
             uint32_t h[C32_N];
             HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
+            if (h[C32_BAD_ITEM]) throw Error(D4_EDEVICE, "internal error: bad donated work item");
             if (h[C32_OVERFLOW]) return;  // the step is rerun
             const uint32_t cap = (uint32_t)(don_a.cap / 2);
             const uint32_t n =

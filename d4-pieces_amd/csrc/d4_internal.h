@@ -20,7 +20,7 @@ constexpr int MAX_ROWS = 64;     // board height
 
 // Counter slots of the per-step device counter block.
 enum : int {
-    C32_RESERVED = 0,
+    C32_BAD_ITEM = 0,  // a donated item named a record outside the frontier
     C32_OVERFLOW = 1,  // hash probe limit hit
     C32_EDGES = 2,     // edges written (enum mode)
     C32_EDGE_OVF = 3,  // edge buffer overflow
@@ -133,6 +133,12 @@ struct MultiArgs {
     uint32_t max_probe;
     uint32_t *c32;
     unsigned long long *c64;
+    // load balance, as in ExtendArgs (candidate ids up to 127)
+    uint32_t budget;
+    const uint64_t *don_in;
+    uint32_t n_don;
+    uint64_t *don_out;
+    uint32_t don_cap;
 };
 
 // Launchers (d4_kernels.hip, d4_multi.hip).

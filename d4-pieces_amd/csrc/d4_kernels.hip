@@ -143,7 +143,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     // the lane's work item owns levels >= base_d of the DFS; sib: whether
     // siblings at level base_d belong to it; steps: DFS steps on the item
     uint32_t base_d = 0, steps = 0;
-    bool sib = true;
+    bool sib = true, no_don = false;
     uint64_t cnt = 0, BW = 0, BWn = 0, E = 0, avail = 0, desc = 0, ok = 0;
 
     // Children are staged per wave in LDS (key, count, open window; in enum
@@ -226,8 +226,16 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
             n += (l > base_d || sib) && kl + 1 < (uint32_t)MAX_CAND;
         }
         if (!n) return false;
+        // once the buffer is full nobody adds to the counter again (so it
+        // cannot wrap), and a lane that found no room keeps its item
+        if (__hip_atomic_load(a.c32 + C32_DON, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+            a.don_cap) {
+            no_don = true;
+            return false;
+        }
         const uint32_t b = atomicAdd(a.c32 + C32_DON, n);
-        if (b + n > a.don_cap) {
+        if ((uint64_t)b + n > (uint64_t)a.don_cap) {
+            no_don = true;
             if (b < a.don_cap) a.c32[C32_DON_END] = b + 1;  // the one range crossing the end
             return false;
         }
@@ -276,17 +284,23 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 const uint32_t nrec = fix ? 1u : buf_items;
                 if (lane < nrec) {
                     uint32_t r0 = buf_rec0 + lane;
+                    bool bad = false;
                     if (don_mode) {
                         bit0 = a.don_in[2 * (buf_item0 + lane)];
                         bit1 = a.don_in[2 * (buf_item0 + lane) + 1];
                         r0 = (uint32_t)bit0;
+                        bad = r0 >= a.n_in;  // never expected: flagged, skipped
+                        if (bad) {
+                            atomicOr(a.c32 + C32_BAD_ITEM, 1u);
+                            r0 = 0;
+                        }
                     }
                     bkey = a.in_key[r0];
                     bcnt = a.in_cnt[r0];
                     // the instance's own row: instances join at their start step
                     uint32_t bi = a.inst_shift < 64 ? (uint32_t)(bkey >> a.inst_shift) : 0u;
                     const uint32_t rr = a.row - a.start[bi];
-                    bterm = rr >= a.hgt[bi];
+                    bterm = bad || rr >= a.hgt[bi];
                     const uint64_t *w = a.bw + (size_t)bi * a.bw_stride + rr;
                     bBW = bterm ? 0 : w[0];
                     bBWn = bterm ? 0 : w[1];
@@ -314,6 +328,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 const bool lead_item = sub == 0 && !don_mode;
                 rec = don_mode ? (uint32_t)it0 : buf_rec0 + (fix ? 0u : q);
                 steps = 0;
+                no_don = false;
                 base_d = 0;
                 sib = fix == 0;
                 cnt = kc;
@@ -370,7 +385,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                             const uint32_t first = (uint32_t)(it0 >> 32) & 0xffu;
                             const bool single = (it0 >> 48) & 1u;
                             for (uint32_t j = 0; j < lvl; j++) {
-                                const uint32_t kj = (uint32_t)((it1 >> (8 * j)) & 0xffu) - 1u;
+                                const uint32_t kj = ((uint32_t)((it1 >> (8 * j)) & 0xffu) - 1u) & 63u;
                                 const PlaceInfo pj = s_pc[c * MAX_CAND + kj];
                                 const uint32_t cj = (uint32_t)pj.info;
                                 E &= ~pj.shape;
@@ -408,7 +423,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
             used2 = used;
             pass = false;
         } else if (searching) {
-            if (a.budget && ++steps > a.budget && donate()) {
+            if (a.budget && !no_don && ++steps > a.budget && donate()) {
                 searching = false;
             } else if (ok) {
                 uint32_t k = (uint32_t)__builtin_ctzll(ok);

@@ -94,6 +94,12 @@ struct d4_mbatch {
     DevBuf<unsigned long long> tab;
     DevBuf<uint32_t> c32, blk;
     DevBuf<unsigned long long> c64;
+    DevBuf<uint64_t> don_a, don_b;  // donated work items (two words each)
+    // DFS steps per work item before donating; 0 (off) by default: the one
+    // GPU run with donation on in this kernel faulted and the cause is not
+    // yet found (a CPU model of the per-lane search and donation agrees
+    // with the no-donation search), so the path stays off
+    uint32_t budget_steps = 0;
     uint64_t n_front = 0;
     double grow_unique = 1.0, grow_children = 1.0;
     // bounded memory: frontier halves parked for later (depth first)
@@ -162,14 +168,48 @@ struct d4_mbatch {
         a.max_probe = 256;
         a.c32 = c32.p;
         a.c64 = c64.p;
+        a.budget = budget_steps;
+        a.don_out = don_a.p;
+        a.don_cap = (uint32_t)(don_a.cap / 2);
         return a;
+    }
+
+    // One extend pass over the frontier plus the follow-up launches that run
+    // the work items lanes over their step budget handed over.
+    void extend(uint32_t row, uint64_t tab_m, bool prune, bool count_only) {
+        don_a.reserve(2u << 20);
+        don_b.reserve(2u << 20);
+        launch_extend_multi(args(row, tab_m), prune, count_only, threads, n_cu, st);
+        HIPCHK(hipGetLastError());
+        if (!budget_steps) return;
+        for (uint32_t round = 0;; round++) {
+            uint32_t h[C32_N];
+            HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (h[C32_BAD_ITEM]) throw Error(D4_EDEVICE, "internal error: bad donated work item");
+            if (h[C32_OVERFLOW]) return;  // the step is rerun
+            const uint32_t cap = (uint32_t)(don_a.cap / 2);
+            const uint32_t n =
+                h[C32_DON] <= cap ? h[C32_DON] : (h[C32_DON_END] ? h[C32_DON_END] - 1 : cap);
+            if (!n) return;
+            stats.donated += n;
+            launch_reset_donations(c32.p, c64.p, st);
+            MultiArgs a = args(row, tab_m);
+            a.don_in = don_a.p;
+            a.n_don = n;
+            a.don_out = don_b.p;
+            a.don_cap = (uint32_t)(don_b.cap / 2);
+            a.budget = round < 24 ? std::max(budget_steps / 4, 32u) : 0;
+            launch_extend_multi(a, prune, count_only, threads, n_cu, st);
+            HIPCHK(hipGetLastError());
+            don_a.swap(don_b);
+        }
     }
 
     // Children the frontier will emit after pruning (search only).
     uint64_t count_children(uint32_t row) {
         launch_reset_counters(c32.p, c64.p, st);
-        launch_extend_multi(args(row, 1), !(flags & D4_FLAG_NO_PRUNE), true, threads, n_cu, st);
-        HIPCHK(hipGetLastError());
+        extend(row, 1, !(flags & D4_FLAG_NO_PRUNE), true);
         unsigned long long h64[C64_N];
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -259,7 +299,7 @@ struct d4_mbatch {
             launch_reset_counters(c32.p, c64.p, st);
             HIPCHK(hipEventRecord(evx0, st));
             if (n_front) {
-                launch_extend_multi(args(row, tab_m), prune, false, threads, n_cu, st);
+                extend(row, tab_m, prune, false);
                 HIPCHK(hipGetLastError());
             }
             HIPCHK(hipEventRecord(evx1, st));
@@ -479,6 +519,13 @@ int d4_mbatch_set_launch(d4_mbatch *b, uint32_t threads, uint32_t refill) {
         if (refill < 1 || refill > 64) throw Error(D4_EINVAL, "refill must be 1..64");
         b->threads = threads;
         b->refill = refill;
+    });
+}
+
+int d4_mbatch_set_budget_steps(d4_mbatch *b, uint32_t steps) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        b->budget_steps = steps;
     });
 }
 

@@ -92,9 +92,10 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
     // and hands them to idle lanes by shuffle.  With split levels fix = 1 / 2
     // a work item is a (record, first placement[, second placement]) triple
     // -- 128 / 16384 items per record -- so small frontiers fill the GPU.
-    const uint32_t fix = a.split;
+    const bool don_mode = a.don_in != nullptr;
+    const uint32_t fix = don_mode ? 0u : a.split;
     const uint32_t per_rec = 1u << (7 * fix);
-    const uint64_t n_items = (uint64_t)a.n_in * per_rec;
+    const uint64_t n_items = don_mode ? (uint64_t)a.n_don : (uint64_t)a.n_in * per_rec;
     bool exhausted = false;
     uint32_t buf_items = 0, buf_pos = 0, buf_rec0 = 0;
     uint64_t buf_item0 = 0;
@@ -102,7 +103,12 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
     uint32_t bterm = 0, blib = 0;
 
     bool searching = false, pass = false;
-    uint32_t inst = 0, lib = 0, used = 0, d = 0, c = 0, anc = 0;
+    uint32_t inst = 0, lib = 0, used = 0, d = 0, c = 0, anc = 0, rec = 0;
+    // the lane's work item owns levels >= base_d of the DFS; sib: whether
+    // siblings at level base_d belong to it; steps: DFS steps on the item
+    uint32_t base_d = 0, steps = 0;
+    bool sib = true, no_don = false;
+    uint64_t bit0 = 0, bit1 = 0;  // donated item words (don_mode)
     uint64_t cnt = 0, BW = 0, BWn = 0, E = 0, desc = 0;
     M2 avail{0, 0}, ok{0, 0};
     const MLib *L = a.libs;
@@ -155,6 +161,44 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
         n_pruned += (t >> 1) & 1u;
         n_new += (t >> 2) & 1u;
     };
+    // Hand the rest of this lane's search tree to a follow-up launch: each
+    // untried candidate of the current level as its own item, and at each
+    // owned level below, the candidates after the one in progress.
+    auto donate = [&]() __attribute__((always_inline)) -> bool {
+        uint32_t n = (uint32_t)(__builtin_popcountll(ok.lo) + __builtin_popcountll(ok.hi));
+        for (uint32_t l = base_d; l < d; l++) {
+            const uint32_t kl = (uint32_t)((desc >> (8 * l)) & 0xffu) - 1u;
+            n += (l > base_d || sib) && kl + 1 < (uint32_t)MAX_LIB_CAND;
+        }
+        if (!n) return false;
+        // once the buffer is full nobody adds to the counter again (so it
+        // cannot wrap), and a lane that found no room keeps its item
+        if (__hip_atomic_load(a.c32 + C32_DON, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+            a.don_cap) {
+            no_don = true;
+            return false;
+        }
+        const uint32_t b = atomicAdd(a.c32 + C32_DON, n);
+        if ((uint64_t)b + n > (uint64_t)a.don_cap) {
+            no_don = true;
+            if (b < a.don_cap) a.c32[C32_DON_END] = b + 1;
+            return false;
+        }
+        uint32_t o = b;
+        auto put = [&](uint32_t lvl, uint32_t first, bool single) __attribute__((always_inline)) {
+            const uint64_t pm = lvl >= 8 ? ~0ull : (1ull << (8 * lvl)) - 1ull;
+            a.don_out[2 * o] = (uint64_t)rec | ((uint64_t)first << 32) | ((uint64_t)lvl << 40) |
+                               ((uint64_t)single << 48);
+            a.don_out[2 * o + 1] = desc & pm;
+            o++;
+        };
+        for (M2 m = ok; m_any(m); m_pop(m)) put(d, m_ctz(m), true);
+        for (uint32_t l = base_d; l < d; l++) {
+            const uint32_t kl = (uint32_t)((desc >> (8 * l)) & 0xffu) - 1u;
+            if ((l > base_d || sib) && kl + 1 < (uint32_t)MAX_LIB_CAND) put(l, kl + 1, false);
+        }
+        return true;
+    };
 
     for (;;) {
         uint64_t im = __ballot(!(searching || pass));
@@ -181,13 +225,28 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
                 buf_rec0 = (uint32_t)(base >> (7 * fix));
                 const uint32_t nrec = fix ? 1u : buf_items;
                 if (lane < nrec) {
-                    bk0 = a.in_k0[buf_rec0 + lane];
-                    bk1 = a.in_k1[buf_rec0 + lane];
-                    bcnt = a.in_cnt[buf_rec0 + lane];
+                    uint32_t r0 = buf_rec0 + lane;
+                    bool bad = false;
+                    if (don_mode) {
+                        bit0 = a.don_in[2 * (buf_item0 + lane)];
+                        bit1 = a.don_in[2 * (buf_item0 + lane) + 1];
+                        r0 = (uint32_t)bit0;
+                        bad = r0 >= a.n_in;  // never expected: flagged, skipped
+                        if (bad) {
+                            atomicOr(a.c32 + C32_BAD_ITEM, 1u);
+                            r0 = 0;
+                        }
+                    }
+                    bk0 = a.in_k0[r0];
+                    bk1 = a.in_k1[r0];
+                    bcnt = a.in_cnt[r0];
                     const uint32_t bi = (uint32_t)(bk0 >> MKEY_INST_SHIFT);
                     blib = a.inst_lib[bi];
                     // 1: finished (absorb), 2: dropped from the search (skip)
-                    bterm = a.row >= a.hgt[bi] ? 1u : (a.dropped && a.dropped[bi]) ? 2u : 0u;
+                    bterm = bad ? 2u
+                                : a.row >= a.hgt[bi]           ? 1u
+                                : (a.dropped && a.dropped[bi]) ? 2u
+                                                               : 0u;
                     const uint64_t *w = a.bw + (size_t)bi * a.bw_stride + a.row;
                     bBW = bterm ? 0 : w[0];
                     bBWn = bterm ? 0 : w[1];
@@ -205,9 +264,19 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
             const uint64_t kBWn = shfl64(bBWn, src);
             const uint32_t kterm = __shfl(bterm, (int)src);
             const uint32_t klib = __shfl(blib, (int)src);
+            uint64_t it0 = 0, it1 = 0;
+            if (don_mode) {
+                it0 = shfl64(bit0, src);
+                it1 = shfl64(bit1, src);
+            }
             if (idle && q < buf_items) {
                 const uint32_t sub = (uint32_t)((buf_item0 + q) & (per_rec - 1));
-                const bool lead_item = sub == 0;
+                const bool lead_item = sub == 0 && !don_mode;
+                rec = don_mode ? (uint32_t)it0 : buf_rec0 + (fix ? 0u : q);
+                steps = 0;
+                no_don = false;
+                base_d = 0;
+                sib = fix == 0;
                 cnt = kc;
                 inst = (uint32_t)(k0 >> MKEY_INST_SHIFT);
                 if (kterm) {
@@ -267,10 +336,39 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
                                     desc = (uint64_t)(k0i + 1);
                                     anc = c;
                                     d = 1;
+                                    base_d = 1;
                                     c = (uint32_t)__builtin_ctzll(E & 0xffu);
                                     ok = m_and(m_and(mfit(L, c, E), avail), m_range(k1i, k1i + 1));
                                 }
                             }
+                        }
+                        if (don_mode) {
+                            // replay the item's placements, then take its
+                            // candidates at its level
+                            const uint32_t lvl = (uint32_t)(it0 >> 40) & 0xffu;
+                            const uint32_t first = (uint32_t)(it0 >> 32) & 0xffu;
+                            const bool single = (it0 >> 48) & 1u;
+                            for (uint32_t j = 0; j < lvl; j++) {
+                                const uint32_t kj = ((uint32_t)((it1 >> (8 * j)) & 0xffu) - 1u) & 127u;
+                                const uint64_t shape = L->cand[kj][0], info = L->cand[kj][1];
+                                E &= ~(shape << (c - ((uint32_t)(info >> 32) & 0xffu)));
+                                used += 1u << (info & 0xffu);
+                                if (((used >> (info & 0xffu)) & ((uint32_t)(info >> 8) & 0xffu)) ==
+                                    ((uint32_t)(info >> 16) & 0xffffu)) {
+                                    const M2 r = m_range((uint32_t)(info >> 40) & 0xffu,
+                                                         (uint32_t)(info >> 48) & 0xffu);
+                                    avail.lo &= ~r.lo;
+                                    avail.hi &= ~r.hi;
+                                }
+                                anc |= c << (3 * j);
+                                c = (uint32_t)__builtin_ctzll(E & 0xffu);
+                            }
+                            desc = it1;
+                            d = lvl;
+                            base_d = lvl;
+                            sib = !single;
+                            ok = m_and(m_and(mfit(L, c, E), avail),
+                                       m_range(first, single ? first + 1 : 128u));
                         }
                         searching = m_any(ok);
                     } else {
@@ -293,7 +391,9 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
             used2 = used;
             pass = false;
         } else if (searching) {
-            if (m_any(ok)) {
+            if (a.budget && !no_don && ++steps > a.budget && donate()) {
+                searching = false;
+            } else if (m_any(ok)) {
                 const uint32_t k = m_ctz(ok);
                 m_pop(ok);
                 const uint64_t shape = L->cand[k][0], info = L->cand[k][1];
@@ -321,8 +421,8 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
                     c = (uint32_t)__builtin_ctzll(E & 0xffu);
                     ok = m_and(mfit(L, c, E), avail);
                 }
-            } else if (d + 1 <= max(fix, 1u)) {
-                // levels below `fix` are fixed by the work item
+            } else if (d <= base_d) {
+                // the levels below base_d are fixed by the work item
                 searching = false;
             } else {
                 d--;
@@ -336,8 +436,8 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
                 const M2 r = m_range((uint32_t)(info >> 40) & 0xffu, (uint32_t)(info >> 48) & 0xffu);
                 avail.lo |= r.lo;
                 avail.hi |= r.hi;
-                ok = d >= fix ? m_and(m_and(mfit(L, c, E), avail), m_range(k + 1, 128))
-                              : M2{0, 0};
+                ok = (d > base_d || sib) ? m_and(m_and(mfit(L, c, E), avail), m_range(k + 1, 128))
+                                         : M2{0, 0};
             }
         }
         const uint64_t em = __ballot(emit);
@@ -410,7 +510,8 @@ void launch_extend_multi(const MultiArgs &a, bool prune, bool count_only, uint32
     if (threads != 256 && threads != 512) threads = 1024;
     // 32 waves per CU; the stage takes 4 KiB of LDS per wave
     const uint64_t per_cu = 32u / (threads / 64);
-    const uint64_t waves = (((uint64_t)a.n_in << (7 * a.split)) + 63) / 64;
+    const uint64_t waves =
+        ((a.don_in ? (uint64_t)a.n_don : ((uint64_t)a.n_in << (7 * a.split))) + 63) / 64;
     uint64_t blocks = (waves + threads / 64 - 1) / (threads / 64);
     blocks = std::min<uint64_t>(blocks, (uint64_t)n_cu * per_cu);
     if (!blocks) blocks = 1;

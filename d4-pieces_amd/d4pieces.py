@@ -118,6 +118,7 @@ SIGNATURES = [
       ctypes.POINTER(_P)]),
     ("d4_mbatch_destroy", None, [_P]),
     ("d4_mbatch_set_launch", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    ("d4_mbatch_set_budget_steps", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_mbatch_set_memory", ctypes.c_int, [_P, ctypes.c_uint64]),
     ("d4_mbatch_set_budget", ctypes.c_int, [_P, ctypes.c_uint64]),
     ("d4_mbatch_run", ctypes.c_int, [_P]),
@@ -419,6 +420,10 @@ class MultiBatch:
 
     def set_memory(self, nbytes: int) -> None:
         _check(load().d4_mbatch_set_memory(self._h, nbytes))
+
+    def set_budget_steps(self, steps: int) -> None:
+        """DFS steps per work item before a lane donates the rest (0: never)."""
+        _check(load().d4_mbatch_set_budget_steps(self._h, steps))
 
     def set_budget(self, max_states: int) -> None:
         """States one instance may hold in a frontier before it is dropped

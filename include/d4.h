@@ -238,6 +238,10 @@ int d4_mbatch_create(uint32_t n_inst, const d4_library *const *libs,
 void d4_mbatch_destroy(d4_mbatch *b);
 /* Extend workgroup size (256, 512, 1024) and refill threshold (1..64). */
 int d4_mbatch_set_launch(d4_mbatch *b, uint32_t threads, uint32_t refill);
+/* DFS steps a lane runs on one work item before it hands the rest to a
+ * follow-up launch (0, the default: never), as d4_library_set_budget.
+ * Experimental for multi-library batches: leave it at 0. */
+int d4_mbatch_set_budget_steps(d4_mbatch *b, uint32_t steps);
 /* Device bytes the frontier, its dedup table and parked frontier parts may
  * take (default: 80% of the free device memory when the batch is created).
  * A row step that would exceed it parks half of the frontier and finishes
