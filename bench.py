@@ -290,10 +290,12 @@ def main():
     ext_launch = sum(s["extend_launches"] for s in stats)
     ext_bytes = sum(s["extend_bytes"] for s in stats)
     achieved = ext_bytes / (ext_ms / 1e3) / 1e9 if ext_ms else 0.0
-    traffic = None
+    traffic, dispatches = None, None
     tpath = os.path.join(ROOT, "profiles", "extend_traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_row_step")
+        prof = json.load(open(tpath))
+        traffic = prof.get("hbm_bytes_per_row_step")
+        dispatches = prof.get("dispatches_per_row_step")
 
     out = {
         "metric": METRIC,
@@ -324,6 +326,8 @@ def main():
         "records_per_step": rec // args.steps,
         "children_per_step": sum(s["children"] for s in stats) // args.steps,
         "peak_frontier": max(s["peak_frontier"] for s in stats),
+        "table_reruns_per_step": sum(s["retries"] for s in stats) / args.steps,
+        "donated_items_per_step": sum(s["donated"] for s in stats) / args.steps,
         "latency_ms": {"batched_p50": dt / args.steps * 1e3,
                        "batched_p99": dt / args.steps * 1e3},
         "roofline": {
@@ -339,7 +343,7 @@ def main():
             "per": "row step",
             "avg_launch_ms": ext_ms / max(ext_launch, 1),
             "bytes_per_launch": ext_bytes / max(ext_launch, 1),
-            "dispatches_per_row_step": None,
+            "dispatches_per_row_step": dispatches,
             "compact_GBs": (sum(s["compact_bytes"] for s in stats) /
                             (sum(s["compact_ms"] for s in stats) / 1e3) / 1e9)
             if sum(s["compact_ms"] for s in stats) else None,

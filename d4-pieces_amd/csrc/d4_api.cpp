@@ -514,13 +514,21 @@ struct Engine {
         // exact count from a search-only pass; a large one is estimated from
         // the previous step's children per record, with a rerun at 4x the
         // size if the table still overflows.
+        // (The row-0 bound ignores the lower rows and the piece counts, so
+        // it can exceed the children many times over; past the first rows
+        // the growth estimate is the tighter of the two.)
         uint64_t est_c = 0, est_u = 0;
         bool exact = false;
+        const uint64_t g_c =
+            (uint64_t)(n_front * std::min(std::max(grow_children, 1.0), 64.0) * 1.5) + 4096;
+        const uint64_t g_u =
+            std::min<uint64_t>(g_c, (uint64_t)(n_front * std::max(grow_unique, 0.25) * 1.5) + 4096);
         if (n_front <= (1u << 20)) {
             const uint64_t bound = next_size();
             if (bound <= (1ull << 26)) {
                 est_c = est_u = bound;
                 exact = true;
+                if (stats.rows_done >= 2 && g_u < bound) est_u = g_u;
             } else if (n_front <= kExactCountMax) {
                 est_c = est_u = count_children();
                 stats.count_passes++;
@@ -528,8 +536,8 @@ struct Engine {
             }
         }
         if (!exact) {
-            est_c = (uint64_t)(n_front * std::min(std::max(grow_children, 1.0), 64.0) * 1.5) + 4096;
-            est_u = std::min<uint64_t>(est_c, (uint64_t)(n_front * std::max(grow_unique, 0.25) * 1.5) + 4096);
+            est_c = g_c;
+            est_u = g_u;
         }
         uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
         uint64_t est_e = en ? est_c + 64 : 0;
