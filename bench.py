@@ -328,6 +328,7 @@ def main():
         "peak_frontier": max(s["peak_frontier"] for s in stats),
         "table_reruns_per_step": sum(s["retries"] for s in stats) / args.steps,
         "donated_items_per_step": sum(s["donated"] for s in stats) / args.steps,
+        "table_growths_per_step": sum(s["rehashes"] for s in stats) / args.steps,
         "latency_ms": {"batched_p50": dt / args.steps * 1e3,
                        "batched_p99": dt / args.steps * 1e3},
         "roofline": {

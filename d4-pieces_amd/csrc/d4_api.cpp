@@ -250,13 +250,16 @@ struct Engine {
     DevBuf<uint64_t> e_desc, e_slot;
     DevBuf<uint64_t> don_a, don_b;  // donated work items (two words each)
     uint64_t n_front = 0, new_roots = 0;
-    double grow_unique = 4.0, grow_children = 8.0;
+    double grow_unique = 4.0, grow_children = 8.0, grow_unique_raw = 4.0;
+    DevBuf<unsigned long long> tk2;  // the larger table a spill moves into
+    DevBuf<uint64_t> spill;          // count mode: children that found no slot
+    static constexpr uint64_t kSpillCap = 1u << 22;
     // bounded memory (count mode, run()): frontier halves parked for later
     uint64_t mem_cap = 0;  // 0: unbounded
     struct Part {
         uint32_t g = 0;
         uint64_t n = 0;
-        double grow_unique = 1.0, grow_children = 1.0;
+        double grow_unique = 1.0, grow_children = 1.0, grow_unique_raw = 1.0;
         DevBuf<uint64_t> k, c;
     };
     std::vector<std::unique_ptr<Part>> parked;
@@ -391,6 +394,8 @@ struct Engine {
         a.seed_mask = lib->maxh >= 2 ? ~((1ull << (8 * (lib->maxh - 1))) - 1ull) : ~0ull;
         a.gcd = lib->gcd;
         a.min_size = lib->min_size;
+        a.spill = mode == D4_MODE_COUNT && spill.p ? spill.p : nullptr;
+        a.spill_cap = a.spill ? (uint32_t)std::min<uint64_t>(spill.cap / 2, kSpillCap) : 0;
         a.budget = lib->ext_budget;
         a.don_in = nullptr;
         a.n_don = 0;
@@ -539,7 +544,19 @@ struct Engine {
             est_c = g_c;
             est_u = g_u;
         }
-        uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
+        // Count mode spills children that find no slot and grows the table
+        // afterwards, so it is sized to the growth estimate without padding
+        // (load 35-70% when the estimate holds); enum mode, which needs every
+        // child's slot, keeps the padded size and reruns on overflow.
+        uint64_t tab_m;
+        if (en) {
+            tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
+        } else {
+            uint64_t t = est_u;
+            if (!exact) t = std::min<uint64_t>(g_c, (uint64_t)(n_front * grow_unique_raw * 1.1) + 4096);
+            tab_m = next_pow2(std::max<uint64_t>((uint64_t)(1.4 * (double)t), 1024));
+            spill.reserve(2 * kSpillCap);
+        }
         uint64_t est_e = en ? est_c + 64 : 0;
         if (en && logs.empty()) log0 = g;
         result_save.reserve(S);
@@ -586,6 +603,37 @@ struct Engine {
             }
             bool ovf = h32[C32_OVERFLOW] != 0;
             bool eovf = en && h32[C32_EDGE_OVF] != 0;
+            if (!ovf && !en && h32[C32_SPILL]) {
+                // move the table into one large enough for it and the spill
+                const uint32_t ns = h32[C32_SPILL];
+                const uint64_t m2 = next_pow2(
+                    std::max<uint64_t>(2 * tab_m, 2 * ((uint64_t)h64[C64_NEW] + ns)));
+                bool room = !(mem_cap && 16ull * (tab_m + m2) + 16ull * n_front + parked_bytes >
+                                             mem_cap);
+                if (room) {
+                    try {
+                        tk2.reserve(2 * m2);
+                    } catch (const Error &e) {
+                        if (e.code != D4_ENOMEM) throw;
+                        room = false;
+                    }
+                }
+                if (room) {
+                    launch_rehash(tk.p, tab_m, tk2.p, m2, spill.p, ns, c32.p, c64.p, st);
+                    HIPCHK(hipGetLastError());
+                    HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipStreamSynchronize(st));
+                    ovf = h32[C32_OVERFLOW] != 0;
+                    if (!ovf) {
+                        tk.swap(tk2);
+                        tab_m = m2;
+                        stats.rehashes++;
+                    }
+                } else {
+                    ovf = true;
+                }
+            }
             if (!ovf && !eovf) break;
             // the step is a pure function of the frontier: rerun it bigger
             stats.retries++;
@@ -608,6 +656,7 @@ struct Engine {
         stats.unique += U;
         if (n_front) {
             grow_unique = std::max(1.0, (double)U / (double)n_front);
+            grow_unique_raw = std::max(0.01, (double)U / (double)n_front);
             grow_children = std::max(1.0, (double)h64[C64_CHILDREN] / (double)n_front);
         }
         if (en) slot_uid.reserve(tab_m);
@@ -662,6 +711,7 @@ struct Engine {
         p->g = g;
         p->n = m;
         p->grow_unique = grow_unique;
+        p->grow_unique_raw = grow_unique_raw;
         p->grow_children = grow_children;
         p->k.reserve(m);
         p->c.reserve(m);
@@ -684,6 +734,7 @@ struct Engine {
         n_front = p->n;
         g = p->g;
         grow_unique = p->grow_unique;
+        grow_unique_raw = p->grow_unique_raw;
         grow_children = p->grow_children;
         new_roots = 0;
     }

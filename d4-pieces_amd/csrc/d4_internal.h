@@ -27,6 +27,7 @@ enum : int {
     C32_OUT = 4,       // compaction output cursor
     C32_DON = 5,       // donated work items written (extend)
     C32_DON_END = 6,   // 1 + end of the valid donated items when C32_DON passed the capacity
+    C32_SPILL = 7,     // children that found no slot within the probe limit (count mode)
     C32_N = 8,
 };
 enum : int {
@@ -91,6 +92,11 @@ struct ExtendArgs {
     uint32_t n_don;
     uint64_t *don_out;       // [don_cap][2]
     uint32_t don_cap;
+    // count mode: a child that finds no slot within the probe limit goes to
+    // the spill list (key, count) instead of failing the step; the host then
+    // moves the table into a larger one and inserts the spill (0: no spill)
+    uint64_t *spill;         // [spill_cap][2]
+    uint32_t spill_cap;
 };
 
 // Tables of one library in a multi-library batch (device layout, 16-byte
@@ -148,6 +154,13 @@ void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
 void launch_reset_work(unsigned long long *c64, hipStream_t st);
 void launch_reset_donations(uint32_t *c32, unsigned long long *c64, hipStream_t st);
+// Move the occupied slots of a {key, count} table of m_old slots into a fresh
+// table of m_new slots, then insert n_spill spilled (key, count) records;
+// new keys of the spill are counted into c64[C64_NEW], a failed insert sets
+// c32[C32_OVERFLOW].
+void launch_rehash(const unsigned long long *old_tab, uint64_t m_old, unsigned long long *new_tab,
+                   uint64_t m_new, const uint64_t *spill, uint32_t n_spill, uint32_t *c32,
+                   unsigned long long *c64, hipStream_t st);
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
                    bool count_only, uint32_t threads, int n_cu, hipStream_t st);
 constexpr uint32_t COMPACT_MAX_BLOCKS = 65536;

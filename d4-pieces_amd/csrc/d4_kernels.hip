@@ -180,6 +180,16 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
             t |= 1u;
             if (!COUNT_ONLY) {
                 int r = table_insert(a, k2, c2, slot);
+                if (r < 0 && !ENUM && a.spill_cap) {
+                    // no slot within the probe limit: park the child for the
+                    // host's table growth (counted as new there)
+                    const uint32_t sp = atomicAdd(a.c32 + C32_SPILL, 1u);
+                    if (sp < a.spill_cap) {
+                        a.spill[2 * sp] = k2;
+                        a.spill[2 * sp + 1] = c2;
+                        r = 0;
+                    }
+                }
                 if (r < 0) failed = true;
                 else t |= r ? 4u : 0u;
             }
@@ -528,6 +538,48 @@ __global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
 
 __global__ void k_reset_work(unsigned long long *c64) { c64[C64_WORK] = 0; }
 
+// Insert (key, cnt) into a {key, count} table of any power-of-two size, as
+// table_insert does.
+__device__ __forceinline__ int insert_at(unsigned long long *tab, uint64_t mask, uint64_t key,
+                                         uint64_t cnt) {
+    uint64_t h = mix64(key) & mask;
+    for (uint32_t p = 0; p < 256; p++) {
+        const uint64_t cur = atomicCAS(tab + 2 * h, (unsigned long long)EMPTY_KEY,
+                                       (unsigned long long)key);
+        if (cur == EMPTY_KEY || cur == key) {
+            atomicAdd(tab + 2 * h + 1, (unsigned long long)cnt);
+            return cur == EMPTY_KEY ? 1 : 0;
+        }
+        h = (h + 1) & mask;
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(256) void k_rehash_table(const unsigned long long *old_tab,
+                                                      uint64_t m_old, unsigned long long *new_tab,
+                                                      uint64_t m_new, uint32_t *c32) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m_old;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = old_tab[2 * i];
+        if (k == EMPTY_KEY) continue;
+        if (insert_at(new_tab, m_new - 1, k, old_tab[2 * i + 1]) < 0)
+            atomicOr(c32 + C32_OVERFLOW, 1u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_insert_spill(const uint64_t *spill, uint32_t n,
+                                                      unsigned long long *tab, uint64_t m,
+                                                      uint32_t *c32, unsigned long long *c64) {
+    uint64_t fresh = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int r = insert_at(tab, m - 1, spill[2 * i], spill[2 * i + 1]);
+        if (r < 0) atomicOr(c32 + C32_OVERFLOW, 1u);
+        else fresh += (uint64_t)r;
+    }
+    fresh = wave_sum_u64(fresh);
+    if (lane_id() == 0 && fresh) atomicAdd(c64 + C64_NEW, (unsigned long long)fresh);
+}
+
 __global__ void k_reset_donations(uint32_t *c32, unsigned long long *c64) {
     c32[C32_DON] = 0;
     c32[C32_DON_END] = 0;
@@ -771,6 +823,20 @@ void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t s
 
 void launch_reset_work(unsigned long long *c64, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, st, c64);
+}
+
+void launch_rehash(const unsigned long long *old_tab, uint64_t m_old, unsigned long long *new_tab,
+                   uint64_t m_new, const uint64_t *spill, uint32_t n_spill, uint32_t *c32,
+                   unsigned long long *c64, hipStream_t st) {
+    launch_fill_table(new_tab, new_tab + 1, m_new, st);
+    const uint64_t b1 = std::min<uint64_t>(std::max<uint64_t>((m_old + 255) / 256, 1), 65536);
+    hipLaunchKernelGGL(k_rehash_table, dim3((uint32_t)b1), dim3(256), 0, st, old_tab, m_old,
+                       new_tab, m_new, c32);
+    if (n_spill) {
+        const uint32_t b2 = (uint32_t)std::min<uint64_t>((n_spill + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_insert_spill, dim3(b2), dim3(256), 0, st, spill, n_spill, new_tab,
+                           m_new, c32, c64);
+    }
 }
 
 void launch_reset_donations(uint32_t *c32, unsigned long long *c64, hipStream_t st) {

@@ -56,6 +56,7 @@ class Stats(ctypes.Structure):
         ("count_passes", ctypes.c_uint64),
         ("splits", ctypes.c_uint64),
         ("donated", ctypes.c_uint64),
+        ("rehashes", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -74,6 +74,7 @@ typedef struct d4_stats {
     uint64_t count_passes;     /* search-only passes sizing the dedup table */
     uint64_t splits;           /* frontier halves parked to stay in the memory cap */
     uint64_t donated;          /* work items handed between extend launches */
+    uint64_t rehashes;         /* dedup tables grown after children spilled */
 } d4_stats;
 
 /* Thread-local text of the last error. */
