@@ -399,6 +399,8 @@ struct Engine {
         a.budget = lib->ext_budget;
         a.don_in = nullptr;
         a.n_don = 0;
+        a.don_read = C32_DON;
+        a.don_write = C32_DON;
         a.don_out = don_a.p;
         a.don_cap = (uint32_t)(don_a.cap / 2);
         return a;
@@ -409,7 +411,12 @@ struct Engine {
     // budget is lifted so the chain ends).
     void run_donations(uint64_t tab_m, uint32_t e_cap, bool en, bool prune, bool count_only) {
         if (!lib->ext_budget) return;
-        for (uint32_t round = 0;; round++) {
+        // Rounds are queued a few at a time without a host round trip: each
+        // reads its item count from the counter pair the round before it
+        // wrote, and the two pairs alternate.
+        constexpr uint32_t kRoundsPerSync = 6;
+        uint32_t rd = C32_DON;  // the pair that counted the items in don_a
+        for (uint32_t round = 0;;) {
             uint32_t h[C32_N];
             HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -417,20 +424,25 @@ struct Engine {
             if (h[C32_OVERFLOW]) return;  // the step is rerun
             const uint32_t cap = (uint32_t)(don_a.cap / 2);
             const uint32_t n =
-                h[C32_DON] <= cap ? h[C32_DON] : (h[C32_DON_END] ? h[C32_DON_END] - 1 : cap);
+                h[rd] <= cap ? h[rd] : (h[rd + 1] ? h[rd + 1] - 1 : cap);
             if (!n) return;
-            stats.donated += n;
-            launch_reset_donations(c32.p, c64.p, st);
-            ExtendArgs a = args(tab_m, e_cap, 0, n_front, 0);
-            a.don_in = don_a.p;
-            a.n_don = n;
-            a.don_out = don_b.p;
-            a.don_cap = (uint32_t)(don_b.cap / 2);
-            // follow-up items are smaller: split them sooner
-            a.budget = round < 24 ? std::max(lib->ext_budget / 4, 32u) : 0;
-            launch_extend(a, en, prune, count_only, lib->ext_threads, lib->n_cu, st);
-            HIPCHK(hipGetLastError());
-            don_a.swap(don_b);
+            for (uint32_t q = 0; q < kRoundsPerSync; q++, round++) {
+                const uint32_t wr = rd == C32_DON ? C32_DON1 : C32_DON;
+                launch_reset_donation_pair(c32.p, c64.p, wr, st);
+                ExtendArgs a = args(tab_m, e_cap, 0, n_front, 0);
+                a.don_in = don_a.p;
+                a.n_don = q == 0 ? n : ~0u;
+                a.don_read = rd;
+                a.don_write = wr;
+                a.don_out = don_b.p;
+                a.don_cap = (uint32_t)(don_b.cap / 2);
+                // follow-up items are smaller: split them sooner
+                a.budget = round < 24 ? std::max(lib->ext_budget / 4, 32u) : 0;
+                launch_extend(a, en, prune, count_only, lib->ext_threads, lib->n_cu, st);
+                HIPCHK(hipGetLastError());
+                don_a.swap(don_b);
+                rd = wr;
+            }
         }
     }
 
@@ -652,6 +664,7 @@ struct Engine {
         }
         stats.children += h64[C64_CHILDREN];
         stats.pruned += h64[C64_PRUNED];
+        stats.donated += h64[C64_DONATED];
         stats.records_extended += n_front;
         stats.unique += U;
         if (n_front) {

@@ -28,7 +28,9 @@ enum : int {
     C32_DON = 5,       // donated work items written (extend)
     C32_DON_END = 6,   // 1 + end of the valid donated items when C32_DON passed the capacity
     C32_SPILL = 7,     // children that found no slot within the probe limit (count mode)
-    C32_N = 8,
+    C32_DON1 = 8,      // second donation counter pair (follow-up rounds alternate)
+    C32_DON_END1 = 9,
+    C32_N = 12,
 };
 enum : int {
     C64_CHILDREN = 0,
@@ -36,7 +38,8 @@ enum : int {
     C64_PRUNED = 2,
     C64_NEXTSIZE = 3,
     C64_WORK = 4,      // dynamic work-item claim (extend)
-    C64_N = 5,
+    C64_DONATED = 5,   // work items handed to follow-up launches
+    C64_N = 6,
 };
 
 // Arguments of the extend kernel (passed by value).
@@ -89,7 +92,9 @@ struct ExtendArgs {
     // instead of records
     uint32_t budget;         // 0: never donate
     const uint64_t *don_in;  // [n_don][2] or null
-    uint32_t n_don;
+    uint32_t n_don;          // items in don_in, or ~0u: read them from c32[don_read]
+    uint32_t don_read;       // counter pair (C32_DON or C32_DON1) that counted don_in
+    uint32_t don_write;      // counter pair this launch's donations count into
     uint64_t *don_out;       // [don_cap][2]
     uint32_t don_cap;
     // count mode: a child that finds no slot within the probe limit goes to
@@ -154,6 +159,9 @@ void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
 void launch_reset_work(unsigned long long *c64, hipStream_t st);
 void launch_reset_donations(uint32_t *c32, unsigned long long *c64, hipStream_t st);
+// reset one donation counter pair (C32_DON or C32_DON1) and the work counter
+void launch_reset_donation_pair(uint32_t *c32, unsigned long long *c64, uint32_t pair,
+                                hipStream_t st);
 // Move the occupied slots of a {key, count} table of m_old slots into a fresh
 // table of m_new slots, then insert n_spill spilled (key, count) records;
 // new keys of the spill are counted into c64[C64_NEW], a failed insert sets
