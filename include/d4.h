@@ -239,6 +239,10 @@ int d4_mbatch_create(uint32_t n_inst, const d4_library *const *libs,
 void d4_mbatch_destroy(d4_mbatch *b);
 /* Extend workgroup size (256, 512, 1024) and refill threshold (1..64). */
 int d4_mbatch_set_launch(d4_mbatch *b, uint32_t threads, uint32_t refill);
+/* Measured autotuner: runs a batch of the first `sample` instances (0: 256)
+ * with each workgroup size and refill threshold, keeps the fastest on b. */
+int d4_mbatch_tune(d4_mbatch *b, uint32_t sample, uint32_t *threads,
+                   uint32_t *refill, double *best_ms);
 /* DFS steps a lane runs on one work item before it hands the rest to a
  * follow-up launch (0, the default: never), as d4_library_set_budget.
  * Experimental for multi-library batches: leave it at 0. */

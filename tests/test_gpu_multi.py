@@ -141,6 +141,25 @@ def test_budget_drops_instances(gpu):
             assert got[i] == 0 and peak2[i] > budget
 
 
+def test_tune_keeps_counts(gpu, oracle):
+    """The multi-batch autotuner picks a launch shape from its sample; the
+    counts do not depend on it."""
+    rng = random.Random(8)
+    pool = [3, 7, 259] + gpu.free_polyominoes(4)
+    insts = []
+    for _ in range(40):
+        shapes = sorted(rng.sample(pool, rng.randint(2, 5)))
+        counts = [rng.randint(1, 2) for _ in shapes]
+        w, h = rng.randint(3, 7), rng.randint(2, 6)
+        insts.append((shapes, counts, rect(w, h), w))
+    libs = [gpu.PieceLibrary(s, c, 0xFF, w) for s, c, _, w in insts]
+    b = gpu.MultiBatch(libs, [x[2] for x in insts])
+    t = b.tune(16)
+    assert t["threads"] in (256, 512, 1024) and t["refill"] in (8, 16, 32)
+    b.run()
+    assert b.counts() == [oracle.count(s, c, r) for s, c, r, _ in insts]
+
+
 def test_edge_cases(gpu, oracle):
     p4 = gpu.free_polyominoes(4)
     insts = [
