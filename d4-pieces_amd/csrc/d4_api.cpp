@@ -546,7 +546,9 @@ struct Engine {
                 est_c = est_u = bound;
                 exact = true;
                 if (stats.rows_done >= 2 && g_u < bound) est_u = g_u;
-            } else if (n_front <= kExactCountMax) {
+            } else if (n_front <= kExactCountMax && (en || stats.rows_done < 2)) {
+                // (count mode past the first rows sizes from the growth
+                // estimate instead: a short table spills and grows)
                 est_c = est_u = count_children();
                 stats.count_passes++;
                 exact = true;

@@ -13,6 +13,8 @@ Workloads (boards are lists of 8-bit row masks, bit c = column c open):
   large_8x15()        12 free pentominoes x2 on 8 wide x 15 rows
   uneven_instances()  4096 puzzles, each with its own 6-14 polyominoes of
                       sizes 3-6 (multiplicity 1-2) and a 4-8 wide board
+  calendar_boards()   the reference's calendar puzzle: 12 x 31 boards, 8 pieces
+  board_configs(text) the reference's board-file region format, expanded
   request_stream(...) Poisson arrivals of 8x8 boards with 30-60 open cells in
                       one region (area a multiple of 5), 70% first-solution
 """
@@ -197,6 +199,117 @@ def uneven_instances(n: int = 4096, seed: int = 3):
             rows[v // W] &= ~(1 << (v % W)) & 0xFF
         out.append((shapes, counts, rows, W))
     return out
+
+
+def normalize8(v: int) -> int:
+    """Translate a shape (8-wide encoding) to the top-left corner."""
+    if not v:
+        return 0
+    while not v & 0xFF:
+        v >>= 8
+    while not v & 0x0101010101010101:
+        v >>= 1
+    return v
+
+
+def transform8(v: int, t: int) -> int:
+    """D4 transform t in the reference's order (src/Shape.cpp:6-40: bit 2 =
+    transpose, bit 0 = flip X, bit 1 = flip Y, applied to the input's
+    coordinates), normalised."""
+    sw, fx, fy = t & 4, t & 1, t & 2
+    out = 0
+    for b in range(64):
+        if v >> b & 1:
+            y, x = divmod(b, 8)
+            yy, xx = (7 - y if fy else y), (7 - x if fx else x)
+            r, c = (xx, yy) if sw else (yy, xx)
+            out |= 1 << (r * 8 + c)
+    return normalize8(out)
+
+
+# The reference's calendar puzzle (src/main.cpp:14-42): 8 pieces, a board of
+# 6 + 6 month cells and 31 day cells (the 8x8 value below turned by 180
+# degrees), and for each month/day the board with those two cells removed.
+CALENDAR_PIECES = [
+    0b11110000_10000000,           # L
+    0b11100000_00110000,           # Z
+    0b01100000_01000000_11000000,  # S
+    0b10000000_10000000_11100000,  # L_
+    0b11100000_11100000,           # rect
+    0b11100000_11000000,           # rect-
+    0b11100000_10100000,           # C
+    0b11110000_00100000,           # |-
+]
+CALENDAR_BOARD = 0b11111100_11111100_11111110_11111110_11111110_11111110_11100000
+
+
+def calendar_boards():
+    """(shapes, counts, [(month, day, rows)], width): the 12 x 31 calendar
+    boards, rows as 8-bit masks (width 7)."""
+    shapes = [normalize8(s) for s in CALENDAR_PIECES]
+    board = transform8(CALENDAR_BOARD, 3)  # flip X and Y (src/main.cpp:37)
+    out = []
+    for m in range(1, 13):
+        for d in range(1, 32):
+            b = board & ~(1 << ((m - 1) // 6 * 8 + (m - 1) % 6))
+            b &= ~(1 << (((d - 1) // 7 + 2) * 8 + (d - 1) % 7))
+            rows = [(b >> (8 * r)) & 0xFF for r in range(8)]
+            while rows and not rows[-1]:
+                rows.pop()
+            out.append((m, d, rows))
+    return shapes, [1] * len(shapes), out, 7
+
+
+def board_configs(text: str, width: int = 8):
+    """The board-file region format (src/board.hpp:15-57): every letter or
+    digit marks a cell of the region of that character, other characters
+    are read as Shape text ('#', '*', '1', '@', 'X', '%' filled; ' ', '.',
+    '0', '-', '_' empty; newline = next row).  The base board fills every
+    region cell; one configuration removes one cell of every region (in
+    region order of first appearance, cells in bit order).  Returns
+    (count, list of row-mask lists)."""
+    def valid(ch):
+        return ch.isascii() and ch.isalnum()
+
+    def parse(sv):
+        if sv.startswith("\n"):
+            sv = sv[1:]
+        value, row, bit = 0, 0, 0
+        for ch in sv:
+            if ch in " 0.-_":
+                bit += 1
+            elif ch in "*1@#X%":
+                value |= 1 << (row * width + bit)
+                bit += 1
+            elif ch == "\n":
+                row, bit = row + 1, 0
+        return value
+
+    keys = []
+    for ch in text:
+        if valid(ch) and ch not in keys:
+            keys.append(ch)
+    base = parse("".join("#" if valid(ch) else ch for ch in text))
+    regions = [parse("".join(("#" if ch == k else ".") if valid(ch) else ch for ch in text))
+               for k in keys]
+    configs = []
+
+    def rec(i, cur):
+        if i == len(regions):
+            rows = [(cur >> (width * r)) & ((1 << width) - 1) for r in range(64 // width)]
+            while rows and not rows[-1]:
+                rows.pop()
+            configs.append(rows)
+            return
+        for b in range(64):
+            if regions[i] >> b & 1:
+                rec(i + 1, cur & ~(1 << b))
+
+    rec(0, base)
+    count = 1
+    for r in regions:
+        count *= bin(r).count("1")
+    return count, configs
 
 
 def request_stream(n: int, rate: float, seed: int = 4):

@@ -243,3 +243,16 @@ def test_doubled_pentominoes(gpu, oracle):
         s.run()
         assert s.count() == want, (w, h)
 
+
+def test_calendar_puzzle_batch(gpu, oracle):
+    """The reference's own workload (src/main.cpp:14-42): the 12 x 31
+    calendar boards of one 8-piece library on a 7-wide board, batched in one
+    frontier, against the oracle board by board."""
+    shapes, counts, boards, width = workloads.calendar_boards()
+    lib = gpu.PieceLibrary(shapes, counts, 0xFF, width)
+    b = gpu.Batch(lib, [rows for _, _, rows in boards])
+    b.run()
+    want = [oracle.count(shapes, counts, rows) for _, _, rows in boards]
+    assert b.counts() == want
+    assert max(want) == 216
+
