@@ -13,6 +13,9 @@ benchmarks read like the reference's own driver code:
                       src/searcher.cpp:86-107, as one shared device frontier
   MultiBatch       <- the same with a piece library per instance (the
                       reference's library is a process global, src/frow.h:8-9)
+  Solution, expand_solution
+                   <- Solution<L> / Step<L> with the cell map
+                      (src/Piece.hpp:31-48, src/Piece.cpp:25-54)
 
 There is no fallback: if libd4.so is missing or no HIP device is visible the
 calls raise.
@@ -196,6 +199,78 @@ def _u8arr(v):
 
 
 Step = Tuple[int, int, int, int]  # (piece_id, trs_id, y, x)
+
+
+def normalize8(v: int) -> int:
+    """Translate a shape (8-wide encoding, bit = row*8 + col) to the top-left
+    corner (Shape::normalize)."""
+    if not v:
+        return 0
+    while not v & 0xFF:
+        v >>= 8
+    while not v & 0x0101010101010101:
+        v >>= 1
+    return v
+
+
+def transform8(v: int, t: int) -> int:
+    """D4 transform t in the reference's order (src/Shape.cpp:6-40: bit 2 =
+    transpose, bit 0 = flip X, bit 1 = flip Y, applied to the input's
+    coordinates), normalised."""
+    sw, fx, fy = t & 4, t & 1, t & 2
+    out = 0
+    for b in range(64):
+        if v >> b & 1:
+            y, x = divmod(b, 8)
+            yy, xx = (7 - y if fy else y), (7 - x if fx else x)
+            r, c = (xx, yy) if sw else (yy, xx)
+            out |= 1 << (r * 8 + c)
+    return normalize8(out)
+
+
+# make_step's matrix per transform (src/Piece.cpp:41-54)
+_STEP_MATRIX = [(1, 0, 0, 1), (-1, 0, 0, 1), (1, 0, 0, -1), (-1, 0, 0, -1),
+                (0, 1, 1, 0), (0, -1, 1, 0), (0, 1, -1, 0), (0, -1, -1, 0)]
+
+
+class FullStep:
+    """Step<L> (src/Piece.hpp:31-36): piece id, transform id, the transform's
+    2x2 matrix a, b, c, d, the translation x, y and the placed shape."""
+    __slots__ = ("piece_id", "trs_id", "a", "b", "c", "d", "x", "y", "shape")
+
+    def __init__(self, piece_id, trs_id, a, b, c, d, x, y, shape):
+        self.piece_id, self.trs_id = piece_id, trs_id
+        self.a, self.b, self.c, self.d, self.x, self.y = a, b, c, d, x, y
+        self.shape = shape
+
+    def __repr__(self):
+        return (f"Step(piece={self.piece_id}, trs={self.trs_id}, x={self.x}, y={self.y}, "
+                f"shape={self.shape:#x})")
+
+
+class Solution:
+    """Solution<L> (src/Piece.hpp:38-48, src/Piece.cpp:25-39): the steps and
+    map[row][col] = index of the step covering the cell, -1 for none
+    (8 columns, one row per board row)."""
+
+    def __init__(self, steps: List[FullStep], height: int):
+        self.steps = steps
+        self.map = [[next((i for i, st in enumerate(steps) if st.shape >> (r * 8 + c) & 1), -1)
+                     for c in range(8)] for r in range(height)]
+
+
+def expand_solution(shapes: Sequence[int], raw: Sequence[Step], height: int,
+                    sym: int = 0xFF) -> Solution:
+    """A solution as returned by solutions() ((piece_id, trs_id, y, x) per
+    step) with the steps' matrices, placed shapes and the cell map.  trs_id
+    counts the transforms enabled in `sym` (Piece.cpp:12-22)."""
+    enabled = [t for t in range(8) if sym >> t & 1]
+    steps = []
+    for pid, trs, y, x in raw:
+        t = enabled[trs]
+        shape = transform8(shapes[pid], t) << (8 * y + x)
+        steps.append(FullStep(pid, trs, *_STEP_MATRIX[t], x, y, shape))
+    return Solution(steps, height)
 
 
 def decode_solutions(buf: bytes) -> List[List[Step]]:

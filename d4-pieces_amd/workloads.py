@@ -201,30 +201,7 @@ def uneven_instances(n: int = 4096, seed: int = 3):
     return out
 
 
-def normalize8(v: int) -> int:
-    """Translate a shape (8-wide encoding) to the top-left corner."""
-    if not v:
-        return 0
-    while not v & 0xFF:
-        v >>= 8
-    while not v & 0x0101010101010101:
-        v >>= 1
-    return v
-
-
-def transform8(v: int, t: int) -> int:
-    """D4 transform t in the reference's order (src/Shape.cpp:6-40: bit 2 =
-    transpose, bit 0 = flip X, bit 1 = flip Y, applied to the input's
-    coordinates), normalised."""
-    sw, fx, fy = t & 4, t & 1, t & 2
-    out = 0
-    for b in range(64):
-        if v >> b & 1:
-            y, x = divmod(b, 8)
-            yy, xx = (7 - y if fy else y), (7 - x if fx else x)
-            r, c = (xx, yy) if sw else (yy, xx)
-            out |= 1 << (r * 8 + c)
-    return normalize8(out)
+from d4pieces import normalize8, transform8  # noqa: E402  (geometry helpers)
 
 
 # The reference's calendar puzzle (src/main.cpp:14-42): 8 pieces, a board of

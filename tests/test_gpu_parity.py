@@ -255,4 +255,13 @@ def test_calendar_puzzle_batch(gpu, oracle):
     want = [oracle.count(shapes, counts, rows) for _, _, rows in boards]
     assert b.counts() == want
     assert max(want) == 216
+    # January 25: the full solution set, and each solution's cell map
+    rows = boards[24][2]
+    s = gpu.CudaSearcher(lib, rows=rows, mode=gpu.MODE_ENUM)
+    s.run()
+    sols = s.solutions()
+    assert sols == oracle.solve(shapes, counts, rows)
+    for raw in sols[:5]:
+        sol = gpu.expand_solution(shapes, raw, len(rows))
+        assert sum(1 for r in sol.map for v in r if v >= 0) == 41
 

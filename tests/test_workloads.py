@@ -34,3 +34,30 @@ def test_board_configs_region_format():
     assert configs[0] == [0b010, 0b110]
     count, configs = workloads.board_configs("ab\n.#")
     assert count == 4  # each region also holds the '#' cell
+
+
+def test_solution_map_and_steps(oracle):
+    """Solution / Step expansion (src/Piece.hpp:31-48, src/Piece.cpp:25-54):
+    the map covers exactly the board's open cells, each step's placed shape
+    is a transform of its piece of the right size, and the matrices are
+    make_step's (oracle: or_step_matrix)."""
+    import ctypes
+    import d4pieces
+    shapes, counts, boards, width = workloads.calendar_boards()
+    m, d, rows = boards[24]  # January 25
+    sols = oracle.solve(shapes, counts, rows)
+    assert len(sols) == 216
+    for raw in sols[::27]:
+        s = d4pieces.expand_solution(shapes, raw, len(rows))
+        cells = {(r, c) for r, v in enumerate(rows) for c in range(8) if v >> c & 1}
+        covered = {(r, c) for r in range(len(rows)) for c in range(8) if s.map[r][c] >= 0}
+        assert covered == cells
+        for i, st in enumerate(s.steps):
+            assert bin(st.shape).count("1") == bin(shapes[st.piece_id]).count("1")
+            assert d4pieces.normalize8(st.shape) == d4pieces.transform8(
+                shapes[st.piece_id], st.trs_id)
+            mat = (ctypes.c_int * 4)()
+            oracle.L.or_step_matrix(st.trs_id, mat)
+            assert (st.a, st.b, st.c, st.d) == tuple(mat)
+            assert all(s.map[r][c] == i for r in range(len(rows)) for c in range(8)
+                       if st.shape >> (8 * r + c) & 1)
