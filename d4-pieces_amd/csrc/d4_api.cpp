@@ -1194,33 +1194,7 @@ int d4_device_synchronize(void) {
 }
 
 size_t d4_free_polyominoes(unsigned n, uint64_t *out, size_t cap) {
-    if (n < 1 || n > 8) return 0;
-    // grow every (n-1)-cell free polyomino by one 4-neighbour cell and keep
-    // the canonical forms
-    std::vector<uint64_t> cur{1};
-    for (unsigned k = 2; k <= n; k++) {
-        std::vector<uint64_t> nxt;
-        for (uint64_t s0 : cur) {
-            uint64_t s = s0 << 9;  // one free row above and column left
-            for (uint64_t b = s; b; b &= b - 1) {
-                int i = __builtin_ctzll(b), r = i / 8, c = i % 8;
-                const int nb[4][2] = {{r - 1, c}, {r + 1, c}, {r, c - 1}, {r, c + 1}};
-                for (auto &q : nb) {
-                    if (q[0] < 0 || q[0] > 7 || q[1] < 0 || q[1] > 7) continue;
-                    uint64_t bit = 1ull << (q[0] * 8 + q[1]);
-                    if (s & bit) continue;
-                    uint64_t v = ~0ull;
-                    for (unsigned t = 0; t < 8; t++) v = std::min(v, transform8(s | bit, t));
-                    nxt.push_back(v);
-                }
-            }
-        }
-        std::sort(nxt.begin(), nxt.end());
-        nxt.erase(std::unique(nxt.begin(), nxt.end()), nxt.end());
-        cur.swap(nxt);
-    }
-    for (size_t i = 0; i < cur.size() && i < cap && out; i++) out[i] = cur[i];
-    return cur.size();
+    return d4_polyominoes(n, 0xffu, out, cap);
 }
 
 int d4_library_create(const uint64_t *shapes, const uint32_t *counts, uint32_t n,

@@ -30,6 +30,7 @@ enum : int {
     C32_SPILL = 7,     // children that found no slot within the probe limit (count mode)
     C32_DON1 = 8,      // second donation counter pair (follow-up rounds alternate)
     C32_DON_END1 = 9,
+    C32_SINK = 10,     // terminal states written to the sink (multi-library batches)
     C32_N = 12,
 };
 enum : int {
@@ -150,6 +151,10 @@ struct MultiArgs {
     uint32_t n_don;
     uint64_t *don_out;
     uint32_t don_cap;
+    // terminal states of finished instances, two words each: instance |
+    // counters << 32, and the state's count (null: not kept)
+    uint64_t *sink;
+    uint32_t sink_cap;
 };
 
 // Launchers (d4_kernels.hip, d4_multi.hip).

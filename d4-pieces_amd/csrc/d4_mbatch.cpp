@@ -121,6 +121,12 @@ struct d4_mbatch {
     std::vector<unsigned long long> h_hist;
     DevBuf<uint8_t> dropped;
     DevBuf<unsigned long long> hist;
+    // terminal states kept for d4_mbatch_terminals: sink_n entries of two
+    // words (instance | counters << 32, count) written by finished steps
+    bool keep_terminals = false;
+    DevBuf<uint64_t> sink;
+    uint64_t sink_n = 0, sink_cap = 0, sink_init = 0;  // entries
+    std::vector<uint64_t> h_sink;
     static constexpr uint64_t kExactCountMax = 1u << 16;
     bool done = false;
     bool trace = getenv("D4_TRACE") != nullptr;  // per-row lines on stderr
@@ -173,6 +179,10 @@ struct d4_mbatch {
         a.budget = budget_steps;
         a.don_out = don_a.p;
         a.don_cap = (uint32_t)(don_a.cap / 2);
+        if (keep_terminals) {
+            a.sink = sink.p + 2 * sink_n;
+            a.sink_cap = (uint32_t)std::min<uint64_t>(sink_cap - sink_n, 0xffffffffull);
+        }
         return a;
     }
 
@@ -221,6 +231,8 @@ struct d4_mbatch {
 
     // Device bytes of a row step with a table of tab_m slots and u children.
     uint64_t need(uint64_t tab_m, uint64_t u) const {
+        // (the terminal-state sink, 16 B per state kept, is output and
+        // outside the cap: splitting the frontier would not shrink it)
         return 32ull * tab_m + 24ull * u + 24ull * n_front + parked_bytes;
     }
 
@@ -322,11 +334,20 @@ struct d4_mbatch {
                         row, (unsigned long long)n_front, (unsigned long long)tab_m,
                         (unsigned long long)h64[C64_CHILDREN], (unsigned long long)h64[C64_NEW],
                         xms, h32[C32_OVERFLOW] ? " overflow" : "");
-            if (!h32[C32_OVERFLOW]) break;
+            if (!h32[C32_OVERFLOW]) {
+                if (!keep_terminals || sink_n + h32[C32_SINK] <= sink_cap) break;
+                // the sink is too small for this step's terminal states:
+                // grow it (keeping what earlier steps wrote) and rerun
+                grow_sink(sink_n + h32[C32_SINK]);
+                stats.retries++;
+                HIPCHK(hipMemcpy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice));
+                continue;
+            }
             stats.retries++;
             HIPCHK(hipMemcpy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice));
             tab_m *= 4;
         }
+        if (keep_terminals) sink_n += h32[C32_SINK];
         const uint64_t U = h64[C64_NEW];
         try {
             n0.reserve(U);
@@ -373,6 +394,43 @@ struct d4_mbatch {
         return true;
     }
 
+    void grow_sink(uint64_t entries) {
+        const uint64_t cap = entries + entries / 2 + 1024;
+        if (2 * cap > sink.cap) {
+            DevBuf<uint64_t> nb;
+            nb.reserve(2 * cap);
+            if (sink_n)
+                HIPCHK(hipMemcpyAsync(nb.p, sink.p, 16ull * sink_n, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+            sink.swap(nb);
+        }
+        sink_cap = cap;
+        stats.sink_growths++;
+    }
+
+    // Distinct terminal states of finished instances, sorted, with the
+    // counts of parts of a split frontier that reached the same state added.
+    void terminals(std::vector<uint64_t> &key, std::vector<uint64_t> &cnt) const {
+        key.clear();
+        cnt.clear();
+        std::vector<std::pair<uint64_t, uint64_t>> v;
+        v.reserve(sink_n);
+        for (uint64_t k = 0; k < sink_n; k++) {
+            const uint64_t e = h_sink[2 * k];
+            const uint32_t i = (uint32_t)e;
+            if (i < N && !h_dropped[i]) v.emplace_back((e >> 32) | ((uint64_t)i << 32), h_sink[2 * k + 1]);
+        }
+        std::sort(v.begin(), v.end());
+        for (const auto &x : v) {
+            if (!key.empty() && key.back() == x.first)
+                cnt.back() += x.second;
+            else {
+                key.push_back(x.first);
+                cnt.push_back(x.second);
+            }
+        }
+    }
+
     // Per-instance frontier shares of the new frontier; instances over the
     // budget are dropped from the next step on.
     void account(uint32_t row) {
@@ -406,6 +464,12 @@ struct d4_mbatch {
         dropped.reserve(N);
         hist.reserve(N);
         HIPCHK(hipMemcpy(dropped.p, h_dropped.data(), N, hipMemcpyHostToDevice));
+        sink_n = 0;
+        h_sink.clear();
+        if (keep_terminals) {
+            sink_cap = sink_init ? sink_init : std::max<uint64_t>(4ull * N, 4096);
+            sink.reserve(2 * sink_cap);
+        }
         roots();
         // step r extends every instance's row r; an instance of height h is
         // absorbed into its count at step h.  Parts of a split frontier run
@@ -429,6 +493,11 @@ struct d4_mbatch {
         // another frontier part crossed the budget: report none of it
         for (uint32_t i = 0; i < N; i++)
             if (h_dropped[i]) counts[i] = 0;
+        if (keep_terminals) {
+            h_sink.resize(2 * sink_n);
+            if (sink_n)
+                HIPCHK(hipMemcpy(h_sink.data(), sink.p, 16ull * sink_n, hipMemcpyDeviceToHost));
+        }
         done = true;
     }
 };
@@ -625,6 +694,31 @@ int d4_mbatch_stats(const d4_mbatch *b, d4_stats *st) {
     return guard([&] {
         if (!b || !st) throw Error(D4_EINVAL, "null argument");
         *st = b->stats;
+    });
+}
+
+int d4_mbatch_set_terminals(d4_mbatch *b, int keep, uint64_t initial_entries) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null argument");
+        b->keep_terminals = keep != 0;
+        b->sink_init = initial_entries;
+    });
+}
+
+int d4_mbatch_terminals(const d4_mbatch *b, uint32_t *inst, uint64_t *counters, uint64_t *tilings,
+                        size_t cap, size_t *n) {
+    return guard([&] {
+        if (!b || !n) throw Error(D4_EINVAL, "null argument");
+        if (!b->done || !b->keep_terminals)
+            throw Error(D4_ESTATE, "run the batch with terminal states kept first");
+        std::vector<uint64_t> key, cnt;
+        b->terminals(key, cnt);
+        *n = key.size();
+        for (size_t k = 0; k < key.size() && k < cap; k++) {
+            if (inst) inst[k] = (uint32_t)(key[k] >> 32);
+            if (counters) counters[k] = key[k] & 0xffffffffull;
+            if (tilings) tilings[k] = cnt[k];
+        }
     });
 }
 

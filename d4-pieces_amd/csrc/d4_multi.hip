@@ -281,8 +281,18 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
                 inst = (uint32_t)(k0 >> MKEY_INST_SHIFT);
                 if (kterm) {
                     // terminal state of a finished instance: all cells covered
-                    if (kterm == 1 && lead_item && !COUNT_ONLY)
+                    if (kterm == 1 && lead_item && !COUNT_ONLY) {
                         atomicAdd(a.result + inst, (unsigned long long)cnt);
+                        if (a.sink) {
+                            // entries past the capacity are counted, not
+                            // written: the host grows the sink and reruns
+                            const uint32_t o = atomicAdd(a.c32 + C32_SINK, 1u);
+                            if (o < a.sink_cap) {
+                                a.sink[2 * (size_t)o] = (uint64_t)inst | ((k1 & 0xffffffffull) << 32);
+                                a.sink[2 * (size_t)o + 1] = cnt;
+                            }
+                        }
+                    }
                 } else {
                     lib = klib;
                     L = a.libs + lib;

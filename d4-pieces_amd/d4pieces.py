@@ -60,6 +60,7 @@ class Stats(ctypes.Structure):
         ("splits", ctypes.c_uint64),
         ("donated", ctypes.c_uint64),
         ("rehashes", ctypes.c_uint64),
+        ("sink_growths", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -131,6 +132,35 @@ SIGNATURES = [
     ("d4_mbatch_instances", ctypes.c_int, [_P, _U64P, _U8P]),
     ("d4_mbatch_counts", ctypes.c_int, [_P, _U64P]),
     ("d4_mbatch_stats", ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    ("d4_mbatch_set_terminals", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint64]),
+    ("d4_mbatch_terminals", ctypes.c_int,
+     [_P, _U32P, _U64P, _U64P, ctypes.c_size_t, _SZP]),
+    ("d4_library_counters", ctypes.c_int, [_P, _U32P, _U32P, _U32P]),
+    ("d4_polyominoes", ctypes.c_size_t,
+     [ctypes.c_uint, ctypes.c_uint32, _U64P, ctypes.c_size_t]),
+    ("d4_naming_create", ctypes.c_int,
+     [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+      _U64P, ctypes.POINTER(_P)]),
+    ("d4_naming_destroy", None, [_P]),
+    ("d4_naming_size", ctypes.c_uint64, [_P]),
+    ("d4_naming_size_pieces", ctypes.c_uint64, [_P]),
+    ("d4_naming_name_piece", ctypes.c_uint64, [_P, ctypes.c_uint64, ctypes.c_uint64]),
+    ("d4_naming_resolve_piece", ctypes.c_int, [_P, ctypes.c_uint64, _U64P, _U64P]),
+    ("d4_naming_name", ctypes.c_int, [_P, _U8P, _U64P]),
+    ("d4_naming_resolve", ctypes.c_int, [_P, ctypes.c_uint64, _U8P]),
+    ("d4_naming_name_counters", ctypes.c_int, [_P, _P, _U64P, ctypes.c_size_t, _U64P]),
+    ("d4_ledger_open", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    ("d4_ledger_close", None, [_P]),
+    ("d4_ledger_should_run", ctypes.c_int,
+     [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    ("d4_ledger_log", ctypes.c_int,
+     [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    ("d4_ledger_log_many", ctypes.c_int, [_P, _U64P, _U64P, ctypes.c_size_t, _U8P]),
+    ("d4_ledger_after_run", ctypes.c_int,
+     [_P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
+    ("d4_ledger_progress", ctypes.c_int, [_P, _U64P, _U64P]),
+    ("d4_ledger_names", ctypes.c_int, [_P, _U64P, ctypes.c_size_t]),
     ("d4_server_create", ctypes.c_int,
      [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_P)]),
     ("d4_server_destroy", None, [_P]),
@@ -326,6 +356,12 @@ class PieceLibrary:
                                       ctypes.byref(r), ctypes.byref(ms)))
         return {"fit_rows": f.value, "threads": t.value, "refill": r.value,
                 "budget": self.get_budget(), "extend_ms": ms.value}
+
+    def counter_layout(self) -> List[Tuple[int, int]]:
+        """(bit offset, width) of each piece's use counter in packed counters."""
+        off, wid, n = (ctypes.c_uint32 * 64)(), (ctypes.c_uint32 * 64)(), ctypes.c_uint32()
+        _check(load().d4_library_counters(self._h, off, wid, ctypes.byref(n)))
+        return [(off[i], wid[i]) for i in range(n.value)]
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -536,6 +572,173 @@ class MultiBatch:
         st = Stats()
         _check(load().d4_mbatch_stats(self._h, ctypes.byref(st)))
         return st.as_dict()
+
+    def set_terminals(self, keep: bool = True, initial_entries: int = 0) -> None:
+        """Keep the terminal states instances reach (for terminals()); the
+        device buffer starts with room for initial_entries (0: default)."""
+        _check(load().d4_mbatch_set_terminals(self._h, 1 if keep else 0, initial_entries))
+
+    def terminals(self) -> Tuple[List[int], List[int], List[int]]:
+        """(instance, packed piece counters, tilings) per distinct terminal
+        state of a finished instance, sorted."""
+        lib = load()
+        n = ctypes.c_size_t()
+        _check(lib.d4_mbatch_terminals(self._h, None, None, None, 0, ctypes.byref(n)))
+        k = max(n.value, 1)
+        inst, ctr, til = (ctypes.c_uint32 * k)(), (ctypes.c_uint64 * k)(), (ctypes.c_uint64 * k)()
+        _check(lib.d4_mbatch_terminals(self._h, inst, ctr, til, n.value, ctypes.byref(n)))
+        m = n.value
+        return list(inst[:m]), list(ctr[:m]), list(til[:m])
+
+
+def polyominoes(n: int, sym: int = 0xFF) -> List[int]:
+    """Canonical polyominoes with n cells under the transforms in sym,
+    ascending: sym 0xFF = known_shapes[n] (src/known1.cpp), sym 0x69 =
+    known_C_shapes[n] (src/known2.cpp)."""
+    lib = load()
+    total = lib.d4_polyominoes(n, sym, None, 0)
+    buf = (ctypes.c_uint64 * max(total, 1))()
+    lib.d4_polyominoes(n, sym, buf, total)
+    return list(buf[:total])
+
+
+NO_NAME = (1 << 64) - 1
+
+
+class Naming:
+    """Naming (src/naming.hpp:15-71): the piece sets of between min_n and
+    max_n pieces of sizes min_m..max_m, at most one of each, that cover tgt
+    cells; arr[m] lists the pieces of size m.  Names are dense ranks in
+    [0, size())."""
+
+    def __init__(self, min_m: int, max_m: int, min_n: int, max_n: int, tgt: int,
+                 arr: Sequence[Sequence[int]]):
+        self.min_m, self.max_m, self.min_n, self.max_n, self.tgt = min_m, max_m, min_n, max_n, tgt
+        self.arr = [list(arr[m]) if m < len(arr) and arr[m] else [] for m in range(max_m + 1)]
+        sizes = [len(self.arr[m]) for m in range(max_m + 1)]
+        h = _P()
+        _check(load().d4_naming_create(min_m, max_m, min_n, max_n, tgt, _u64arr(sizes),
+                                       ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.d4_naming_destroy(self._h)
+            self._h = None
+
+    def size(self) -> int:
+        return load().d4_naming_size(self._h)
+
+    def size_pieces(self) -> int:
+        return load().d4_naming_size_pieces(self._h)
+
+    def name_piece(self, m: int, i: int) -> int:
+        return load().d4_naming_name_piece(self._h, m, i)
+
+    def resolve_piece(self, nm: int) -> Tuple[int, int]:
+        """(m, i) of piece id nm; (0, 0) when out of range, as the reference."""
+        m, i = ctypes.c_uint64(), ctypes.c_uint64()
+        rc = load().d4_naming_resolve_piece(self._h, nm, ctypes.byref(m), ctypes.byref(i))
+        if rc == D4_ERANGE:
+            return 0, 0
+        _check(rc)
+        return m.value, i.value
+
+    def pieces(self) -> List[int]:
+        """Every piece's shape, in piece-id order."""
+        return [v for m in range(self.min_m, self.max_m + 1) for v in self.arr[m]]
+
+    def name_used(self, used: Sequence[int]) -> Optional[int]:
+        out = ctypes.c_uint64()
+        _check(load().d4_naming_name(self._h, _u8arr(list(used)), ctypes.byref(out)))
+        return None if out.value == NO_NAME else out.value
+
+    def name(self, func) -> Optional[int]:
+        """Naming::name: func(m, i) is true iff the i-th piece of size m is
+        in the set; None when the set is not named."""
+        used = [1 if func(m, i) else 0
+                for m in range(self.min_m, self.max_m + 1) for i in range(len(self.arr[m]))]
+        return self.name_used(used)
+
+    def resolve_used(self, nm: int) -> Optional[List[int]]:
+        used = (ctypes.c_uint8 * max(self.size_pieces(), 1))()
+        rc = load().d4_naming_resolve(self._h, nm, used)
+        if rc == D4_ERANGE:
+            return None
+        _check(rc)
+        return list(used[:self.size_pieces()])
+
+    def resolve(self, nm: int, func) -> bool:
+        """Naming::resolve: calls func(m, i) for each piece of set nm;
+        False when nm >= size()."""
+        used = self.resolve_used(nm)
+        if used is None:
+            return False
+        for pid, u in enumerate(used):
+            if u:
+                func(*self.resolve_piece(pid))
+        return True
+
+    def name_counters(self, library: "PieceLibrary", counters: Sequence[int]) -> List[int]:
+        """Names (NO_NAME where none) of terminal states' packed counters of
+        a search over `library`, whose piece p is piece id p."""
+        n = len(counters)
+        out = (ctypes.c_uint64 * max(n, 1))()
+        _check(load().d4_naming_name_counters(self._h, library._h, _u64arr(list(counters)) if n
+                                              else None, n, out))
+        return list(out[:n])
+
+
+LEDGER_VERIFY, LEDGER_COUNT, LEDGER_BIT, LEDGER_BYTE = 0, 1, 2, 3
+
+
+class Ledger:
+    """Where names are logged per board config (d4_ledger_*): Verify,
+    AtomicCount, or the FileBit / FileByte mmap layout of src/search.cpp."""
+
+    def __init__(self, kind: int, n_configs: int, n_names: int, path: Optional[str] = None):
+        h = _P()
+        _check(load().d4_ledger_open(kind, path.encode() if path else None, n_configs, n_names,
+                                     ctypes.byref(h)))
+        self._h, self.kind, self.n_configs, self.n_names = h, kind, n_configs, n_names
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.d4_ledger_close(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def should_run(self, config: int, shape: int) -> bool:
+        r = ctypes.c_int()
+        _check(load().d4_ledger_should_run(self._h, config, shape, ctypes.byref(r)))
+        return bool(r.value)
+
+    def log(self, config: int, name: int) -> bool:
+        r = ctypes.c_int()
+        _check(load().d4_ledger_log(self._h, config, name, ctypes.byref(r)))
+        return bool(r.value)
+
+    def log_many(self, configs: Sequence[int], names: Sequence[int]) -> List[bool]:
+        n = len(configs)
+        fresh = (ctypes.c_uint8 * max(n, 1))()
+        if n:
+            _check(load().d4_ledger_log_many(self._h, _u64arr(list(configs)),
+                                             _u64arr(list(names)), n, fresh))
+        return [bool(x) for x in fresh[:n]]
+
+    def after_run(self, config: int, shape: int, count: int, ms: int) -> None:
+        _check(load().d4_ledger_after_run(self._h, config, shape, count, ms))
+
+    def progress(self) -> Tuple[int, int]:
+        w, c = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(load().d4_ledger_progress(self._h, ctypes.byref(w), ctypes.byref(c)))
+        return w.value, c.value
+
+    def names(self) -> List[int]:
+        buf = (ctypes.c_uint64 * max(self.n_names, 1))()
+        _check(load().d4_ledger_names(self._h, buf, self.n_names))
+        return list(buf[:self.n_names])
 
 
 class Server:

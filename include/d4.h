@@ -75,6 +75,7 @@ typedef struct d4_stats {
     uint64_t splits;           /* frontier halves parked to stay in the memory cap */
     uint64_t donated;          /* work items handed between extend launches */
     uint64_t rehashes;         /* dedup tables grown after children spilled */
+    uint64_t sink_growths;     /* terminal-state sinks grown (the step reruns) */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -264,6 +265,123 @@ int d4_mbatch_counts(const d4_mbatch *b, uint64_t *counts);
 int d4_mbatch_instances(const d4_mbatch *b, uint64_t *peak_states,
                         uint8_t *finished);
 int d4_mbatch_stats(const d4_mbatch *b, d4_stats *st);
+/* Keep the terminal states (every cell covered) that instances reach, for
+ * d4_mbatch_terminals (off by default).  The device buffer starts with
+ * room for initial_entries states (0: 4 per instance, at least 4096) and
+ * grows as needed (a row step that overflows it is rerun).  Replaces the
+ * records the
+ * commented-out consumer of src/searcher.cpp:38-65 reads from the
+ * CudaSearcher frontier once its height reaches 0. */
+int d4_mbatch_set_terminals(d4_mbatch *b, int keep, uint64_t initial_entries);
+/* After d4_mbatch_run with terminals kept: one entry per distinct (instance,
+ * piece use counters) terminal state of a finished instance, sorted by
+ * instance then counters; tilings[k] = number of tilings ending in it.
+ * Counters are packed as the library packs them (d4_library_counters).
+ * Writes min(total, cap) entries and sets *n = total. */
+int d4_mbatch_terminals(const d4_mbatch *b, uint32_t *inst, uint64_t *counters,
+                        uint64_t *tilings, size_t cap, size_t *n);
+/* Bit offset and width of each piece's use counter in a library's packed
+ * counters (n entries each, n = the library's piece count). */
+int d4_library_counters(const d4_library *lib, uint32_t *offset, uint32_t *width,
+                        uint32_t *n);
+
+/* ---------------------------------------------------------------------- */
+/* piece-set names and ledgers: the consumer of terminal states            */
+/* (src/naming.hpp:15-71, src/naming.inl:6-72, src/naming.cpp:9-81,        */
+/*  src/search.cpp:25-226)                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* Canonical polyominoes with n cells (1 <= n <= 8) under the transforms in
+ * `sym` (bit t = D4 transform t): each as the minimum over those transforms
+ * of the normalised 8-wide encoding, ascending.  sym 0xFF gives the free
+ * tables (src/known1.cpp:3-14); sym 0x69 (identity and the three rotations)
+ * gives one-sided ones (src/known2.cpp:3-14, generated by
+ * src/enum.cpp:28-57).  Writes min(total, cap) and returns the total. */
+size_t d4_polyominoes(unsigned n, uint32_t sym, uint64_t *out, size_t cap);
+
+typedef struct d4_naming d4_naming;
+#define D4_NO_NAME UINT64_MAX
+
+/* Naming(m, M, n, N, t, arr, sz) (src/naming.hpp:17, src/naming.cpp:35-81):
+ * the piece sets that take between min_n and max_n pieces of sizes min_m ..
+ * max_m, at most one of each, arr_sizes[m] pieces of size m (arr_sizes has
+ * max_m + 1 entries; index < min_m unused), covering tgt cells in all.
+ * Names are ranks: partitions (pieces per size) in lexicographic order of
+ * their per-size counts, then per size, from min_m (most significant) to
+ * max_m, the rank of the chosen indices among k-subsets in lexicographic
+ * order.  D4_ERANGE when a count exceeds 2^64. */
+int d4_naming_create(uint64_t min_m, uint64_t max_m, uint64_t min_n, uint64_t max_n,
+                     uint64_t tgt, const uint64_t *arr_sizes, d4_naming **out);
+void d4_naming_destroy(d4_naming *nm);
+/* Naming::size / size_pieces (src/naming.hpp:30-39). */
+uint64_t d4_naming_size(const d4_naming *nm);
+uint64_t d4_naming_size_pieces(const d4_naming *nm);
+/* Naming::name_piece (src/naming.hpp:20-24): the id of piece i of size m;
+ * D4_NO_NAME when out of range. */
+uint64_t d4_naming_name_piece(const d4_naming *nm, uint64_t m, uint64_t i);
+/* Naming::resolve_piece (src/naming.cpp:25-33); D4_ERANGE (with m = i = 0)
+ * for an id >= size_pieces. */
+int d4_naming_resolve_piece(const d4_naming *nm, uint64_t id, uint64_t *m, uint64_t *i);
+/* Naming::name (src/naming.inl:6-29): used[id] != 0 for each piece id in the
+ * set (size_pieces entries); *name = D4_NO_NAME when the set is not one of
+ * the named ones (the reference's empty optional). */
+int d4_naming_name(const d4_naming *nm, const uint8_t *used, uint64_t *name);
+/* Naming::resolve (src/naming.inl:61-72): used[id] = 1 for the pieces of
+ * the named set, 0 otherwise; D4_ERANGE for name >= size (the reference's
+ * false). */
+int d4_naming_resolve(const d4_naming *nm, uint64_t name, uint8_t *used);
+/* Names of n terminal states of a search over a library whose piece p is
+ * naming piece id p (a library of size_pieces pieces): counters[k] is
+ * decoded with the library's counter layout; a state that uses a piece
+ * more than once, or a set outside the naming, gets D4_NO_NAME (the
+ * reference prints "Naming rejected" for those, src/searcher.cpp:58-60). */
+int d4_naming_name_counters(const d4_naming *nm, const d4_library *lib,
+                            const uint64_t *counters, size_t n, uint64_t *names);
+
+typedef struct d4_ledger d4_ledger;
+/* Ledger kinds, the reference's S=v|ac|fb|fB (src/search.cpp:297-315). */
+enum {
+    D4_LEDGER_VERIFY = 0, /* Verify: counts work only (search.cpp:25-35)    */
+    D4_LEDGER_COUNT = 1,  /* AtomicCount: u64 per name (search.cpp:37-54)   */
+    D4_LEDGER_BIT = 2,    /* FileBit: file, one bit per (name, config)      */
+    D4_LEDGER_BYTE = 3    /* FileByte: file, one byte per (name, config)    */
+};
+
+/* A ledger over n_configs board configurations and n_names piece sets.
+ * The two file kinds map `path` shared (FileSearcherBase,
+ * src/search.cpp:56-174) with the reference's layout: n_configs 32-byte
+ * descriptors {shape, count, solved (ms, 0 = unsolved), 0}, then n_names
+ * u64 per-name config counts, then the ledger: n_names rows of n_configs
+ * bytes (FileByte, 0xFF = set) or ceil(n_configs / 8) bytes (FileBit, bit
+ * c % 8 of byte c / 8).  An empty file is extended to that size; a longer
+ * one is refused ("file too long"), a shorter non-empty one too ("file too
+ * short"); D4_EINVAL. */
+int d4_ledger_open(int kind, const char *path, uint64_t n_configs, uint64_t n_names,
+                   d4_ledger **out);
+/* Unmaps and closes (the mapping is shared: the file holds every change). */
+void d4_ledger_close(d4_ledger *L);
+/* SearcherFactory::should_run (src/searcher.hpp:70, search.cpp:156-159):
+ * the file kinds record the config's shape and run it only when unsolved. */
+int d4_ledger_should_run(d4_ledger *L, uint64_t config, uint64_t shape, int *run);
+/* Searcher::log (src/search.cpp:28-31, 44-50, 182-191, 207-220) for one
+ * (config, name) pair: *fresh = 1 when it was not logged for the config
+ * before (then the work counter advances), else 0.  Names D4_NO_NAME are
+ * skipped (*fresh = 0). */
+int d4_ledger_log(d4_ledger *L, uint64_t config, uint64_t name, int *fresh);
+/* log for n pairs; fresh[k] as d4_ledger_log (may be NULL). */
+int d4_ledger_log_many(d4_ledger *L, const uint64_t *configs, const uint64_t *names,
+                       size_t n, uint8_t *fresh);
+/* SearcherFactory::after_run (search.cpp:161-165, searcher.cpp:110-115):
+ * the file kinds store the config's count (log calls that returned 1) and
+ * the milliseconds it took (at least 1, so that it reads as solved). */
+int d4_ledger_after_run(d4_ledger *L, uint64_t config, uint64_t shape, uint64_t count,
+                        uint64_t ms);
+/* Work counter (log calls that returned 1) and configs finished. */
+int d4_ledger_progress(const d4_ledger *L, uint64_t *work, uint64_t *configs_done);
+/* Per-name totals: AtomicCount's ledger or the file kinds' name header
+ * (configs each name was logged for); Verify has none (D4_ESTATE).  Writes
+ * min(n_names, cap) values. */
+int d4_ledger_names(const d4_ledger *L, uint64_t *out, size_t cap);
 
 /* ---------------------------------------------------------------------- */
 /* serving: continuous batching of requests (no reference counterpart)     */

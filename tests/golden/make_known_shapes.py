@@ -1,5 +1,5 @@
-"""Writes known_shapes.json: the free-polyomino tables the reference ships
-(src/known1.cpp:3-14, free; src/known2.cpp:3-14, one-sided counts) as data.
+"""Writes known_shapes.json: the polyomino tables the reference ships
+(src/known1.cpp:3-14, free; src/known2.cpp:3-14, one-sided) as data.
 
 Run in a container that has the reference at /root/reference; the JSON is
 committed so nothing reads the reference at test time."""
@@ -27,5 +27,6 @@ json.dump({
     "source": "src/known1.cpp:3-14 (free), src/known2.cpp:3-14 (one-sided)",
     "free": {str(k): v for k, v in sorted(free.items())},
     "free_counts": free_counts,
+    "one_sided": {str(k): v for k, v in sorted(one_sided.items())},
     "one_sided_counts": one_sided_counts,
 }, open(os.path.join(HERE, "known_shapes.json"), "w"), indent=0)
