@@ -159,9 +159,15 @@ def uneven_section(d4, workloads, n, budget, plain_sample):
     instances one per batch; parity against the oracle's counts of the
     instances it finished (tests/golden/uneven_4096_seed3.json)."""
     insts = workloads.uneven_instances(4096, seed=3)[:n]
+    t = time.perf_counter()
     libs = multi_libraries(d4, insts)
+    t_lib = time.perf_counter() - t
     log(f"uneven: {n} instances, state budget {budget} per instance")
+    t = time.perf_counter()
     b = d4.MultiBatch(libs, [x[2] for x in insts])
+    t_create = time.perf_counter() - t
+    log(f"uneven: {len(set(map(id, libs)))} libraries built on the host in {t_lib:.2f} s, "
+        f"batch tables built and uploaded in {t_create:.2f} s")
     b.set_budget(budget)
     t = time.perf_counter()
     tuned = b.tune(64)  # measured launch shape on a sample (outside the timing)
@@ -174,6 +180,7 @@ def uneven_section(d4, workloads, n, budget, plain_sample):
     peak, fin = b.instances()
     st = b.stats()
     out = {"instances": n, "state_budget": budget, "finished": sum(fin), "wall_s": wall,
+           "library_build_s": t_lib, "batch_create_s": t_create,
            "launch": tuned,
            "records_per_s": st["records_extended"] / wall,
            "solutions_per_s": sum(counts) / wall,
