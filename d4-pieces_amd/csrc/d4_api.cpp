@@ -171,7 +171,11 @@ static void build_fast(d4_library &L) {
 template <class T>
 static void upload(T *&dst, const T *src, size_t n) {
     HIPCHK(hipMalloc((void **)&dst, std::max<size_t>(n, 1) * sizeof(T)));
-    if (n) HIPCHK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    if (n) {
+        // complete before any engine's (non-blocking) stream reads the table
+        HIPCHK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+        HIPCHK(hipStreamSynchronize(nullptr));
+    }
 }
 
 // Build and upload the single-library tables once (engines call this).
@@ -238,6 +242,15 @@ struct Engine {
     uint32_t bw_stride = 0;
     // device state
     hipStream_t st = nullptr;
+
+    // Every copy of an engine runs on its own stream: the stream is
+    // non-blocking, so a copy on the null stream would not be ordered with
+    // the engine's kernels (a count restore could land after the rerun it
+    // prepares).  The host waits for it, so host buffers may be reused.
+    void copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) const {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evx0 = nullptr, evx1 = nullptr, evc0 = nullptr;
     DevBuf<uint64_t> bw;
     DevBuf<uint32_t> hgt, dstart;
@@ -460,31 +473,31 @@ struct Engine {
                     w |= (uint64_t)bd[r + j] << (8 * j);
                 win[r] = w;
             }
-            HIPCHK(hipMemcpy(bw.p + (size_t)slot * bw_stride, win.data(), bw_stride * 8,
-                             hipMemcpyHostToDevice));
+            copy(bw.p + (size_t)slot * bw_stride, win.data(), bw_stride * 8,
+                 hipMemcpyHostToDevice);
             start[slot] = g;
             k0.push_back(inst_shift < 64 ? (uint64_t)slot << inst_shift : 0);
             c0.push_back(1);
         }
-        HIPCHK(hipMemcpy(hgt.p, heights.data(), (size_t)S * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dstart.p, start.data(), (size_t)S * 4, hipMemcpyHostToDevice));
+        copy(hgt.p, heights.data(), (size_t)S * 4, hipMemcpyHostToDevice);
+        copy(dstart.p, start.data(), (size_t)S * 4, hipMemcpyHostToDevice);
         std::vector<unsigned long long> zero(1, 0);
         for (uint32_t slot : pending)
-            HIPCHK(hipMemcpy(result.p + slot, zero.data(), 8, hipMemcpyHostToDevice));
+            copy(result.p + slot, zero.data(), 8, hipMemcpyHostToDevice);
         const uint64_t n = n_front + k0.size();
         if (n > fk.cap || n > fc.cap) {  // grow, keeping the current frontier
             DevBuf<uint64_t> k2, c2;
             k2.reserve(n);
             c2.reserve(n);
             if (n_front) {
-                HIPCHK(hipMemcpy(k2.p, fk.p, n_front * 8, hipMemcpyDeviceToDevice));
-                HIPCHK(hipMemcpy(c2.p, fc.p, n_front * 8, hipMemcpyDeviceToDevice));
+                copy(k2.p, fk.p, n_front * 8, hipMemcpyDeviceToDevice);
+                copy(c2.p, fc.p, n_front * 8, hipMemcpyDeviceToDevice);
             }
             fk.swap(k2);
             fc.swap(c2);
         }
-        HIPCHK(hipMemcpy(fk.p + n_front, k0.data(), k0.size() * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(fc.p + n_front, c0.data(), c0.size() * 8, hipMemcpyHostToDevice));
+        copy(fk.p + n_front, k0.data(), k0.size() * 8, hipMemcpyHostToDevice);
+        copy(fc.p + n_front, c0.data(), c0.size() * 8, hipMemcpyHostToDevice);
         n_front = n;
         new_roots = k0.size();
         stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, n_front);
@@ -502,7 +515,7 @@ struct Engine {
         for (uint32_t i = 0; i < S; i++)
             if (active[i] && start[i] + heights[i] == g) any = true;
         if (any) {
-            HIPCHK(hipMemcpy(host_result.data(), result.p, (size_t)S * 8, hipMemcpyDeviceToHost));
+            copy(host_result.data(), result.p, (size_t)S * 8, hipMemcpyDeviceToHost);
             for (uint32_t i = 0; i < S; i++)
                 if (active[i] && start[i] + heights[i] == g) {
                     completed.push_back(i);
@@ -651,7 +664,7 @@ struct Engine {
             if (!ovf && !eovf) break;
             // the step is a pure function of the frontier: rerun it bigger
             stats.retries++;
-            HIPCHK(hipMemcpy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice));
+            copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
             if (ovf) tab_m *= 4;
             if (eovf) est_e = (uint64_t)h32[C32_EDGES] + 1024;
         }
@@ -661,7 +674,7 @@ struct Engine {
             nc.reserve(U);
         } catch (const Error &e) {
             if (e.code != D4_ENOMEM || !bounded) throw;
-            HIPCHK(hipMemcpy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice));
+            copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
             return false;
         }
         stats.children += h64[C64_CHILDREN];
@@ -780,7 +793,7 @@ struct Engine {
         }
         g = gend + 1;
         n_front = 0;
-        HIPCHK(hipMemcpy(host_result.data(), result.p, (size_t)S * 8, hipMemcpyDeviceToHost));
+        copy(host_result.data(), result.p, (size_t)S * 8, hipMemcpyDeviceToHost);
         for (uint32_t i = 0; i < S; i++)
             if (active[i]) {
                 completed.push_back(i);
@@ -924,15 +937,15 @@ struct Engine {
         const DevLog &lg = logs[j];
         out.keys.resize(lg.n_keys);
         if (lg.n_keys)
-            HIPCHK(hipMemcpy(out.keys.data(), lg.keys.p, lg.n_keys * 8, hipMemcpyDeviceToHost));
+            copy(out.keys.data(), lg.keys.p, lg.n_keys * 8, hipMemcpyDeviceToHost);
         if (!edges) return;
         out.e_parent.resize(lg.n_edges);
         out.e_child.resize(lg.n_edges);
         out.e_desc.resize(lg.n_edges);
         if (lg.n_edges) {
-            HIPCHK(hipMemcpy(out.e_parent.data(), lg.par.p, lg.n_edges * 4, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(out.e_child.data(), lg.chi.p, lg.n_edges * 4, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(out.e_desc.data(), lg.desc.p, lg.n_edges * 8, hipMemcpyDeviceToHost));
+            copy(out.e_parent.data(), lg.par.p, lg.n_edges * 4, hipMemcpyDeviceToHost);
+            copy(out.e_child.data(), lg.chi.p, lg.n_edges * 4, hipMemcpyDeviceToHost);
+            copy(out.e_desc.data(), lg.desc.p, lg.n_edges * 8, hipMemcpyDeviceToHost);
         }
     }
 
@@ -1426,8 +1439,8 @@ int d4_searcher_export(const d4_searcher *s, void *rx, size_t cap, size_t *n,
             if (L.counts[i] > 1) throw Error(D4_ERANGE, "the record format has no piece counts");
         std::vector<uint64_t> keys(e.n_front), cnts(e.n_front);
         if (e.n_front) {
-            HIPCHK(hipMemcpy(keys.data(), e.fk.p, e.n_front * 8, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(cnts.data(), e.fc.p, e.n_front * 8, hipMemcpyDeviceToHost));
+            e.copy(keys.data(), e.fk.p, e.n_front * 8, hipMemcpyDeviceToHost);
+            e.copy(cnts.data(), e.fc.p, e.n_front * 8, hipMemcpyDeviceToHost);
         }
         const uint32_t row = e.g - e.start[0];
         uint64_t BW = 0;

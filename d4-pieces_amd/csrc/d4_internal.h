@@ -111,6 +111,9 @@ struct MLib {
     // fit[512 - (512 >> c) + (b >> c)]: candidates anchored at column c whose
     // row-0 cells lie inside row pattern b (b's bits below c are clear)
     uint64_t fit[512][2];
+    // fitr[t][c][b]: candidates anchored at column c whose cells in window
+    // row t + 1 lie inside row pattern b
+    uint64_t fitr[2][8][256][2];
     // per candidate: normalised shape, and info = counter offset |
     // field mask << 8 | count << 16 | anchor column of the shape << 32 |
     // piece's candidate range lo << 40 | hi << 48

@@ -47,6 +47,25 @@ void build_mlib(const d4_library &L, MLib &M) {
             M.fit[base + v][1] = m[1];
         }
     }
+    // rows 1 and 2 of the window: mark each placement at its own row pattern,
+    // then close over supersets (a pattern admits every placement whose
+    // cells in that row are a subset of it)
+    for (uint32_t t = 0; t < 2; t++)
+        for (uint32_t c = 0; c < 8; c++) {
+            auto &F = M.fitr[t][c];
+            for (uint32_t j = 0; j < k; j++) {
+                const uint64_t pl = placed(c, j);
+                if (!pl) continue;
+                const uint32_t row = (uint32_t)(pl >> (8 * (t + 1))) & 0xffu;
+                F[row][j >> 6] |= 1ull << (j & 63u);
+            }
+            for (uint32_t bit = 0; bit < 8; bit++)
+                for (uint32_t b = 0; b < 256; b++)
+                    if (b >> bit & 1u) {
+                        F[b][0] |= F[b ^ (1u << bit)][0];
+                        F[b][1] |= F[b ^ (1u << bit)][1];
+                    }
+        }
     for (uint32_t j = 0; j < k; j++) {
         M.cand[j][0] = L.cnormal[j];
         M.cand[j][1] = (uint64_t)(L.cinfo[j] & 0xffffu) | ((uint64_t)(L.cinfo[j] >> 16) << 16) |
@@ -131,6 +150,14 @@ struct d4_mbatch {
     bool done = false;
     bool trace = getenv("D4_TRACE") != nullptr;  // per-row lines on stderr
     d4_stats stats{};
+
+    // Copies run on the batch's own (non-blocking) stream, so they are
+    // ordered with its kernels (a count restore before the rerun it
+    // prepares); the host waits for them.
+    void copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) const {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
 
     ~d4_mbatch() {
         if (st) (void)hipStreamSynchronize(st);
@@ -340,14 +367,13 @@ struct d4_mbatch {
                 // grow it (keeping what earlier steps wrote) and rerun
                 grow_sink(sink_n + h32[C32_SINK]);
                 stats.retries++;
-                HIPCHK(hipMemcpy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice));
+                copy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice);
                 continue;
             }
             stats.retries++;
-            HIPCHK(hipMemcpy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice));
+            copy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice);
             tab_m *= 4;
         }
-        if (keep_terminals) sink_n += h32[C32_SINK];
         const uint64_t U = h64[C64_NEW];
         try {
             n0.reserve(U);
@@ -355,9 +381,10 @@ struct d4_mbatch {
             nc.reserve(U);
         } catch (const Error &e) {
             if (e.code != D4_ENOMEM || n_front < 2) throw;
-            HIPCHK(hipMemcpy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice));
-            return false;
+            copy(result.p, result_save.p, 8ull * N, hipMemcpyDeviceToDevice);
+            return false;  // (the step's terminal states are written again)
         }
+        if (keep_terminals) sink_n += h32[C32_SINK];
         stats.children += h64[C64_CHILDREN];
         stats.pruned += h64[C64_PRUNED];
         stats.records_extended += n_front;
@@ -451,7 +478,7 @@ struct d4_mbatch {
             }
         }
         if (changed)
-            HIPCHK(hipMemcpy(dropped.p, h_dropped.data(), N, hipMemcpyHostToDevice));
+            copy(dropped.p, h_dropped.data(), N, hipMemcpyHostToDevice);
     }
 
     void run() {
@@ -463,7 +490,7 @@ struct d4_mbatch {
         h_hist.assign(N, 0);
         dropped.reserve(N);
         hist.reserve(N);
-        HIPCHK(hipMemcpy(dropped.p, h_dropped.data(), N, hipMemcpyHostToDevice));
+        copy(dropped.p, h_dropped.data(), N, hipMemcpyHostToDevice);
         sink_n = 0;
         h_sink.clear();
         if (keep_terminals) {
@@ -488,7 +515,7 @@ struct d4_mbatch {
             r = resume();
         }
         counts.assign(N, 0);
-        HIPCHK(hipMemcpy(counts.data(), result.p, 8ull * N, hipMemcpyDeviceToHost));
+        copy(counts.data(), result.p, 8ull * N, hipMemcpyDeviceToHost);
         // a dropped instance may have absorbed part of its count before
         // another frontier part crossed the budget: report none of it
         for (uint32_t i = 0; i < N; i++)
@@ -496,7 +523,7 @@ struct d4_mbatch {
         if (keep_terminals) {
             h_sink.resize(2 * sink_n);
             if (sink_n)
-                HIPCHK(hipMemcpy(h_sink.data(), sink.p, 16ull * sink_n, hipMemcpyDeviceToHost));
+                copy(h_sink.data(), sink.p, 16ull * sink_n, hipMemcpyDeviceToHost);
         }
         done = true;
     }

@@ -5,14 +5,16 @@
 // The single-library kernel (d4_kernels.hip) stages one library's tables in
 // LDS and packs a state into one u64.  Here every record names its instance,
 // the instance names its library, and the lane reads that library's tables
-// from HBM through L2 (about 10 KiB per library, so a few thousand libraries
-// stay cache resident): a 128-bit candidate mask per (anchor column, row-0
-// pattern) entry, the candidates' shapes and counter fields, and the pieces'
-// counter fields.  States take two words, occupancy + instance and the use
-// counters, so up to 14 hexominoes with multiplicity 2 on an 8-wide board
-// still fit.  The search itself is the same depth-first enumeration of the
-// row's completions with anchor = lowest open cell, in (piece, transform)
-// order (src/Piece.cpp:82-98, src/Piece.inl:5-24).
+// from HBM through the caches (74 KiB per library): 128-bit candidate masks
+// per (anchor column, row pattern) for the first three window rows -- one AND
+// per row leaves the placements that fit there, as the single-library
+// kernel's byte-sliced tables do (with the row-0 masks alone the search
+// takes 3.6 times as long) -- the candidates' shapes and counter fields, and
+// the pieces' counter fields.  States take two words, occupancy + instance
+// and the use counters, so up to 14 hexominoes with multiplicity 2 on an
+// 8-wide board still fit.  The search itself is the same depth-first
+// enumeration of the row's completions with anchor = lowest open cell, in
+// (piece, transform) order (src/Piece.cpp:82-98, src/Piece.inl:5-24).
 #include "d4_internal.h"
 #include "d4_device.h"
 
@@ -52,7 +54,9 @@ __device__ __forceinline__ M2 m_range(uint32_t lo, uint32_t hi) {
 
 __device__ __forceinline__ M2 mfit(const MLib *L, uint32_t c, uint64_t E) {
     const uint32_t i = 512u - (512u >> c) + ((uint32_t)(E & 0xffu) >> c);
-    return {L->fit[i][0], L->fit[i][1]};
+    const uint32_t b1 = (uint32_t)(E >> 8) & 0xffu, b2 = (uint32_t)(E >> 16) & 0xffu;
+    return {L->fit[i][0] & L->fitr[0][c][b1][0] & L->fitr[1][c][b2][0],
+            L->fit[i][1] & L->fitr[0][c][b1][1] & L->fitr[1][c][b2][1]};
 }
 
 // Insert a two-word key.  Each word of a slot changes only EMPTY -> value,
