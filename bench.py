@@ -153,7 +153,7 @@ def multi_libraries(d4, insts):
     return libs
 
 
-def uneven_section(d4, workloads, n, budget, plain_sample):
+def uneven_section(d4, workloads, n, budget, plain_sample, tune=True):
     """The uneven puzzles (own piece set and width per instance) in one
     multi-library batch with a per-instance state budget, against the same
     instances one per batch; parity against the oracle's counts of the
@@ -169,9 +169,13 @@ def uneven_section(d4, workloads, n, budget, plain_sample):
     log(f"uneven: {len(set(map(id, libs)))} libraries built on the host in {t_lib:.2f} s, "
         f"batch tables built and uploaded in {t_create:.2f} s")
     b.set_budget(budget)
-    t = time.perf_counter()
-    tuned = b.tune(64)  # measured launch shape on a sample (outside the timing)
-    log(f"uneven autotune {tuned} in {time.perf_counter() - t:.1f} s")
+    if tune:
+        t = time.perf_counter()
+        tuned = b.tune(64)  # measured launch shape on a sample (outside the timing)
+        log(f"uneven autotune {tuned} in {time.perf_counter() - t:.1f} s")
+    else:
+        tuned = {"threads": 256, "refill": 16}
+        b.set_launch(tuned["threads"], tuned["refill"])
     t = time.perf_counter()
     b.run()
     d4.synchronize()
@@ -228,7 +232,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tune", action="store_true",
                     help="run the measured autotuner instead of --launch")
-    ap.add_argument("--no-tune", action="store_true", help="library defaults, no --launch")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="library defaults, no --launch; no autotune of the uneven batch")
     ap.add_argument("--launch", default="3,256,16,768",
                     help="fit_rows,threads,refill,budget of the extend kernel (the "
                          "autotuner's choice for this workload on MI355X)")
@@ -402,7 +407,7 @@ def main():
 
     if args.uneven:
         out["uneven"] = uneven_section(d4, workloads, args.uneven, args.uneven_budget,
-                                       args.uneven_plain)
+                                       args.uneven_plain, tune=not args.no_tune)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

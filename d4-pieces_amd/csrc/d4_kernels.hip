@@ -529,13 +529,15 @@ __global__ __launch_bounds__(256) void k_fill_table(unsigned long long *key,
     }
 }
 
-// 4-word slots {k0, k1, count, unused} of the multi-library batch.
+// 4-word slots {k0, k1, count, unused} of the multi-library batch, written
+// whole (two 16-byte stores per slot), so every cache line is written in full.
 __global__ __launch_bounds__(256) void k_fill_table4(unsigned long long *tab, uint64_t n) {
+    ulonglong2 *t2 = reinterpret_cast<ulonglong2 *>(tab);
+    const ulonglong2 keys = make_ulonglong2(EMPTY_KEY, EMPTY_KEY), zero = make_ulonglong2(0, 0);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        tab[4 * i] = EMPTY_KEY;
-        tab[4 * i + 1] = EMPTY_KEY;
-        tab[4 * i + 2] = 0;
+        t2[2 * i] = keys;
+        t2[2 * i + 1] = zero;
     }
 }
 
