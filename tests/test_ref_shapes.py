@@ -1,0 +1,132 @@
+"""The oracle's and the product's shape geometry against the reference's own
+code: src/Shape.cpp, src/Group.hpp and the known-shape tables
+(src/known1.cpp, src/known2.cpp), compiled by `make -C oracle ref` into
+oracle/_ref/libref.so behind a C shim (oracle/ref_shim.cpp).  Skipped where
+that library was not built (it needs the reference's sources)."""
+import ctypes
+import os
+import random
+
+import pytest
+
+from conftest import ROOT
+
+REF = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built")
+
+# SymmetryGroup values (src/Group.hpp:8-19)
+GROUPS = [0b00000001, 0b00001001, 0b01101001, 0b00000011, 0b00000101, 0b00010001,
+          0b10000001, 0b00001111, 0b10011001, 0b11111111]
+
+
+@pytest.fixture(scope="module")
+def ref():
+    L = ctypes.CDLL(REF)
+    L.ref_transforms.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
+    for name in ("ref_canonical",):
+        getattr(L, name).restype = ctypes.c_uint64
+        getattr(L, name).argtypes = [ctypes.c_uint64, ctypes.c_uint]
+    for name in ("ref_normalize", "ref_extend1", "ref_sml_island"):
+        getattr(L, name).restype = ctypes.c_uint64
+        getattr(L, name).argtypes = [ctypes.c_uint64]
+    L.ref_symmetry.restype = ctypes.c_uint
+    L.ref_symmetry.argtypes = [ctypes.c_uint64]
+    L.ref_connected.restype = ctypes.c_int
+    L.ref_connected.argtypes = [ctypes.c_uint64]
+    L.ref_known.restype = ctypes.c_size_t
+    L.ref_known.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_void_p, ctypes.c_size_t]
+    return L
+
+
+def transforms(ref, s):
+    out = (ctypes.c_uint64 * 8)()
+    ref.ref_transforms(s, 1, out)
+    return list(out)
+
+
+def known(ref, one_sided, n):
+    buf = (ctypes.c_uint64 * 1024)()
+    k = ref.ref_known(one_sided, n, buf, 1024)
+    return list(buf[:k])
+
+
+def sample_shapes():
+    """Every free polyomino up to 6 cells in every orientation, plus random
+    (often disconnected) 8x8 masks, normalised and not."""
+    import d4pieces as d4
+    out = []
+    for n in range(1, 7):
+        for s in d4.free_polyominoes(n):
+            out += [d4.transform8(s, t) for t in range(8)]
+    rng = random.Random(17)
+    for _ in range(400):
+        v = rng.getrandbits(64) & rng.getrandbits(64) & rng.getrandbits(64)
+        out += [v, d4.normalize8(v)] if v else []
+    return out
+
+
+def test_transform_order_matches_reference(ref, oracle):
+    import d4pieces as d4
+    for s in sample_shapes():
+        want = transforms(ref, s)
+        assert [oracle.transform(s, t) for t in range(8)] == want, hex(s)
+        assert [d4.transform8(s, t) for t in range(8)] == want, hex(s)
+
+
+def test_canonical_forms_match_reference(ref, oracle):
+    for s in sample_shapes():
+        for forms in GROUPS:
+            assert oracle.canonical(s, forms) == ref.ref_canonical(s, forms), (hex(s), forms)
+
+
+def test_normalize_matches_reference(ref):
+    import d4pieces as d4
+    for s in sample_shapes():
+        assert d4.normalize8(s) == ref.ref_normalize(s)
+
+
+def test_unique_orientations_match_symmetry(ref, oracle):
+    """Piece's unique transforms (src/Piece.cpp:12-22) number 8 / |stabiliser|,
+    the stabiliser being Shape::symmetry()."""
+    import d4pieces as d4
+    for n in range(1, 8):
+        for s in d4.free_polyominoes(n):
+            stab = bin(ref.ref_symmetry(s)).count("1")
+            assert len(oracle.orients(s, 0xFF)) * stab == 8
+
+
+def test_polyomino_tables_match_reference(ref, oracle):
+    """The product's generated tables and the oracle's enumeration against the
+    reference's compiled known-shape tables."""
+    import d4pieces as d4
+    for n in range(1, 9):
+        assert d4.polyominoes(n, 0xFF) == known(ref, 0, n) == oracle.enum_free(n)
+        assert d4.polyominoes(n, 0x69) == known(ref, 1, n)
+
+
+def test_regions_match_reference_islands(ref):
+    """workloads.regions (the board filter of the seeded workloads) against
+    Shape::connected and Shape::sml_island."""
+    import workloads
+    rng = random.Random(23)
+    for _ in range(500):
+        rows = [rng.getrandbits(8) | rng.getrandbits(8) for _ in range(8)]
+        v = sum(r << (8 * i) for i, r in enumerate(rows))
+        areas = workloads.regions(rows)
+        assert sum(areas) == bin(v).count("1")
+        assert bool(ref.ref_connected(v)) == (len(areas) <= 1)
+        if v:
+            assert min(areas) == bin(ref.ref_sml_island(v)).count("1")
+
+
+def test_orientations_under_subgroups(ref, oracle):
+    """The oracle keeps |G| / |G and stabiliser| orientations of a piece under
+    each named symmetry group G (orbit-stabiliser), the stabiliser being the
+    reference's Shape::symmetry()."""
+    import d4pieces as d4
+    for n in range(1, 7):
+        for s in d4.free_polyominoes(n):
+            stab = ref.ref_symmetry(s)
+            for g in GROUPS:
+                k = len(oracle.orients(s, g))
+                assert k * bin(g & stab).count("1") == bin(g).count("1"), (hex(s), g)
