@@ -54,7 +54,8 @@ static void build_fit(d4_library &L, uint32_t fit_rows) {
 
 // Piece counters, candidates and the pruning constants (any library of up
 // to MAX_LIB_CAND candidates).
-static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *counts) {
+static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *counts,
+                         const uint8_t *enabled) {
     uint32_t k = 0, off = 0, g = 0, mins = UINT32_MAX;
     for (uint32_t i = 0; i < L.n; i++) {
         uint64_t s = shapes[i];
@@ -74,7 +75,9 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
         }
         // unique transforms enabled by sym, in order (Piece.cpp:12-22); the
         // transform id is the index within the sym-filtered list, as
-        // Piece::cover passes it to make_step.
+        // Piece::cover passes it to make_step.  An `enabled` mask switches
+        // placements on by that index instead (Piece::Placement::enabled,
+        // duplicates included when their bit is set).
         std::vector<uint64_t> seen;
         uint32_t first = k;
         unsigned sym = L.sym, nall = 0;
@@ -83,7 +86,7 @@ static void build_tables(d4_library &L, const uint64_t *shapes, const uint32_t *
             uint64_t sh = transform8(s, t);
             bool dup = std::find(seen.begin(), seen.end(), sh) != seen.end();
             seen.push_back(sh);
-            if (!dup) {
+            if (enabled ? ((enabled[i] >> nall) & 1u) != 0 : !dup) {
                 if (k >= (uint32_t)MAX_LIB_CAND)
                     throw Error(D4_EINVAL, "more than 128 (piece, transform) pairs");
                 L.cpiece[k] = (uint8_t)i;
@@ -1212,6 +1215,11 @@ size_t d4_free_polyominoes(unsigned n, uint64_t *out, size_t cap) {
 
 int d4_library_create(const uint64_t *shapes, const uint32_t *counts, uint32_t n,
                       uint32_t sym, uint32_t width, d4_library **out) {
+    return d4_library_create_ex(shapes, counts, n, sym, nullptr, width, out);
+}
+
+int d4_library_create_ex(const uint64_t *shapes, const uint32_t *counts, uint32_t n,
+                         uint32_t sym, const uint8_t *enabled, uint32_t width, d4_library **out) {
     return guard([&] {
         if (!out || !shapes || !counts) throw Error(D4_EINVAL, "null argument");
         *out = nullptr;
@@ -1222,7 +1230,7 @@ int d4_library_create(const uint64_t *shapes, const uint32_t *counts, uint32_t n
         L->n = n;
         L->sym = sym & 0xffu;
         L->W = width;
-        build_tables(*L, shapes, counts);
+        build_tables(*L, shapes, counts, enabled);
         int dev = 0, ndev = 0;
         HIPCHK(hipGetDeviceCount(&ndev));
         if (!ndev) throw Error(D4_EDEVICE, "no HIP device");

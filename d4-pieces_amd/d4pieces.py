@@ -82,6 +82,9 @@ SIGNATURES = [
     ("d4_free_polyominoes", ctypes.c_size_t, [ctypes.c_uint, _U64P, ctypes.c_size_t]),
     ("d4_library_create", ctypes.c_int,
      [_U64P, _U32P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
+    ("d4_library_create_ex", ctypes.c_int,
+     [_U64P, _U32P, ctypes.c_uint32, ctypes.c_uint32, _U8P, ctypes.c_uint32,
+      ctypes.POINTER(_P)]),
     ("d4_library_destroy", None, [_P]),
     ("d4_library_info", ctypes.c_int, [_P, _U32P, _U32P]),
     ("d4_library_set_launch", ctypes.c_int,
@@ -293,13 +296,15 @@ def expand_solution(shapes: Sequence[int], raw: Sequence[Step], height: int,
                     sym: int = 0xFF) -> Solution:
     """A solution as returned by solutions() ((piece_id, trs_id, y, x) per
     step) with the steps' matrices, placed shapes and the cell map.  trs_id
-    counts the transforms enabled in `sym` (Piece.cpp:12-22)."""
+    counts the transforms enabled in `sym` (Piece.cpp:12-22); the placed
+    shape is that transform's, while the matrix is make_step's for trs_id
+    itself (Piece.cpp:41-54 switches on the placement index, so under a
+    partial sym mask it is not the placed transform's matrix)."""
     enabled = [t for t in range(8) if sym >> t & 1]
     steps = []
     for pid, trs, y, x in raw:
-        t = enabled[trs]
-        shape = transform8(shapes[pid], t) << (8 * y + x)
-        steps.append(FullStep(pid, trs, *_STEP_MATRIX[t], x, y, shape))
+        shape = transform8(shapes[pid], enabled[trs]) << (8 * y + x)
+        steps.append(FullStep(pid, trs, *_STEP_MATRIX[trs], x, y, shape))
     return Solution(steps, height)
 
 
@@ -317,13 +322,19 @@ class PieceLibrary:
     """Piece set + placement tables on the current device."""
 
     def __init__(self, shapes: Sequence[int], counts: Optional[Sequence[int]] = None,
-                 sym: int = 0xFF, width: int = 8):
+                 sym: int = 0xFF, width: int = 8, enabled: Optional[Sequence[int]] = None):
+        """enabled[i]: piece i's placement mask (bit p = the p-th transform sym
+        admits); None = every distinct placement."""
         lib = load()
         counts = list(counts) if counts is not None else [1] * len(shapes)
         self.shapes, self.counts, self.sym, self.width = list(shapes), counts, sym, width
         h = _P()
-        _check(lib.d4_library_create(_u64arr(self.shapes), _u32arr(counts), len(shapes),
-                                     sym, width, ctypes.byref(h)))
+        if enabled is None:
+            _check(lib.d4_library_create(_u64arr(self.shapes), _u32arr(counts), len(shapes),
+                                         sym, width, ctypes.byref(h)))
+        else:
+            _check(lib.d4_library_create_ex(_u64arr(self.shapes), _u32arr(counts), len(shapes),
+                                            sym, _u8arr(list(enabled)), width, ctypes.byref(h)))
         self._h = h
         nc, mh = ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib.d4_library_info(h, ctypes.byref(nc), ctypes.byref(mh)))

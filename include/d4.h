@@ -110,6 +110,15 @@ size_t d4_free_polyominoes(unsigned n, uint64_t *out, size_t cap);
 int d4_library_create(const uint64_t *shapes, const uint32_t *counts,
                       uint32_t n, uint32_t sym, uint32_t width,
                       d4_library **out);
+/* d4_library_create with each piece's placements switched on by its own
+ * mask, as the UI does through Piece::Placement::enabled (src/Piece.hpp:16-20,
+ * bound in src/wasm.cpp:37-47; read by Piece::cover, src/Piece.inl:8-10):
+ * bit p of enabled[i] = the p-th transform sym admits (the placement's
+ * trs_id); a duplicate shape is a placement of its own when its bit is set.
+ * enabled == NULL is d4_library_create (duplicates off). */
+int d4_library_create_ex(const uint64_t *shapes, const uint32_t *counts, uint32_t n,
+                         uint32_t sym, const uint8_t *enabled, uint32_t width,
+                         d4_library **out);
 void d4_library_destroy(d4_library *lib);
 /* n_cand = unique (piece, transform) pairs, max_height = tallest piece. */
 int d4_library_info(const d4_library *lib, uint32_t *n_cand,

@@ -256,8 +256,27 @@ static int dfs(ctx_t *c, unsigned first_row) {
     return 0;
 }
 
-static int setup(ctx_t *c, const uint64_t *shapes, const uint32_t *counts,
-                 uint32_t n, uint32_t sym, const uint8_t *rows, uint32_t H) {
+/* Placements a piece's `enabled` mask switches on (Piece::Placement::enabled,
+ * Piece.hpp:16-20, read by Piece::cover, Piece.inl:8-10): bit p = the p-th
+ * transform the sym mask admits, duplicates included when their bit is set. */
+static unsigned orients_enabled(uint64_t shape, unsigned sym, unsigned en, uint64_t *normals,
+                                unsigned *trs) {
+    unsigned nall = 0, k = 0;
+    for (unsigned t = 0; t < 8; t++, sym >>= 1) {
+        if (!(sym & 1u)) continue;
+        if ((en >> nall) & 1u) {
+            normals[k] = or_transform(shape, t);
+            trs[k] = nall;
+            k++;
+        }
+        nall++;
+    }
+    return k;
+}
+
+static int setup_ex(ctx_t *c, const uint64_t *shapes, const uint32_t *counts,
+                    const uint8_t *enabled, uint32_t n, uint32_t sym, const uint8_t *rows,
+                    uint32_t H) {
     if (n > OR_MAXP || H > 64) return -1;
     memset(c, 0, sizeof *c);
     c->n = n;
@@ -265,7 +284,8 @@ static int setup(ctx_t *c, const uint64_t *shapes, const uint32_t *counts,
     for (uint32_t i = 0; i < n; i++) {
         c->count[i] = counts[i];
         c->size[i] = (uint32_t)__builtin_popcountll(shapes[i]);
-        c->nor[i] = or_piece_orients(shapes[i], sym, c->normal[i], c->trs[i]);
+        c->nor[i] = enabled ? orients_enabled(shapes[i], sym, enabled[i], c->normal[i], c->trs[i])
+                            : or_piece_orients(shapes[i], sym, c->normal[i], c->trs[i]);
         for (unsigned o = 0; o < c->nor[i]; o++) {
             /* p.max = (bottom(), right()) of the normalised shape: the
              * largest translation that stays on the board. */
@@ -279,6 +299,11 @@ static int setup(ctx_t *c, const uint64_t *shapes, const uint32_t *counts,
         c->open_cnt += (uint32_t)__builtin_popcount(rows[r]);
     }
     return 0;
+}
+
+static int setup(ctx_t *c, const uint64_t *shapes, const uint32_t *counts,
+                 uint32_t n, uint32_t sym, const uint8_t *rows, uint32_t H) {
+    return setup_ex(c, shapes, counts, NULL, n, sym, rows, H);
 }
 
 uint64_t or_solve_count(const uint64_t *shapes, const uint32_t *counts,
@@ -313,9 +338,15 @@ uint64_t or_solve_count_budget(const uint64_t *shapes, const uint32_t *counts,
 uint64_t or_solve(const uint64_t *shapes, const uint32_t *counts, uint32_t n,
                   uint32_t sym, const uint8_t *rows, uint32_t H, int single,
                   uint8_t *out, size_t cap, size_t *out_len) {
+    return or_solve_ex(shapes, counts, NULL, n, sym, rows, H, single, out, cap, out_len);
+}
+
+uint64_t or_solve_ex(const uint64_t *shapes, const uint32_t *counts, const uint8_t *enabled,
+                     uint32_t n, uint32_t sym, const uint8_t *rows, uint32_t H, int single,
+                     uint8_t *out, size_t cap, size_t *out_len) {
     ctx_t *c = malloc(sizeof *c);
     uint64_t r = 0;
-    if (!setup(c, shapes, counts, n, sym, rows, H)) {
+    if (!setup_ex(c, shapes, counts, enabled, n, sym, rows, H)) {
         c->single = single != 0;
         c->out = out ? out : (uint8_t *)c; /* non-null: record lengths */
         c->cap = out ? cap : 0;

@@ -94,6 +94,14 @@ uint64_t or_solve(const uint64_t *shapes, const uint32_t *counts, uint32_t n,
                   uint32_t sym, const uint8_t *rows, uint32_t H, int single,
                   uint8_t *out, size_t cap, size_t *out_len);
 
+/* or_solve with each piece's placements switched on by its own mask
+ * (Piece::Placement::enabled, Piece.hpp:16-20): bit p of enabled[i] = the
+ * p-th transform sym admits (its trs_id), duplicate shapes included when
+ * their bit is set.  enabled == NULL gives or_solve (duplicates off). */
+uint64_t or_solve_ex(const uint64_t *shapes, const uint32_t *counts, const uint8_t *enabled,
+                     uint32_t n, uint32_t sym, const uint8_t *rows, uint32_t H, int single,
+                     uint8_t *out, size_t cap, size_t *out_len);
+
 /* make_step's 2x2 matrix for transform trs (Piece.cpp:41-54): a, b, c, d. */
 void or_step_matrix(unsigned trs, int *abcd);
 

@@ -20,6 +20,17 @@ void ref_transforms(uint64_t s, int norm, uint64_t *out) {
     for (int i = 0; i < 8; i++) out[i] = t[i].get_value();
 }
 
+// Shape<8>::left / width / right / top / height / bottom (src/Shape.hpp:126-168)
+void ref_geometry(uint64_t s, uint64_t *out) {
+    const Shape<8> sh{s};
+    out[0] = sh.left();
+    out[1] = sh.width();
+    out[2] = sh.right();
+    out[3] = sh.top();
+    out[4] = sh.height();
+    out[5] = sh.bottom();
+}
+
 // Shape<8>::canonical_form(forms) (src/Shape.cpp:57-66)
 uint64_t ref_canonical(uint64_t s, unsigned forms) { return Shape<8>{s}.canonical_form(forms).get_value(); }
 
@@ -33,6 +44,11 @@ uint64_t ref_normalize(uint64_t s) { return Shape<8>{s}.normalize().get_value();
 uint64_t ref_extend1(uint64_t s) { return Shape<8>{s}.extend1().get_value(); }
 int ref_connected(uint64_t s) { return Shape<8>{s}.connected() ? 1 : 0; }
 uint64_t ref_sml_island(uint64_t s) { return Shape<8>{s}.sml_island().get_value(); }
+
+// SymmetryGroup product (src/Group.hpp:27-50)
+unsigned ref_group_product(unsigned a, unsigned b) {
+    return static_cast<unsigned>(static_cast<SymmetryGroup>(a) * static_cast<SymmetryGroup>(b));
+}
 
 // known_shapes / known_C_shapes (src/known1.cpp, src/known2.cpp): n-cell
 // table, min(total, cap) values written, total returned.

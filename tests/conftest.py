@@ -41,6 +41,11 @@ class Oracle:
                                ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
                                ctypes.c_void_p, ctypes.c_size_t,
                                ctypes.POINTER(ctypes.c_size_t)]
+        L.or_solve_ex.restype = ctypes.c_uint64
+        L.or_solve_ex.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                  ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.or_solve_count_budget.restype = ctypes.c_uint64
         L.or_solve_count_budget.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                             ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
@@ -104,14 +109,16 @@ class Oracle:
                                          ctypes.byref(done))
         return n, bool(done.value)
 
-    def solve(self, shapes, counts, rows, single=False, sym=0xFF):
+    def solve(self, shapes, counts, rows, single=False, sym=0xFF, enabled=None):
+        """enabled: per-piece placement masks (or_solve_ex), None = defaults."""
         sh, cn, rw = self._args(shapes, counts, rows)
+        en = (ctypes.c_uint8 * len(shapes))(*enabled) if enabled is not None else None
         need = ctypes.c_size_t(0)
-        self.L.or_solve(sh, cn, len(shapes), sym, rw, len(rows), int(single), None, 0,
-                        ctypes.byref(need))
+        self.L.or_solve_ex(sh, cn, en, len(shapes), sym, rw, len(rows), int(single), None, 0,
+                           ctypes.byref(need))
         buf = (ctypes.c_uint8 * max(need.value, 1))()
-        n = self.L.or_solve(sh, cn, len(shapes), sym, rw, len(rows), int(single), buf,
-                            need.value, ctypes.byref(need))
+        n = self.L.or_solve_ex(sh, cn, en, len(shapes), sym, rw, len(rows), int(single), buf,
+                               need.value, ctypes.byref(need))
         data = bytes(buf[:need.value])
         out, i = [], 0
         while i < len(data):

@@ -230,6 +230,53 @@ def test_subset_tilings(oracle):
     assert oracle.count([1, 15 & 0x3 | (0x3 << 8)], [4, 1], rect(2, 2)) == 2
 
 
+def default_masks(oracle, shapes, sym=0xFF):
+    """Piece::Placement::enabled as the Piece constructor sets it: every
+    placement that is not a duplicate of an earlier one."""
+    return [sum(1 << t for _, t in oracle.orients(s, sym)) for s in shapes]
+
+
+def test_enabled_placements(oracle, p5):
+    """or_solve_ex (per-placement masks, Piece.hpp:16-20 / Piece.inl:8-10):
+    the constructor's masks give or_solve's output; enabling the X
+    pentomino's 7 duplicate placements multiplies every tiling by 8; keeping
+    only the rotations equals the C4 symmetry mask; a piece with no placement
+    is a piece left out."""
+    scott = rect(8, 8, [(3, 3), (3, 4), (4, 3), (4, 4)])
+    base = oracle.solve(p5, [1] * 12, scott)
+    assert len(base) == 520
+    assert oracle.solve(p5, [1] * 12, scott, enabled=default_masks(oracle, p5)) == base
+    x = p5.index(0x020702)  # the X pentomino: all 8 transforms coincide
+    assert len(oracle.orients(p5[x], 0xFF)) == 1
+    en = default_masks(oracle, p5)
+    en[x] = 0xFF
+    assert len(oracle.solve(p5, [1] * 12, scott, enabled=en)) == 8 * 520
+    rot = 0b01101001
+
+    def rotations_only(shapes):
+        out = []
+        for s in shapes:
+            seen, m = [], 0
+            for t in range(8):
+                if rot >> t & 1:
+                    img = oracle.transform(s, t)
+                    if img not in seen:
+                        m |= 1 << t
+                    seen.append(img)
+            out.append(m)
+        return out
+
+    shapes4 = oracle.enum_free(4)
+    for shapes, counts, rows in [(p5, [1] * 12, rect(5, 6, [(0, 0), (5, 4)])),
+                                 (shapes4, [2] * 5, rect(4, 5)),
+                                 (shapes4, [1, 2, 1, 2, 1], rect(5, 4))]:
+        got = oracle.solve(shapes, counts, rows, enabled=rotations_only(shapes))
+        assert len(got) == oracle.count(shapes, counts, rows, sym=rot)
+    dom = [3, 1]  # domino with its placements off, monomino
+    assert len(oracle.solve(dom, [8, 16], rect(4, 4), enabled=[0, 1])) == 1
+    assert len(oracle.solve(dom, [8, 15], rect(4, 4), enabled=[0, 1])) == 0
+
+
 def test_record_codec_round_trip(oracle):
     """parse_R<H>(assemble_R<H>(cfg)) restores cfg (the reference's
     bmark/record.cu property) for every height and id count it can hold."""

@@ -130,3 +130,57 @@ def test_orientations_under_subgroups(ref, oracle):
             for g in GROUPS:
                 k = len(oracle.orients(s, g))
                 assert k * bin(g & stab).count("1") == bin(g).count("1"), (hex(s), g)
+
+
+def test_embind_shape_surface_matches_reference(ref):
+    """pieces_api.Shape8 (the embind Shape8 surface, src/wasm.cpp:48-74)
+    against the reference's compiled Shape<8>."""
+    import pieces_api as pa
+    ref.ref_geometry.argtypes = [ctypes.c_uint64, ctypes.c_void_p]
+    for s in sample_shapes():
+        if not s:
+            continue
+        sh = pa.Shape8(s)
+        g = (ctypes.c_uint64 * 6)()
+        ref.ref_geometry(s, g)
+        assert [sh.left, sh.width, sh.right, sh.top, sh.height, sh.bottom] == list(g), hex(s)
+        raw = (ctypes.c_uint64 * 8)()
+        ref.ref_transforms(s, 0, raw)
+        fns = [sh.transform0, sh.transform1, sh.transform2, sh.transform3, sh.transform4,
+               sh.transform5, sh.transform6, sh.transform7]
+        assert [f(False).value for f in fns] == list(raw)
+        assert [f(True).value for f in fns] == transforms(ref, s)
+        assert sh.symmetry == ref.ref_symmetry(s)
+        assert int(sh.classify) == ref.ref_symmetry(s)
+        assert sh.normalize().value == ref.ref_normalize(s)
+        for forms in GROUPS:
+            assert sh.canonical_form(forms).value == ref.ref_canonical(s, forms)
+
+
+def test_embind_piece_and_groups(ref, oracle):
+    """Piece8 placements (normal, enabled, duplicate) as the Piece constructor
+    (src/Piece.cpp:12-22) makes them, compared with the oracle's unique
+    orientations; groupProduct against the reference's rule table where that
+    gives a group, else the group closure of the reference's mask."""
+    import d4pieces as d4
+    import pieces_api as pa
+    ref.ref_group_product.restype = ctypes.c_uint
+    ref.ref_group_product.argtypes = [ctypes.c_uint, ctypes.c_uint]
+    for n in range(1, 7):
+        for s in d4.free_polyominoes(n):
+            for sym in GROUPS:
+                p = pa.Piece8(pa.Shape8(s), sym)
+                on = [(pl.normal.value, t) for t, pl in enumerate(p.placements) if pl.enabled]
+                assert on == oracle.orients(s, sym)
+                assert all(pl.duplicate != pl.enabled for pl in p.placements)
+                assert len(p.placements) == bin(sym).count("1") and p.count == 1
+    for a in GROUPS:
+        for b in GROUPS:
+            r = ref.ref_group_product(a, b)
+            got = pa.groupProduct(a, b)
+            assert got in GROUPS
+            assert got == r if r in GROUPS else got == pa.groupProduct(r, r)
+            assert pa.subgroup(got, a) and pa.subgroup(got, b)
+    assert pa.groupProduct(pa.SymmetryGroup.C4, pa.SymmetryGroup.D1_P) == pa.SymmetryGroup.D4
+    assert [pa.shape_count(n) for n in range(1, 9)] == [1, 1, 2, 5, 12, 35, 108, 369]
+    assert pa.shape_at8(5, 0).value == known(ref, 0, 5)[0]
