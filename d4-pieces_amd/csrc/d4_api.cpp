@@ -282,6 +282,9 @@ struct Engine {
     uint64_t parked_bytes = 0;
     bool trace = getenv("D4_TRACE") != nullptr;  // per-step lines of run() on stderr
     static constexpr uint64_t kExactCountMax = 1u << 16;
+    // children per record allowed for when a first row is sized without a
+    // count (count mode; more spill into a grown table)
+    static constexpr uint64_t kFirstRowFanout = 8192;
     d4_stats stats{};
     std::deque<DevLog> logs;  // logs[j] = global step log0 + j
     uint32_t log0 = 0;
@@ -467,26 +470,40 @@ struct Engine {
     // occupancy, count 1) is appended to the frontier.
     void upload_pending() {
         if (pending.empty()) return;
-        std::vector<uint64_t> win(bw_stride), k0, c0;
+        std::vector<uint64_t> k0, c0;
         for (uint32_t slot : pending) {
-            const uint8_t *bd = &boards[(size_t)slot * (MAX_ROWS + 8)];
-            for (uint32_t r = 0; r < bw_stride; r++) {
-                uint64_t w = 0;
-                for (uint32_t j = 0; j < 8 && r + j < (uint32_t)MAX_ROWS + 8; j++)
-                    w |= (uint64_t)bd[r + j] << (8 * j);
-                win[r] = w;
-            }
-            copy(bw.p + (size_t)slot * bw_stride, win.data(), bw_stride * 8,
-                 hipMemcpyHostToDevice);
             start[slot] = g;
             k0.push_back(inst_shift < 64 ? (uint64_t)slot << inst_shift : 0);
             c0.push_back(1);
         }
+        // board windows and zeroed counts, one copy per run of consecutive
+        // slots (a batch admits all its boards as one run)
+        std::vector<uint32_t> order(pending.begin(), pending.end());
+        std::sort(order.begin(), order.end());
+        std::vector<uint64_t> win;
+        std::vector<unsigned long long> zero;
+        for (size_t i = 0; i < order.size();) {
+            size_t j = i + 1;
+            while (j < order.size() && order[j] == order[j - 1] + 1) j++;
+            const uint32_t first = order[i], run = (uint32_t)(j - i);
+            win.assign((size_t)run * bw_stride, 0);
+            for (uint32_t q = 0; q < run; q++) {
+                const uint8_t *bd = &boards[(size_t)(first + q) * (MAX_ROWS + 8)];
+                for (uint32_t r = 0; r < bw_stride; r++) {
+                    uint64_t w = 0;
+                    for (uint32_t b = 0; b < 8 && r + b < (uint32_t)MAX_ROWS + 8; b++)
+                        w |= (uint64_t)bd[r + b] << (8 * b);
+                    win[(size_t)q * bw_stride + r] = w;
+                }
+            }
+            copy(bw.p + (size_t)first * bw_stride, win.data(), win.size() * 8,
+                 hipMemcpyHostToDevice);
+            zero.assign(run, 0);
+            copy(result.p + first, zero.data(), (size_t)run * 8, hipMemcpyHostToDevice);
+            i = j;
+        }
         copy(hgt.p, heights.data(), (size_t)S * 4, hipMemcpyHostToDevice);
         copy(dstart.p, start.data(), (size_t)S * 4, hipMemcpyHostToDevice);
-        std::vector<unsigned long long> zero(1, 0);
-        for (uint32_t slot : pending)
-            copy(result.p + slot, zero.data(), 8, hipMemcpyHostToDevice);
         const uint64_t n = n_front + k0.size();
         if (n > fk.cap || n > fc.cap) {  // grow, keeping the current frontier
             DevBuf<uint64_t> k2, c2;
@@ -562,11 +579,16 @@ struct Engine {
                 est_c = est_u = bound;
                 exact = true;
                 if (stats.rows_done >= 2 && g_u < bound) est_u = g_u;
-            } else if (n_front <= kExactCountMax && (en || stats.rows_done < 2)) {
-                // (count mode past the first rows sizes from the growth
-                // estimate instead: a short table spills and grows)
+            } else if (n_front <= kExactCountMax && en) {
+                // enum mode needs every child's slot: count them exactly
                 est_c = est_u = count_children();
                 stats.count_passes++;
+                exact = true;
+            } else if (n_front <= kExactCountMax && stats.rows_done < 2) {
+                // count mode: a short table spills and grows, so the first
+                // rows are sized from a generous per-record allowance rather
+                // than by searching the row twice
+                est_c = est_u = std::min<uint64_t>(bound, n_front * kFirstRowFanout + 65536);
                 exact = true;
             }
         }
