@@ -66,14 +66,26 @@ __device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
     return -1;
 }
 
+// FR > 0: the table has FR rows, known when compiling, so the lookups are
+// independent LDS loads with constant offsets; FR == 0: `rows` at run time
+// (a loop, one load after the other).
+template <int FR>
 __device__ __forceinline__ uint64_t fitmask(const uint64_t *s_fit,
                                             uint32_t rows, uint32_t c,
                                             uint64_t E) {
-    const uint64_t *f = s_fit + (size_t)c * rows * 256u;
-    uint64_t m = f[E & 0xffu];
-    for (uint32_t j = 1; j < rows; j++)
-        m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
-    return m;
+    if constexpr (FR > 0) {
+        const uint64_t *f = s_fit + (size_t)c * FR * 256u;
+        uint64_t m = f[E & 0xffu];
+#pragma unroll
+        for (int j = 1; j < FR; j++) m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
+        return m;
+    } else {
+        const uint64_t *f = s_fit + (size_t)c * rows * 256u;
+        uint64_t m = f[E & 0xffu];
+        for (uint32_t j = 1; j < rows; j++)
+            m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
+        return m;
+    }
 }
 
 // Candidates still available under the use counters `used`: counter fields
@@ -99,7 +111,7 @@ __device__ __forceinline__ uint64_t piece_range(uint64_t info) {
     return (hi >= 64 ? ~0ull : (1ull << hi) - 1ull) & (~0ull << lo);
 }
 
-template <bool ENUM, bool PRUNE, bool COUNT_ONLY, int THREADS>
+template <bool ENUM, bool PRUNE, bool COUNT_ONLY, int THREADS, int FR>
 __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     constexpr uint32_t SW = ENUM ? 5 : 3;  // staged u64 words per child
     extern __shared__ uint64_t lds[];
@@ -366,7 +378,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                     if (E & 0xffu) {
                         avail = avail_of(s_xa, used);
                         c = (uint32_t)__builtin_ctzll(E & 0xffu);
-                        ok = fitmask(s_fit, rows, c, E) & avail;
+                        ok = fitmask<FR>(s_fit, rows, c, E) & avail;
                         if (fix == 1) {
                             ok &= 1ull << sub;
                         } else if (fix == 2) {
@@ -393,7 +405,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                                     d = 1;
                                     base_d = 1;
                                     c = (uint32_t)__builtin_ctzll(E & 0xffu);
-                                    ok = fitmask(s_fit, rows, c, E) & avail & (1ull << k1);
+                                    ok = fitmask<FR>(s_fit, rows, c, E) & avail & (1ull << k1);
                                 }
                             }
                         } else if (don_mode) {
@@ -417,7 +429,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                             d = lvl;
                             base_d = lvl;
                             sib = !single;
-                            ok = fitmask(s_fit, rows, c, E) & avail &
+                            ok = fitmask<FR>(s_fit, rows, c, E) & avail &
                                  (single ? 1ull << first : ~0ull << first);
                         }
                         searching = ok != 0;
@@ -466,7 +478,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                     anc |= c << (3 * d);
                     d++;
                     c = (uint32_t)__builtin_ctzll(E & 0xffu);
-                    ok = fitmask(s_fit, rows, c, E) & avail;
+                    ok = fitmask<FR>(s_fit, rows, c, E) & avail;
                 }
             } else if (d <= base_d) {
                 // the levels below base_d are fixed by the work item
@@ -481,7 +493,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 E |= pk.shape;
                 used -= 1u << (pk.info & 0xffu);
                 avail |= piece_range(pk.info);
-                ok = (d > base_d || sib) ? fitmask(s_fit, rows, c, E) & avail & ((~0ull << k) << 1)
+                ok = (d > base_d || sib) ? fitmask<FR>(s_fit, rows, c, E) & avail & ((~0ull << k) << 1)
                                          : 0;
             }
         }
@@ -864,19 +876,35 @@ void launch_reset_donations(uint32_t *c32, unsigned long long *c64, hipStream_t 
     hipLaunchKernelGGL(k_reset_donations, dim3(1), dim3(1), 0, st, c32, c64);
 }
 
+template <bool ENUM, bool PRUNE, bool COUNT_ONLY, int FR>
+static void launch_extend_fr(const ExtendArgs &a, uint32_t threads, dim3 g, size_t lds,
+                             hipStream_t st) {
+    switch (threads) {
+        case 256:
+            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 256, FR>), g, dim3(256), lds, st,
+                               a);
+            break;
+        case 512:
+            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 512, FR>), g, dim3(512), lds, st,
+                               a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 1024, FR>), g, dim3(1024), lds,
+                               st, a);
+            break;
+    }
+}
+
+// Table rows 1-3 (the range the tuner picks for pieces up to 5 rows tall)
+// get kernels with the row count compiled in; more rows use the loop.
 template <bool ENUM, bool PRUNE, bool COUNT_ONLY>
 static void launch_extend_t(const ExtendArgs &a, uint32_t threads, dim3 g, size_t lds,
                             hipStream_t st) {
-    switch (threads) {
-        case 256:
-            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 256>), g, dim3(256), lds, st, a);
-            break;
-        case 512:
-            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 512>), g, dim3(512), lds, st, a);
-            break;
-        default:
-            hipLaunchKernelGGL((k_extend<ENUM, PRUNE, COUNT_ONLY, 1024>), g, dim3(1024), lds, st, a);
-            break;
+    switch (a.fit_rows) {
+        case 1: launch_extend_fr<ENUM, PRUNE, COUNT_ONLY, 1>(a, threads, g, lds, st); break;
+        case 2: launch_extend_fr<ENUM, PRUNE, COUNT_ONLY, 2>(a, threads, g, lds, st); break;
+        case 3: launch_extend_fr<ENUM, PRUNE, COUNT_ONLY, 3>(a, threads, g, lds, st); break;
+        default: launch_extend_fr<ENUM, PRUNE, COUNT_ONLY, 0>(a, threads, g, lds, st); break;
     }
 }
 
