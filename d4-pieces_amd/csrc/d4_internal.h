@@ -114,6 +114,9 @@ struct MLib {
     // fitr[t][c][b]: candidates anchored at column c whose cells in window
     // row t + 1 lie inside row pattern b
     uint64_t fitr[2][8][256][2];
+    // xav[b][v]: candidates of the pieces whose use counter lies in byte b
+    // of the counters and is exhausted when that byte reads v
+    uint64_t xav[4][256][2];
     // per candidate: normalised shape, and info = counter offset |
     // field mask << 8 | count << 16 | anchor column of the shape << 32 |
     // piece's candidate range lo << 40 | hi << 48

@@ -83,6 +83,19 @@ void build_mlib(const d4_library &L, MLib &M) {
         M.piece[i] = (uint64_t)L.off[i] | ((uint64_t)((1u << L.fbits[i]) - 1u) << 8) |
                      ((uint64_t)L.counts[i] << 16) | (lo << 32) | (hi << 40);
     }
+    // xav[b][v]: the candidates of every piece whose counter lies in byte b
+    // and is exhausted when that byte reads v (a piece of count 0 always is)
+    uint64_t pm[MAX_PIECES][2] = {};
+    for (uint32_t j = 0; j < k; j++) pm[L.cpiece[j]][j >> 6] |= 1ull << (j & 63u);
+    for (uint32_t b = 0; b < 4; b++)
+        for (uint32_t v = 0; v < 256; v++)
+            for (uint32_t i = 0; i < L.n; i++) {
+                if (L.off[i] / 8 != b && !(L.fbits[i] == 0 && b == 0)) continue;
+                const uint32_t f = L.fbits[i] ? (v >> (L.off[i] % 8)) & ((1u << L.fbits[i]) - 1u) : 0;
+                if (f < L.counts[i]) continue;
+                M.xav[b][v][0] |= pm[i][0];
+                M.xav[b][v][1] |= pm[i][1];
+            }
     const uint32_t occ_bits = 8 * (L.maxh > 1 ? L.maxh - 1 : 0);
     M.occ_mask = occ_bits ? (1ull << occ_bits) - 1ull : 0;
     M.seed_mask = L.maxh >= 2 ? ~((1ull << (8 * (L.maxh - 1))) - 1ull) : ~0ull;

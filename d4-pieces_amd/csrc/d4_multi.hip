@@ -5,12 +5,13 @@
 // The single-library kernel (d4_kernels.hip) stages one library's tables in
 // LDS and packs a state into one u64.  Here every record names its instance,
 // the instance names its library, and the lane reads that library's tables
-// from HBM through the caches (74 KiB per library): 128-bit candidate masks
+// from HBM through the caches (90 KiB per library): 128-bit candidate masks
 // per (anchor column, row pattern) for the first three window rows -- one AND
 // per row leaves the placements that fit there, as the single-library
 // kernel's byte-sliced tables do (with the row-0 masks alone the search
 // takes 3.6 times as long) -- the candidates' shapes and counter fields, and
-// the pieces' counter fields.  States take two words, occupancy + instance
+// per counter byte value the candidates of the pieces it shows exhausted (a
+// record's available candidates in four lookups).  States take two words, occupancy + instance
 // and the use counters, so up to 14 hexominoes with multiplicity 2 on an
 // 8-wide board still fit.  The search itself is the same depth-first
 // enumeration of the row's completions with anchor = lowest open cell, in
@@ -308,17 +309,12 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
                     anc = 0;
                     desc = 0;
                     if (E & 0xffu) {
-                        avail = {~0ull, ~0ull};
-                        const uint32_t np = L->n_pieces;
-                        for (uint32_t i = 0; i < np; i++) {
-                            const uint64_t p = L->piece[i];
-                            const uint32_t f = (used >> (p & 0xffu)) & ((uint32_t)(p >> 8) & 0xffu);
-                            if (f >= ((uint32_t)(p >> 16) & 0xffffu)) {
-                                const M2 r = m_range((uint32_t)(p >> 32) & 0xffu,
-                                                     (uint32_t)(p >> 40) & 0xffu);
-                                avail.lo &= ~r.lo;
-                                avail.hi &= ~r.hi;
-                            }
+                        {
+                            const uint64_t *x0 = L->xav[0][used & 0xffu];
+                            const uint64_t *x1 = L->xav[1][(used >> 8) & 0xffu];
+                            const uint64_t *x2 = L->xav[2][(used >> 16) & 0xffu];
+                            const uint64_t *x3 = L->xav[3][used >> 24];
+                            avail = {~(x0[0] | x1[0] | x2[0] | x3[0]), ~(x0[1] | x1[1] | x2[1] | x3[1])};
                         }
                         c = (uint32_t)__builtin_ctzll(E & 0xffu);
                         ok = m_and(mfit(L, c, E), avail);
