@@ -119,11 +119,14 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     uint64_t *s_fit = lds;
     const PlaceInfo *s_pc = reinterpret_cast<const PlaceInfo *>(s_fit + nfit);
     uint64_t *s_xa = s_fit + nfit + 2 * 8 * MAX_CAND;
-    uint64_t *s_stage = s_xa + 4 * 256;
+    uint64_t *s_rng = s_xa + 4 * 256;  // per candidate: its piece's candidate range
+    uint64_t *s_stage = s_rng + MAX_CAND;
     for (uint32_t i = threadIdx.x; i < nfit; i += THREADS) s_fit[i] = a.fit[i];
     for (uint32_t i = threadIdx.x; i < 2 * 8 * MAX_CAND; i += THREADS)
         s_fit[nfit + i] = a.placeinfo[i];
     for (uint32_t i = threadIdx.x; i < 4 * 256; i += THREADS) s_xa[i] = a.xavail[i];
+    for (uint32_t i = threadIdx.x; i < MAX_CAND; i += THREADS)
+        s_rng[i] = piece_range(a.placeinfo[2 * i + 1]);
     __syncthreads();
 
     const uint32_t lane = lane_id();
@@ -399,7 +402,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                                     E = E1;
                                     used += 1u << (ci & 0xffu);
                                     if (((used >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
-                                        avail &= ~piece_range(pi0.info);
+                                        avail &= ~s_rng[k0];
                                     desc = (uint64_t)(k0 + 1);
                                     anc = c;
                                     d = 1;
@@ -421,7 +424,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                                 E &= ~pj.shape;
                                 used += 1u << (cj & 0xffu);
                                 if (((used >> (cj & 0xffu)) & ((cj >> 8) & 0xffu)) == (cj >> 16))
-                                    avail &= ~piece_range(pj.info);
+                                    avail &= ~s_rng[kj];
                                 anc |= c << (3 * j);
                                 c = (uint32_t)__builtin_ctzll(E & 0xffu);
                             }
@@ -473,7 +476,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                     E = E2;
                     used += inc;
                     if (((used >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
-                        avail &= ~piece_range(pk.info);
+                        avail &= ~s_rng[k];
                     desc |= (uint64_t)(k + 1) << (8 * d);
                     anc |= c << (3 * d);
                     d++;
@@ -492,7 +495,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 const PlaceInfo pk = s_pc[c * MAX_CAND + k];
                 E |= pk.shape;
                 used -= 1u << (pk.info & 0xffu);
-                avail |= piece_range(pk.info);
+                avail |= s_rng[k];
                 ok = (d > base_d || sib) ? fitmask<FR>(s_fit, rows, c, E) & avail & ((~0ull << k) << 1)
                                          : 0;
             }
@@ -833,7 +836,7 @@ void launch_first_solutions_walk(const uint32_t *slots, uint32_t n_slots, const 
 }
 
 size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool enum_mode) {
-    return sizeof(uint64_t) * (8u * fit_rows * 256u + 2u * 8u * MAX_CAND + 4u * 256u) +
+    return sizeof(uint64_t) * (8u * fit_rows * 256u + 2u * 8u * MAX_CAND + 4u * 256u + MAX_CAND) +
            sizeof(uint64_t) * (enum_mode ? 5u : 3u) * STAGE * (threads / 64);
 }
 
