@@ -234,7 +234,7 @@ def main():
                     help="run the measured autotuner instead of --launch")
     ap.add_argument("--no-tune", action="store_true",
                     help="library defaults, no --launch; no autotune of the uneven batch")
-    ap.add_argument("--launch", default="3,256,16,768",
+    ap.add_argument("--launch", default="3,256,16,1024",
                     help="fit_rows,threads,refill,budget of the extend kernel (the "
                          "autotuner's choice for this workload on MI355X)")
     ap.add_argument("--tune-sample", type=int, default=256)

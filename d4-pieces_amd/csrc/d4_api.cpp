@@ -432,8 +432,13 @@ struct Engine {
         if (!lib->ext_budget) return;
         // Rounds are queued a few at a time without a host round trip: each
         // reads its item count from the counter pair the round before it
-        // wrote, and the two pairs alternate.
-        constexpr uint32_t kRoundsPerSync = 6;
+        // wrote, and the two pairs alternate.  A round lasts about as long
+        // as its budget of dependent DFS steps on one lane, so follow-up
+        // items get a short budget and split again quickly; after
+        // kBudgetedRounds rounds the rest runs to completion.
+        constexpr uint32_t kRoundsPerSync = 8;
+        constexpr uint32_t kFollowBudget = 64;
+        constexpr uint32_t kBudgetedRounds = 64;
         uint32_t rd = C32_DON;  // the pair that counted the items in don_a
         for (uint32_t round = 0;;) {
             uint32_t h[C32_N];
@@ -455,8 +460,7 @@ struct Engine {
                 a.don_write = wr;
                 a.don_out = don_b.p;
                 a.don_cap = (uint32_t)(don_b.cap / 2);
-                // follow-up items are smaller: split them sooner
-                a.budget = round < 24 ? std::max(lib->ext_budget / 4, 32u) : 0;
+                a.budget = round < kBudgetedRounds ? kFollowBudget : 0;
                 launch_extend(a, en, prune, count_only, lib->ext_threads, lib->n_cu, st);
                 HIPCHK(hipGetLastError());
                 don_a.swap(don_b);
@@ -1362,7 +1366,7 @@ int d4_library_tune(d4_library *lib, uint32_t n_inst, const uint8_t *boards, uin
             throw Error(D4_EDEVICE, g_err);
         // then the donation budget (work split for the launch's tail)
         uint32_t best_b = lib->ext_budget;
-        for (uint32_t b : {0u, 256u, 384u, 512u, 768u, 1024u, 2048u}) {
+        for (uint32_t b : {0u, 512u, 768u, 1024u, 1536u, 2048u}) {
             lib->ext_budget = b;
             double ms = measure();
             if (ms < best) {

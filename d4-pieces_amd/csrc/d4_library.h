@@ -27,7 +27,7 @@ struct d4_library {
     int n_cu = 1;
     uint32_t ext_threads = 1024;  // extend workgroup size (autotuned)
     uint32_t ext_refill = 16;     // idle lanes per refill round (autotuned)
-    uint32_t ext_budget = 512;    // DFS steps per work item before donating (0: never)
+    uint32_t ext_budget = 1024;   // DFS steps per work item before donating (0: never)
     // single-library tables (at most MAX_CAND candidates: one u64 mask),
     // built and uploaded on first use by an engine
     std::mutex fast_mu;
