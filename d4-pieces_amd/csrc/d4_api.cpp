@@ -1126,11 +1126,7 @@ static std::unique_ptr<Engine> make_batch(const d4_library *lib, uint32_t n, con
     auto e = std::make_unique<Engine>(lib, n, hm, mode, flags, true);
     for (uint32_t i = 0; i < n; i++) e->admit(i, b + (size_t)i * stride, h[i]);
     e->upload_pending();
-    if (mode == D4_MODE_COUNT) {
-        size_t free_b = 0, total_b = 0;
-        HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        e->mem_cap = (uint64_t)(0.8 * (double)free_b);
-    }
+    if (mode == D4_MODE_COUNT) e->mem_cap = (uint64_t)(0.8 * (double)available_device_bytes());
     return e;
 }
 
@@ -1558,11 +1554,7 @@ int d4_server_create(const d4_library *lib, uint32_t slots, uint32_t max_height,
         sv->slots = slots;
         sv->max_h = max_height;
         HIPCHK(hipSetDevice(lib->device));
-        if (!mem_budget) {
-            size_t fr = 0, tot = 0;
-            HIPCHK(hipMemGetInfo(&fr, &tot));
-            mem_budget = fr / 2;
-        }
+        if (!mem_budget) mem_budget = available_device_bytes() / 2;
         sv->budget = mem_budget;
         for (int k = 0; k < 2; k++) {
             sv->eng[k] = std::make_unique<Engine>(lib, slots, max_height,

@@ -154,11 +154,30 @@ class Pool {
         std::lock_guard<std::mutex> lk(mu_);
         free_.emplace(round(bytes), std::make_pair(dev, p));
     }
+    // Bytes cached for reuse on a device: free for the library's purposes,
+    // though the driver counts them as in use.
+    size_t cached(int dev) {
+        std::lock_guard<std::mutex> lk(mu_);
+        size_t n = 0;
+        for (auto &kv : free_)
+            if (kv.second.first == dev) n += kv.first;
+        return n;
+    }
 };
 
 inline Pool &pool() {
     static Pool *p = new Pool();  // never destroyed: outlives every handle
     return *p;
+}
+
+// Device memory a new search may plan with: the driver's free bytes plus
+// what the allocator holds cached on the current device.
+inline uint64_t available_device_bytes() {
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    return (uint64_t)free_b + (uint64_t)pool().cached(dev);
 }
 
 template <class T>

@@ -154,7 +154,9 @@ struct MultiArgs {
     // load balance, as in ExtendArgs (candidate ids up to 127)
     uint32_t budget;
     const uint64_t *don_in;
-    uint32_t n_don;
+    uint32_t n_don;          // items in don_in, or ~0u: read them from c32[don_read]
+    uint32_t don_read;       // counter pair (C32_DON or C32_DON1) that counted don_in
+    uint32_t don_write;      // counter pair this launch's donations count into
     uint64_t *don_out;
     uint32_t don_cap;
     // terminal states of finished instances, two words each: instance |

@@ -129,11 +129,9 @@ struct d4_mbatch {
     DevBuf<uint32_t> c32, blk;
     DevBuf<unsigned long long> c64;
     DevBuf<uint64_t> don_a, don_b;  // donated work items (two words each)
-    // DFS steps per work item before donating; 0 (off) by default: the one
-    // GPU run with donation on in this kernel faulted and the cause is not
-    // yet found (a CPU model of the per-lane search and donation agrees
-    // with the no-donation search), so the path stays off
-    uint32_t budget_steps = 0;
+    // DFS steps per work item of the main launch before its lane hands the
+    // rest of its search tree to follow-up launches (0: never)
+    uint32_t budget_steps = 4096;
     uint64_t n_front = 0;
     double grow_unique = 1.0, grow_children = 1.0;
     // bounded memory: frontier halves parked for later (depth first)
@@ -219,6 +217,8 @@ struct d4_mbatch {
         a.budget = budget_steps;
         a.don_out = don_a.p;
         a.don_cap = (uint32_t)(don_a.cap / 2);
+        a.don_read = C32_DON;
+        a.don_write = C32_DON;
         if (keep_terminals) {
             a.sink = sink.p + 2 * sink_n;
             a.sink_cap = (uint32_t)std::min<uint64_t>(sink_cap - sink_n, 0xffffffffull);
@@ -234,27 +234,40 @@ struct d4_mbatch {
         launch_extend_multi(args(row, tab_m), prune, count_only, threads, n_cu, st);
         HIPCHK(hipGetLastError());
         if (!budget_steps) return;
-        for (uint32_t round = 0;; round++) {
+        // Rounds are queued a few at a time without a host round trip: each
+        // reads its item count from the counter pair the round before it
+        // wrote, and the two pairs alternate.  A round lasts about as long
+        // as its budget of dependent DFS steps on one lane, so follow-up
+        // items get a short budget and split again quickly; after
+        // kBudgetedRounds rounds the rest runs to completion.
+        constexpr uint32_t kRoundsPerSync = 8;
+        constexpr uint32_t kFollowBudget = 64;
+        constexpr uint32_t kBudgetedRounds = 96;
+        uint32_t rd = C32_DON;
+        for (uint32_t round = 0;;) {
             uint32_t h[C32_N];
-            HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
+            copy(h, c32.p, sizeof h, hipMemcpyDeviceToHost);
             if (h[C32_BAD_ITEM]) throw Error(D4_EDEVICE, "internal error: bad donated work item");
             if (h[C32_OVERFLOW]) return;  // the step is rerun
             const uint32_t cap = (uint32_t)(don_a.cap / 2);
-            const uint32_t n =
-                h[C32_DON] <= cap ? h[C32_DON] : (h[C32_DON_END] ? h[C32_DON_END] - 1 : cap);
+            const uint32_t n = h[rd] <= cap ? h[rd] : (h[rd + 1] ? h[rd + 1] - 1 : cap);
             if (!n) return;
-            stats.donated += n;
-            launch_reset_donations(c32.p, c64.p, st);
-            MultiArgs a = args(row, tab_m);
-            a.don_in = don_a.p;
-            a.n_don = n;
-            a.don_out = don_b.p;
-            a.don_cap = (uint32_t)(don_b.cap / 2);
-            a.budget = round < 24 ? std::max(budget_steps / 4, 32u) : 0;
-            launch_extend_multi(a, prune, count_only, threads, n_cu, st);
-            HIPCHK(hipGetLastError());
-            don_a.swap(don_b);
+            for (uint32_t q = 0; q < kRoundsPerSync; q++, round++) {
+                const uint32_t wr = rd == C32_DON ? C32_DON1 : C32_DON;
+                launch_reset_donation_pair(c32.p, c64.p, wr, st);
+                MultiArgs a = args(row, tab_m);
+                a.don_in = don_a.p;
+                a.n_don = q == 0 ? n : ~0u;
+                a.don_read = rd;
+                a.don_write = wr;
+                a.don_out = don_b.p;
+                a.don_cap = (uint32_t)(don_b.cap / 2);
+                a.budget = round < kBudgetedRounds ? kFollowBudget : 0;
+                launch_extend_multi(a, prune, count_only, threads, n_cu, st);
+                HIPCHK(hipGetLastError());
+                don_a.swap(don_b);
+                rd = wr;
+            }
         }
     }
 
@@ -323,7 +336,10 @@ struct d4_mbatch {
         // exact children count from a search-only pass; a large one an
         // estimate from the previous step's ratios (first rows grow by
         // thousands per record, later ones by a few), with a rerun at 4x the
-        // size if the table still overflows.
+        // size if the table still overflows.  (The estimate errs large after
+        // the first rows; sized from a sampled count instead, the tables are
+        // 4x smaller but inserts collide more on shared lines, and the step
+        // is no faster.)
         uint64_t est_u;
         if (n_front <= kExactCountMax) {
             est_u = count_children(row);
@@ -400,6 +416,7 @@ struct d4_mbatch {
         if (keep_terminals) sink_n += h32[C32_SINK];
         stats.children += h64[C64_CHILDREN];
         stats.pruned += h64[C64_PRUNED];
+        stats.donated += h64[C64_DONATED];
         stats.records_extended += n_front;
         stats.unique += U;
         if (n_front) {
@@ -660,9 +677,7 @@ static std::unique_ptr<d4_mbatch> make_mbatch(uint32_t n_inst, const d4_library 
         B->c32.reserve(C32_N);
         B->c64.reserve(C64_N);
         HIPCHK(hipStreamSynchronize(B->st));
-        size_t free_b = 0, total_b = 0;
-        HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        B->mem_cap = (uint64_t)(0.8 * (double)free_b);
+        B->mem_cap = (uint64_t)(0.8 * (double)available_device_bytes());
         B->lib_of.assign(libs, libs + n_inst);
         B->board_bytes.assign(boards, boards + (size_t)n_inst * stride);
         B->stride = stride;

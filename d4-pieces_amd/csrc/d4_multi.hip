@@ -100,7 +100,14 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
     const bool don_mode = a.don_in != nullptr;
     const uint32_t fix = don_mode ? 0u : a.split;
     const uint32_t per_rec = 1u << (7 * fix);
-    const uint64_t n_items = don_mode ? (uint64_t)a.n_don : (uint64_t)a.n_in * per_rec;
+    // a follow-up launch queued without a host round trip reads its item
+    // count from the counters of the launch before it
+    uint32_t n_don = a.n_don;
+    if (don_mode && n_don == ~0u) {
+        const uint32_t dn = a.c32[a.don_read], de = a.c32[a.don_read + 1];
+        n_don = dn <= a.don_cap ? dn : (de ? de - 1 : a.don_cap);
+    }
+    const uint64_t n_items = don_mode ? (uint64_t)n_don : (uint64_t)a.n_in * per_rec;
     bool exhausted = false;
     uint32_t buf_items = 0, buf_pos = 0, buf_rec0 = 0;
     uint64_t buf_item0 = 0;
@@ -178,17 +185,18 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
         if (!n) return false;
         // once the buffer is full nobody adds to the counter again (so it
         // cannot wrap), and a lane that found no room keeps its item
-        if (__hip_atomic_load(a.c32 + C32_DON, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+        if (__hip_atomic_load(a.c32 + a.don_write, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
             a.don_cap) {
             no_don = true;
             return false;
         }
-        const uint32_t b = atomicAdd(a.c32 + C32_DON, n);
+        const uint32_t b = atomicAdd(a.c32 + a.don_write, n);
         if ((uint64_t)b + n > (uint64_t)a.don_cap) {
             no_don = true;
-            if (b < a.don_cap) a.c32[C32_DON_END] = b + 1;
+            if (b < a.don_cap) a.c32[a.don_write + 1] = b + 1;  // the one range crossing the end
             return false;
         }
+        atomicAdd(a.c64 + C64_DONATED, (unsigned long long)n);
         uint32_t o = b;
         auto put = [&](uint32_t lvl, uint32_t first, bool single) __attribute__((always_inline)) {
             const uint64_t pm = lvl >= 8 ? ~0ull : (1ull << (8 * lvl)) - 1ull;
@@ -520,8 +528,12 @@ void launch_extend_multi(const MultiArgs &a, bool prune, bool count_only, uint32
     if (threads != 256 && threads != 512) threads = 1024;
     // 32 waves per CU; the stage takes 4 KiB of LDS per wave
     const uint64_t per_cu = 32u / (threads / 64);
-    const uint64_t waves =
-        ((a.don_in ? (uint64_t)a.n_don : ((uint64_t)a.n_in << (7 * a.split))) + 63) / 64;
+    // (a follow-up launch whose count is read on the device gets the full
+    // resident grid; its waves leave at once when there is nothing to do)
+    const uint64_t n_work = !a.don_in          ? ((uint64_t)a.n_in << (7 * a.split))
+                            : a.n_don == ~0u ? ~0ull / 2
+                                             : (uint64_t)a.n_don;
+    const uint64_t waves = (n_work + 63) / 64;
     uint64_t blocks = (waves + threads / 64 - 1) / (threads / 64);
     blocks = std::min<uint64_t>(blocks, (uint64_t)n_cu * per_cu);
     if (!blocks) blocks = 1;

@@ -555,7 +555,8 @@ class MultiBatch:
         return {"threads": t.value, "refill": r.value, "extend_ms": ms.value}
 
     def set_budget_steps(self, steps: int) -> None:
-        """DFS steps per work item before a lane donates the rest (0: never)."""
+        """DFS steps per work item of a row's main launch before a lane donates
+        the rest (default 4096; 0: never)."""
         _check(load().d4_mbatch_set_budget_steps(self._h, steps))
 
     def set_budget(self, max_states: int) -> None:

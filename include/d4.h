@@ -253,9 +253,9 @@ int d4_mbatch_set_launch(d4_mbatch *b, uint32_t threads, uint32_t refill);
  * with each workgroup size and refill threshold, keeps the fastest on b. */
 int d4_mbatch_tune(d4_mbatch *b, uint32_t sample, uint32_t *threads,
                    uint32_t *refill, double *best_ms);
-/* DFS steps a lane runs on one work item before it hands the rest to a
- * follow-up launch (0, the default: never), as d4_library_set_budget.
- * Experimental for multi-library batches: leave it at 0. */
+/* DFS steps a lane runs on one work item of a row's main launch before it
+ * hands the rest of its search tree to follow-up launches, as
+ * d4_library_set_budget (default 4096; 0: never). */
 int d4_mbatch_set_budget_steps(d4_mbatch *b, uint32_t steps);
 /* Device bytes the frontier, its dedup table and parked frontier parts may
  * take (default: 80% of the free device memory when the batch is created).
