@@ -171,6 +171,13 @@ static void build_fast(d4_library &L) {
     }
 }
 
+void d4::check_compacted(uint32_t found, uint64_t expected) {
+    if (found != (uint32_t)expected)
+        throw Error(D4_EDEVICE, "internal error: compaction found " + std::to_string(found) +
+                                    " states, the row step counted " +
+                                    std::to_string(expected));
+}
+
 template <class T>
 static void upload(T *&dst, const T *src, size_t n) {
     HIPCHK(hipMalloc((void **)&dst, std::max<size_t>(n, 1) * sizeof(T)));
@@ -719,8 +726,10 @@ struct Engine {
         if (en) slot_uid.reserve(tab_m);
         HIPCHK(hipEventRecord(evc0, st));
         blk.reserve(2 * COMPACT_MAX_BLOCKS);
-        launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, c32.p, blk.p, en ? slot_uid.p : nullptr,
-                       st);
+        launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, U, c32.p, blk.p,
+                       en ? slot_uid.p : nullptr, st);
+        uint32_t n_out = 0;
+        HIPCHK(hipMemcpyAsync(&n_out, c32.p + C32_OUT, 4, hipMemcpyDeviceToHost, st));
         uint32_t n_edges = en ? h32[C32_EDGES] : 0;
         if (en && n_edges) {
             e_child.reserve(n_edges);
@@ -729,6 +738,7 @@ struct Engine {
         HIPCHK(hipEventRecord(ev1, st));
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
+        check_compacted(n_out, U);
         float ms = 0, cms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         HIPCHK(hipEventElapsedTime(&cms, evc0, ev1));

@@ -187,13 +187,16 @@ void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
 constexpr uint32_t COMPACT_MAX_BLOCKS = 65536;
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
-                    uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
+                    uint64_t *out_key, uint64_t *out_cnt, uint64_t out_cap, uint32_t *c32,
                     uint32_t *blk, uint32_t *slot_uid, hipStream_t st);
 // multi-library batch: 4-word table slots, two-word keys
 void launch_fill_table4(unsigned long long *tab, uint64_t n, hipStream_t st);
 void launch_compact4(const unsigned long long *tab, uint64_t m, uint64_t *out_k0,
-                     uint64_t *out_k1, uint64_t *out_cnt, uint32_t *c32, uint32_t *blk,
-                     hipStream_t st);
+                     uint64_t *out_k1, uint64_t *out_cnt, uint64_t out_cap, uint32_t *c32,
+                     uint32_t *blk, hipStream_t st);
+// the number of states a compaction found (c32[C32_OUT]) must equal the
+// step's new-slot count; otherwise the table and the counts disagree
+void check_compacted(uint32_t found, uint64_t expected);
 void launch_extend_multi(const MultiArgs &a, bool prune, bool count_only, uint32_t threads,
                          int n_cu, hipStream_t st);
 void launch_inst_hist(const uint64_t *k0, uint64_t n, uint32_t n_inst, unsigned long long *hist,

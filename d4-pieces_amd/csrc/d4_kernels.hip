@@ -178,6 +178,43 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     uint64_t *st_desc = st_E + STAGE;     // enum mode: row descriptor
     uint64_t *st_rec = st_desc + STAGE;   // enum mode: parent record
     uint32_t sc = 0;  // staged entries (wave-uniform)
+    // Count mode pipelines the table insert: a flush issues the first probe
+    // (the CAS) of each kept child and leaves it in flight; the next flush
+    // settles it (merge or claim, further probes, spill), so the round trip
+    // overlaps the DFS steps in between instead of stalling the wave.
+    constexpr bool PIPE = !ENUM && !COUNT_ONLY;
+    bool p_on = false;
+    uint64_t p_key = 0, p_cnt = 0, p_h = 0, p_cur = 0;
+    // Settles this lane's pending insert: returns tally bit 2 (new) and sets
+    // `failed` when no slot was found within the probe limit and the spill
+    // list is full.
+    auto settle = [&](bool &failed) __attribute__((always_inline)) -> uint32_t {
+        if (!p_on) return 0u;
+        p_on = false;
+        int r = -1;
+        uint64_t h = p_h, cur = p_cur;
+        for (uint32_t pr = 1;; pr++) {
+            if (cur == EMPTY_KEY || cur == p_key) {
+                atomicAdd(a.tab_cnt + 2 * h, (unsigned long long)p_cnt);
+                r = cur == EMPTY_KEY ? 1 : 0;
+                break;
+            }
+            if (pr >= a.max_probe) break;
+            h = (h + 1) & a.tab_mask;
+            cur = atomicCAS(a.tab_key + 2 * h, (unsigned long long)EMPTY_KEY,
+                            (unsigned long long)p_key);
+        }
+        if (r < 0 && a.spill_cap) {
+            const uint32_t sp = atomicAdd(a.c32 + C32_SPILL, 1u);
+            if (sp < a.spill_cap) {
+                a.spill[2 * sp] = p_key;
+                a.spill[2 * sp + 1] = p_cnt;
+                r = 0;
+            }
+        }
+        if (r < 0) failed = true;
+        return r > 0 ? 4u : 0u;
+    };
     // Returns this lane's tallies: bit 0 child kept, bit 1 pruned, bit 2 new.
     auto flush = [&](uint32_t n) __attribute__((always_inline)) -> uint32_t {
         __builtin_amdgcn_wave_barrier();
@@ -198,9 +235,17 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
         }
         bool failed = false;
         uint64_t slot = 0;
+        if (PIPE) t |= settle(failed);
         if (keep) {
             t |= 1u;
-            if (!COUNT_ONLY) {
+            if (PIPE) {
+                p_h = mix64(k2) & a.tab_mask;
+                p_cur = atomicCAS(a.tab_key + 2 * p_h, (unsigned long long)EMPTY_KEY,
+                                  (unsigned long long)k2);
+                p_key = k2;
+                p_cnt = c2;
+                p_on = true;
+            } else if (!COUNT_ONLY) {
                 int r = table_insert(a, k2, c2, slot);
                 if (r < 0 && !ENUM && a.spill_cap) {
                     // no slot within the probe limit: park the child for the
@@ -523,6 +568,11 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
         }
     }
     while (sc) tally(flush(min(sc, 64u)));
+    if (PIPE) {
+        bool failed = false;
+        tally(settle(failed));
+        if (failed) atomicOr(a.c32 + C32_OVERFLOW, 1u);
+    }
 
     n_children = wave_sum_u64(n_children);
     n_new = wave_sum_u64(n_new);
@@ -668,12 +718,14 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t *blk_cnt, u
 }
 
 // SW = 2: {key, count} slots (tab_cnt = tab_key + 1); SW = 4: {k0, k1,
-// count, unused} slots (tab_cnt = tab_key + 2), k1 goes to out_k1.
+// count, unused} slots (tab_cnt = tab_key + 2), k1 goes to out_k1.  States
+// past out_cap (the host's unique count) are not written; the host compares
+// c32[C32_OUT] with that count.
 template <int SW>
 __global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
     const unsigned long long *tab_key, const unsigned long long *tab_cnt, uint64_t m,
     uint64_t per_block, const uint32_t *blk_off, uint64_t *out_key, uint64_t *out_k1,
-    uint64_t *out_cnt, uint32_t *slot_uid) {
+    uint64_t *out_cnt, uint32_t *slot_uid, uint64_t out_cap) {
     __shared__ uint32_t s_w[CMP_THREADS / 64];
     const uint64_t lo = blockIdx.x * per_block;
     const uint64_t hi = min(m, lo + per_block);
@@ -691,8 +743,8 @@ __global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
             before += w < wv ? s_w[w] : 0;
             total += s_w[w];
         }
-        if (valid) {
-            const uint32_t o = base + before + mbcnt(bm);
+        const uint32_t o = base + before + mbcnt(bm);
+        if (valid && o < out_cap) {
             out_key[o] = k;
             if (SW == 4) out_k1[o] = tab_key[SW * i + 1];
             out_cnt[o] = tab_cnt[SW * i];
@@ -945,7 +997,7 @@ void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_o
 
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
-                    uint64_t *out_key, uint64_t *out_cnt, uint32_t *c32,
+                    uint64_t *out_key, uint64_t *out_cnt, uint64_t out_cap, uint32_t *c32,
                     uint32_t *blk, uint32_t *slot_uid, hipStream_t st) {
     // blk holds 2 * COMPACT_MAX_BLOCKS words: counts, then offsets
     uint64_t per_block = 4096;
@@ -957,7 +1009,7 @@ void launch_compact(const unsigned long long *tab_key,
                        blk + COMPACT_MAX_BLOCKS, c32);
     hipLaunchKernelGGL(k_compact_write<2>, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, tab_cnt,
                        m, per_block, blk + COMPACT_MAX_BLOCKS, out_key, nullptr, out_cnt,
-                       slot_uid);
+                       slot_uid, out_cap);
 }
 
 void launch_fill_table4(unsigned long long *tab, uint64_t n, hipStream_t st) {
@@ -966,8 +1018,8 @@ void launch_fill_table4(unsigned long long *tab, uint64_t n, hipStream_t st) {
 }
 
 void launch_compact4(const unsigned long long *tab, uint64_t m, uint64_t *out_k0,
-                     uint64_t *out_k1, uint64_t *out_cnt, uint32_t *c32, uint32_t *blk,
-                     hipStream_t st) {
+                     uint64_t *out_k1, uint64_t *out_cnt, uint64_t out_cap, uint32_t *c32,
+                     uint32_t *blk, hipStream_t st) {
     uint64_t per_block = 2048;
     while ((m + per_block - 1) / per_block > COMPACT_MAX_BLOCKS) per_block *= 2;
     const uint32_t nb = (uint32_t)((m + per_block - 1) / per_block);
@@ -976,7 +1028,8 @@ void launch_compact4(const unsigned long long *tab, uint64_t m, uint64_t *out_k0
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blk, nb,
                        blk + COMPACT_MAX_BLOCKS, c32);
     hipLaunchKernelGGL(k_compact_write<4>, dim3(nb), dim3(CMP_THREADS), 0, st, tab, tab + 2, m,
-                       per_block, blk + COMPACT_MAX_BLOCKS, out_k0, out_k1, out_cnt, nullptr);
+                       per_block, blk + COMPACT_MAX_BLOCKS, out_k0, out_k1, out_cnt, nullptr,
+                       out_cap);
 }
 
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,

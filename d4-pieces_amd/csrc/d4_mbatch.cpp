@@ -425,10 +425,13 @@ struct d4_mbatch {
         }
         blk.reserve(2 * COMPACT_MAX_BLOCKS);
         HIPCHK(hipEventRecord(evc0, st));
-        launch_compact4(tab.p, tab_m, n0.p, n1.p, nc.p, c32.p, blk.p, st);
+        launch_compact4(tab.p, tab_m, n0.p, n1.p, nc.p, U, c32.p, blk.p, st);
+        uint32_t n_out = 0;
+        HIPCHK(hipMemcpyAsync(&n_out, c32.p + C32_OUT, 4, hipMemcpyDeviceToHost, st));
         HIPCHK(hipEventRecord(ev1, st));
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
+        check_compacted(n_out, U);
         float ms = 0, cms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         HIPCHK(hipEventElapsedTime(&cms, evc0, ev1));
