@@ -408,7 +408,7 @@ struct Engine {
         a.used_mask = used_mask;
         a.tab_key = tk.p;
         a.tab_cnt = tk.p + 1;  // interleaved {key, count} slots
-        a.tab_mask = tab_m - 1;
+        a.tab_size = tab_m;
         a.max_probe = 256;
         a.c32 = c32.p;
         a.c64 = c64.p;
@@ -613,11 +613,11 @@ struct Engine {
         // child's slot, keeps the padded size and reruns on overflow.
         uint64_t tab_m;
         if (en) {
-            tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
+            tab_m = round_slots(2 * est_u);
         } else {
             uint64_t t = est_u;
             if (!exact) t = std::min<uint64_t>(g_c, (uint64_t)(n_front * grow_unique_raw * 1.1) + 4096);
-            tab_m = next_pow2(std::max<uint64_t>((uint64_t)(1.4 * (double)t), 1024));
+            tab_m = round_slots((uint64_t)(1.4 * (double)t));
             spill.reserve(2 * kSpillCap);
         }
         uint64_t est_e = en ? est_c + 64 : 0;
@@ -669,8 +669,8 @@ struct Engine {
             if (!ovf && !en && h32[C32_SPILL]) {
                 // move the table into one large enough for it and the spill
                 const uint32_t ns = h32[C32_SPILL];
-                const uint64_t m2 = next_pow2(
-                    std::max<uint64_t>(2 * tab_m, 2 * ((uint64_t)h64[C64_NEW] + ns)));
+                const uint64_t m2 =
+                    round_slots(std::max<uint64_t>(2 * tab_m, 2 * ((uint64_t)h64[C64_NEW] + ns)));
                 bool room = !(mem_cap && 16ull * (tab_m + m2) + 16ull * n_front + parked_bytes >
                                              mem_cap);
                 if (room) {

@@ -95,6 +95,12 @@ inline uint64_t next_pow2(uint64_t v) {
     return p;
 }
 
+// Slots of a single-library dedup table: at least 1024, a multiple of 1024
+// (the kernels map hashes onto any slot count).
+inline uint64_t round_slots(uint64_t v) {
+    return std::max<uint64_t>((v + 1023) & ~1023ull, 1024);
+}
+
 // Caching device allocator: freed blocks stay with the process and are
 // reused by later searches of similar size (hipMalloc / hipFree of multi-GB
 // tables per row would otherwise dominate short searches).  Blocks are

@@ -72,7 +72,7 @@ struct ExtendArgs {
     // dedup table
     unsigned long long *tab_key;
     unsigned long long *tab_cnt;
-    uint64_t tab_mask;
+    uint64_t tab_size;       // slots (any number >= 1)
     uint32_t max_probe;
     // counters
     uint32_t *c32;

@@ -43,7 +43,15 @@ constexpr int CMP_THREADS = 256;
 
 // The dedup table interleaves each slot's key and count (16 B per slot:
 // tab_key[2h], tab_cnt[2h] with tab_cnt = tab_key + 1), so an insert touches
-// one cache line.
+// one cache line.  A table has any number m of slots (the host sizes it to
+// its estimate, not to a power of two): a key's home slot is the high word
+// of mix64(key) * m, and probes step linearly, wrapping at m.
+__device__ __forceinline__ uint64_t home_slot(uint64_t key, uint64_t m) {
+    return __umul64hi(mix64(key), m);
+}
+__device__ __forceinline__ uint64_t next_slot(uint64_t h, uint64_t m) {
+    return h + 1 < m ? h + 1 : 0;
+}
 //
 // Insert (key, cnt) into the table: returns 1 when the key is new, 0 when it
 // merged into an existing slot, -1 when the probe limit was reached.  Keys
@@ -52,7 +60,7 @@ constexpr int CMP_THREADS = 256;
 // trip per probe; the count update does not wait for its result.
 __device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
                                             uint64_t cnt, uint64_t &slot) {
-    uint64_t h = mix64(key) & a.tab_mask;
+    uint64_t h = home_slot(key, a.tab_size);
     for (uint32_t p = 0; p < a.max_probe; p++) {
         uint64_t cur = atomicCAS(a.tab_key + 2 * h, (unsigned long long)EMPTY_KEY,
                                  (unsigned long long)key);
@@ -61,7 +69,7 @@ __device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
             slot = h;
             return cur == EMPTY_KEY ? 1 : 0;
         }
-        h = (h + 1) & a.tab_mask;
+        h = next_slot(h, a.tab_size);
     }
     return -1;
 }
@@ -200,7 +208,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
                 break;
             }
             if (pr >= a.max_probe) break;
-            h = (h + 1) & a.tab_mask;
+            h = next_slot(h, a.tab_size);
             cur = atomicCAS(a.tab_key + 2 * h, (unsigned long long)EMPTY_KEY,
                             (unsigned long long)p_key);
         }
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
         if (keep) {
             t |= 1u;
             if (PIPE) {
-                p_h = mix64(k2) & a.tab_mask;
+                p_h = home_slot(k2, a.tab_size);
                 p_cur = atomicCAS(a.tab_key + 2 * p_h, (unsigned long long)EMPTY_KEY,
                                   (unsigned long long)k2);
                 p_key = k2;
@@ -613,11 +621,11 @@ __global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
 
 __global__ void k_reset_work(unsigned long long *c64) { c64[C64_WORK] = 0; }
 
-// Insert (key, cnt) into a {key, count} table of any power-of-two size, as
-// table_insert does.
-__device__ __forceinline__ int insert_at(unsigned long long *tab, uint64_t mask, uint64_t key,
+// Insert (key, cnt) into a {key, count} table of m slots, as table_insert
+// does.
+__device__ __forceinline__ int insert_at(unsigned long long *tab, uint64_t m, uint64_t key,
                                          uint64_t cnt) {
-    uint64_t h = mix64(key) & mask;
+    uint64_t h = home_slot(key, m);
     for (uint32_t p = 0; p < 256; p++) {
         const uint64_t cur = atomicCAS(tab + 2 * h, (unsigned long long)EMPTY_KEY,
                                        (unsigned long long)key);
@@ -625,7 +633,7 @@ __device__ __forceinline__ int insert_at(unsigned long long *tab, uint64_t mask,
             atomicAdd(tab + 2 * h + 1, (unsigned long long)cnt);
             return cur == EMPTY_KEY ? 1 : 0;
         }
-        h = (h + 1) & mask;
+        h = next_slot(h, m);
     }
     return -1;
 }
@@ -637,7 +645,7 @@ __global__ __launch_bounds__(256) void k_rehash_table(const unsigned long long *
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t k = old_tab[2 * i];
         if (k == EMPTY_KEY) continue;
-        if (insert_at(new_tab, m_new - 1, k, old_tab[2 * i + 1]) < 0)
+        if (insert_at(new_tab, m_new, k, old_tab[2 * i + 1]) < 0)
             atomicOr(c32 + C32_OVERFLOW, 1u);
     }
 }
@@ -647,7 +655,7 @@ __global__ __launch_bounds__(256) void k_insert_spill(const uint64_t *spill, uin
                                                       uint32_t *c32, unsigned long long *c64) {
     uint64_t fresh = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int r = insert_at(tab, m - 1, spill[2 * i], spill[2 * i + 1]);
+        const int r = insert_at(tab, m, spill[2 * i], spill[2 * i + 1]);
         if (r < 0) atomicOr(c32 + C32_OVERFLOW, 1u);
         else fresh += (uint64_t)r;
     }
