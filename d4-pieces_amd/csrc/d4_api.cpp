@@ -8,12 +8,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "d4_host.h"
@@ -1197,14 +1199,39 @@ struct d4_server {
         }
     }
 
+    // Every in-flight request advances one row.  The count engine (all
+    // solutions) and the enum engine (first solution) step at the same time,
+    // each from its own host thread on its own stream, so a step lasts as
+    // long as the slower of the two rather than their sum.
     void step() {
         admit();
+        bool stepped[2] = {false, false};
+        std::exception_ptr err[2];
+        std::map<uint32_t, std::vector<std::vector<uint8_t>>> firsts;
+        auto advance = [&](int k) {
+            try {
+                HIPCHK(hipSetDevice(lib->device));
+                Engine &e = *eng[k];
+                e.step();
+                if (k == 1 && !e.completed.empty()) firsts = e.first_solutions(e.completed);
+            } catch (...) {
+                err[k] = std::current_exception();
+            }
+        };
+        for (int k = 0; k < 2; k++) stepped[k] = eng[k]->n_active != 0;
+        if (stepped[0] && stepped[1]) {
+            std::thread t(advance, 0);
+            advance(1);
+            t.join();
+        } else {
+            for (int k = 0; k < 2; k++)
+                if (stepped[k]) advance(k);
+        }
+        // requests the other engine finished are delivered before an error
+        // of one engine is raised
         for (int k = 0; k < 2; k++) {
+            if (!stepped[k] || err[k]) continue;
             Engine &e = *eng[k];
-            if (!e.n_active) continue;
-            e.step();
-            std::map<uint32_t, std::vector<std::vector<uint8_t>>> firsts;
-            if (k == 1 && !e.completed.empty()) firsts = e.first_solutions(e.completed);
             for (uint32_t slot : e.completed) {
                 Done d{slot_req[k][slot], e.host_result[slot], {}};
                 if (k == 1)
@@ -1214,6 +1241,8 @@ struct d4_server {
             }
             e.trim_logs();
         }
+        for (int k = 0; k < 2; k++)
+            if (err[k]) std::rethrow_exception(err[k]);
     }
 };
 struct d4_batch {
