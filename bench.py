@@ -123,6 +123,7 @@ def serve_section(d4, lib, workloads, rates, n_req, slots):
         else:
             s.count()
     plain_rps = ns / (time.perf_counter() - t)
+    s = None
     out["plain_requests_per_s"] = plain_rps
     for rate in rates:
         reqs = workloads.request_stream(n_req, rate=rate, seed=4)
@@ -300,6 +301,10 @@ def main():
         counts = c
     sync()
     dt = time.perf_counter() - t0
+    # each engine holds a HIP stream, and streams share the process's few
+    # hardware queues: release a section's engines before the next section
+    # (the server's two engines only overlap on queues of their own)
+    b = None
 
     rec = sum(s["records_extended"] for s in stats)
     n_sol = sum(counts) * args.steps
@@ -385,6 +390,7 @@ def main():
             psol += c
             log(f"plain board {i} {lat[-1]:.1f} ms")
         pdt = time.perf_counter() - t1
+        s = None
         out["plain"] = {
             "boards": n,
             "records_per_s": prec / pdt,
