@@ -235,7 +235,7 @@ def main():
                     help="run the measured autotuner instead of --launch")
     ap.add_argument("--no-tune", action="store_true",
                     help="library defaults, no --launch; no autotune of the uneven batch")
-    ap.add_argument("--launch", default="3,256,16,1024",
+    ap.add_argument("--launch", default="3,512,16,1536",
                     help="fit_rows,threads,refill,budget of the extend kernel (the "
                          "autotuner's choice for this workload on MI355X)")
     ap.add_argument("--tune-sample", type=int, default=256)
@@ -243,6 +243,9 @@ def main():
                     help="comma-separated arrival rates (requests/s); empty to skip")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-slots", type=int, default=1024)
+    ap.add_argument("--serve-launch", default="3,256,16,1024",
+                    help="fit_rows,threads,refill,budget of the server's extend kernel "
+                         "(measured best for the request stream on MI355X)")
     ap.add_argument("--uneven", type=int, default=4096,
                     help="uneven puzzles in the multi-library section (0: skip)")
     ap.add_argument("--uneven-budget", type=int, default=1 << 20)
@@ -404,8 +407,16 @@ def main():
 
     if args.serve_rates:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
-        out["served"] = serve_section(d4, lib, workloads, rates, args.serve_requests,
+        # the server's frontiers are small (a few requests in flight): it gets
+        # a library with its own measured launch shape
+        slib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
+        f, t, r, bud = (int(x) for x in args.serve_launch.split(","))
+        slib.set_launch(f, t, r)
+        slib.set_budget(bud)
+        out["served"] = serve_section(d4, slib, workloads, rates, args.serve_requests,
                                       args.serve_slots)
+        out["served"]["launch"] = dict(zip(("fit_rows", "threads", "refill", "budget"),
+                                           (f, t, r, bud)))
         best = max(out["served"]["rates"], key=lambda r: r["requests_per_s"])
         out["latency_ms"]["served_p50"] = best["latency_ms_p50"]
         out["latency_ms"]["served_p99"] = best["latency_ms_p99"]
