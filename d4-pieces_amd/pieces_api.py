@@ -1,6 +1,6 @@
 """The reference's embind surface (src/wasm.cpp:10-146, the `pieces` module
-its web UI calls) over this build's device search: the same names, fields and
-argument meanings, with solve8 answered by the GPU searcher instead of the
+its web UI calls) over the device search of d4pieces: the same names, fields
+and argument meanings, with solve8 answered by the GPU searcher instead of the
 CPU backtracker.
 
     Shape8(value)          value, LEN, size, normalize(), canonical_form(forms),
