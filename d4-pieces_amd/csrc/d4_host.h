@@ -161,7 +161,7 @@ class Pool {
         free_.emplace(round(bytes), std::make_pair(dev, p));
     }
     // Bytes cached for reuse on a device: free for the library's purposes,
-    // though the driver counts them as in use.
+    // though hipMemGetInfo counts them as in use.
     size_t cached(int dev) {
         std::lock_guard<std::mutex> lk(mu_);
         size_t n = 0;
@@ -176,7 +176,7 @@ inline Pool &pool() {
     return *p;
 }
 
-// Device memory a new search may plan with: the driver's free bytes plus
+// Device memory a new search may plan with: hipMemGetInfo's free bytes plus
 // what the allocator holds cached on the current device.
 inline uint64_t available_device_bytes() {
     size_t free_b = 0, total_b = 0;
