@@ -276,6 +276,7 @@ struct Engine {
     DevBuf<uint64_t> don_a, don_b;  // donated work items (two words each)
     uint64_t n_front = 0, new_roots = 0;
     double grow_unique = 4.0, grow_children = 8.0, grow_unique_raw = 4.0;
+    double prev_unique_raw = 0.0;  // the step before's unique ratio (0: none yet)
     DevBuf<unsigned long long> tk2;  // the larger table a spill moves into
     DevBuf<uint64_t> spill;          // count mode: children that found no slot
     static constexpr uint64_t kSpillCap = 1u << 22;
@@ -285,6 +286,7 @@ struct Engine {
         uint32_t g = 0;
         uint64_t n = 0;
         double grow_unique = 1.0, grow_children = 1.0, grow_unique_raw = 1.0;
+        double prev_unique_raw = 0.0;
         DevBuf<uint64_t> k, c;
     };
     std::vector<std::unique_ptr<Part>> parked;
@@ -627,11 +629,31 @@ struct Engine {
         result_save.reserve(S);
         uint32_t h32[C32_N];
         unsigned long long h64[C64_N];
+        bool trend = false;
         for (;;) {
             if (bounded && (tab_m > (1ull << 34) ||
                             16ull * tab_m + 16ull * std::min(est_u, tab_m) + 16ull * n_front +
-                                    parked_bytes > mem_cap))
+                                    parked_bytes > mem_cap)) {
+                // Growth slows from row to row, and a split frontier cannot
+                // merge its parts' equal states (each part of a split at the
+                // peak rows of 8x15 pentominoes x2 rebuilt nearly the whole
+                // next frontier).  Before splitting, size the step from the
+                // unique ratio extrapolated from the last two (their quotient
+                // applied once more); if the children outgrow it, the spill
+                // and table growth or a rerun catch it and the split follows.
+                if (!exact && !trend && prev_unique_raw > grow_unique_raw) {
+                    trend = true;
+                    const double r = grow_unique_raw * grow_unique_raw / prev_unique_raw;
+                    const uint64_t m2 =
+                        round_slots((uint64_t)(1.4 * ((double)n_front * r * 1.1 + 4096)));
+                    if (m2 < tab_m) {
+                        tab_m = m2;
+                        est_u = std::min<uint64_t>(est_u, (uint64_t)(n_front * r * 1.5) + 4096);
+                        continue;
+                    }
+                }
                 return false;
+            }
             if (tab_m > (1ull << 34)) throw Error(D4_ERANGE, "dedup table exceeds 2^34 slots");
             try {
                 tk.reserve(2 * tab_m);
@@ -692,6 +714,9 @@ struct Engine {
                     ovf = h32[C32_OVERFLOW] != 0;
                     if (!ovf) {
                         tk.swap(tk2);
+                        // the old table goes back to the allocator, where
+                        // the next frontier can take its memory
+                        tk2.release();
                         tab_m = m2;
                         stats.rehashes++;
                     }
@@ -721,6 +746,7 @@ struct Engine {
         stats.records_extended += n_front;
         stats.unique += U;
         if (n_front) {
+            prev_unique_raw = stats.rows_done ? grow_unique_raw : 0.0;
             grow_unique = std::max(1.0, (double)U / (double)n_front);
             grow_unique_raw = std::max(0.01, (double)U / (double)n_front);
             grow_children = std::max(1.0, (double)h64[C64_CHILDREN] / (double)n_front);
@@ -775,12 +801,19 @@ struct Engine {
 
     // Park the upper half of the frontier, to resume at this step later.
     void split() {
+        // the failed step's table and next-frontier buffers hold no state:
+        // return them to the allocator before the parked half is allocated
+        tk.release();
+        tk2.release();
+        nk.release();
+        nc.release();
         const uint64_t keep = n_front / 2, m = n_front - keep;
         auto p = std::make_unique<Part>();
         p->g = g;
         p->n = m;
         p->grow_unique = grow_unique;
         p->grow_unique_raw = grow_unique_raw;
+        p->prev_unique_raw = prev_unique_raw;
         p->grow_children = grow_children;
         p->k.reserve(m);
         p->c.reserve(m);
@@ -804,6 +837,7 @@ struct Engine {
         g = p->g;
         grow_unique = p->grow_unique;
         grow_unique_raw = p->grow_unique_raw;
+        prev_unique_raw = p->prev_unique_raw;
         grow_children = p->grow_children;
         new_roots = 0;
     }
