@@ -134,11 +134,15 @@ struct d4_mbatch {
     uint32_t budget_steps = 4096;
     uint64_t n_front = 0;
     double grow_unique = 1.0, grow_children = 1.0;
+    // unique children per record of the last step and of the step before
+    // (0: not measured yet)
+    double unique_raw = 0.0, prev_unique_raw = 0.0;
     // bounded memory: frontier halves parked for later (depth first)
     struct Part {
         uint32_t row = 0;
         uint64_t n = 0;
         double grow_unique = 1.0, grow_children = 1.0;
+        double unique_raw = 0.0, prev_unique_raw = 0.0;
         DevBuf<uint64_t> k0, k1, c;
     };
     std::vector<std::unique_ptr<Part>> parked;
@@ -191,6 +195,7 @@ struct d4_mbatch {
         HIPCHK(hipStreamSynchronize(st));
         n_front = N;
         grow_unique = grow_children = 1.0;
+        unique_raw = prev_unique_raw = 0.0;
     }
 
     MultiArgs args(uint32_t row, uint64_t tab_m) const {
@@ -291,12 +296,20 @@ struct d4_mbatch {
 
     // Park the upper half of the frontier (resumed at this row later).
     void split(uint32_t row) {
+        // the failed step's table and next-frontier buffers hold no state:
+        // return them to the allocator before the parked half is allocated
+        tab.release();
+        n0.release();
+        n1.release();
+        nc.release();
         const uint64_t keep = n_front / 2, m = n_front - keep;
         auto p = std::make_unique<Part>();
         p->row = row;
         p->n = m;
         p->grow_unique = grow_unique;
         p->grow_children = grow_children;
+        p->unique_raw = unique_raw;
+        p->prev_unique_raw = prev_unique_raw;
         p->k0.reserve(m);
         p->k1.reserve(m);
         p->c.reserve(m);
@@ -324,6 +337,8 @@ struct d4_mbatch {
         n_front = p->n;
         grow_unique = p->grow_unique;
         grow_children = p->grow_children;
+        unique_raw = p->unique_raw;
+        prev_unique_raw = p->prev_unique_raw;
         return p->row;
     }
 
@@ -353,9 +368,27 @@ struct d4_mbatch {
         uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
         uint32_t h32[C32_N];
         unsigned long long h64[C64_N];
+        bool trend = false;
         for (;;) {
-            if (n_front >= 2 && (tab_m > (1ull << 33) || need(tab_m, std::min(est_u, tab_m)) > mem_cap))
+            if (n_front >= 2 && (tab_m > (1ull << 33) || need(tab_m, std::min(est_u, tab_m)) > mem_cap)) {
+                // A split part cannot merge its equal states with the other
+                // part's, so before splitting the step is sized once more
+                // from the unique ratio extrapolated from the last two steps
+                // (growth slows from row to row); if the table overflows, the
+                // rerun at 4x the size does not fit and the split follows.
+                if (!trend && n_front > kExactCountMax && prev_unique_raw > unique_raw) {
+                    trend = true;
+                    const double r = unique_raw * unique_raw / prev_unique_raw;
+                    const uint64_t u2 = (uint64_t)((double)n_front * r * 1.5) + 4096;
+                    const uint64_t m2 = next_pow2(std::max<uint64_t>(2 * u2, 1024));
+                    if (m2 < tab_m) {
+                        tab_m = m2;
+                        est_u = std::min(est_u, u2);
+                        continue;
+                    }
+                }
                 return false;
+            }
             if (tab_m > (1ull << 33)) throw Error(D4_ERANGE, "dedup table exceeds 2^33 slots");
             try {
                 tab.reserve(4 * tab_m);
@@ -422,6 +455,8 @@ struct d4_mbatch {
         if (n_front) {
             grow_unique = std::max(1.0, (double)U / (double)n_front);
             grow_children = std::max(1.0, (double)h64[C64_CHILDREN] / (double)n_front);
+            prev_unique_raw = unique_raw;
+            unique_raw = std::max(0.01, (double)U / (double)n_front);
         }
         blk.reserve(2 * COMPACT_MAX_BLOCKS);
         HIPCHK(hipEventRecord(evc0, st));
