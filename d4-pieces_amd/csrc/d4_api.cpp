@@ -594,8 +594,11 @@ struct Engine {
                 est_c = est_u = bound;
                 exact = true;
                 if (stats.rows_done >= 2 && g_u < bound) est_u = g_u;
-            } else if (n_front <= kExactCountMax && en) {
-                // enum mode needs every child's slot: count them exactly
+            } else if (n_front <= kExactCountMax && en && (stats.rows_done < 2 || new_roots)) {
+                // enum mode needs every child's slot: the first rows and new
+                // roots (thousands of children per record) are counted
+                // exactly; later rows take the growth estimate below, and a
+                // table or edge buffer that overflows reruns the step
                 est_c = est_u = count_children();
                 stats.count_passes++;
                 exact = true;
