@@ -239,6 +239,13 @@ def main():
                     help="fit_rows,threads,refill,budget of the extend kernel (the "
                          "autotuner's choice for this workload on MI355X)")
     ap.add_argument("--tune-sample", type=int, default=256)
+    ap.add_argument("--kernel", choices=("level", "expand", "dfs"), default="level",
+                    help="count-mode row-step kernel: level-synchronous (level), partial-row "
+                         "snapshots on a per-wave stack (expand) or one depth-first search "
+                         "per lane (dfs)")
+    ap.add_argument("--tail-rounds", type=int, default=256,
+                    help="expand kernel: rounds after a launch's records run out before "
+                         "a wave's partial rows move to a follow-up launch")
     ap.add_argument("--serve-rates", default="100,20000",
                     help="comma-separated arrival rates (requests/s); empty to skip")
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -270,6 +277,9 @@ def main():
     boards = workloads.seeded_8x8(args.instances, seed=2)
     lib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
     tuned = None
+    kernel = {"level": d4.KERNEL_LEVEL, "expand": d4.KERNEL_EXPAND,
+              "dfs": d4.KERNEL_DFS}[args.kernel]
+    lib.set_kernel(kernel, args.tail_rounds)
     if args.launch and not args.tune and not args.no_tune:
         f, t, r, b = (int(x) for x in args.launch.split(","))
         lib.set_launch(f, t, r)

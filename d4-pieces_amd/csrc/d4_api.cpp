@@ -259,9 +259,14 @@ struct Engine {
     // non-blocking, so a copy on the null stream would not be ordered with
     // the engine's kernels (a count restore could land after the rerun it
     // prepares).  The host waits for it, so host buffers may be reused.
+    // Waits for the engine's stream (counted: host round trips per row step).
+    void sync_host() const {
+        HIPCHK(hipStreamSynchronize(st));
+        stats.host_syncs++;
+    }
     void copy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) const {
         HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
-        HIPCHK(hipStreamSynchronize(st));
+        sync_host();
     }
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evx0 = nullptr, evx1 = nullptr, evc0 = nullptr;
     DevBuf<uint64_t> bw;
@@ -274,6 +279,14 @@ struct Engine {
     DevBuf<uint32_t> e_parent, e_child, slot_uid, blk;
     DevBuf<uint64_t> e_desc, e_slot;
     DevBuf<uint64_t> don_a, don_b;  // donated work items (two words each)
+    // k_expand's spilled partial-row states (EXPAND_WORDS words each), items per buffer
+    uint64_t xp_cap = 1u << 20;
+    // level kernels: partial-row buffers (two, alternating), the children
+    // buffer, and per-parent sizes seen on the last row step
+    DevBuf<uint64_t> lv_a, lv_b, ch_k, ch_c;
+    double lv_ratio = 0.0, ch_ratio = 0.0;  // max level / children written per parent
+    uint64_t lv_cap = 0, ch_cap = 0;  // last sizes used
+    std::vector<hipEvent_t> lv_ev;  // chunk boundaries: levels / dedup timing
     uint64_t n_front = 0, new_roots = 0;
     double grow_unique = 4.0, grow_children = 8.0, grow_unique_raw = 4.0;
     double prev_unique_raw = 0.0;  // the step before's unique ratio (0: none yet)
@@ -296,7 +309,7 @@ struct Engine {
     // children per record allowed for when a first row is sized without a
     // count (count mode; more spill into a grown table)
     static constexpr uint64_t kFirstRowFanout = 8192;
-    d4_stats stats{};
+    mutable d4_stats stats{};  // mutable: const reads of device logs count their waits
     std::deque<DevLog> logs;  // logs[j] = global step log0 + j
     uint32_t log0 = 0;
 
@@ -345,6 +358,7 @@ struct Engine {
         (void)hipStreamSynchronize(st);
         for (hipEvent_t e : {ev0, ev1, evx0, evx1, evc0})
             if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : lv_ev) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -440,6 +454,10 @@ struct Engine {
     // launch after launch, until none are left (after a few rounds the
     // budget is lifted so the chain ends).
     void run_donations(uint64_t tab_m, uint32_t e_cap, bool en, bool prune, bool count_only) {
+        if (expand_kernel(en, count_only)) {
+            run_expansions(tab_m, prune);
+            return;
+        }
         if (!lib->ext_budget) return;
         // Rounds are queued a few at a time without a host round trip: each
         // reads its item count from the counter pair the round before it
@@ -454,7 +472,7 @@ struct Engine {
         for (uint32_t round = 0;;) {
             uint32_t h[C32_N];
             HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
+            sync_host();
             if (h[C32_BAD_ITEM]) throw Error(D4_EDEVICE, "internal error: bad donated work item");
             if (h[C32_OVERFLOW]) return;  // the step is rerun
             const uint32_t cap = (uint32_t)(don_a.cap / 2);
@@ -473,6 +491,237 @@ struct Engine {
                 a.don_cap = (uint32_t)(don_b.cap / 2);
                 a.budget = round < kBudgetedRounds ? kFollowBudget : 0;
                 launch_extend(a, en, prune, count_only, lib->ext_threads, lib->n_cu, st);
+                HIPCHK(hipGetLastError());
+                don_a.swap(don_b);
+                rd = wr;
+            }
+        }
+    }
+
+    // Count-mode row steps run k_expand unless the library selects the
+    // depth-first kernel.
+    bool expand_kernel(bool en, bool count_only) const {
+        return !en && !count_only && lib->xp_kernel == D4_KERNEL_EXPAND;
+    }
+    bool level_kernel(bool en, bool count_only) const {
+        return !en && !count_only && lib->xp_kernel == D4_KERNEL_LEVEL &&
+               level_lds_bytes(lib->fit_rows) <= lib->lds_limit;
+    }
+
+    // ---- level kernels (count mode) -------------------------------------
+    // Buffer budget: a quarter of the memory cap (16 GiB without a cap, at
+    // most that).  A chunk of parents is sized from the per-parent partial
+    // rows (widest level) and children of the last row step that had a
+    // large frontier (a tiny frontier's first rows fan out by thousands per
+    // parent and would understate the chunk afterwards); an overflow reruns
+    // only the chunk that overflowed, with four times the estimate.
+    uint64_t level_budget() const {
+        const uint64_t cap = mem_cap ? mem_cap / 4 : 16ull << 30;
+        return std::max<uint64_t>(std::min<uint64_t>(cap, 16ull << 30), 1ull << 20);
+    }
+    // per-parent estimates at the start of a step
+    double lv_pp0() const { return std::max(lv_ratio, 2.0); }
+    double ch_pp0() const { return std::max(ch_ratio, 4.0); }
+    uint64_t level_min_cap() const {
+        return std::min<uint64_t>(std::max<uint64_t>(level_budget() / 1024, 4096), 1u << 20);
+    }
+    // parents per chunk: the buffers of a chunk fit in the budget
+    uint64_t level_chunk(uint64_t n, double lv_pp, double ch_pp) const {
+        const double pp = 8.0 * 1.5 * (2.0 * LVL_FIELDS * lv_pp + 2.0 * ch_pp);
+        const uint64_t c = (uint64_t)((double)level_budget() / pp);
+        return std::min<uint64_t>(std::max<uint64_t>(c, 1), n);
+    }
+    // capacities are powers of two, so that searches of similar size find
+    // their buffers in the allocator's cache instead of allocating anew
+    uint64_t level_cap(uint64_t c, double pp) const {
+        return next_pow2(std::max<uint64_t>((uint64_t)((double)c * pp * 1.5), level_min_cap()));
+    }
+    // Device bytes the level buffers take for a row step of n parents.
+    uint64_t level_bytes(uint64_t n) const {
+        if (!n) return 0;
+        const uint64_t c = level_chunk(n, lv_pp0(), ch_pp0());
+        return 8ull * (2ull * LVL_FIELDS * level_cap(c, lv_pp0()) + 2ull * level_cap(c, ch_pp0()));
+    }
+
+    // Level-synchronous extend + dedup of the frontier (count mode).  Per
+    // chunk of parents: one k_level launch per level (each reads its item
+    // count on the device), then k_dedup merges the chunk's children into
+    // the table; all chunks are queued with no host round trip.  A chunk
+    // whose buffers overflow is rerun (chunks before it are complete; the
+    // dedup of each completed chunk counts itself in c32[C32_LVDONE]).
+    // Terminal records (instances that finish at this step) are counted by
+    // one pass of their own, so a rerun chunk adds nothing twice.
+    void run_levels(uint64_t tab_m, bool prune) {
+        spill.reserve(2 * kSpillCap);
+        bool any_term = false;
+        for (uint32_t i = 0; i < S; i++)
+            if (active[i] && start[i] + heights[i] == g) any_term = true;
+        if (any_term)
+            launch_terminal(fk.p, fc.p, n_front, g, dstart.p, hgt.p, inst_shift, result.p, lib->n_cu, st);
+        LevelArgs a{};
+        a.fit = lib->d_fit;
+        a.placeinfo = lib->d_placeinfo;
+        a.xavail = lib->d_xavail;
+        a.fit_rows = lib->fit_rows;
+        a.bw = bw.p;
+        a.bw_stride = bw_stride;
+        a.hgt = hgt.p;
+        a.start = dstart.p;
+        a.result = result.p;
+        a.row = g;
+        a.occ_bits = occ_bits;
+        a.inst_shift = inst_shift;
+        a.used_mask = used_mask;
+        a.occ_mask = occ_mask;
+        a.c32 = c32.p;
+        a.c64 = c64.p;
+        DedupArgs d{};
+        d.bw = bw.p;
+        d.bw_stride = bw_stride;
+        d.start = dstart.p;
+        d.row = g;
+        d.occ_bits = occ_bits;
+        d.inst_shift = inst_shift;
+        d.occ_mask = occ_mask;
+        d.seed_mask = lib->maxh >= 2 ? ~((1ull << (8 * (lib->maxh - 1))) - 1ull) : ~0ull;
+        d.gcd = lib->gcd;
+        d.min_size = lib->min_size;
+        d.tab = tk.p;
+        d.tab_size = tab_m;
+        d.max_probe = 256;
+        d.spill = spill.p;
+        d.spill_cap = (uint32_t)std::min<uint64_t>(spill.cap / 2, kSpillCap);
+        d.c32 = c32.p;
+        d.c64 = c64.p;
+        uint64_t b0 = 0;
+        uint32_t reruns = 0;
+        uint64_t mx_lv = 0, mx_ch = 0, mx_chunk = 0;
+        double lv_pp = lv_pp0(), ch_pp = ch_pp0();
+        for (;;) {
+            const uint64_t chunk = level_chunk(n_front - b0, lv_pp, ch_pp);
+            lv_cap = level_cap(chunk, lv_pp);
+            ch_cap = level_cap(chunk, ch_pp);
+            lv_a.reserve(LVL_FIELDS * lv_cap);
+            lv_b.reserve(LVL_FIELDS * lv_cap);
+            ch_k.reserve(ch_cap);
+            ch_c.reserve(ch_cap);
+            a.ch_key = ch_k.p;
+            a.ch_cnt = ch_c.p;
+            d.ch_key = ch_k.p;
+            d.ch_cnt = ch_c.p;
+            a.ch_cap = d.ch_cap = ch_cap;
+            a.in_cap = a.out_cap = lv_cap;
+            const uint32_t nch = (uint32_t)((n_front - b0 + chunk - 1) / chunk);
+            while (lv_ev.size() < 2 * (size_t)nch + 1) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                lv_ev.push_back(e);
+            }
+            launch_reset_levels(c32.p, c64.p, true, st);
+            HIPCHK(hipEventRecord(lv_ev[0], st));
+            for (uint32_t j = 0; j < nch; j++) {
+                const uint64_t b = b0 + (uint64_t)j * chunk;
+                if (j) launch_reset_levels(c32.p, c64.p, false, st);
+                a.in_key = fk.p + b;
+                a.in_cnt = fc.p + b;
+                a.n_rec = (uint32_t)std::min<uint64_t>(chunk, n_front - b);
+                for (uint32_t L = 0; L < LVL_MAX; L++) {
+                    a.level = L;
+                    a.in_st = (L & 1) ? lv_a.p : lv_b.p;
+                    a.out_st = (L & 1) ? lv_b.p : lv_a.p;
+                    launch_level(a, lib->n_cu, st);
+                }
+                HIPCHK(hipEventRecord(lv_ev[2 * j + 1], st));
+                launch_dedup(d, prune, lib->n_cu, st);
+                HIPCHK(hipEventRecord(lv_ev[2 * j + 2], st));
+            }
+            HIPCHK(hipGetLastError());
+            uint32_t h32[C32_N];
+            unsigned long long h64[C64_N];
+            HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
+            sync_host();
+            const uint32_t done = std::min<uint32_t>(h32[C32_LVDONE], nch);
+            for (uint32_t j = 0; j < nch; j++) {
+                float lms = 0, dms = 0;
+                HIPCHK(hipEventElapsedTime(&lms, j ? lv_ev[2 * j] : lv_ev[0], lv_ev[2 * j + 1]));
+                HIPCHK(hipEventElapsedTime(&dms, lv_ev[2 * j + 1], lv_ev[2 * j + 2]));
+                stats.extend_ms += lms;
+                stats.dedup_ms += dms;
+                stats.dedup_launches++;
+                stats.extend_launches++;
+            }
+            if (h32[C32_OVERFLOW]) return;  // the table: the row step reruns larger
+            // sizes of the last completed chunk (its counters are still set)
+            if (done && !h32[C32_XOVF]) {
+                uint64_t m = 0;
+                for (uint32_t L = 0; L < LVL_MAX; L++) m = std::max<uint64_t>(m, h64[C64_LVL + L]);
+                const uint64_t last = std::min<uint64_t>(chunk, n_front - (b0 + (uint64_t)(nch - 1) * chunk));
+                if (last >= mx_chunk) {
+                    mx_chunk = last;
+                    mx_lv = m;
+                    mx_ch = h64[C64_CHW];
+                }
+            }
+            if (trace)
+                fprintf(stderr,
+                        "levels step %u: parents %llu from %llu, chunk %llu, caps %llu / %llu, "
+                        "done %u of %u, overflow %u, children %llu, level rows %llu %llu %llu %llu "
+                        "%llu %llu %llu %llu\n",
+                        g, (unsigned long long)n_front, (unsigned long long)b0,
+                        (unsigned long long)chunk, (unsigned long long)lv_cap,
+                        (unsigned long long)ch_cap, done, nch, h32[C32_XOVF], h64[C64_CHW],
+                        h64[C64_LVL], h64[C64_LVL + 1], h64[C64_LVL + 2], h64[C64_LVL + 3],
+                        h64[C64_LVL + 4], h64[C64_LVL + 5], h64[C64_LVL + 6], h64[C64_LVL + 7]);
+            if (!h32[C32_XOVF]) break;
+            // rerun from the chunk that overflowed, with four times the room
+            // per parent for the stream that ran out (a counter that passed
+            // its capacity)
+            stats.chunk_reruns++;
+            b0 += (uint64_t)done * chunk;
+            const uint64_t c_fail = std::min<uint64_t>(chunk, n_front - b0);
+            bool lv_over = false;
+            for (uint32_t L = 0; L < LVL_MAX; L++) lv_over |= h64[C64_LVL + L] > lv_cap;
+            if (lv_over) lv_pp = 4.0 * std::max(lv_pp, (double)lv_cap / (double)c_fail);
+            if (h64[C64_CHW] > ch_cap || !lv_over)
+                ch_pp = 4.0 * std::max(ch_pp, (double)ch_cap / (double)c_fail);
+            if (++reruns > 32) throw Error(D4_ERANGE, "a row step's partial rows outgrow every buffer");
+        }
+        if (mx_chunk >= 4096) {
+            lv_ratio = (double)mx_lv / (double)mx_chunk;
+            ch_ratio = (double)mx_ch / (double)mx_chunk;
+        }
+    }
+
+    // k_expand's follow-up launches: the partial-row states spilled by the
+    // launch before (a full stack, or a wave's tail), eight launches per host
+    // round trip, each reading its item count on the device, until a round
+    // spills nothing.  Tail spills stop after kBudgetedRounds rounds, so the
+    // chain ends (a full stack still spills).
+    void run_expansions(uint64_t tab_m, bool prune) {
+        constexpr uint32_t kRoundsPerSync = 8;
+        constexpr uint32_t kBudgetedRounds = 64;
+        uint32_t rd = C32_DON;
+        for (uint32_t round = 0;;) {
+            uint32_t h[C32_N];
+            HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
+            sync_host();
+            if (h[C32_OVERFLOW] || h[C32_XOVF]) return;  // the step is rerun
+            const uint32_t cap = (uint32_t)(don_a.cap / EXPAND_WORDS);
+            const uint32_t n = h[rd] <= cap ? h[rd] : (h[rd + 1] ? h[rd + 1] - 1 : cap);
+            if (!n) return;
+            for (uint32_t q = 0; q < kRoundsPerSync; q++, round++) {
+                const uint32_t wr = rd == C32_DON ? C32_DON1 : C32_DON;
+                launch_reset_donation_pair(c32.p, c64.p, wr, st);
+                ExtendArgs a = args(tab_m, 0, 0, n_front, 0);
+                a.don_in = don_a.p;
+                a.n_don = q == 0 ? n : ~0u;
+                a.don_read = rd;
+                a.don_write = wr;
+                a.don_out = don_b.p;
+                a.don_cap = (uint32_t)(don_b.cap / EXPAND_WORDS);
+                a.budget = round < kBudgetedRounds ? lib->xp_budget : 0;
+                launch_expand(a, prune, lib->ext_threads, lib->n_cu, st);
                 HIPCHK(hipGetLastError());
                 don_a.swap(don_b);
                 rd = wr;
@@ -636,6 +885,7 @@ struct Engine {
         for (;;) {
             if (bounded && (tab_m > (1ull << 34) ||
                             16ull * tab_m + 16ull * std::min(est_u, tab_m) + 16ull * n_front +
+                                    level_bytes(level_kernel(en, false) ? n_front : 0) +
                                     parked_bytes > mem_cap)) {
                 // Growth slows from row to row, and a split frontier cannot
                 // merge its parts' equal states (each part of a split at the
@@ -683,16 +933,27 @@ struct Engine {
             HIPCHK(hipEventRecord(evx1, st));
             HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (n_front) {
+            sync_host();
+            const bool lv = level_kernel(en, false);
+            if (n_front && !lv) {
                 float xms = 0;
                 HIPCHK(hipEventElapsedTime(&xms, evx0, evx1));
                 stats.extend_ms += xms;
                 stats.extend_launches++;
+            }
+            if (n_front) {
                 stats.extend_bytes += 16ull * n_front + 16ull * h64[C64_CHILDREN];
+                // level kernels: the dedup reads every child written (kept or
+                // pruned) and writes the new states
+                if (lv)
+                    stats.dedup_bytes += 16ull * (h64[C64_CHILDREN] + h64[C64_PRUNED]) +
+                                         16ull * h64[C64_NEW];
             }
             bool ovf = h32[C32_OVERFLOW] != 0;
             bool eovf = en && h32[C32_EDGE_OVF] != 0;
+            // k_expand's spill buffer was full: rerun with four times the room
+            // (the level kernels rerun their own chunks)
+            const bool xovf = !ovf && !lv && h32[C32_XOVF] != 0;
             if (!ovf && !en && h32[C32_SPILL]) {
                 // move the table into one large enough for it and the spill
                 const uint32_t ns = h32[C32_SPILL];
@@ -713,7 +974,7 @@ struct Engine {
                     HIPCHK(hipGetLastError());
                     HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
                     HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
-                    HIPCHK(hipStreamSynchronize(st));
+                    sync_host();
                     ovf = h32[C32_OVERFLOW] != 0;
                     if (!ovf) {
                         tk.swap(tk2);
@@ -727,11 +988,12 @@ struct Engine {
                     ovf = true;
                 }
             }
-            if (!ovf && !eovf) break;
+            if (!ovf && !eovf && !xovf) break;
             // the step is a pure function of the frontier: rerun it bigger
             stats.retries++;
             copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
             if (ovf) tab_m *= 4;
+            if (xovf) xp_cap *= 4;
             if (eovf) est_e = (uint64_t)h32[C32_EDGES] + 1024;
         }
         const uint64_t U = h64[C64_NEW];
@@ -768,7 +1030,7 @@ struct Engine {
         }
         HIPCHK(hipEventRecord(ev1, st));
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
+        sync_host();
         check_compacted(n_out, U);
         float ms = 0, cms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -822,7 +1084,7 @@ struct Engine {
         p->c.reserve(m);
         HIPCHK(hipMemcpyAsync(p->k.p, fk.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(p->c.p, fc.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));
+        sync_host();
         parked_bytes += 16 * m;
         parked.push_back(std::move(p));
         new_roots = std::min<uint64_t>(new_roots, keep);
@@ -887,14 +1149,33 @@ struct Engine {
     // mixed frontier would otherwise leave each of them to a single lane.
     void launch_ranges(uint64_t tab_m, uint32_t e_cap, bool en, bool prune, bool count_only) {
         if (!n_front) return;
-        don_a.reserve(2u << 20);  // 1 Mi donated items per launch
-        don_b.reserve(2u << 20);
+        const bool xp = expand_kernel(en, count_only);
+        don_a.reserve(xp ? EXPAND_WORDS * xp_cap : 2u << 20);  // 1 Mi donated items per launch
+        don_b.reserve(xp ? EXPAND_WORDS * xp_cap : 2u << 20);
         auto split_of = [](uint64_t n) {
             return n <= 1024 ? 2u : n <= (1u << 16) ? 1u : 0u;
         };
         const uint64_t roots = std::min<uint64_t>(new_roots, n_front);
         const uint64_t old = n_front - roots;
-        if (old && roots) {
+        if (level_kernel(en, count_only)) {
+            run_levels(tab_m, prune);
+            return;
+        }
+        if (xp) {
+            auto go = [&](uint64_t b, uint64_t n, uint32_t split) {
+                ExtendArgs a = args(tab_m, 0, b, n, split);
+                a.don_cap = (uint32_t)(don_a.cap / EXPAND_WORDS);
+                a.budget = lib->xp_budget;
+                launch_expand(a, prune, lib->ext_threads, lib->n_cu, st);
+            };
+            if (old && roots) {
+                go(0, old, split_of(old));
+                launch_reset_work(c64.p, st);
+                go(old, roots, 2);
+            } else {
+                go(0, n_front, split_of(n_front));
+            }
+        } else if (old && roots) {
             launch_extend(args(tab_m, e_cap, 0, old, split_of(old)), en, prune, count_only,
                           lib->ext_threads, lib->n_cu, st);
             launch_reset_work(c64.p, st);
@@ -917,7 +1198,7 @@ struct Engine {
         HIPCHK(hipGetLastError());
         unsigned long long h64[C64_N];
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        sync_host();
         return h64[C64_CHILDREN];
     }
 
@@ -929,7 +1210,7 @@ struct Engine {
                          occ_mask, lib->d_comp, c32.p, c64.p, lib->n_cu, st);
         unsigned long long h64[C64_N];
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        sync_host();
         return h64[C64_NEXTSIZE];
     }
 
@@ -1103,7 +1384,7 @@ struct Engine {
             HIPCHK(hipGetLastError());
             std::vector<uint64_t> ho(mem.size() * (H + 1));
             HIPCHK(hipMemcpyAsync(ho.data(), d_out.p, ho.size() * 8, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
+            sync_host();
             for (size_t j = 0; j < mem.size(); j++) {
                 const uint64_t *o = &ho[j * (H + 1)];
                 if (!o[H]) {
@@ -1380,6 +1661,23 @@ int d4_library_set_budget(d4_library *lib, uint32_t steps) {
     return guard([&] {
         if (!lib) throw Error(D4_EINVAL, "null library");
         lib->ext_budget = steps;
+    });
+}
+
+int d4_library_set_kernel(d4_library *lib, uint32_t kernel, uint32_t tail_rounds) {
+    return guard([&] {
+        if (!lib) throw Error(D4_EINVAL, "null library");
+        if (kernel > D4_KERNEL_LEVEL) throw Error(D4_EINVAL, "unknown kernel");
+        lib->xp_kernel = kernel;
+        lib->xp_budget = tail_rounds;
+    });
+}
+
+int d4_library_get_kernel(const d4_library *lib, uint32_t *kernel, uint32_t *tail_rounds) {
+    return guard([&] {
+        if (!lib) throw Error(D4_EINVAL, "null library");
+        if (kernel) *kernel = lib->xp_kernel;
+        if (tail_rounds) *tail_rounds = lib->xp_budget;
     });
 }
 

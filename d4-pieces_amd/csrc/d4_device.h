@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "d4_internal.h"
+
 namespace d4 {
 namespace dev {
 
@@ -89,6 +91,82 @@ __device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g,
         closed &= ~comp;
     }
     return true;
+}
+
+// The dedup table interleaves each slot's key and count (16 B per slot:
+// tab_key[2h], tab_cnt[2h] with tab_cnt = tab_key + 1), so an insert touches
+// one cache line.  A table has any number m of slots (the host sizes it to
+// its estimate, not to a power of two): a key's home slot is the high word
+// of mix64(key) * m, and probes step linearly, wrapping at m.
+__device__ __forceinline__ uint64_t home_slot(uint64_t key, uint64_t m) {
+    return __umul64hi(mix64(key), m);
+}
+__device__ __forceinline__ uint64_t next_slot(uint64_t h, uint64_t m) {
+    return h + 1 < m ? h + 1 : 0;
+}
+//
+// Insert (key, cnt) into the table: returns 1 when the key is new, 0 when it
+// merged into an existing slot, -1 when the probe limit was reached.  Keys
+// only ever change EMPTY -> key, so the CAS's returned value decides: EMPTY
+// means this lane claimed the slot, the own key means a merge.  One round
+// trip per probe; the count update does not wait for its result.
+__device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
+                                            uint64_t cnt, uint64_t &slot) {
+    uint64_t h = home_slot(key, a.tab_size);
+    for (uint32_t p = 0; p < a.max_probe; p++) {
+        uint64_t cur = atomicCAS(a.tab_key + 2 * h, (unsigned long long)EMPTY_KEY,
+                                 (unsigned long long)key);
+        if (cur == EMPTY_KEY || cur == key) {
+            atomicAdd(a.tab_cnt + 2 * h, (unsigned long long)cnt);
+            slot = h;
+            return cur == EMPTY_KEY ? 1 : 0;
+        }
+        h = next_slot(h, a.tab_size);
+    }
+    return -1;
+}
+
+// FR > 0: the table has FR rows, known when compiling, so the lookups are
+// independent LDS loads with constant offsets; FR == 0: `rows` at run time
+// (a loop, one load after the other).
+template <int FR>
+__device__ __forceinline__ uint64_t fitmask(const uint64_t *s_fit,
+                                            uint32_t rows, uint32_t c,
+                                            uint64_t E) {
+    if constexpr (FR > 0) {
+        const uint64_t *f = s_fit + (size_t)c * FR * 256u;
+        uint64_t m = f[E & 0xffu];
+#pragma unroll
+        for (int j = 1; j < FR; j++) m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
+        return m;
+    } else {
+        const uint64_t *f = s_fit + (size_t)c * rows * 256u;
+        uint64_t m = f[E & 0xffu];
+        for (uint32_t j = 1; j < rows; j++)
+            m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
+        return m;
+    }
+}
+
+// Candidates still available under the use counters `used`: counter fields
+// never straddle a byte, so each byte indexes a table of the candidates its
+// exhausted pieces rule out.
+__device__ __forceinline__ uint64_t avail_of(const uint64_t *s_xa, uint32_t used) {
+    return ~(s_xa[used & 0xffu] | s_xa[256 + ((used >> 8) & 0xffu)] |
+             s_xa[512 + ((used >> 16) & 0xffu)] | s_xa[768 + (used >> 24)]);
+}
+
+// Packed placement entry: place[c][k] (shape) and the candidate's counter
+// info (offset | field mask << 8 | count << 16, piece's candidate range
+// [lo, hi) in bits 32..39 / 40..47), read with one 16-byte LDS load.
+struct __attribute__((aligned(16))) PlaceInfo {
+    uint64_t shape;
+    uint64_t info;
+};
+
+__device__ __forceinline__ uint64_t piece_range(uint64_t info) {
+    const uint32_t lo = (uint32_t)(info >> 32) & 0xffu, hi = (uint32_t)(info >> 40) & 0xffu;
+    return (hi >= 64 ? ~0ull : (1ull << hi) - 1ull) & (~0ull << lo);
 }
 
 }  // namespace dev

@@ -31,7 +31,9 @@ enum : int {
     C32_DON1 = 8,      // second donation counter pair (follow-up rounds alternate)
     C32_DON_END1 = 9,
     C32_SINK = 10,     // terminal states written to the sink (multi-library batches)
-    C32_N = 12,
+    C32_XOVF = 11,     // a partial row found no room (k_expand spill, k_level buffers)
+    C32_LVDONE = 12,   // chunks the level kernels' dedup completed this step
+    C32_N = 13,
 };
 enum : int {
     C64_CHILDREN = 0,
@@ -40,7 +42,9 @@ enum : int {
     C64_NEXTSIZE = 3,
     C64_WORK = 4,      // dynamic work-item claim (extend)
     C64_DONATED = 5,   // work items handed to follow-up launches
-    C64_N = 6,
+    C64_CHW = 6,       // children written by the level kernels (k_level, incl. holes)
+    C64_LVL = 7,       // C64_LVL + L: partial rows written for level L (k_level)
+    C64_N = 16,
 };
 
 // Arguments of the extend kernel (passed by value).
@@ -104,6 +108,61 @@ struct ExtendArgs {
     uint64_t *spill;         // [spill_cap][2]
     uint32_t spill_cap;
 };
+
+// Level-synchronous count-mode row step (d4_level.hip).  A partial row
+// (state) is {open window E, remaining candidates ok, tiling count, occupancy
+// of the rows below cov, use counters | instance << 32}, stored SoA in a
+// buffer of `cap` entries per field; children are {key, count} SoA.
+constexpr uint32_t LVL_FIELDS = 5;
+constexpr uint32_t LVL_MAX = 8;  // placements per row (one level each)
+struct LevelArgs {
+    // level 0 reads records; later levels read the states level - 1 wrote
+    const uint64_t *in_key, *in_cnt;
+    uint32_t n_rec;
+    const uint64_t *in_st;   // [LVL_FIELDS][in_cap]
+    uint64_t in_cap;
+    uint32_t level;          // states read come from c64[C64_LVL + level - 1]
+    uint64_t *out_st;        // [LVL_FIELDS][out_cap], counted in c64[C64_LVL + level]
+    uint64_t out_cap;
+    uint64_t *ch_key, *ch_cnt;  // children, counted in c64[C64_CHW]
+    uint64_t ch_cap;
+    // library tables (staged into LDS)
+    const uint64_t *fit, *placeinfo, *xavail;
+    uint32_t fit_rows;
+    // instances
+    const uint64_t *bw;
+    uint32_t bw_stride;
+    const uint32_t *hgt, *start;
+    unsigned long long *result;
+    uint32_t row;
+    // key layout
+    uint32_t occ_bits, inst_shift, used_mask;
+    uint64_t occ_mask;
+    uint32_t *c32;
+    unsigned long long *c64;
+};
+struct DedupArgs {
+    const uint64_t *ch_key, *ch_cnt;
+    uint64_t ch_cap;         // children: min(c64[C64_CHW], ch_cap)
+    const uint64_t *bw;
+    uint32_t bw_stride;
+    const uint32_t *start;
+    uint32_t row;            // the children's row is row + 1
+    uint32_t occ_bits, inst_shift;
+    uint64_t occ_mask;
+    uint64_t seed_mask;
+    uint32_t gcd, min_size;
+    unsigned long long *tab;  // {key, count} slots
+    uint64_t tab_size;
+    uint32_t max_probe;
+    uint64_t *spill;
+    uint32_t spill_cap;
+    uint32_t *c32;
+    unsigned long long *c64;
+};
+void launch_level(const LevelArgs &a, int n_cu, hipStream_t st);
+void launch_dedup(const DedupArgs &a, bool prune, int n_cu, hipStream_t st);
+size_t level_lds_bytes(uint32_t fit_rows);
 
 // Tables of one library in a multi-library batch (device layout, 16-byte
 // aligned).  Candidate masks are 128 bits: word 0 = candidates 0..63.
@@ -171,6 +230,13 @@ void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
 void launch_reset_work(unsigned long long *c64, hipStream_t st);
+// zero the level kernels' per-chunk counters; first: also the overflow flag
+// and the completed-chunk count
+void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st);
+// tiling counts of the records whose instance finishes at this step
+void launch_terminal(const uint64_t *key, const uint64_t *cnt, uint64_t n, uint32_t row,
+                     const uint32_t *start, const uint32_t *hgt, uint32_t inst_shift,
+                     unsigned long long *result, int n_cu, hipStream_t st);
 void launch_reset_donations(uint32_t *c32, unsigned long long *c64, hipStream_t st);
 // reset one donation counter pair (C32_DON or C32_DON1) and the work counter
 void launch_reset_donation_pair(uint32_t *c32, unsigned long long *c64, uint32_t pair,
@@ -184,6 +250,13 @@ void launch_rehash(const unsigned long long *old_tab, uint64_t m_old, unsigned l
                    unsigned long long *c64, hipStream_t st);
 void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune,
                    bool count_only, uint32_t threads, int n_cu, hipStream_t st);
+// Count-mode extend over partial-row snapshots (d4_expand.hip): same inputs
+// and table semantics as launch_extend with enum_mode = count_only = false.
+// In a follow-up launch (don_in set) the items are spilled partial-row
+// states of EXPAND_WORDS words each, and don_out receives such states.
+constexpr uint32_t EXPAND_WORDS = 5;
+void launch_expand(const ExtendArgs &a, bool prune, uint32_t threads, int n_cu, hipStream_t st);
+size_t expand_lds_bytes(uint32_t fit_rows, uint32_t threads);
 constexpr uint32_t COMPACT_MAX_BLOCKS = 65536;
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,

@@ -41,84 +41,8 @@ using namespace dev;
 constexpr uint32_t STAGE = 128;  // staged children per wave
 constexpr int CMP_THREADS = 256;
 
-// The dedup table interleaves each slot's key and count (16 B per slot:
-// tab_key[2h], tab_cnt[2h] with tab_cnt = tab_key + 1), so an insert touches
-// one cache line.  A table has any number m of slots (the host sizes it to
-// its estimate, not to a power of two): a key's home slot is the high word
-// of mix64(key) * m, and probes step linearly, wrapping at m.
-__device__ __forceinline__ uint64_t home_slot(uint64_t key, uint64_t m) {
-    return __umul64hi(mix64(key), m);
-}
-__device__ __forceinline__ uint64_t next_slot(uint64_t h, uint64_t m) {
-    return h + 1 < m ? h + 1 : 0;
-}
-//
-// Insert (key, cnt) into the table: returns 1 when the key is new, 0 when it
-// merged into an existing slot, -1 when the probe limit was reached.  Keys
-// only ever change EMPTY -> key, so the CAS's returned value decides: EMPTY
-// means this lane claimed the slot, the own key means a merge.  One round
-// trip per probe; the count update does not wait for its result.
-__device__ __forceinline__ int table_insert(const ExtendArgs &a, uint64_t key,
-                                            uint64_t cnt, uint64_t &slot) {
-    uint64_t h = home_slot(key, a.tab_size);
-    for (uint32_t p = 0; p < a.max_probe; p++) {
-        uint64_t cur = atomicCAS(a.tab_key + 2 * h, (unsigned long long)EMPTY_KEY,
-                                 (unsigned long long)key);
-        if (cur == EMPTY_KEY || cur == key) {
-            atomicAdd(a.tab_cnt + 2 * h, (unsigned long long)cnt);
-            slot = h;
-            return cur == EMPTY_KEY ? 1 : 0;
-        }
-        h = next_slot(h, a.tab_size);
-    }
-    return -1;
-}
-
-// FR > 0: the table has FR rows, known when compiling, so the lookups are
-// independent LDS loads with constant offsets; FR == 0: `rows` at run time
-// (a loop, one load after the other).
-template <int FR>
-__device__ __forceinline__ uint64_t fitmask(const uint64_t *s_fit,
-                                            uint32_t rows, uint32_t c,
-                                            uint64_t E) {
-    if constexpr (FR > 0) {
-        const uint64_t *f = s_fit + (size_t)c * FR * 256u;
-        uint64_t m = f[E & 0xffu];
-#pragma unroll
-        for (int j = 1; j < FR; j++) m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
-        return m;
-    } else {
-        const uint64_t *f = s_fit + (size_t)c * rows * 256u;
-        uint64_t m = f[E & 0xffu];
-        for (uint32_t j = 1; j < rows; j++)
-            m &= f[j * 256u + ((E >> (8 * j)) & 0xffu)];
-        return m;
-    }
-}
-
-// Candidates still available under the use counters `used`: counter fields
-// never straddle a byte, so each byte indexes a table of the candidates its
-// exhausted pieces rule out.
-__device__ __forceinline__ uint64_t avail_of(const uint64_t *s_xa, uint32_t used) {
-    return ~(s_xa[used & 0xffu] | s_xa[256 + ((used >> 8) & 0xffu)] |
-             s_xa[512 + ((used >> 16) & 0xffu)] | s_xa[768 + (used >> 24)]);
-}
-
 // COUNT_ONLY runs the same search but only counts the surviving children
 // (used to size the dedup table exactly when the frontier is small).
-// Packed placement entry: place[c][k] (shape) and the candidate's counter
-// info (offset | field mask << 8 | count << 16, piece's candidate range
-// [lo, hi) in bits 32..39 / 40..47), read with one 16-byte LDS load.
-struct __attribute__((aligned(16))) PlaceInfo {
-    uint64_t shape;
-    uint64_t info;
-};
-
-__device__ __forceinline__ uint64_t piece_range(uint64_t info) {
-    const uint32_t lo = (uint32_t)(info >> 32) & 0xffu, hi = (uint32_t)(info >> 40) & 0xffu;
-    return (hi >= 64 ? ~0ull : (1ull << hi) - 1ull) & (~0ull << lo);
-}
-
 template <bool ENUM, bool PRUNE, bool COUNT_ONLY, int THREADS, int FR>
 __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     constexpr uint32_t SW = ENUM ? 5 : 3;  // staged u64 words per child
@@ -621,6 +545,31 @@ __global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
 
 __global__ void k_reset_work(unsigned long long *c64) { c64[C64_WORK] = 0; }
 
+// Terminal states (every cell of the instance covered) add their count to
+// the instance's result.
+__global__ __launch_bounds__(256) void k_terminal(const uint64_t *key, const uint64_t *cnt,
+                                                  uint64_t n, uint32_t row, const uint32_t *start,
+                                                  const uint32_t *hgt, uint32_t inst_shift,
+                                                  unsigned long long *result) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const uint64_t k = key[i];
+        const uint32_t inst = inst_shift < 64 ? (uint32_t)(k >> inst_shift) : 0u;
+        if (row - start[inst] >= hgt[inst]) atomicAdd(result + inst, (unsigned long long)cnt[i]);
+    }
+}
+
+// the level kernels' per-chunk counters (children written, partial rows per
+// level); at a step's first chunk also the overflow flag and the chunk count
+__global__ void k_reset_levels(uint32_t *c32, unsigned long long *c64, uint32_t first) {
+    // after an overflow the counters keep the failed chunk's sizes for the host
+    if (!first && c32[C32_XOVF]) return;
+    if (threadIdx.x <= LVL_MAX) c64[C64_CHW + threadIdx.x] = 0;
+    if (first && threadIdx.x == 0) {
+        c32[C32_XOVF] = 0;
+        c32[C32_LVDONE] = 0;
+    }
+}
+
 // Insert (key, cnt) into a {key, count} table of m slots, as table_insert
 // does.
 __device__ __forceinline__ int insert_at(unsigned long long *tab, uint64_t m, uint64_t key,
@@ -910,6 +859,18 @@ void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
 
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c32, c64);
+}
+
+void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_levels, dim3(1), dim3(64), 0, st, c32, c64, (uint32_t)first);
+}
+
+void launch_terminal(const uint64_t *key, const uint64_t *cnt, uint64_t n, uint32_t row,
+                     const uint32_t *start, const uint32_t *hgt, uint32_t inst_shift,
+                     unsigned long long *result, int n_cu, hipStream_t st) {
+    const uint64_t b = std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), (uint64_t)n_cu * 8);
+    hipLaunchKernelGGL(k_terminal, dim3((uint32_t)b), dim3(256), 0, st, key, cnt, n, row, start,
+                       hgt, inst_shift, result);
 }
 
 void launch_reset_work(unsigned long long *c64, hipStream_t st) {

@@ -1,0 +1,384 @@
+// d4_level.hip -- level-synchronous count-mode row step (gfx950).
+//
+// A row's completions form a tree: level L holds the partial rows with L
+// placements (each placement covers the row's lowest open cell, the
+// reference CPU searcher's order, src/Piece.cpp:82-98), and a row has at
+// most 8 levels.  Instead of one depth-first search per lane (k_extend) the
+// row step runs one launch per level over every partial row of the frontier:
+//
+//   k_level   a wave loads 64 partial rows (level 0: records, expanded to
+//             their root partial row), lists their (row, candidate) pairs in
+//             LDS (a wave prefix sum of the candidate counts gives each row
+//             its range of the list) and then processes the pairs 64 at a
+//             time, one per lane, so no lane idles on a row with few
+//             candidates or waits on a sibling's deeper subtree.  A placement
+//             that completes the row yields a child {key, count}; one that
+//             leaves it open yields a partial row of the next level together
+//             with its candidate mask -- or nothing, when that mask is empty.
+//             Outputs are written densely (ballot + mbcnt) into chunks that
+//             each wave reserves with one atomic per chunk, not per record.
+//   k_dedup   prunes each child (connectivity) and merges it into the
+//             open-addressing table (CAS, counts summed), with nothing else
+//             on its lanes' path, so many probes are in flight at once.
+//
+// The children and the table are the ones k_extend produces (each child is
+// one placement sequence; only the order of discovery differs).  The
+// reference's counterparts are tiled_row_search / impl<H>
+// (src/kernel.cuh:79-160) and the host Sorter (src/sorter.cpp:156-214).
+#include "d4_internal.h"
+#include "d4_device.h"
+
+#include <algorithm>
+
+namespace d4 {
+
+namespace {
+
+using namespace dev;
+
+constexpr int LT = 1024;          // threads per workgroup (16 waves: 4 per SIMD)
+constexpr uint32_t PCAP = 1024;   // listed (row, candidate) pairs per wave and pass
+constexpr uint32_t OWN = 5;       // per-wave LDS words per loaded row: E, cov, cnt, ui, avail
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, (unsigned)o);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+// Dense output of one stream (wave-uniform state): lanes with `pred` get
+// consecutive positions of the wave's current chunk; a chunk is reserved
+// with one atomic on the stream's u64 counter.  Returns false when the
+// buffer is exhausted (C32_XOVF is set; the host reruns the step larger).
+struct Writer {
+    uint64_t pos = 0, end = 0;
+};
+
+__device__ __forceinline__ bool reserve_chunk(unsigned long long *ctr, uint64_t chunk, uint64_t cap,
+                                              uint32_t *c32, uint64_t &base) {
+    uint64_t b = 0;
+    if (lane_id() == 0) b = atomicAdd(ctr, (unsigned long long)chunk);
+    b = shfl64(b, 0);
+    if (b + chunk > cap) {
+        if (lane_id() == 0) atomicOr(c32 + C32_XOVF, 1u);
+        return false;
+    }
+    base = b;
+    return true;
+}
+
+// Positions for the lanes in m: returns each lane's slot; `ok` false on
+// overflow.  At most one chunk switch per call (a call places <= 64 items
+// and chunks hold >= 64).
+__device__ __forceinline__ uint64_t place(Writer &w, uint64_t m, unsigned long long *ctr,
+                                          uint64_t chunk, uint64_t cap, uint32_t *c32, bool &ok) {
+    const uint32_t n = (uint32_t)__builtin_popcountll(m);
+    const uint32_t q = mbcnt(m);
+    uint64_t slot = w.pos + q;
+    if (w.pos + n > w.end) {
+        uint64_t base = 0;
+        if (!reserve_chunk(ctr, chunk, cap, c32, base)) {
+            ok = false;
+            return 0;
+        }
+        const uint64_t room = w.end - w.pos;  // slots left in the old chunk
+        if (q >= room) slot = base + (q - room);
+        w.pos = base + (n - room);
+        w.end = base + chunk;
+        return slot;
+    }
+    w.pos += n;
+    return slot;
+}
+
+template <bool FROM_REC, int FR>
+__global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
+    extern __shared__ uint64_t lds[];
+    const uint32_t nfit = 8u * a.fit_rows * 256u;
+    uint64_t *s_fit = lds;
+    const PlaceInfo *s_pc = reinterpret_cast<const PlaceInfo *>(s_fit + nfit);
+    uint64_t *s_xa = s_fit + nfit + 2 * 8 * MAX_CAND;
+    uint64_t *s_rng = s_xa + 4 * 256;
+    for (uint32_t i = threadIdx.x; i < nfit; i += LT) s_fit[i] = a.fit[i];
+    for (uint32_t i = threadIdx.x; i < 2 * 8 * MAX_CAND; i += LT) s_fit[nfit + i] = a.placeinfo[i];
+    for (uint32_t i = threadIdx.x; i < 4 * 256; i += LT) s_xa[i] = a.xavail[i];
+    for (uint32_t i = threadIdx.x; i < MAX_CAND; i += LT)
+        s_rng[i] = piece_range(a.placeinfo[2 * i + 1]);
+    __syncthreads();
+
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    uint64_t *o_E = s_rng + MAX_CAND + (size_t)wv * (OWN * 64 + PCAP / 4);
+    uint64_t *o_cov = o_E + 64, *o_cnt = o_cov + 64, *o_ui = o_cnt + 64, *o_av = o_ui + 64;
+    uint16_t *plist = reinterpret_cast<uint16_t *>(o_av + 64);
+
+    if (__hip_atomic_load(a.c32 + C32_XOVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    uint64_t n_in;
+    if (FROM_REC) {
+        n_in = a.n_rec;
+    } else {
+        n_in = a.c64[C64_LVL + a.level - 1];
+        if (n_in > a.in_cap) n_in = a.in_cap;
+    }
+    const uint64_t waves = (uint64_t)gridDim.x * (LT / 64);
+    const uint64_t wid = (uint64_t)blockIdx.x * (LT / 64) + wv;
+    // chunk size: small levels get small chunks (fewer holes)
+    uint64_t chunk = n_in / (waves ? waves : 1);
+    chunk = chunk < 64 ? 64 : chunk > 1024 ? 1024 : chunk;
+    Writer ws, wc;
+    bool alive = true;
+    unsigned long long *ctr_s = a.c64 + C64_LVL + a.level, *ctr_c = a.c64 + C64_CHW;
+    uint64_t *oE = a.out_st, *oOk = oE + a.out_cap, *oCnt = oOk + a.out_cap,
+             *oCov = oCnt + a.out_cap, *oUi = oCov + a.out_cap;
+
+    for (uint64_t b0 = wid * 64; alive && b0 < n_in; b0 += waves * 64) {
+        const uint64_t i = b0 + lane;
+        const bool valid = i < n_in;
+        uint64_t E = 0, ok = 0, cnt = 0, cov = 0;
+        uint32_t used = 0, inst = 0;
+        bool pass = false;
+        if (FROM_REC) {
+            if (valid) {
+                const uint64_t key = a.in_key[i];
+                cnt = a.in_cnt[i];
+                inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
+                used = (uint32_t)(key >> a.occ_bits) & a.used_mask;
+                const uint32_t rr = a.row - a.start[inst];
+                // a terminal state (its instance finishes at this step) is
+                // counted by the step's terminal pass and has no children
+                if (rr < a.hgt[inst]) {
+                    const uint64_t occ = key & a.occ_mask;
+                    E = a.bw[(size_t)inst * a.bw_stride + rr] & ~occ;
+                    cov = occ >> 8;
+                    if (E & 0xffu) {
+                        const uint32_t c = (uint32_t)__builtin_ctzll(E & 0xffu);
+                        ok = fitmask<FR>(s_fit, a.fit_rows, c, E) & avail_of(s_xa, used);
+                    } else {
+                        pass = true;  // row already covered from above: its own child
+                    }
+                }
+            }
+        } else if (valid) {
+            const uint64_t *s = a.in_st;
+            E = s[i];
+            ok = s[a.in_cap + i];
+            cnt = s[2 * a.in_cap + i];
+            cov = s[3 * a.in_cap + i];
+            const uint64_t ui = s[4 * a.in_cap + i];
+            used = (uint32_t)ui;
+            inst = (uint32_t)(ui >> 32);
+        }
+        if (FROM_REC) {
+            const uint64_t pm = __ballot(pass);
+            if (pm) {
+                bool okw = true;
+                const uint64_t sl = place(wc, pm, ctr_c, chunk, a.ch_cap, a.c32, okw);
+                if (!okw) {
+                    alive = false;
+                    break;
+                }
+                if (pass) {
+                    uint64_t key2 = cov | ((uint64_t)used << a.occ_bits);
+                    if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
+                    a.ch_key[sl] = key2;
+                    a.ch_cnt[sl] = cnt;
+                }
+            }
+        }
+        // the loaded rows go to LDS; their pairs are listed pass by pass
+        __builtin_amdgcn_wave_barrier();
+        o_E[lane] = E;
+        o_cov[lane] = cov;
+        o_cnt[lane] = cnt;
+        o_ui[lane] = (uint64_t)used | ((uint64_t)inst << 32);
+        o_av[lane] = ok ? avail_of(s_xa, used) : 0ull;
+        const uint32_t n = (uint32_t)__builtin_popcountll(ok);
+        const uint32_t incl = wave_incl_scan(n);
+        const uint32_t excl = incl - n;
+        bool pending = n != 0;
+        while (alive) {
+            const uint64_t pendm = __ballot(pending);
+            if (!pendm) break;
+            const uint32_t first = (uint32_t)__builtin_ctzll(pendm);
+            const uint32_t off = __shfl(excl, (int)first);
+            const bool fits = pending && incl - off <= PCAP;
+            const uint64_t fm = __ballot(fits);
+            const uint32_t lastf = 63u - (uint32_t)__builtin_clzll(fm);
+            const uint32_t total = __shfl(incl, (int)lastf) - off;
+            __builtin_amdgcn_wave_barrier();
+            if (fits) {
+                uint32_t t = excl - off;
+                for (uint64_t m = ok; m; m &= m - 1)
+                    plist[t++] = (uint16_t)((lane << 6) | (uint32_t)__builtin_ctzll(m));
+            }
+            __builtin_amdgcn_wave_barrier();
+            pending = pending && !fits;
+            for (uint32_t r0 = 0; r0 < total && alive; r0 += 64) {
+                const uint32_t p = r0 + lane;
+                bool emitC = false, emitS = false;
+                uint64_t E2 = 0, cov2 = 0, ok2 = 0, pc = 0;
+                uint32_t used2 = 0, pi = 0;
+                if (p < total) {
+                    const uint32_t e = plist[p];
+                    const uint32_t o = e >> 6, k = e & 63u;
+                    const uint64_t pE = o_E[o];
+                    const uint64_t pui = o_ui[o];
+                    const uint32_t c = (uint32_t)__builtin_ctzll(pE & 0xffu);
+                    const PlaceInfo pk = s_pc[c * MAX_CAND + k];
+                    const uint64_t pl = pk.shape;
+                    if (!(pl & ~pE)) {
+                        const uint32_t ci = (uint32_t)pk.info;
+                        E2 = pE & ~pl;
+                        used2 = (uint32_t)pui + (1u << (ci & 0xffu));
+                        pi = (uint32_t)(pui >> 32);
+                        cov2 = o_cov[o] | (pl >> 8);
+                        pc = o_cnt[o];
+                        if (!(E2 & 0xffu)) {
+                            emitC = true;
+                        } else {
+                            uint64_t av2 = o_av[o];
+                            if (((used2 >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
+                                av2 &= ~s_rng[k];
+                            const uint32_t c2 = (uint32_t)__builtin_ctzll(E2 & 0xffu);
+                            ok2 = fitmask<FR>(s_fit, a.fit_rows, c2, E2) & av2;
+                            emitS = ok2 != 0;
+                        }
+                    }
+                }
+                const uint64_t cm = __ballot(emitC);
+                if (cm) {
+                    bool okw = true;
+                    const uint64_t sl = place(wc, cm, ctr_c, chunk, a.ch_cap, a.c32, okw);
+                    if (!okw) {
+                        alive = false;
+                        break;
+                    }
+                    if (emitC) {
+                        uint64_t key2 = cov2 | ((uint64_t)used2 << a.occ_bits);
+                        if (a.inst_shift < 64) key2 |= (uint64_t)pi << a.inst_shift;
+                        a.ch_key[sl] = key2;
+                        a.ch_cnt[sl] = pc;
+                    }
+                }
+                const uint64_t sm = __ballot(emitS);
+                if (sm) {
+                    bool okw = true;
+                    const uint64_t sl = place(ws, sm, ctr_s, chunk, a.out_cap, a.c32, okw);
+                    if (!okw) {
+                        alive = false;
+                        break;
+                    }
+                    if (emitS) {
+                        oE[sl] = E2;
+                        oOk[sl] = ok2;
+                        oCnt[sl] = pc;
+                        oCov[sl] = cov2;
+                        oUi[sl] = (uint64_t)used2 | ((uint64_t)pi << 32);
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (!alive) return;  // the step is rerun
+    // unused slots of the wave's last chunks: no candidates / empty keys
+    for (uint64_t j = ws.pos + lane; j < ws.end; j += 64) oOk[j] = 0;
+    for (uint64_t j = wc.pos + lane; j < wc.end; j += 64) a.ch_key[j] = EMPTY_KEY;
+}
+
+template <bool PRUNE>
+__global__ __launch_bounds__(256) void k_dedup(DedupArgs a) {
+    uint64_t n = a.c64[C64_CHW];
+    if (n > a.ch_cap) n = a.ch_cap;
+    // a chunk whose level kernels overflowed is rerun by the host: nothing
+    // of it enters the table; a complete chunk counts itself once
+    if (__hip_atomic_load(a.c32 + C32_XOVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) n = 0;
+    else if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.c32 + C32_LVDONE, 1u);
+    uint64_t kept = 0, fresh = 0, pruned = 0;
+    bool failed = false;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const uint64_t key = a.ch_key[i];
+        if (key == EMPTY_KEY) continue;
+        const uint64_t cnt = a.ch_cnt[i];
+        if (PRUNE) {
+            const uint32_t inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
+            const uint32_t rr = a.row + 1 - a.start[inst];
+            const uint64_t E = a.bw[(size_t)inst * a.bw_stride + rr] & ~(key & a.occ_mask);
+            if (!viable(E, a.seed_mask, a.gcd, a.min_size)) {
+                pruned++;
+                continue;
+            }
+        }
+        kept++;
+        uint64_t h = home_slot(key, a.tab_size);
+        int r = -1;
+        for (uint32_t p = 0; p < a.max_probe; p++) {
+            const uint64_t cur =
+                atomicCAS(a.tab + 2 * h, (unsigned long long)EMPTY_KEY, (unsigned long long)key);
+            if (cur == EMPTY_KEY || cur == key) {
+                atomicAdd(a.tab + 2 * h + 1, (unsigned long long)cnt);
+                r = cur == EMPTY_KEY ? 1 : 0;
+                break;
+            }
+            h = next_slot(h, a.tab_size);
+        }
+        if (r < 0 && a.spill_cap) {
+            // no slot within the probe limit: park it for the host's table growth
+            const uint32_t s = atomicAdd(a.c32 + C32_SPILL, 1u);
+            if (s < a.spill_cap) {
+                a.spill[2 * s] = key;
+                a.spill[2 * s + 1] = cnt;
+                r = 0;
+            }
+        }
+        if (r < 0) failed = true;
+        fresh += r > 0;
+    }
+    if (failed) atomicOr(a.c32 + C32_OVERFLOW, 1u);
+    kept = wave_sum_u64(kept);
+    fresh = wave_sum_u64(fresh);
+    pruned = wave_sum_u64(pruned);
+    if (lane_id() == 0) {
+        if (kept) atomicAdd(a.c64 + C64_CHILDREN, (unsigned long long)kept);
+        if (fresh) atomicAdd(a.c64 + C64_NEW, (unsigned long long)fresh);
+        if (pruned) atomicAdd(a.c64 + C64_PRUNED, (unsigned long long)pruned);
+    }
+}
+
+template <bool FROM_REC>
+static void launch_lv(const LevelArgs &a, dim3 g, size_t lds, hipStream_t st) {
+    switch (a.fit_rows) {
+        case 1: hipLaunchKernelGGL((k_level<FROM_REC, 1>), g, dim3(LT), lds, st, a); break;
+        case 2: hipLaunchKernelGGL((k_level<FROM_REC, 2>), g, dim3(LT), lds, st, a); break;
+        case 3: hipLaunchKernelGGL((k_level<FROM_REC, 3>), g, dim3(LT), lds, st, a); break;
+        default: hipLaunchKernelGGL((k_level<FROM_REC, 0>), g, dim3(LT), lds, st, a); break;
+    }
+}
+
+}  // namespace
+
+size_t level_lds_bytes(uint32_t fit_rows) {
+    return sizeof(uint64_t) * (8u * fit_rows * 256u + 2u * 8u * MAX_CAND + 4u * 256u + MAX_CAND) +
+           (sizeof(uint64_t) * OWN * 64 + sizeof(uint16_t) * PCAP) * (LT / 64);
+}
+
+void launch_level(const LevelArgs &a, int n_cu, hipStream_t st) {
+    const size_t lds = level_lds_bytes(a.fit_rows);
+    // one workgroup per CU (LDS-bound); level 0 knows its record count
+    uint64_t blocks = (uint64_t)n_cu;
+    if (a.level == 0) blocks = std::min<uint64_t>(blocks, ((uint64_t)a.n_rec + LT - 1) / LT);
+    if (!blocks) blocks = 1;
+    if (a.level == 0) launch_lv<true>(a, dim3((uint32_t)blocks), lds, st);
+    else launch_lv<false>(a, dim3((uint32_t)blocks), lds, st);
+}
+
+void launch_dedup(const DedupArgs &a, bool prune, int n_cu, hipStream_t st) {
+    const dim3 g((uint32_t)n_cu * 8);
+    if (prune) hipLaunchKernelGGL(k_dedup<true>, g, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_dedup<false>, g, dim3(256), 0, st, a);
+}
+
+}  // namespace d4

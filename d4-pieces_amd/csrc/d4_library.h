@@ -28,6 +28,11 @@ struct d4_library {
     uint32_t ext_threads = 1024;  // extend workgroup size (autotuned)
     uint32_t ext_refill = 16;     // idle lanes per refill round (autotuned)
     uint32_t ext_budget = 1024;   // DFS steps per work item before donating (0: never)
+    // count-mode row steps: D4_KERNEL_LEVEL (level-synchronous k_level +
+    // k_dedup), D4_KERNEL_EXPAND (k_expand: snapshots on a per-wave stack) or
+    // D4_KERNEL_DFS (k_extend: one depth-first search per lane)
+    uint32_t xp_kernel = D4_KERNEL_LEVEL;
+    uint32_t xp_budget = 256;     // k_expand: tail rounds before a wave's snapshots move on
     // single-library tables (at most MAX_CAND candidates: one u64 mask),
     // built and uploaded on first use by an engine
     std::mutex fast_mu;

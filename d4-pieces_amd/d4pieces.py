@@ -32,6 +32,7 @@ LIB_PATH = os.path.join(_HERE, "libd4.so")
 D4_OK, D4_EINVAL, D4_ENOMEM, D4_EDEVICE, D4_ESTATE, D4_ERANGE = 0, -1, -2, -3, -4, -5
 MODE_COUNT, MODE_ENUM = 0, 1
 FLAG_NO_PRUNE = 1
+KERNEL_DFS, KERNEL_EXPAND, KERNEL_LEVEL = 0, 1, 2
 
 
 class D4Error(RuntimeError):
@@ -61,6 +62,11 @@ class Stats(ctypes.Structure):
         ("donated", ctypes.c_uint64),
         ("rehashes", ctypes.c_uint64),
         ("sink_growths", ctypes.c_uint64),
+        ("host_syncs", ctypes.c_uint64),
+        ("dedup_ms", ctypes.c_double),
+        ("dedup_launches", ctypes.c_uint64),
+        ("dedup_bytes", ctypes.c_uint64),
+        ("chunk_reruns", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -91,6 +97,8 @@ SIGNATURES = [
      [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("d4_library_get_launch", ctypes.c_int, [_P, _U32P, _U32P, _U32P]),
     ("d4_library_set_budget", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_library_set_kernel", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    ("d4_library_get_kernel", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     ("d4_library_get_budget", ctypes.c_int, [_P, _U32P]),
     ("d4_library_tune", ctypes.c_int,
      [_P, ctypes.c_uint32, _U8P, ctypes.c_uint32, _U32P, _U32P, _U32P, _U32P,
@@ -351,6 +359,17 @@ class PieceLibrary:
         v = ctypes.c_uint32()
         _check(load().d4_library_get_budget(self._h, ctypes.byref(v)))
         return v.value
+
+    def set_kernel(self, kernel: int, tail_rounds: int = 256) -> None:
+        """Count-mode row-step kernel: KERNEL_EXPAND (default) or KERNEL_DFS;
+        tail_rounds: k_expand rounds after the records run out before a wave's
+        partial rows move to a follow-up launch (0: never)."""
+        _check(load().d4_library_set_kernel(self._h, kernel, tail_rounds))
+
+    def get_kernel(self) -> Tuple[int, int]:
+        k, t = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(load().d4_library_get_kernel(self._h, ctypes.byref(k), ctypes.byref(t)))
+        return k.value, t.value
 
     def get_launch(self) -> Tuple[int, int, int]:
         f, t, r = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()

@@ -76,6 +76,14 @@ typedef struct d4_stats {
     uint64_t donated;          /* work items handed between extend launches */
     uint64_t rehashes;         /* dedup tables grown after children spilled */
     uint64_t sink_growths;     /* terminal-state sinks grown (the step reruns) */
+    uint64_t host_syncs;       /* host round trips (waits on the device) inside
+                                  the row steps                             */
+    double   dedup_ms;         /* device time of the separate dedup kernel
+                                  (level kernels; 0 where dedup is fused
+                                  into the extend kernel)                   */
+    uint64_t dedup_launches;
+    uint64_t dedup_bytes;      /* 16 per child read + 16 per new state      */
+    uint64_t chunk_reruns;     /* level-kernel chunks rerun with larger buffers */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -138,6 +146,25 @@ int d4_library_set_launch(d4_library *lib, uint32_t fit_rows, uint32_t threads,
  * counterpart: the reference's tiles are fixed per launch, src/kernel.cu.) */
 int d4_library_set_budget(d4_library *lib, uint32_t steps);
 int d4_library_get_budget(const d4_library *lib, uint32_t *steps);
+/* Kernel of count-mode row steps (searchers, batches, the server's
+ * all-solutions frontier, bounded runs).
+ *   D4_KERNEL_LEVEL (the default): level-synchronous -- one launch per
+ *     placement level over every partial row of the frontier, (row,
+ *     candidate) pairs spread one per lane, partial rows and children
+ *     written densely to device memory, then a separate dedup kernel
+ *     (prune + table merge).  Needs fit tables of at most 4 rows.
+ *   D4_KERNEL_EXPAND: partial-row snapshots on a per-wave LDS stack, one
+ *     candidate per lane per round; a wave's snapshots move to a follow-up
+ *     launch once the launch's records are gone and the wave has run
+ *     tail_rounds more rounds (0: never).
+ *   D4_KERNEL_DFS: one depth-first search per lane with the donation budget
+ *     of d4_library_set_budget.
+ * All replace the reference's tiled_row_search (src/kernel.cuh:79-160);
+ * counts are identical.  Enum mode (solutions) always uses the depth-first
+ * kernel. */
+enum { D4_KERNEL_DFS = 0, D4_KERNEL_EXPAND = 1, D4_KERNEL_LEVEL = 2 };
+int d4_library_set_kernel(d4_library *lib, uint32_t kernel, uint32_t tail_rounds);
+int d4_library_get_kernel(const d4_library *lib, uint32_t *kernel, uint32_t *tail_rounds);
 int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows,
                           uint32_t *threads, uint32_t *refill);
 /* Measured autotuner, replacing the analytic cost model of

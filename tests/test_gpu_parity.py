@@ -265,3 +265,45 @@ def test_calendar_puzzle_batch(gpu, oracle):
         sol = gpu.expand_solution(shapes, raw, len(rows))
         assert sum(1 for r in sol.map for v in r if v >= 0) == 41
 
+
+
+@pytest.mark.parametrize("kernel", ["dfs", "expand", "level"])
+def test_row_step_kernels_agree(gpu, p5, oracle, kernel):
+    """The three count-mode row-step kernels (depth-first per lane, partial
+    rows on a per-wave stack, level-synchronous) give the oracle's counts:
+    the 256 seeded 8x8 boards (fixture), 6x10, random small instances with
+    multiplicities, and doubled pentominoes on 8x5 under a memory cap small
+    enough to split the frontier and chunk the level buffers."""
+    import json
+    import os
+    k = {"dfs": gpu.KERNEL_DFS, "expand": gpu.KERNEL_EXPAND, "level": gpu.KERNEL_LEVEL}[kernel]
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    lib.set_kernel(k, 256)
+    assert lib.get_kernel()[0] == k
+    b = gpu.Batch(lib, fx["boards"])
+    b.run()
+    assert b.counts() == fx["counts"]
+    lib6 = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 6)
+    lib6.set_kernel(k, 256)
+    assert gpu_count(gpu, lib6, rect(6, 10)) == 9356
+    rng = random.Random(11)
+    for _ in range(6):
+        shapes = rng.sample(gpu.free_polyominoes(4) + gpu.free_polyominoes(3), 4)
+        counts = [rng.randint(1, 2) for _ in shapes]
+        w = rng.randint(3, 6)
+        area = sum(bin(s).count("1") * c for s, c in zip(shapes, counts))
+        h = -(-area // w)
+        rows = [(1 << w) - 1] * h
+        extra = w * h - area
+        rows[-1] &= ((1 << w) - 1) >> extra
+        lb = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+        lb.set_kernel(k, 256)
+        assert gpu_count(gpu, lb, rows) == oracle.count(shapes, counts, rows)
+    lib2 = gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8)
+    lib2.set_kernel(k, 256)
+    s = gpu.CudaSearcher(lib2, rows=rect(8, 5))
+    s.set_memory(4 << 20)
+    s.run()
+    assert s.count() == 695086
