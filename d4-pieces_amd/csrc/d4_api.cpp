@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -210,6 +211,55 @@ static void ensure_fast(const d4_library *cl) {
 }
 
 // --------------------------------------------------------------------------
+// allocation-free scratch (d4_searcher_count_noalloc)
+// --------------------------------------------------------------------------
+
+// The allocation-free entry point's device memory: one array per device for
+// the whole process, carved into the count-mode engine's buffers.  Host code
+// must never pass `scratch` itself to a kernel or copy (in host code the name
+// is a stand-in, not a device address): the library looks up each device's
+// address once, at library creation, with hipGetSymbolAddress.
+constexpr uint64_t kScratchBytes = 128ull << 20;
+__device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+// slices of the scratch, in entries
+constexpr uint64_t kNaFront = 512u << 10;  // frontier states (x4 arrays: current, next)
+constexpr uint64_t kNaTable = 2u << 20;    // dedup table slots (16 B)
+constexpr uint64_t kNaLevel = 512u << 10;  // partial rows per level buffer (x2, 40 B)
+constexpr uint64_t kNaChild = 2u << 20;    // children per chunk (16 B)
+constexpr uint64_t kNaSpill = 16u << 10;   // spilled children (16 B)
+
+struct NoAllocDev {
+    std::mutex mu;  // serialises this process's calls on the device
+    bool ready = false;
+    uint8_t *ptr = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[5] = {};
+};
+NoAllocDev g_noalloc[kMaxDevices];
+std::mutex g_noalloc_mu;
+
+// Once per device, with that device current: the scratch's address on it,
+// and the stream and events the entry point runs on.
+void noalloc_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    NoAllocDev &d = g_noalloc[dev];
+    if (d.ready) return;
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    HIPCHK(hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking));
+    for (hipEvent_t &e : d.ev) HIPCHK(hipEventCreate(&e));
+    d.ptr = static_cast<uint8_t *>(p);
+    d.ready = true;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
 // frontier engine
 // --------------------------------------------------------------------------
 
@@ -354,8 +404,72 @@ struct Engine {
         HIPCHK(hipStreamSynchronize(st));
     }
 
+    // A count-mode engine for one board whose buffers are slices of the
+    // device's allocation-free scratch, on the stream and events made for
+    // it at set-up: nothing is allocated on the device.
+    bool noalloc = false;
+    Engine(const d4_library *l, uint32_t max_height, uint32_t f, NoAllocDev &na)
+        : lib(l), S(1), mode(D4_MODE_COUNT), flags(f), max_h(max_height), keep_logs(false) {
+        if (!lib->fast_ready) throw Error(D4_ESTATE, "library tables are not on the device");
+        if (max_h > (uint32_t)MAX_ROWS) throw Error(D4_EINVAL, "board taller than 64 rows");
+        noalloc = true;
+        heights.assign(1, 0);
+        start.assign(1, 0);
+        active.assign(1, 0);
+        host_result.assign(1, 0);
+        boards.assign(MAX_ROWS + 8, 0);
+        occ_bits = 8 * (lib->maxh > 1 ? lib->maxh - 1 : 0);
+        occ_mask = occ_bits ? (occ_bits >= 64 ? ~0ull : (1ull << occ_bits) - 1) : 0;
+        used_mask = lib->used_bits >= 32 ? ~0u : (1u << lib->used_bits) - 1;
+        if (occ_bits + lib->used_bits > 63) throw Error(D4_EINVAL, "state does not fit a 63-bit key");
+        inst_shift = 64;
+        st = na.st;
+        ev0 = na.ev[0];
+        ev1 = na.ev[1];
+        evx0 = na.ev[2];
+        evx1 = na.ev[3];
+        evc0 = na.ev[4];
+        bw_stride = max_h + 2;
+        uint8_t *q = na.ptr;
+        auto carve = [&](auto &buf, uint64_t n) {
+            using T = std::remove_reference_t<decltype(*buf.p)>;
+            const uint64_t bytes = (n * sizeof(T) + 255) & ~255ull;
+            if ((uint64_t)(q - na.ptr) + bytes > kScratchBytes)
+                throw Error(D4_EDEVICE, "internal error: scratch layout exceeds the array");
+            buf.attach(reinterpret_cast<T *>(q), n);
+            q += bytes;
+        };
+        carve(c32, 64);
+        carve(c64, 64);
+        carve(bw, (uint64_t)MAX_ROWS + 2);
+        carve(hgt, 1);
+        carve(dstart, 1);
+        carve(result, 1);
+        carve(result_save, 1);
+        carve(blk, 2ull * COMPACT_MAX_BLOCKS);
+        carve(spill, 2 * kNaSpill);
+        carve(fk, kNaFront);
+        carve(fc, kNaFront);
+        carve(nk, kNaFront);
+        carve(nc, kNaFront);
+        carve(tk, 2 * kNaTable);
+        carve(lv_a, (uint64_t)LVL_FIELDS * kNaLevel);
+        carve(lv_b, (uint64_t)LVL_FIELDS * kNaLevel);
+        carve(ch_k, kNaChild);
+        carve(ch_c, kNaChild);
+        // the scratch keeps the last call's contents: clear what starts at 0
+        // (fill kernels, not memset nodes, in case a caller records them)
+        launch_zero_u32(reinterpret_cast<uint32_t *>(bw.p), 2ull * bw_stride, st);
+        launch_zero_u32(hgt.p, 1, st);
+        launch_zero_u32(dstart.p, 1, st);
+        launch_zero_u32(reinterpret_cast<uint32_t *>(result.p), 2, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
     ~Engine() {
         (void)hipStreamSynchronize(st);
+        if (noalloc) return;  // the stream and events belong to the scratch
         for (hipEvent_t e : {ev0, ev1, evx0, evx1, evc0})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : lv_ev) (void)hipEventDestroy(e);
@@ -516,6 +630,7 @@ struct Engine {
     // parent and would understate the chunk afterwards); an overflow reruns
     // only the chunk that overflowed, with four times the estimate.
     uint64_t level_budget() const {
+        if (noalloc) return 8ull * (2ull * lv_a.cap + 2ull * ch_k.cap);
         const uint64_t cap = mem_cap ? mem_cap / 4 : 16ull << 30;
         return std::max<uint64_t>(std::min<uint64_t>(cap, 16ull << 30), 1ull << 20);
     }
@@ -552,7 +667,7 @@ struct Engine {
     // Terminal records (instances that finish at this step) are counted by
     // one pass of their own, so a rerun chunk adds nothing twice.
     void run_levels(uint64_t tab_m, bool prune) {
-        spill.reserve(2 * kSpillCap);
+        if (!spill.ext) spill.reserve(2 * kSpillCap);
         bool any_term = false;
         for (uint32_t i = 0; i < S; i++)
             if (active[i] && start[i] + heights[i] == g) any_term = true;
@@ -601,6 +716,10 @@ struct Engine {
             const uint64_t chunk = level_chunk(n_front - b0, lv_pp, ch_pp);
             lv_cap = level_cap(chunk, lv_pp);
             ch_cap = level_cap(chunk, ch_pp);
+            if (noalloc) {
+                lv_cap = std::min<uint64_t>(lv_cap, lv_a.cap / LVL_FIELDS);
+                ch_cap = std::min<uint64_t>(ch_cap, ch_k.cap);
+            }
             lv_a.reserve(LVL_FIELDS * lv_cap);
             lv_b.reserve(LVL_FIELDS * lv_cap);
             ch_k.reserve(ch_cap);
@@ -612,13 +731,14 @@ struct Engine {
             a.ch_cap = d.ch_cap = ch_cap;
             a.in_cap = a.out_cap = lv_cap;
             const uint32_t nch = (uint32_t)((n_front - b0 + chunk - 1) / chunk);
-            while (lv_ev.size() < 2 * (size_t)nch + 1) {
+            const bool timed = !noalloc;  // (no events are made for the scratch's chunks)
+            while (timed && lv_ev.size() < 2 * (size_t)nch + 1) {
                 hipEvent_t e;
                 HIPCHK(hipEventCreate(&e));
                 lv_ev.push_back(e);
             }
             launch_reset_levels(c32.p, c64.p, true, st);
-            HIPCHK(hipEventRecord(lv_ev[0], st));
+            if (timed) HIPCHK(hipEventRecord(lv_ev[0], st));
             for (uint32_t j = 0; j < nch; j++) {
                 const uint64_t b = b0 + (uint64_t)j * chunk;
                 if (j) launch_reset_levels(c32.p, c64.p, false, st);
@@ -631,9 +751,9 @@ struct Engine {
                     a.out_st = (L & 1) ? lv_b.p : lv_a.p;
                     launch_level(a, lib->n_cu, st);
                 }
-                HIPCHK(hipEventRecord(lv_ev[2 * j + 1], st));
+                if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 1], st));
                 launch_dedup(d, prune, lib->n_cu, st);
-                HIPCHK(hipEventRecord(lv_ev[2 * j + 2], st));
+                if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 2], st));
             }
             HIPCHK(hipGetLastError());
             uint32_t h32[C32_N];
@@ -642,7 +762,7 @@ struct Engine {
             HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
             sync_host();
             const uint32_t done = std::min<uint32_t>(h32[C32_LVDONE], nch);
-            for (uint32_t j = 0; j < nch; j++) {
+            for (uint32_t j = 0; timed && j < nch; j++) {
                 float lms = 0, dms = 0;
                 HIPCHK(hipEventElapsedTime(&lms, j ? lv_ev[2 * j] : lv_ev[0], lv_ev[2 * j + 1]));
                 HIPCHK(hipEventElapsedTime(&dms, lv_ev[2 * j + 1], lv_ev[2 * j + 2]));
@@ -685,7 +805,10 @@ struct Engine {
             if (lv_over) lv_pp = 4.0 * std::max(lv_pp, (double)lv_cap / (double)c_fail);
             if (h64[C64_CHW] > ch_cap || !lv_over)
                 ch_pp = 4.0 * std::max(ch_pp, (double)ch_cap / (double)c_fail);
-            if (++reruns > 32) throw Error(D4_ERANGE, "a row step's partial rows outgrow every buffer");
+            if (++reruns > 32)
+                throw Error(D4_ERANGE, noalloc ? "allocation-free search: a row step's partial "
+                                                 "rows exceed the scratch"
+                                               : "a row step's partial rows outgrow every buffer");
         }
         if (mx_chunk >= 4096) {
             lv_ratio = (double)mx_lv / (double)mx_chunk;
@@ -874,7 +997,7 @@ struct Engine {
             uint64_t t = est_u;
             if (!exact) t = std::min<uint64_t>(g_c, (uint64_t)(n_front * grow_unique_raw * 1.1) + 4096);
             tab_m = round_slots((uint64_t)(1.4 * (double)t));
-            spill.reserve(2 * kSpillCap);
+            if (!spill.ext) spill.reserve(2 * kSpillCap);
         }
         uint64_t est_e = en ? est_c + 64 : 0;
         if (en && logs.empty()) log0 = g;
@@ -908,6 +1031,8 @@ struct Engine {
                 return false;
             }
             if (tab_m > (1ull << 34)) throw Error(D4_ERANGE, "dedup table exceeds 2^34 slots");
+            // the allocation-free search always uses its whole table slice
+            if (tk.ext) tab_m = tk.cap / 2;
             try {
                 tk.reserve(2 * tab_m);
             } catch (const Error &e) {
@@ -954,6 +1079,9 @@ struct Engine {
             // k_expand's spill buffer was full: rerun with four times the room
             // (the level kernels rerun their own chunks)
             const bool xovf = !ovf && !lv && h32[C32_XOVF] != 0;
+            if (!ovf && !en && h32[C32_SPILL] && tk.ext)
+                throw Error(D4_ERANGE, "allocation-free search: children spilled past the "
+                                       "scratch's dedup table");
             if (!ovf && !en && h32[C32_SPILL]) {
                 // move the table into one large enough for it and the spill
                 const uint32_t ns = h32[C32_SPILL];
@@ -992,6 +1120,9 @@ struct Engine {
             // the step is a pure function of the frontier: rerun it bigger
             stats.retries++;
             copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
+            if (ovf && tk.ext)
+                throw Error(D4_ERANGE, "allocation-free search: the row's states exceed the "
+                                       "scratch's dedup table");
             if (ovf) tab_m *= 4;
             if (xovf) xp_cap *= 4;
             if (eovf) est_e = (uint64_t)h32[C32_EDGES] + 1024;
@@ -1149,6 +1280,10 @@ struct Engine {
     // mixed frontier would otherwise leave each of them to a single lane.
     void launch_ranges(uint64_t tab_m, uint32_t e_cap, bool en, bool prune, bool count_only) {
         if (!n_front) return;
+        if (level_kernel(en, count_only)) {
+            run_levels(tab_m, prune);
+            return;
+        }
         const bool xp = expand_kernel(en, count_only);
         don_a.reserve(xp ? EXPAND_WORDS * xp_cap : 2u << 20);  // 1 Mi donated items per launch
         don_b.reserve(xp ? EXPAND_WORDS * xp_cap : 2u << 20);
@@ -1157,10 +1292,6 @@ struct Engine {
         };
         const uint64_t roots = std::min<uint64_t>(new_roots, n_front);
         const uint64_t old = n_front - roots;
-        if (level_kernel(en, count_only)) {
-            run_levels(tab_m, prune);
-            return;
-        }
         if (xp) {
             auto go = [&](uint64_t b, uint64_t n, uint32_t split) {
                 ExtendArgs a = args(tab_m, 0, b, n, split);
@@ -1619,6 +1750,10 @@ int d4_library_create_ex(const uint64_t *shapes, const uint32_t *counts, uint32_
         HIPCHK(hipGetDeviceProperties(&prop, dev));
         L->n_cu = prop.multiProcessorCount;
         L->lds_limit = (uint32_t)prop.sharedMemPerBlock;
+        // single-library tables go to the device now (the allocation-free
+        // entry point needs them there), and this device's scratch is set up
+        if (L->ncand <= (uint32_t)MAX_CAND) ensure_fast(L.get());
+        noalloc_setup(dev);
         *out = L.release();
     });
 }
@@ -1783,6 +1918,29 @@ int d4_searcher_set_memory(d4_searcher *s, uint64_t bytes) {
 }
 
 void d4_searcher_destroy(d4_searcher *s) { delete s; }
+
+int d4_searcher_count_noalloc(const d4_library *lib, const uint8_t *rows, uint32_t height,
+                              uint32_t flags, uint64_t *count) {
+    return guard([&] {
+        if (!lib || !count || (!rows && height)) throw Error(D4_EINVAL, "null argument");
+        *count = 0;
+        if (height > (uint32_t)MAX_ROWS) throw Error(D4_ERANGE, "board taller than 64 rows");
+        if (!height) {
+            *count = 1;  // the empty board has its one (empty) tiling
+            return;
+        }
+        const int dev = lib->device;
+        if (dev < 0 || dev >= kMaxDevices || !g_noalloc[dev].ready)
+            throw Error(D4_ESTATE, "no allocation-free scratch on the library's device");
+        NoAllocDev &na = g_noalloc[dev];
+        std::lock_guard<std::mutex> lk(na.mu);
+        HIPCHK(hipSetDevice(dev));
+        Engine e(lib, height, flags, na);
+        e.admit(0, rows, height);
+        e.run();
+        *count = e.host_result[0];
+    });
+}
 
 uint32_t d4_searcher_height(const d4_searcher *s) { return s ? s->e->height_of(0) : 0; }
 

@@ -186,21 +186,37 @@ inline uint64_t available_device_bytes() {
     return (uint64_t)free_b + (uint64_t)pool().cached(dev);
 }
 
+// A device buffer from the caching allocator, or attached to a fixed slice
+// of memory the caller owns (the allocation-free entry point's scratch):
+// an attached buffer never allocates or frees, and a request beyond its
+// slice is an error.
 template <class T>
 struct DevBuf {
     T *p = nullptr;
     uint64_t cap = 0;
+    bool ext = false;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
+    void attach(T *q, uint64_t n) {
+        release();
+        p = q;
+        cap = n;
+        ext = true;
+    }
     void release() {
+        if (ext) return;
         if (p) pool().release(p, cap * sizeof(T));
         p = nullptr;
         cap = 0;
     }
     void reserve(uint64_t n) {
         if (n <= cap) return;
+        if (ext)
+            throw Error(D4_ERANGE, "allocation-free search: " + std::to_string(n) +
+                                       " entries needed, the scratch slice holds " +
+                                       std::to_string(cap));
         release();
         uint64_t c = Pool::round(std::max<uint64_t>(n, 64) * sizeof(T)) / sizeof(T);
         p = static_cast<T *>(pool().alloc(c * sizeof(T)));
@@ -209,6 +225,7 @@ struct DevBuf {
     void swap(DevBuf &o) {
         std::swap(p, o.p);
         std::swap(cap, o.cap);
+        std::swap(ext, o.ext);
     }
 };
 

@@ -230,6 +230,7 @@ void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
                            hipStream_t st);
 void launch_reset_work(unsigned long long *c64, hipStream_t st);
+void launch_zero_u32(uint32_t *p, uint64_t n, hipStream_t st);
 // zero the level kernels' per-chunk counters; first: also the overflow flag
 // and the completed-chunk count
 void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st);

@@ -545,6 +545,11 @@ __global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
 
 __global__ void k_reset_work(unsigned long long *c64) { c64[C64_WORK] = 0; }
 
+__global__ __launch_bounds__(256) void k_zero_u32(uint32_t *p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        p[i] = 0;
+}
+
 // Terminal states (every cell of the instance covered) add their count to
 // the instance's result.
 __global__ __launch_bounds__(256) void k_terminal(const uint64_t *key, const uint64_t *cnt,
@@ -871,6 +876,11 @@ void launch_terminal(const uint64_t *key, const uint64_t *cnt, uint64_t n, uint3
     const uint64_t b = std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), (uint64_t)n_cu * 8);
     hipLaunchKernelGGL(k_terminal, dim3((uint32_t)b), dim3(256), 0, st, key, cnt, n, row, start,
                        hgt, inst_shift, result);
+}
+
+void launch_zero_u32(uint32_t *p, uint64_t n, hipStream_t st) {
+    const uint64_t b = std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), 1024);
+    hipLaunchKernelGGL(k_zero_u32, dim3((uint32_t)b), dim3(256), 0, st, p, n);
 }
 
 void launch_reset_work(unsigned long long *c64, hipStream_t st) {

@@ -98,6 +98,8 @@ SIGNATURES = [
     ("d4_library_get_launch", ctypes.c_int, [_P, _U32P, _U32P, _U32P]),
     ("d4_library_set_budget", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_library_set_kernel", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    ("d4_searcher_count_noalloc", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_uint32,
+                                                 ctypes.c_uint32, ctypes.c_void_p]),
     ("d4_library_get_kernel", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     ("d4_library_get_budget", ctypes.c_int, [_P, _U32P]),
     ("d4_library_tune", ctypes.c_int,
@@ -409,6 +411,16 @@ def _flatten(boards):
 
 def _rows_arg(rows: Sequence[int]):
     return _u8arr(list(rows))
+
+
+def count_noalloc(lib: "PieceLibrary", rows: Sequence[int], prune: bool = True) -> int:
+    """Tilings of one board counted in the device's fixed scratch, with no
+    device allocation (d4_searcher_count_noalloc)."""
+    rows = list(rows)
+    out = ctypes.c_uint64()
+    _check(load().d4_searcher_count_noalloc(lib._h, _rows_arg(rows), len(rows),
+                                            0 if prune else FLAG_NO_PRUNE, ctypes.byref(out)))
+    return out.value
 
 
 class CudaSearcher:

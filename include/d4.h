@@ -212,6 +212,23 @@ int d4_searcher_run(d4_searcher *s);
  * other half first (depth first), streaming a frontier larger than memory
  * through it; the count is unchanged.  d4_searcher_step is not bounded. */
 int d4_searcher_set_memory(d4_searcher *s, uint64_t bytes);
+/* Allocation-free count: the number of tilings of one board (as
+ * d4_searcher_create + d4_searcher_run + d4_searcher_count in count mode),
+ * computed without allocating device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Its
+ * buffers are slices of one 128 MiB device array per GPU (set up when the
+ * first library is created on that GPU and held until the process ends):
+ * at most 512 Ki frontier states, a 2 Mi-slot dedup table, 512 Ki partial
+ * rows per level and 2 Mi children per chunk; a search that needs more
+ * fails with D4_ERANGE.  The library's tables must be on the device
+ * (libraries of at most 64 (piece, transform) pairs are uploaded at
+ * creation).  Every call on a GPU shares that GPU's array, which keeps the
+ * last call's contents (each call clears what must start at zero): calls
+ * in one process are serialised by a lock; callers that share the array in
+ * other ways race. */
+int d4_searcher_count_noalloc(const d4_library *lib, const uint8_t *rows,
+                              uint32_t height, uint32_t flags, uint64_t *count);
 /* Number of tilings (valid once height == 0): sum of terminal counts. */
 int d4_searcher_count(const d4_searcher *s, uint64_t *count);
 /* Solutions (D4_MODE_ENUM only, height == 0), in the CPU searcher's DFS
