@@ -174,12 +174,6 @@ static void build_fast(d4_library &L) {
     }
 }
 
-void d4::check_compacted(uint32_t found, uint64_t expected) {
-    if (found != (uint32_t)expected)
-        throw Error(D4_EDEVICE, "internal error: compaction found " + std::to_string(found) +
-                                    " states, the row step counted " +
-                                    std::to_string(expected));
-}
 
 template <class T>
 static void upload(T *&dst, const T *src, size_t n) {
@@ -330,7 +324,7 @@ struct Engine {
     DevBuf<uint64_t> e_desc, e_slot;
     DevBuf<uint64_t> don_a, don_b;  // donated work items (two words each)
     // k_expand's spilled partial-row states (EXPAND_WORDS words each), items per buffer
-    uint64_t xp_cap = 1u << 20;
+    uint64_t xp_cap = 0;  // (set from the library's donation capacity)
     // level kernels: partial-row buffers (two, alternating), the children
     // buffer, and per-parent sizes seen on the last row step
     DevBuf<uint64_t> lv_a, lv_b, ch_k, ch_c;
@@ -385,6 +379,7 @@ struct Engine {
                                        std::to_string(lib->used_bits) + " + instance " +
                                        std::to_string(ib) + " bits)");
         inst_shift = ib ? occ_bits + lib->used_bits : 64;
+        xp_cap = lib->don_items;
         HIPCHK(hipSetDevice(lib->device));
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         for (hipEvent_t *e : {&ev0, &ev1, &evx0, &evx1, &evc0}) HIPCHK(hipEventCreate(e));
@@ -560,7 +555,7 @@ struct Engine {
         a.don_read = C32_DON;
         a.don_write = C32_DON;
         a.don_out = don_a.p;
-        a.don_cap = (uint32_t)(don_a.cap / 2);
+        a.don_cap = (uint32_t)std::min<uint64_t>(don_a.cap / 2, lib->don_items);
         return a;
     }
 
@@ -589,7 +584,7 @@ struct Engine {
             sync_host();
             if (h[C32_BAD_ITEM]) throw Error(D4_EDEVICE, "internal error: bad donated work item");
             if (h[C32_OVERFLOW]) return;  // the step is rerun
-            const uint32_t cap = (uint32_t)(don_a.cap / 2);
+            const uint32_t cap = (uint32_t)std::min<uint64_t>(don_a.cap / 2, lib->don_items);
             const uint32_t n =
                 h[rd] <= cap ? h[rd] : (h[rd + 1] ? h[rd + 1] - 1 : cap);
             if (!n) return;
@@ -602,7 +597,7 @@ struct Engine {
                 a.don_read = rd;
                 a.don_write = wr;
                 a.don_out = don_b.p;
-                a.don_cap = (uint32_t)(don_b.cap / 2);
+                a.don_cap = (uint32_t)std::min<uint64_t>(don_b.cap / 2, lib->don_items);
                 a.budget = round < kBudgetedRounds ? kFollowBudget : 0;
                 launch_extend(a, en, prune, count_only, lib->ext_threads, lib->n_cu, st);
                 HIPCHK(hipGetLastError());
@@ -830,7 +825,7 @@ struct Engine {
             HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
             sync_host();
             if (h[C32_OVERFLOW] || h[C32_XOVF]) return;  // the step is rerun
-            const uint32_t cap = (uint32_t)(don_a.cap / EXPAND_WORDS);
+            const uint32_t cap = (uint32_t)std::min<uint64_t>(don_a.cap / EXPAND_WORDS, xp_cap);
             const uint32_t n = h[rd] <= cap ? h[rd] : (h[rd + 1] ? h[rd + 1] - 1 : cap);
             if (!n) return;
             for (uint32_t q = 0; q < kRoundsPerSync; q++, round++) {
@@ -842,7 +837,7 @@ struct Engine {
                 a.don_read = rd;
                 a.don_write = wr;
                 a.don_out = don_b.p;
-                a.don_cap = (uint32_t)(don_b.cap / EXPAND_WORDS);
+                a.don_cap = (uint32_t)std::min<uint64_t>(don_b.cap / EXPAND_WORDS, xp_cap);
                 a.budget = round < kBudgetedRounds ? lib->xp_budget : 0;
                 launch_expand(a, prune, lib->ext_threads, lib->n_cu, st);
                 HIPCHK(hipGetLastError());
@@ -1066,6 +1061,7 @@ struct Engine {
                 stats.extend_ms += xms;
                 stats.extend_launches++;
             }
+            stats.donation_full += h32[C32_DONFULL];  // (every attempt, reruns included)
             if (n_front) {
                 stats.extend_bytes += 16ull * n_front + 16ull * h64[C64_CHILDREN];
                 // level kernels: the dedup reads every child written (kept or
@@ -1128,6 +1124,7 @@ struct Engine {
             if (eovf) est_e = (uint64_t)h32[C32_EDGES] + 1024;
         }
         const uint64_t U = h64[C64_NEW];
+        check_row_states(U);
         try {
             nk.reserve(U);
             nc.reserve(U);
@@ -1295,7 +1292,7 @@ struct Engine {
         if (xp) {
             auto go = [&](uint64_t b, uint64_t n, uint32_t split) {
                 ExtendArgs a = args(tab_m, 0, b, n, split);
-                a.don_cap = (uint32_t)(don_a.cap / EXPAND_WORDS);
+                a.don_cap = (uint32_t)std::min<uint64_t>(don_a.cap / EXPAND_WORDS, xp_cap);
                 a.budget = lib->xp_budget;
                 launch_expand(a, prune, lib->ext_threads, lib->n_cu, st);
             };
@@ -1331,6 +1328,28 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
         sync_host();
         return h64[C64_CHILDREN];
+    }
+
+    // Per-bucket sizes of the current frontier (bucket = the current row's
+    // open cells), cached until the frontier changes.
+    unsigned long long bucket_n[256] = {}, bucket_next[256] = {};
+    uint64_t bucket_g = ~0ull;
+    DevBuf<unsigned long long> bucket_d;
+    void buckets() {
+        upload_pending();
+        if (bucket_g == ((uint64_t)g << 32 | stats.rows_done)) return;
+        bucket_d.reserve(512);
+        if (n_front)
+            launch_buckets(fk.p, n_front, g, dstart.p, bw.p, bw_stride, hgt.p, inst_shift,
+                           occ_mask, lib->d_comp, bucket_d.p, lib->n_cu, st);
+        else
+            launch_zero_u32(reinterpret_cast<uint32_t *>(bucket_d.p), 2 * 512, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(bucket_n, bucket_d.p, sizeof bucket_n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(bucket_next, bucket_d.p + 256, sizeof bucket_next,
+                              hipMemcpyDeviceToHost, st));
+        sync_host();
+        bucket_g = (uint64_t)g << 32 | stats.rows_done;
     }
 
     uint64_t next_size() {
@@ -1808,6 +1827,14 @@ int d4_library_set_kernel(d4_library *lib, uint32_t kernel, uint32_t tail_rounds
     });
 }
 
+int d4_library_set_donation_capacity(d4_library *lib, uint32_t items) {
+    return guard([&] {
+        if (!lib) throw Error(D4_EINVAL, "null library");
+        if (items < 64) throw Error(D4_EINVAL, "donation capacity below 64 items");
+        lib->don_items = items;
+    });
+}
+
 int d4_library_get_kernel(const d4_library *lib, uint32_t *kernel, uint32_t *tail_rounds) {
     return guard([&] {
         if (!lib) throw Error(D4_EINVAL, "null library");
@@ -1950,6 +1977,28 @@ uint64_t d4_searcher_next_size(const d4_searcher *s) {
     uint64_t v = 0;
     if (!s) return 0;
     if (guard([&] { v = s->e->next_size(); }) != D4_OK) return 0;
+    return v;
+}
+
+uint64_t d4_searcher_size_bucket(const d4_searcher *s, unsigned ea) {
+    uint64_t v = 0;
+    if (!s || ea > 255) return 0;
+    if (guard([&] {
+            s->e->buckets();
+            v = s->e->bucket_n[ea];
+        }) != D4_OK)
+        return 0;
+    return v;
+}
+
+uint64_t d4_searcher_next_size_bucket(const d4_searcher *s, unsigned ea) {
+    uint64_t v = 0;
+    if (!s || ea > 255) return 0;
+    if (guard([&] {
+            s->e->buckets();
+            v = s->e->bucket_next[ea];
+        }) != D4_OK)
+        return 0;
     return v;
 }
 

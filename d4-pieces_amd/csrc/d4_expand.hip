@@ -197,6 +197,7 @@ __global__ __launch_bounds__(THREADS) void k_expand(ExtendArgs a) {
                 if ((uint64_t)b + n <= (uint64_t)a.don_cap) okr = 1;
                 else if (b < a.don_cap) a.c32[a.don_write + 1] = b + 1;
             }
+            if (!okr) atomicAdd(a.c32 + C32_DONFULL, 1u);
         }
         base = __shfl(b, 0);
         return __shfl(okr, 0) != 0;

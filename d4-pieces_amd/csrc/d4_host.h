@@ -45,6 +45,24 @@ inline int guard(F &&f) {
     }
 }
 
+// A row step's unique states before compaction: the compaction writes them
+// at 32-bit offsets, so a row with 2^32 or more is refused before it runs.
+inline void check_row_states(uint64_t unique) {
+    if (unique > 0xffffffffull)
+        throw Error(D4_ERANGE, "a row's unique states (" + std::to_string(unique) +
+                                   ") exceed the compaction's 2^32");
+}
+
+// The number of states a compaction found (c32[C32_OUT]) must equal the
+// step's new-slot count, compared in 64 bits; otherwise the table and the
+// counts disagree.
+inline void check_compacted(uint64_t found, uint64_t expected) {
+    if (found != expected)
+        throw Error(D4_EDEVICE, "internal error: compaction found " + std::to_string(found) +
+                                    " states, the row step counted " +
+                                    std::to_string(expected));
+}
+
 // --------------------------------------------------------------------------
 // geometry (8-wide bitboards, bit = row*8 + col)
 // --------------------------------------------------------------------------

@@ -33,7 +33,8 @@ enum : int {
     C32_SINK = 10,     // terminal states written to the sink (multi-library batches)
     C32_XOVF = 11,     // a partial row found no room (k_expand spill, k_level buffers)
     C32_LVDONE = 12,   // chunks the level kernels' dedup completed this step
-    C32_N = 13,
+    C32_DONFULL = 13,  // donations / spills refused because the buffer was full
+    C32_N = 14,
 };
 enum : int {
     C64_CHILDREN = 0,
@@ -268,15 +269,18 @@ void launch_fill_table4(unsigned long long *tab, uint64_t n, hipStream_t st);
 void launch_compact4(const unsigned long long *tab, uint64_t m, uint64_t *out_k0,
                      uint64_t *out_k1, uint64_t *out_cnt, uint64_t out_cap, uint32_t *c32,
                      uint32_t *blk, hipStream_t st);
-// the number of states a compaction found (c32[C32_OUT]) must equal the
-// step's new-slot count; otherwise the table and the counts disagree
-void check_compacted(uint32_t found, uint64_t expected);
 void launch_extend_multi(const MultiArgs &a, bool prune, bool count_only, uint32_t threads,
                          int n_cu, hipStream_t st);
 void launch_inst_hist(const uint64_t *k0, uint64_t n, uint32_t n_inst, unsigned long long *hist,
                       int n_cu, hipStream_t st);
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
                         uint32_t *e_child, uint32_t n, hipStream_t st);
+// out[0..255] = records per current-row pattern, out[256..511] = their
+// children's row-0 completion bound
+void launch_buckets(const uint64_t *key, uint64_t n, uint32_t row, const uint32_t *start,
+                    const uint64_t *bw, uint32_t bw_stride, const uint32_t *hgt,
+                    uint32_t inst_shift, uint64_t occ_mask, const unsigned long long *comp,
+                    unsigned long long *out, int n_cu, hipStream_t st);
 void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,
                       const uint32_t *start, const uint64_t *bw, uint32_t bw_stride,
                       const uint32_t *hgt, uint32_t inst_shift,

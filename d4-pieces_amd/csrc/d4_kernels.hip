@@ -240,12 +240,14 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
         if (__hip_atomic_load(a.c32 + a.don_write, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
             a.don_cap) {
             no_don = true;
+            atomicAdd(a.c32 + C32_DONFULL, 1u);
             return false;
         }
         const uint32_t b = atomicAdd(a.c32 + a.don_write, n);
         if ((uint64_t)b + n > (uint64_t)a.don_cap) {
             no_don = true;
             if (b < a.don_cap) a.c32[a.don_write + 1] = b + 1;  // the one range crossing the end
+            atomicAdd(a.c32 + C32_DONFULL, 1u);
             return false;
         }
         atomicAdd(a.c64 + C64_DONATED, (unsigned long long)n);
@@ -745,6 +747,32 @@ __global__ __launch_bounds__(256) void k_next_size(
     if (lane_id() == 0 && acc) atomicAdd(c64 + C64_NEXTSIZE, (unsigned long long)acc);
 }
 
+// Per bucket (the current row's open cells, the reference's `pos` / `ea`
+// byte): records, and the row-0 completion bound of their children.
+__global__ __launch_bounds__(256) void k_buckets(
+    const uint64_t *key, uint64_t n, uint32_t row, const uint32_t *start, const uint64_t *bw,
+    uint32_t bw_stride, const uint32_t *hgt, uint32_t inst_shift, uint64_t occ_mask,
+    const unsigned long long *comp, unsigned long long *out) {
+    __shared__ unsigned long long s_n[256], s_c[256];
+    s_n[threadIdx.x] = 0;
+    s_c[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const uint64_t k = key[i];
+        const uint32_t inst = inst_shift < 64 ? (uint32_t)(k >> inst_shift) : 0u;
+        const uint32_t rr = row - start[inst];
+        if (rr >= hgt[inst]) continue;
+        const uint32_t ea = (uint32_t)(bw[(size_t)inst * bw_stride + rr] & ~(k & occ_mask)) & 0xffu;
+        atomicAdd(s_n + ea, 1ull);
+        atomicAdd(s_c + ea, comp[ea]);
+    }
+    __syncthreads();
+    if (s_n[threadIdx.x]) {
+        atomicAdd(out + threadIdx.x, s_n[threadIdx.x]);
+        atomicAdd(out + 256 + threadIdx.x, s_c[threadIdx.x]);
+    }
+}
+
 // ---- first solutions on the device (enum mode) ---------------------------
 // live[u] = the state's instance is in the group (terminal states).
 __global__ __launch_bounds__(256) void k_fs_terminal(const uint64_t *keys, uint64_t n,
@@ -1018,6 +1046,16 @@ void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
     if (!blocks) blocks = 1;
     hipLaunchKernelGGL(k_remap_edges, dim3(blocks), dim3(256), 0, st, e_slot, slot_uid,
                        e_child, n);
+}
+
+void launch_buckets(const uint64_t *key, uint64_t n, uint32_t row, const uint32_t *start,
+                    const uint64_t *bw, uint32_t bw_stride, const uint32_t *hgt,
+                    uint32_t inst_shift, uint64_t occ_mask, const unsigned long long *comp,
+                    unsigned long long *out, int n_cu, hipStream_t st) {
+    launch_zero_u32(reinterpret_cast<uint32_t *>(out), 2 * 512, st);
+    const uint64_t b = std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), (uint64_t)n_cu * 4);
+    hipLaunchKernelGGL(k_buckets, dim3((uint32_t)b), dim3(256), 0, st, key, n, row, start, bw,
+                       bw_stride, hgt, inst_shift, occ_mask, comp, out);
 }
 
 void launch_next_size(const uint64_t *key, uint32_t n, uint32_t row,

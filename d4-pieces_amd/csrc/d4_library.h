@@ -33,6 +33,7 @@ struct d4_library {
     // D4_KERNEL_DFS (k_extend: one depth-first search per lane)
     uint32_t xp_kernel = D4_KERNEL_LEVEL;
     uint32_t xp_budget = 256;     // k_expand: tail rounds before a wave's snapshots move on
+    uint32_t don_items = 1u << 20;  // donated work items / spilled partial rows per buffer
     // single-library tables (at most MAX_CAND candidates: one u64 mask),
     // built and uploaded on first use by an engine
     std::mutex fast_mu;

@@ -132,6 +132,7 @@ struct d4_mbatch {
     // DFS steps per work item of the main launch before its lane hands the
     // rest of its search tree to follow-up launches (0: never)
     uint32_t budget_steps = 4096;
+    uint32_t don_items = 1u << 20;  // donated work items per buffer
     uint64_t n_front = 0;
     double grow_unique = 1.0, grow_children = 1.0;
     // unique children per record of the last step and of the step before
@@ -221,7 +222,7 @@ struct d4_mbatch {
         a.c64 = c64.p;
         a.budget = budget_steps;
         a.don_out = don_a.p;
-        a.don_cap = (uint32_t)(don_a.cap / 2);
+        a.don_cap = (uint32_t)std::min<uint64_t>(don_a.cap / 2, don_items);
         a.don_read = C32_DON;
         a.don_write = C32_DON;
         if (keep_terminals) {
@@ -254,7 +255,7 @@ struct d4_mbatch {
             copy(h, c32.p, sizeof h, hipMemcpyDeviceToHost);
             if (h[C32_BAD_ITEM]) throw Error(D4_EDEVICE, "internal error: bad donated work item");
             if (h[C32_OVERFLOW]) return;  // the step is rerun
-            const uint32_t cap = (uint32_t)(don_a.cap / 2);
+            const uint32_t cap = (uint32_t)std::min<uint64_t>(don_a.cap / 2, don_items);
             const uint32_t n = h[rd] <= cap ? h[rd] : (h[rd + 1] ? h[rd + 1] - 1 : cap);
             if (!n) return;
             for (uint32_t q = 0; q < kRoundsPerSync; q++, round++) {
@@ -266,7 +267,7 @@ struct d4_mbatch {
                 a.don_read = rd;
                 a.don_write = wr;
                 a.don_out = don_b.p;
-                a.don_cap = (uint32_t)(don_b.cap / 2);
+                a.don_cap = (uint32_t)std::min<uint64_t>(don_b.cap / 2, don_items);
                 a.budget = round < kBudgetedRounds ? kFollowBudget : 0;
                 launch_extend_multi(a, prune, count_only, threads, n_cu, st);
                 HIPCHK(hipGetLastError());
@@ -409,6 +410,7 @@ struct d4_mbatch {
             HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
+            stats.donation_full += h32[C32_DONFULL];  // (every attempt, reruns included)
             float xms = 0;
             if (n_front) {
                 HIPCHK(hipEventElapsedTime(&xms, evx0, evx1));
@@ -437,6 +439,7 @@ struct d4_mbatch {
             tab_m *= 4;
         }
         const uint64_t U = h64[C64_NEW];
+        check_row_states(U);
         try {
             n0.reserve(U);
             n1.reserve(U);
@@ -738,6 +741,14 @@ int d4_mbatch_set_budget_steps(d4_mbatch *b, uint32_t steps) {
     return guard([&] {
         if (!b) throw Error(D4_EINVAL, "null batch");
         b->budget_steps = steps;
+    });
+}
+
+int d4_mbatch_set_donation_capacity(d4_mbatch *b, uint32_t items) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        if (items < 64) throw Error(D4_EINVAL, "donation capacity below 64 items");
+        b->don_items = items;
     });
 }
 

@@ -188,12 +188,14 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
         if (__hip_atomic_load(a.c32 + a.don_write, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
             a.don_cap) {
             no_don = true;
+            atomicAdd(a.c32 + C32_DONFULL, 1u);
             return false;
         }
         const uint32_t b = atomicAdd(a.c32 + a.don_write, n);
         if ((uint64_t)b + n > (uint64_t)a.don_cap) {
             no_don = true;
             if (b < a.don_cap) a.c32[a.don_write + 1] = b + 1;  // the one range crossing the end
+            atomicAdd(a.c32 + C32_DONFULL, 1u);
             return false;
         }
         atomicAdd(a.c64 + C64_DONATED, (unsigned long long)n);

@@ -67,6 +67,7 @@ class Stats(ctypes.Structure):
         ("dedup_launches", ctypes.c_uint64),
         ("dedup_bytes", ctypes.c_uint64),
         ("chunk_reruns", ctypes.c_uint64),
+        ("donation_full", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -98,6 +99,10 @@ SIGNATURES = [
     ("d4_library_get_launch", ctypes.c_int, [_P, _U32P, _U32P, _U32P]),
     ("d4_library_set_budget", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_library_set_kernel", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
+    ("d4_library_set_donation_capacity", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_searcher_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
+    ("d4_searcher_next_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
+    ("d4_mbatch_set_donation_capacity", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_searcher_count_noalloc", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_uint32,
                                                  ctypes.c_uint32, ctypes.c_void_p]),
     ("d4_library_get_kernel", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
@@ -368,6 +373,10 @@ class PieceLibrary:
         partial rows move to a follow-up launch (0: never)."""
         _check(load().d4_library_set_kernel(self._h, kernel, tail_rounds))
 
+    def set_donation_capacity(self, items: int) -> None:
+        """Items per donation buffer (default 1 Mi)."""
+        _check(load().d4_library_set_donation_capacity(self._h, items))
+
     def get_kernel(self) -> Tuple[int, int]:
         k, t = ctypes.c_uint32(), ctypes.c_uint32()
         _check(load().d4_library_get_kernel(self._h, ctypes.byref(k), ctypes.byref(t)))
@@ -450,11 +459,18 @@ class CudaSearcher:
     def get_height(self) -> int:
         return load().d4_searcher_height(self._h)
 
-    def size(self) -> int:
-        return load().d4_searcher_size(self._h)
+    def size(self, i: Optional[int] = None) -> int:
+        """Records in the frontier (size()), or in bucket i, the records whose
+        current row has open cells i (size(unsigned i))."""
+        if i is None:
+            return load().d4_searcher_size(self._h)
+        return load().d4_searcher_size_bucket(self._h, i)
 
-    def next_size(self) -> int:
-        return load().d4_searcher_next_size(self._h)
+    def next_size(self, i: Optional[int] = None) -> int:
+        """Upper bound on the next step's children, in total or for bucket i."""
+        if i is None:
+            return load().d4_searcher_next_size(self._h)
+        return load().d4_searcher_next_size_bucket(self._h, i)
 
     def set_memory(self, nbytes: int) -> None:
         """Count mode: device bytes run() may use (frontier parts beyond it
@@ -589,6 +605,10 @@ class MultiBatch:
         """DFS steps per work item of a row's main launch before a lane donates
         the rest (default 4096; 0: never)."""
         _check(load().d4_mbatch_set_budget_steps(self._h, steps))
+
+    def set_donation_capacity(self, items: int) -> None:
+        """Items per donation buffer (default 1 Mi)."""
+        _check(load().d4_mbatch_set_donation_capacity(self._h, items))
 
     def set_budget(self, max_states: int) -> None:
         """States one instance may hold in a frontier before it is dropped

@@ -84,6 +84,7 @@ typedef struct d4_stats {
     uint64_t dedup_launches;
     uint64_t dedup_bytes;      /* 16 per child read + 16 per new state      */
     uint64_t chunk_reruns;     /* level-kernel chunks rerun with larger buffers */
+    uint64_t donation_full;    /* donations / spills refused by a full buffer */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -165,6 +166,10 @@ int d4_library_get_budget(const d4_library *lib, uint32_t *steps);
 enum { D4_KERNEL_DFS = 0, D4_KERNEL_EXPAND = 1, D4_KERNEL_LEVEL = 2 };
 int d4_library_set_kernel(d4_library *lib, uint32_t kernel, uint32_t tail_rounds);
 int d4_library_get_kernel(const d4_library *lib, uint32_t *kernel, uint32_t *tail_rounds);
+/* Items per donation buffer of the depth-first and snapshot kernels (default
+ * 1 Mi; at least 64).  A lane or wave that finds the buffer full keeps its
+ * work (counted in d4_stats.donation_full); small values exercise that path. */
+int d4_library_set_donation_capacity(d4_library *lib, uint32_t items);
 int d4_library_get_launch(const d4_library *lib, uint32_t *fit_rows,
                           uint32_t *threads, uint32_t *refill);
 /* Measured autotuner, replacing the analytic cost model of
@@ -201,6 +206,14 @@ uint64_t d4_searcher_size(const d4_searcher *s);
  * the number of ways to fill the record's current row ignoring lower rows and
  * piece counts). */
 uint64_t d4_searcher_next_size(const d4_searcher *s);
+/* size(unsigned i), src/searcher_cuda.h:16-21: records of the current
+ * frontier whose current row has open cells `ea` (the reference's WL bucket
+ * `pos`, 0..255; bit c = column c open).  0 for ea > 255 or on error. */
+uint64_t d4_searcher_size_bucket(const d4_searcher *s, unsigned ea);
+/* next_size(unsigned i), src/searcher_cuda.cu:84-90: the bound of
+ * d4_searcher_next_size restricted to bucket `ea`.  Summed over the 256
+ * buckets both equal d4_searcher_size / d4_searcher_next_size. */
+uint64_t d4_searcher_next_size_bucket(const d4_searcher *s, unsigned ea);
 /* search_GPU(), src/searcher_cuda.cu:32-82: advance the frontier by one row
  * (extend + prune + dedup + compact on the device). */
 int d4_searcher_step(d4_searcher *s);
@@ -301,6 +314,9 @@ int d4_mbatch_tune(d4_mbatch *b, uint32_t sample, uint32_t *threads,
  * hands the rest of its search tree to follow-up launches, as
  * d4_library_set_budget (default 4096; 0: never). */
 int d4_mbatch_set_budget_steps(d4_mbatch *b, uint32_t steps);
+/* Items per donation buffer (default 1 Mi; at least 64), as
+ * d4_library_set_donation_capacity. */
+int d4_mbatch_set_donation_capacity(d4_mbatch *b, uint32_t items);
 /* Device bytes the frontier, its dedup table and parked frontier parts may
  * take (default: 80% of the free device memory when the batch is created).
  * A row step that would exceed it parks half of the frontier and finishes

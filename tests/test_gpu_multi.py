@@ -218,3 +218,30 @@ def test_uneven_instances(gpu, uneven, uneven_fixture):
         assert len(both) >= n // 2
         bad = [(i, got[i], got_r[i]) for i in both if got_r[i] != got[i]]
         assert not bad, bad[:10]
+
+
+def test_donation_buffer_full(gpu, uneven, uneven_fixture):
+    """Regression for the donation-buffer-full branch of k_extend_multi (the
+    path whose counter once wrapped past 2^32): a 64-item buffer and an
+    8-step budget make lanes find it full many times; counts still equal the
+    oracle's where instances finish, and the refusals are counted."""
+    n, budget = 256, 1 << 20
+    insts = uneven[:n]
+    cache, libs = {}, []
+    for shapes, counts, _, w in insts:
+        key = (tuple(shapes), tuple(counts), w)
+        if key not in cache:
+            cache[key] = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+        libs.append(cache[key])
+    b = gpu.MultiBatch(libs, [x[2] for x in insts])
+    b.set_budget(budget)
+    b.set_budget_steps(8)
+    b.set_donation_capacity(64)
+    b.run()
+    got, fin = b.counts(), b.instances()[1]
+    st = b.stats()
+    assert st["donation_full"] > 0 and st["donated"] > 0
+    pinned = {int(k): v for k, v in uneven_fixture["counts"].items() if int(k) < n}
+    checked = [i for i in pinned if fin[i]]
+    assert len(checked) > 100
+    assert all(got[i] == pinned[i] for i in checked)

@@ -307,3 +307,28 @@ def test_row_step_kernels_agree(gpu, p5, oracle, kernel):
     s.set_memory(4 << 20)
     s.run()
     assert s.count() == 695086
+
+
+@pytest.mark.parametrize("kernel", ["dfs", "expand"])
+def test_donation_buffer_full(gpu, p5, kernel):
+    """Regression for the buffer-full branches: the depth-first kernel's
+    donation buffer (the path whose counter once wrapped past 2^32) and the
+    snapshot kernel's spill buffer shrunk to 64 items, with budgets that make
+    lanes and waves hand work over constantly; counts equal the fixture and
+    the refusals are counted (the snapshot kernel reruns a step whose spill
+    found no room, with a larger buffer)."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    if kernel == "dfs":
+        lib.set_kernel(gpu.KERNEL_DFS, 0)
+        lib.set_budget(8)
+    else:
+        lib.set_kernel(gpu.KERNEL_EXPAND, 1)
+    lib.set_donation_capacity(64)
+    b = gpu.Batch(lib, fx["boards"][:64])
+    b.run()
+    assert b.counts() == fx["counts"][:64]
+    assert b.stats()["donation_full"] > 0

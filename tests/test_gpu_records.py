@@ -74,3 +74,35 @@ def test_export_matches_row_bfs(gpu, oracle, width, rows, upto):
         assert got == want, r
         if r < upto:
             s.search_GPU()
+
+
+def test_size_buckets(gpu):
+    """size(i) / next_size(i) (src/searcher_cuda.h:16-27): the records of
+    bucket i are the exported RX records with ea == i (the reference's WL
+    bucket `pos`), the buckets add up to size() and next_size(), and each
+    bucket's bound is the row-0 completion count of i times its records."""
+    import collections
+    p5 = gpu.free_polyominoes(5)
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    rows = [0xFF, 0xFF, 0xFF, 0xE7, 0xE7, 0xFF, 0xFF, 0xFF]
+    s = gpu.CudaSearcher(lib, rows=rows)
+    per_pos = {}
+    steps = 0
+    while s.get_height():
+        hist = collections.Counter(ea for ea, _, _ in s.export())
+        sizes = [s.size(i) for i in range(256)]
+        nexts = [s.next_size(i) for i in range(256)]
+        assert sizes == [hist.get(i, 0) for i in range(256)]
+        assert sum(sizes) == s.size()
+        assert sum(nexts) == s.next_size()
+        for i in range(256):
+            if sizes[i]:
+                per = nexts[i] // sizes[i]
+                assert nexts[i] == per * sizes[i]
+                assert per_pos.setdefault(i, per) == per  # depends on i alone
+            else:
+                assert nexts[i] == 0
+        assert s.size(256) == 0
+        s.search_GPU()
+        steps += 1
+    assert steps == 8 and per_pos[0xFF] > 1

@@ -39,3 +39,22 @@ def test_ledger_naming_polyominoes_sanitized(tmp_path):
     assert "polyominoes n=8 sym=0xff: 369" in r.stdout
     assert "polyominoes n=8 sym=0x69: 704" in r.stdout
     assert "polyominoes n=8 sym=0x1: 2725" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_row_state_guards(tmp_path):
+    """The 2^32 unique-state guard before compaction and the 64-bit
+    compaction check (tests/host/guard_check.cpp), built host-only."""
+    exe = str(tmp_path / "guard_check")
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=all",
+           "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(ROOT, "d4-pieces_amd", "csrc"),
+           "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+           os.path.join(HOST, "guard_check.cpp"), os.path.join(HOST, "set_error_stub.cpp"),
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
+    b = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert b.returncode == 0, b.stderr[-4000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "fails 0" in r.stdout

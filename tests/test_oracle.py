@@ -312,3 +312,53 @@ def test_record_codec_layout(oracle):
     r, ea = oracle.assemble_R(4, 0x00_112233_44 << 8, 3, [7, 8, 9])
     assert r[0] == (0x112233 & 0xFFFFFF) | (3 << 24)
     assert r[2] == 0xFF090807 and r[1] == 0xFFFFFFFF
+
+
+def test_step_matrix_maps_transform_back(oracle):
+    """make_step's matrices (src/Piece.cpp:41-54, restated as
+    or_step_matrix) against the transforms themselves (or_transform, pinned
+    by the reference's compiled Shape::transforms): for transform t, the
+    matrix M maps every cell (x, y) = (column, row) of the transformed shape
+    back onto the original shape's cells, up to translation (input = M .
+    output).  Shapes with no symmetry make a wrong sign, a swapped entry or
+    a matrix listed under the wrong t fail, and the product's table
+    (d4pieces) must equal the pinned one."""
+    import ctypes
+
+    def cells(s):
+        return [(b % 8, b // 8) for b in range(64) if s >> b & 1]
+
+    def norm(pts):
+        mx = min(x for x, _ in pts)
+        my = min(y for _, y in pts)
+        return sorted((x - mx, y - my) for x, y in pts)
+
+    # every free polyomino of 4-6 cells with no symmetry (8 distinct
+    # transforms)
+    asym = [s for n in (4, 5, 6) for s in oracle.enum_free(n)
+            if len({oracle.transform(s, t) for t in range(8)}) == 8]
+    assert len(asym) == 1 + 5 + 20  # L; F, L, N, P, Y; 20 hexominoes
+    for s in asym:
+        orig = norm(cells(s))
+        for t in range(8):
+            m = (ctypes.c_int * 4)()
+            oracle.L.or_step_matrix(t, m)
+            a, b, c, d = m
+            assert sorted(map(abs, (a, b, c, d))) == [0, 0, 1, 1]
+            back = [(a * x + b * y, c * x + d * y) for x, y in cells(oracle.transform(s, t))]
+            assert norm(back) == orig, (hex(s), t, tuple(m))
+            # no other listed matrix maps transform t back
+            for u in range(8):
+                if u == t:
+                    continue
+                mu = (ctypes.c_int * 4)()
+                oracle.L.or_step_matrix(u, mu)
+                au, bu, cu, du = mu
+                wrong = [(au * x + bu * y, cu * x + du * y)
+                         for x, y in cells(oracle.transform(s, t))]
+                assert norm(wrong) != orig
+    import d4pieces
+    for t in range(8):
+        m = (ctypes.c_int * 4)()
+        oracle.L.or_step_matrix(t, m)
+        assert tuple(d4pieces._STEP_MATRIX[t]) == tuple(m)
