@@ -11,11 +11,21 @@ frontier).  Reported:
   solutions_per_s  tilings found per second
   plain            the same boards one CudaSearcher at a time (a sample),
                    records/s, solutions/s and per-board latency
-  roofline         the extend kernel's algorithmic HBM bytes / its average
-                   launch time (HIP events on the engine's stream), against
-                   MI355X's 8 TB/s
+  roofline         the dominant kernel's algorithmic HBM bytes / its time
+                   (HIP events on the engine's stream) against MI355X's
+                   8 TB/s, with the extend, dedup and compaction kernels each
+                   under roofline.kernels
+  served           Poisson request streams at 8 rates (70% first-solution)
+  uneven           4096 puzzles, each with its own piece set and width, in
+                   one multi-library batch, then the ones a state budget
+                   dropped in a second pass with a larger budget
+  large            12 pentominoes x2 on 8x15 (one bounded-memory search),
+                   and a sample of its frontier checked state by state
+                   against the oracle
   cpu_baseline     the oracle (CPU restatement of the reference backtracker)
-                   on a bounded sample of the same boards, P host threads
+                   on the host's cores: the 256 boards, 6x10 single-threaded,
+                   the request stream offline, and the large search's rate
+                   on its sampled states
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -47,63 +57,123 @@ def percentile(v, q):
     return v[lo] + (v[hi] - v[lo]) * (k - lo)
 
 
-def cpu_baseline(boards, gpu_counts, shapes, seconds, threads):
-    """Oracle solve_count on the first boards of the workload until about
-    `seconds` of wall time; returns the baseline object and a parity flag."""
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def load_oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from conftest import Oracle
-    path = os.path.join(ROOT, "oracle", "liboracle.so")
-    oracle = Oracle(path)
+    return Oracle(os.path.join(ROOT, "oracle", "liboracle.so"))
+
+
+# --------------------------------------------------------------------------
+# CPU baseline (the oracle, test infrastructure only: it is the checker and
+# the reported baseline, never part of the GPU measurement)
+# --------------------------------------------------------------------------
+
+def cpu_boards(oracle, boards, gpu_counts, shapes, threads, seconds):
+    """solve_count on the boards, one board per task on `threads` threads,
+    until all are done or `seconds` pass."""
     done, sols, ok = 0, 0, True
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
         while done < len(boards) and time.perf_counter() - t0 < seconds:
-            chunk = boards[done:done + threads]
+            chunk = boards[done:done + 4 * threads]
             res = list(ex.map(lambda b: oracle.count(shapes, [1] * 12, b), chunk))
             for j, c in enumerate(res):
                 ok &= c == gpu_counts[done + j]
             sols += sum(res)
             done += len(chunk)
-            log(f"cpu baseline {done} boards {time.perf_counter() - t0:.1f} s")
+            log(f"cpu baseline: {done}/{len(boards)} boards {time.perf_counter() - t0:.1f} s")
     dt = time.perf_counter() - t0
-    return {
-        "value": sols / dt,
-        "unit": "solutions/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"first {done} of {len(boards)} boards, one board per thread, "
-                  f"{dt:.1f} s wall, {done / dt:.3f} boards/s",
-    }, ok, done
+    return {"boards": done, "of": len(boards), "wall_s": dt, "solutions_per_s": sols / dt,
+            "boards_per_s": done / dt, "threads": threads, "parity_equal": ok}
 
+
+def cpu_plain_6x10(oracle, workloads):
+    shapes, counts, rows, _ = workloads.plain_6x10()
+    t = time.perf_counter()
+    n = oracle.count(shapes, counts, rows)
+    t_count = time.perf_counter() - t
+    t = time.perf_counter()
+    sols = oracle.solve(shapes, counts, rows)
+    t_set = time.perf_counter() - t
+    return {"count": n, "solutions": len(sols), "count_s": t_count, "full_set_s": t_set,
+            "threads": 1}
+
+
+def cpu_requests(oracle, workloads, n_req, threads):
+    """The request stream offline: every request solved by the oracle (first
+    solution or count), one request per task."""
+    shapes = workloads.pentominoes()
+    reqs = workloads.request_stream(n_req, rate=1.0, seed=4)
+
+    def one(r):
+        _, rows, single = r
+        if single:
+            return len(oracle.solve(shapes, [1] * 12, rows, single=True))
+        return oracle.count(shapes, [1] * 12, rows)
+
+    t = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(one, reqs))
+    dt = time.perf_counter() - t
+    return {"requests": n_req, "wall_s": dt, "requests_per_s": n_req / dt, "threads": threads}
+
+
+# --------------------------------------------------------------------------
+# serving
+# --------------------------------------------------------------------------
 
 def serve_run(d4, lib, reqs, slots):
     """Open-loop replay of a request stream (arrival times in seconds from
-    the start) through one Server; returns latencies (ms), wall time and the
-    results."""
+    the start) through one Server; returns per-request latencies (ms), wall
+    time, results, server stats and the host time spent in server steps."""
     srv = d4.Server(lib, slots=slots, max_height=8)
     n = len(reqs)
-    t_arr, sub, res, lat, last_log = {}, 0, {}, [], -1
+    t_arr, sub, res, lat, last_log = {}, 0, {}, {}, -1
+    rid_of = {}
+    step_s, steps = 0.0, 0
     t0 = time.perf_counter()
     while len(res) < n:
         now = time.perf_counter() - t0
         while sub < n and reqs[sub][0] <= now:
             rid = srv.submit(reqs[sub][1], reqs[sub][2])
             t_arr[rid] = reqs[sub][0]
+            rid_of[rid] = sub
             sub += 1
-        fl, wt = srv.step() if (sub > len(res)) else (0, 0)
+        if sub > len(res):
+            ts = time.perf_counter()
+            srv.step()
+            step_s += time.perf_counter() - ts
+            steps += 1
         while True:
             r = srv.poll()
             if r is None:
                 break
             res[r[0]] = (r[1], r[2])
-            lat.append((time.perf_counter() - t0 - t_arr[r[0]]) * 1e3)
-        if len(res) and len(res) % 200 == 0 and len(res) != last_log:
+            lat[rid_of[r[0]]] = (time.perf_counter() - t0 - t_arr[r[0]]) * 1e3
+        if len(res) and len(res) % 250 == 0 and len(res) != last_log:
             last_log = len(res)
             log(f"  served {len(res)}/{n}")
         if sub == len(res) and sub < n:
             # idle until the next arrival
             time.sleep(max(0.0, reqs[sub][0] - (time.perf_counter() - t0)))
-    return lat, time.perf_counter() - t0, res, srv.stats()
+    return lat, time.perf_counter() - t0, res, srv.stats(), step_s, steps
 
 
 def serve_section(d4, lib, workloads, rates, n_req, slots):
@@ -111,7 +181,6 @@ def serve_section(d4, lib, workloads, rates, n_req, slots):
     same requests solved one per CudaSearcher, one after another."""
     out = {"rates": []}
     base_reqs = workloads.request_stream(n_req, rate=1.0, seed=4)
-    # one-request-per-launch baseline on a sample
     ns = min(64, n_req)
     log(f"serving: one-request-per-launch baseline on {ns} requests")
     t = time.perf_counter()
@@ -125,24 +194,49 @@ def serve_section(d4, lib, workloads, rates, n_req, slots):
     plain_rps = ns / (time.perf_counter() - t)
     s = None
     out["plain_requests_per_s"] = plain_rps
+    bands = ((30, 39), (40, 49), (50, 64))
     for rate in rates:
         reqs = workloads.request_stream(n_req, rate=rate, seed=4)
-        lat, wall, res, st = serve_run(d4, lib, reqs, slots)
+        lat, wall, res, st, step_s, steps = serve_run(d4, lib, reqs, slots)
         tilings = sum(c for c, _ in res.values())
         found = sum(1 for c, sol in res.values() if c)
+        by_kind = {}
+        for kind, want in (("first_solution", True), ("all_solutions", False)):
+            v = [lat[i] for i in range(n_req) if reqs[i][2] == want]
+            by_kind[kind] = {"requests": len(v), "p50_ms": percentile(v, 0.5),
+                             "p99_ms": percentile(v, 0.99)}
+        by_band = {}
+        for lo, hi in bands:
+            v = [lat[i] for i in range(n_req)
+                 if lo <= sum(bin(r).count("1") for r in reqs[i][1]) <= hi]
+            by_band[f"{lo}-{hi}_free_cells"] = {"requests": len(v), "p50_ms": percentile(v, 0.5),
+                                                "p99_ms": percentile(v, 0.99)}
+        lat_all = [lat[i] for i in range(n_req)]
+        gpu_ms = st["all"]["gpu_ms"] + st["one"]["gpu_ms"]
         row = {"rate": rate, "requests": n_req, "wall_s": wall,
                "requests_per_s": n_req / wall,
+               "keeps_up": n_req / wall >= 0.95 * rate,
                "solutions_per_s": tilings / wall,
-               "latency_ms_p50": percentile(lat, 0.5),
-               "latency_ms_p99": percentile(lat, 0.99),
+               "latency_ms_p50": percentile(lat_all, 0.5),
+               "latency_ms_p99": percentile(lat_all, 0.99),
+               "by_kind": by_kind, "by_free_cells": by_band,
+               "server_steps": steps,
+               "host_ms_per_step": step_s / max(steps, 1) * 1e3,
+               # the two engines' device time (they overlap on the GPU)
+               "gpu_ms_per_step": gpu_ms / max(steps, 1),
                "solvable": found,
                "served_vs_plain": (n_req / wall) / plain_rps,
                "mem_waits": st["mem_waits"]}
-        log(f"serve rate {rate}/s: {row['requests_per_s']:.0f} req/s, p50 "
-            f"{row['latency_ms_p50']:.1f} ms, p99 {row['latency_ms_p99']:.1f} ms")
+        log(f"serve rate {rate:g}/s: {row['requests_per_s']:.0f} req/s, p50 "
+            f"{row['latency_ms_p50']:.1f} ms, p99 {row['latency_ms_p99']:.1f} ms, "
+            f"host {row['host_ms_per_step']:.2f} / gpu {row['gpu_ms_per_step']:.2f} ms per step")
         out["rates"].append(row)
     return out
 
+
+# --------------------------------------------------------------------------
+# uneven instances (multi-library batch)
+# --------------------------------------------------------------------------
 
 def multi_libraries(d4, insts):
     cache, libs = {}, []
@@ -154,11 +248,14 @@ def multi_libraries(d4, insts):
     return libs
 
 
-def uneven_section(d4, workloads, n, budget, plain_sample, tune=True):
+def uneven_section(d4, workloads, n, budget, budget2, plain_sample, tune=True,
+                   pass2_seconds=60.0):
     """The uneven puzzles (own piece set and width per instance) in one
-    multi-library batch with a per-instance state budget, against the same
-    instances one per batch; parity against the oracle's counts of the
-    instances it finished (tests/golden/uneven_4096_seed3.json)."""
+    multi-library batch with a per-instance state budget; the instances that
+    budget drops run again in a second batch with budget2; the rest is
+    reported with the frontier size at which it stopped.  Against the same
+    instances one per batch, and the oracle's counts of the instances it
+    finished (tests/golden/uneven_4096_seed3.json)."""
     insts = workloads.uneven_instances(4096, seed=3)[:n]
     t = time.perf_counter()
     libs = multi_libraries(d4, insts)
@@ -184,17 +281,57 @@ def uneven_section(d4, workloads, n, budget, plain_sample, tune=True):
     counts = b.counts()
     peak, fin = b.instances()
     st = b.stats()
+    b = None
     out = {"instances": n, "state_budget": budget, "finished": sum(fin), "wall_s": wall,
            "library_build_s": t_lib, "batch_create_s": t_create,
            "launch": tuned,
            "records_per_s": st["records_extended"] / wall,
-           "solutions_per_s": sum(counts) / wall,
+           "solutions_per_s_finished": sum(counts) / wall,
            "records_extended": st["records_extended"], "peak_frontier": st["peak_frontier"],
            "splits": st["splits"],
            "extend_GBs": st["extend_bytes"] / (st["extend_ms"] / 1e3) / 1e9
            if st["extend_ms"] else None}
     log(f"uneven batched: {wall:.1f} s, {sum(fin)}/{n} finished, "
         f"{out['records_per_s'] / 1e6:.0f} M records/s")
+    # second pass: the dropped instances with a larger budget, in batches of
+    # 128 (one progress line each) until pass2_seconds have gone by
+    rest = [i for i in range(n) if not fin[i]]
+    if rest and budget2 > budget:
+        t = time.perf_counter()
+        ran, fin2_n, rec2, splits2 = 0, 0, 0, 0
+        for c0 in range(0, len(rest), 128):
+            if time.perf_counter() - t > pass2_seconds:
+                break
+            part = rest[c0:c0 + 128]
+            b2 = d4.MultiBatch([libs[i] for i in part], [insts[i][2] for i in part])
+            b2.set_budget(budget2)
+            b2.set_launch(tuned["threads"], tuned["refill"])
+            b2.run()
+            c2 = b2.counts()
+            peak2, fin2 = b2.instances()
+            st2 = b2.stats()
+            b2 = None
+            for j, i in enumerate(part):
+                if fin2[j]:
+                    counts[i], fin[i] = c2[j], True
+                peak[i] = max(peak[i], peak2[j])
+            ran += len(part)
+            fin2_n += sum(fin2)
+            rec2 += st2["records_extended"]
+            splits2 += st2["splits"]
+            log(f"uneven second pass: {ran}/{len(rest)} instances, {fin2_n} finished, "
+                f"{time.perf_counter() - t:.1f} s")
+        d4.synchronize()
+        wall2 = time.perf_counter() - t
+        out["second_pass"] = {"instances": len(rest), "ran": ran, "state_budget": budget2,
+                              "finished": fin2_n, "wall_s": wall2, "records_extended": rec2,
+                              "splits": splits2, "batch": 128}
+    out["finished_total"] = sum(fin)
+    # (states_at_stop: the largest frontier share the instance reached in the
+    # pass that dropped it; an instance the time bound left unrun keeps the
+    # first pass's figure)
+    out["unfinished"] = [{"instance": i, "states_at_stop": peak[i]}
+                         for i in range(n) if not fin[i]]
     fx_path = os.path.join(ROOT, "tests", "golden", "uneven_4096_seed3.json")
     if os.path.exists(fx_path):
         pinned = {int(k): v for k, v in json.load(open(fx_path))["counts"].items()
@@ -220,15 +357,101 @@ def uneven_section(d4, workloads, n, budget, plain_sample, tune=True):
     return out
 
 
+# --------------------------------------------------------------------------
+# large single enumeration
+# --------------------------------------------------------------------------
+
+def sub_board(rows, g, key, maxh, layout, counts):
+    """The board and piece counts left to a frontier state of step g."""
+    occ_bits = 8 * (maxh - 1)
+    occ = key & ((1 << occ_bits) - 1)
+    sub = list(rows[g:])
+    for j in range(min(maxh - 1, len(sub))):
+        sub[j] &= ~(occ >> (8 * j)) & 0xFF
+    left = []
+    for (off, w), c in zip(layout, counts):
+        used = (key >> (occ_bits + off)) & ((1 << w) - 1) if w else 0
+        left.append(c - used)
+    return sub, left
+
+
+def large_section(d4, workloads, oracle, sample_height, n_sample, cpu_seconds):
+    """12 pentominoes x2 on 8x15: one count-mode search under the default
+    memory cap (80% of free device memory; a row that would not fit parks
+    half of its frontier), timed; a sample of its frontier at sample_height
+    rows left is counted state by state on the GPU and by the oracle."""
+    shapes, counts, rows, w = workloads.large_8x15()
+    lib = d4.PieceLibrary(shapes, counts, 0xFF, w)
+    # one line per row on stderr from the engine (progress of a long search)
+    os.environ["D4_TRACE"] = "1"
+    s = d4.CudaSearcher(lib, rows=rows)
+    del os.environ["D4_TRACE"]
+    s.set_sample(sample_height, n_sample)
+    d4.synchronize()
+    t = time.perf_counter()
+    s.run()
+    d4.synchronize()
+    wall = time.perf_counter() - t
+    total = s.count()
+    st = s.stats()
+    sample = s.sample()
+    s = None
+    out = {"workload": "12 free pentominoes x2 (24 pieces, 120 cells) on 8 wide x 15 rows",
+           "count": total, "wall_s": wall, "records_extended": st["records_extended"],
+           "records_per_s": st["records_extended"] / wall, "solutions_per_s": total / wall,
+           "peak_frontier": st["peak_frontier"], "splits": st["splits"],
+           "rows": st["rows_done"], "table_growths": st["rehashes"]}
+    log(f"large 8x15: {total} tilings in {wall:.1f} s, peak frontier {st['peak_frontier']}, "
+        f"{st['splits']} splits")
+    # state by state: the GPU and the oracle on what each sampled state leaves
+    maxh = lib.max_height
+    layout = lib.counter_layout()
+    g = len(rows) - sample_height
+    checked, equal, cpu_s, cpu_tilings, gpu_s = 0, True, 0.0, 0, 0.0
+    for key, cnt in sample:
+        if cpu_s > cpu_seconds:
+            break
+        sub, left = sub_board(rows, g, key, maxh, layout, counts)
+        t = time.perf_counter()
+        want = oracle.count(shapes, left, sub)
+        cpu_s += time.perf_counter() - t
+        t = time.perf_counter()
+        sl = d4.PieceLibrary(shapes, left, 0xFF, w)
+        ss = d4.CudaSearcher(sl, rows=sub)
+        ss.run()
+        got = ss.count()
+        gpu_s += time.perf_counter() - t
+        equal &= got == want
+        checked += 1
+        cpu_tilings += cnt * want
+    out["sample"] = {"height": sample_height, "states": checked, "gpu_equals_oracle": equal,
+                     "oracle_s": cpu_s, "tilings_through_sample": cpu_tilings}
+    log(f"large sample: {checked} states at height {sample_height}, GPU == oracle: {equal}, "
+        f"oracle {cpu_s:.1f} s")
+    return out, (cpu_tilings / cpu_s if cpu_s else None)
+
+
+# --------------------------------------------------------------------------
+# main
+# --------------------------------------------------------------------------
+
+def kernel_roofline(name, bytes_, ms, traffic=None):
+    gbs = bytes_ / (ms / 1e3) / 1e9 if ms else 0.0
+    return {"kernel": name, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+            "ms_per_row_step": None, "bytes_per_row_step": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=16)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="wall-time bound of the oracle on the 256 boards")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every host core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tune", action="store_true",
@@ -236,8 +459,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="library defaults, no --launch; no autotune of the uneven batch")
     ap.add_argument("--launch", default="3,512,16,1536",
-                    help="fit_rows,threads,refill,budget of the extend kernel (the "
-                         "autotuner's choice for this workload on MI355X)")
+                    help="fit_rows,threads,refill,budget of the depth-first kernel (the "
+                         "autotuner's choice for this workload on MI355X; the level "
+                         "kernel uses the fit rows)")
     ap.add_argument("--tune-sample", type=int, default=256)
     ap.add_argument("--kernel", choices=("level", "expand", "dfs"), default="level",
                     help="count-mode row-step kernel: level-synchronous (level), partial-row "
@@ -246,7 +470,7 @@ def main():
     ap.add_argument("--tail-rounds", type=int, default=256,
                     help="expand kernel: rounds after a launch's records run out before "
                          "a wave's partial rows move to a follow-up launch")
-    ap.add_argument("--serve-rates", default="100,20000",
+    ap.add_argument("--serve-rates", default="100,200,500,1000,2000,5000,10000,20000",
                     help="comma-separated arrival rates (requests/s); empty to skip")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-slots", type=int, default=1024)
@@ -256,7 +480,12 @@ def main():
     ap.add_argument("--uneven", type=int, default=4096,
                     help="uneven puzzles in the multi-library section (0: skip)")
     ap.add_argument("--uneven-budget", type=int, default=1 << 20)
+    ap.add_argument("--uneven-budget2", type=int, default=1 << 22,
+                    help="state budget of the second pass over the dropped instances")
     ap.add_argument("--uneven-plain", type=int, default=32)
+    ap.add_argument("--large", type=int, default=1, help="run the 8x15 search (0: skip)")
+    ap.add_argument("--large-sample-height", type=int, default=4)
+    ap.add_argument("--large-sample", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the "
@@ -269,8 +498,7 @@ def main():
 
     if d4.device_count() < 1:
         raise SystemExit("no HIP device visible")
-    # device-wide barrier through the library's own HIP runtime (torch ships
-    # a separate runtime copy, which is not needed here)
+    # device-wide barrier through the library's own HIP runtime
     sync = d4.synchronize
 
     shapes = workloads.pentominoes()
@@ -285,8 +513,6 @@ def main():
         lib.set_launch(f, t, r)
         lib.set_budget(b)
     elif args.tune:
-        # measured autotune of the extend launch shape on a sample of the
-        # workload (outside the timed region)
         t = time.perf_counter()
         tuned = lib.tune(boards[:args.tune_sample])
         log(f"autotune {tuned} in {time.perf_counter() - t:.1f} s")
@@ -299,7 +525,7 @@ def main():
     for i in range(args.warmup):
         t = time.perf_counter()
         step()
-        log(f"warmup step {i} {time.perf_counter() - t:.2f} s")
+        log(f"warmup step {i} {time.perf_counter() - t:.3f} s")
     sync()
     t0 = time.perf_counter()
     stats, counts = [], None
@@ -307,7 +533,7 @@ def main():
         t = time.perf_counter()
         b = step()
         stats.append(b.stats())
-        log(f"timed step {i} {time.perf_counter() - t:.2f} s")
+        log(f"timed step {i} {time.perf_counter() - t:.3f} s")
         c = b.counts()
         if counts is not None and c != counts:
             raise SystemExit("batched counts differ between steps")
@@ -316,21 +542,32 @@ def main():
     dt = time.perf_counter() - t0
     # each engine holds a HIP stream, and streams share the process's few
     # hardware queues: release a section's engines before the next section
-    # (the server's two engines only overlap on queues of their own)
     b = None
 
-    rec = sum(s["records_extended"] for s in stats)
+    def tot(k):
+        return sum(s[k] for s in stats)
+
+    rec = tot("records_extended")
     n_sol = sum(counts) * args.steps
-    ext_ms = sum(s["extend_ms"] for s in stats)
-    ext_launch = sum(s["extend_launches"] for s in stats)
-    ext_bytes = sum(s["extend_bytes"] for s in stats)
-    achieved = ext_bytes / (ext_ms / 1e3) / 1e9 if ext_ms else 0.0
-    traffic, dispatches = None, None
-    tpath = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    rows = tot("rows_done")
+    prof = {}
+    tpath = os.path.join(ROOT, "profiles", "row_step_traffic.json")
     if os.path.exists(tpath):
         prof = json.load(open(tpath))
-        traffic = prof.get("hbm_bytes_per_row_step")
-        dispatches = prof.get("dispatches_per_row_step")
+    kernels = {}
+    for name, bkey, mkey in (("extend", "extend_bytes", "extend_ms"),
+                             ("dedup", "dedup_bytes", "dedup_ms"),
+                             ("compact", "compact_bytes", "compact_ms")):
+        if tot(mkey):
+            k = kernel_roofline(name, tot(bkey), tot(mkey),
+                                prof.get(name, {}).get("hbm_bytes_per_row_step"))
+            k["ms_per_row_step"] = tot(mkey) / rows
+            k["bytes_per_row_step"] = tot(bkey) / rows
+            kernels[name] = k
+    dominant = max(kernels, key=lambda k: kernels[k]["ms_per_row_step"])
+    roof = dict(kernels[dominant])
+    roof["per"] = "row step"
+    roof["kernels"] = kernels
 
     out = {
         "metric": METRIC,
@@ -352,6 +589,7 @@ def main():
             "board": "8x8",
             "pieces": "12 free pentominoes x1, D4",
             "mode": "count",
+            "kernel": args.kernel,
             "parallelism": "instances batched in one frontier, 1 GPU",
             "launch": dict(zip(("fit_rows", "threads", "refill"), lib.get_launch()),
                            budget=lib.get_budget(), autotuned=tuned is not None),
@@ -359,32 +597,25 @@ def main():
         "solutions_per_s": n_sol / dt,
         "solutions_per_step": sum(counts),
         "records_per_step": rec // args.steps,
-        "children_per_step": sum(s["children"] for s in stats) // args.steps,
+        "children_per_step": tot("children") // args.steps,
         "peak_frontier": max(s["peak_frontier"] for s in stats),
-        "table_reruns_per_step": sum(s["retries"] for s in stats) / args.steps,
-        "donated_items_per_step": sum(s["donated"] for s in stats) / args.steps,
-        "table_growths_per_step": sum(s["rehashes"] for s in stats) / args.steps,
+        "table_reruns_per_step": tot("retries") / args.steps,
+        "chunk_reruns_per_step": tot("chunk_reruns") / args.steps,
+        "host_syncs_per_row_step": tot("host_syncs") / rows,
+        "table_growths_per_step": tot("rehashes") / args.steps,
         "latency_ms": {"batched_p50": dt / args.steps * 1e3,
                        "batched_p99": dt / args.steps * 1e3},
-        "roofline": {
-            "kernel": "k_extend",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            # one unit = one row step's extend: the main launch plus the
-            # load-balance follow-up launches (HIP events around them)
-            "per": "row step",
-            "avg_launch_ms": ext_ms / max(ext_launch, 1),
-            "bytes_per_launch": ext_bytes / max(ext_launch, 1),
-            "dispatches_per_row_step": dispatches,
-            "compact_GBs": (sum(s["compact_bytes"] for s in stats) /
-                            (sum(s["compact_ms"] for s in stats) / 1e3) / 1e9)
-            if sum(s["compact_ms"] for s in stats) else None,
-        },
+        "roofline": roof,
     }
+
+    # the large search runs while the device's memory is still unfragmented
+    oracle = None if args.no_cpu else load_oracle()
+    large_rate = None
+    if args.large:
+        if oracle is None:
+            oracle = load_oracle()
+        out["large"], large_rate = large_section(d4, workloads, oracle, args.large_sample_height,
+                                                 args.large_sample, 30.0)
 
     if not args.no_plain:
         n = min(args.plain_sample, len(boards))
@@ -401,9 +632,9 @@ def main():
                 raise SystemExit(f"plain count differs from batched on board {i}")
             prec += s.stats()["records_extended"]
             psol += c
-            log(f"plain board {i} {lat[-1]:.1f} ms")
         pdt = time.perf_counter() - t1
         s = None
+        log(f"plain: {n} boards one per CudaSearcher, p50 {percentile(lat, 0.5):.1f} ms")
         out["plain"] = {
             "boards": n,
             "records_per_s": prec / pdt,
@@ -420,6 +651,7 @@ def main():
         # the server's frontiers are small (a few requests in flight): it gets
         # a library with its own measured launch shape
         slib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
+        slib.set_kernel(kernel, args.tail_rounds)
         f, t, r, bud = (int(x) for x in args.serve_launch.split(","))
         slib.set_launch(f, t, r)
         slib.set_budget(bud)
@@ -427,21 +659,57 @@ def main():
                                       args.serve_slots)
         out["served"]["launch"] = dict(zip(("fit_rows", "threads", "refill", "budget"),
                                            (f, t, r, bud)))
-        best = max(out["served"]["rates"], key=lambda r: r["requests_per_s"])
-        out["latency_ms"]["served_p50"] = best["latency_ms_p50"]
-        out["latency_ms"]["served_p99"] = best["latency_ms_p99"]
-        out["served_vs_plain"] = best["served_vs_plain"]
+        rows_ = out["served"]["rates"]
+        keep = [x for x in rows_ if x["keeps_up"]] or rows_[:1]
+        head = max(keep, key=lambda x: x["rate"])  # highest rate served as it arrives
+        sat = max(rows_, key=lambda x: x["requests_per_s"])
+        out["served"]["headline_rate"] = head["rate"]
+        out["latency_ms"]["served_rate"] = head["rate"]
+        out["latency_ms"]["served_p50"] = head["latency_ms_p50"]
+        out["latency_ms"]["served_p99"] = head["latency_ms_p99"]
+        out["served_vs_plain"] = sat["served_vs_plain"]
+        out["served_saturated_requests_per_s"] = sat["requests_per_s"]
 
     if args.uneven:
         out["uneven"] = uneven_section(d4, workloads, args.uneven, args.uneven_budget,
-                                       args.uneven_plain, tune=not args.no_tune)
+                                       args.uneven_budget2, args.uneven_plain,
+                                       tune=not args.no_tune)
+
 
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        base, ok, n_checked = cpu_baseline(boards, counts, shapes, args.cpu_seconds, threads)
-        out["cpu_baseline"] = base
-        out["parity"] = {"boards_checked_against_oracle": n_checked, "equal": ok}
-        if not ok:
+        threads = args.cpu_threads or host_cores()
+        c2 = cpu_boards(oracle, boards, counts, shapes, threads, args.cpu_seconds)
+        per = {"seeded_8x8_256": c2, "plain_6x10": cpu_plain_6x10(oracle, workloads)}
+        log(f"cpu baseline: 6x10 count {per['plain_6x10']['count_s']:.2f} s, "
+            f"full set {per['plain_6x10']['full_set_s']:.2f} s (1 thread)")
+        if args.serve_rates:
+            per["requests_offline"] = cpu_requests(oracle, workloads, args.serve_requests,
+                                                   threads)
+            log(f"cpu baseline: {args.serve_requests} requests offline in "
+                f"{per['requests_offline']['wall_s']:.1f} s")
+        if large_rate:
+            tot_l = out["large"]["count"]
+            per["large_8x15"] = {
+                "method": "oracle on the sampled frontier states' remaining boards, one "
+                          "thread; rate = tilings through the sample (state count x its "
+                          "completions) / oracle time",
+                "tilings_per_s": large_rate,
+                "estimated_single_thread_s": tot_l / large_rate,
+                "states": out["large"]["sample"]["states"], "threads": 1}
+        out["cpu_baseline"] = {
+            "value": c2["solutions_per_s"],
+            "unit": "solutions/s",
+            "cores": threads,
+            "cpu_model": cpu_model(),
+            "kind": "port",
+            "sample": f"{c2['boards']} of {len(boards)} boards of the workload, one board per "
+                      f"task on {threads} threads, {c2['wall_s']:.1f} s wall",
+            "per_config": per,
+        }
+        out["vs_cpu"] = out["solutions_per_s"] / c2["solutions_per_s"]
+        out["parity"] = {"boards_checked_against_oracle": c2["boards"],
+                         "equal": c2["parity_equal"]}
+        if not c2["parity_equal"]:
             print(json.dumps(out))
             raise SystemExit("GPU counts differ from the oracle")
     print(json.dumps(out))

@@ -913,6 +913,9 @@ struct Engine {
         upload_pending();
         if (!n_active) throw Error(D4_ESTATE, "no instance left to search");
         row_step();
+        g++;
+        take_sample();
+        g--;
         bool any = false;
         for (uint32_t i = 0; i < S; i++)
             if (active[i] && start[i] + heights[i] == g) any = true;
@@ -1247,6 +1250,9 @@ struct Engine {
         for (;;) {
             while (n_front && g <= gend) {
                 if (row_step(true)) {
+                    g++;
+                    take_sample();
+                    g--;
                     if (trace)
                         fprintf(stderr, "run step %u: frontier %llu, %zu parts parked\n", g,
                                 (unsigned long long)n_front, parked.size());
@@ -1328,6 +1334,24 @@ struct Engine {
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
         sync_host();
         return h64[C64_CHILDREN];
+    }
+
+    // Sample of the states of a single instance's frontier at a given
+    // height (rows left), taken as the search passes it (bounded runs
+    // included): the first states of the frontier, up to sample_cap.
+    uint32_t sample_h = 0;
+    uint64_t sample_cap = 0;
+    std::vector<uint64_t> sample_keys, sample_cnt;
+    void take_sample() {
+        if (!sample_h || S != 1 || !active[0] || sample_keys.size() >= sample_cap) return;
+        if (heights[0] - std::min(heights[0], g - start[0]) != sample_h) return;
+        const uint64_t k = std::min<uint64_t>(sample_cap - sample_keys.size(), n_front);
+        if (!k) return;
+        const size_t at = sample_keys.size();
+        sample_keys.resize(at + k);
+        sample_cnt.resize(at + k);
+        copy(sample_keys.data() + at, fk.p, 8 * k, hipMemcpyDeviceToHost);
+        copy(sample_cnt.data() + at, fc.p, 8 * k, hipMemcpyDeviceToHost);
     }
 
     // Per-bucket sizes of the current frontier (bucket = the current row's
@@ -1978,6 +2002,44 @@ uint64_t d4_searcher_next_size(const d4_searcher *s) {
     if (!s) return 0;
     if (guard([&] { v = s->e->next_size(); }) != D4_OK) return 0;
     return v;
+}
+
+int d4_searcher_set_sample(d4_searcher *s, uint32_t height, uint64_t cap) {
+    return guard([&] {
+        if (!s) throw Error(D4_EINVAL, "null searcher");
+        s->e->sample_h = height;
+        s->e->sample_cap = cap;
+        s->e->sample_keys.clear();
+        s->e->sample_cnt.clear();
+        s->e->take_sample();
+    });
+}
+
+int d4_searcher_sample(const d4_searcher *s, uint64_t *keys, uint64_t *counts, uint64_t cap,
+                       uint64_t *n) {
+    return guard([&] {
+        if (!s || !n) throw Error(D4_EINVAL, "null argument");
+        const Engine &e = *s->e;
+        *n = e.sample_keys.size();
+        for (uint64_t k = 0; k < *n && k < cap; k++) {
+            if (keys) keys[k] = e.sample_keys[k];
+            if (counts) counts[k] = e.sample_cnt[k];
+        }
+    });
+}
+
+int d4_searcher_frontier(const d4_searcher *s, uint64_t first, uint64_t cap, uint64_t *keys,
+                         uint64_t *counts, uint64_t *n) {
+    return guard([&] {
+        if (!s || !n) throw Error(D4_EINVAL, "null argument");
+        Engine &e = *s->e;
+        e.upload_pending();
+        *n = e.n_front;
+        if (first >= e.n_front || !cap) return;
+        const uint64_t k = std::min<uint64_t>(cap, e.n_front - first);
+        if (keys) e.copy(keys, e.fk.p + first, 8 * k, hipMemcpyDeviceToHost);
+        if (counts) e.copy(counts, e.fc.p + first, 8 * k, hipMemcpyDeviceToHost);
+    });
 }
 
 uint64_t d4_searcher_size_bucket(const d4_searcher *s, unsigned ea) {

@@ -101,6 +101,11 @@ SIGNATURES = [
     ("d4_library_set_kernel", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     ("d4_library_set_donation_capacity", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_searcher_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
+    ("d4_searcher_set_sample", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint64]),
+    ("d4_searcher_sample", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                          ctypes.c_void_p]),
+    ("d4_searcher_frontier", ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("d4_searcher_next_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
     ("d4_mbatch_set_donation_capacity", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_searcher_count_noalloc", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_uint32,
@@ -458,6 +463,29 @@ class CudaSearcher:
 
     def get_height(self) -> int:
         return load().d4_searcher_height(self._h)
+
+    def set_sample(self, height: int, cap: int) -> None:
+        """Keep up to cap frontier states at `height` rows left as the search
+        passes that height (sample() returns them)."""
+        _check(load().d4_searcher_set_sample(self._h, height, cap))
+
+    def sample(self) -> List[Tuple[int, int]]:
+        n = ctypes.c_uint64()
+        _check(load().d4_searcher_sample(self._h, None, None, 0, ctypes.byref(n)))
+        keys = (ctypes.c_uint64 * max(n.value, 1))()
+        cnts = (ctypes.c_uint64 * max(n.value, 1))()
+        _check(load().d4_searcher_sample(self._h, keys, cnts, n.value, ctypes.byref(n)))
+        return list(zip(keys[:n.value], cnts[:n.value]))
+
+    def frontier(self, first: int = 0, cap: int = 1 << 16) -> List[Tuple[int, int]]:
+        """(key, tiling count) of frontier states first .. first + cap - 1."""
+        n = ctypes.c_uint64()
+        _check(load().d4_searcher_frontier(self._h, first, 0, None, None, ctypes.byref(n)))
+        k = max(0, min(cap, n.value - first))
+        keys = (ctypes.c_uint64 * max(k, 1))()
+        cnts = (ctypes.c_uint64 * max(k, 1))()
+        _check(load().d4_searcher_frontier(self._h, first, k, keys, cnts, ctypes.byref(n)))
+        return list(zip(keys[:k], cnts[:k]))
 
     def size(self, i: Optional[int] = None) -> int:
         """Records in the frontier (size()), or in bucket i, the records whose

@@ -214,6 +214,23 @@ uint64_t d4_searcher_size_bucket(const d4_searcher *s, unsigned ea);
  * d4_searcher_next_size restricted to bucket `ea`.  Summed over the 256
  * buckets both equal d4_searcher_size / d4_searcher_next_size. */
 uint64_t d4_searcher_next_size_bucket(const d4_searcher *s, unsigned ea);
+/* The current frontier's states as the engine keeps them: keys[k] = the
+ * occupancy by placed pieces of the current row and the rows below it (8
+ * bits per row, max_height - 1 rows, d4_library_info) followed by the
+ * library's packed use counters (d4_library_counters), counts[k] = the
+ * state's tiling count; states first .. first + cap - 1 in the frontier's
+ * order (dedup-table order, which does not follow the states' values).
+ * *n = frontier size.  keys / counts may be NULL. */
+int d4_searcher_frontier(const d4_searcher *s, uint64_t first, uint64_t cap, uint64_t *keys,
+                         uint64_t *counts, uint64_t *n);
+/* Keep up to cap states (keys and counts as d4_searcher_frontier) of the
+ * frontier the search reaches at `height` rows left, taken as d4_searcher_run
+ * (bounded runs included) or d4_searcher_step pass it; 0 turns it off.  With
+ * a split frontier the states come from the parts that reach the height
+ * first.  d4_searcher_sample returns them (*n = states kept). */
+int d4_searcher_set_sample(d4_searcher *s, uint32_t height, uint64_t cap);
+int d4_searcher_sample(const d4_searcher *s, uint64_t *keys, uint64_t *counts, uint64_t cap,
+                       uint64_t *n);
 /* search_GPU(), src/searcher_cuda.cu:32-82: advance the frontier by one row
  * (extend + prune + dedup + compact on the device). */
 int d4_searcher_step(d4_searcher *s);

@@ -332,3 +332,31 @@ def test_donation_buffer_full(gpu, p5, kernel):
     b.run()
     assert b.counts() == fx["counts"][:64]
     assert b.stats()["donation_full"] > 0
+
+
+def test_large_sample_states_match_oracle(gpu, p5, oracle):
+    """The doubled-pentomino library on 8x5 (subset tilings allowed): the
+    frontier states the search passes at 3 rows left (taken from a bounded
+    run under a memory cap that splits it) each leave a sub-board whose
+    count, on the GPU and by the oracle, agrees; the sampled key layout is
+    the documented one."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    rows = rect(8, 5)
+    lib = gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8)
+    s = gpu.CudaSearcher(lib, rows=rows)
+    s.set_memory(2 << 20)
+    s.set_sample(3, 24)
+    s.run()
+    assert s.count() == 695086
+    sample = s.sample()
+    assert len(sample) == 24 and s.stats()["splits"] > 0
+    layout = lib.counter_layout()
+    for key, cnt in sample[:12]:
+        sub, left = bench.sub_board(rows, len(rows) - 3, key, lib.max_height, layout, [2] * 12)
+        assert len(sub) == 3 and all(0 <= c <= 2 for c in left) and cnt > 0
+        want = oracle.count(p5, left, sub)
+        sl = gpu.PieceLibrary(p5, left, 0xFF, 8)
+        assert gpu_count(gpu, sl, sub) == want
