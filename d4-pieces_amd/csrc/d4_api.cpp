@@ -696,9 +696,17 @@ struct Engine {
         d.seed_mask = lib->maxh >= 2 ? ~((1ull << (8 * (lib->maxh - 1))) - 1ull) : ~0ull;
         d.gcd = lib->gcd;
         d.min_size = lib->min_size;
+        d.closed_rows = lib->maxh >= 1 ? lib->maxh - 1 : 0;
+        d.n_inst = S;
+        d.hgt = hgt.p;
         d.tab = tk.p;
         d.tab_size = tab_m;
-        d.max_probe = 256;
+        // linear probing at the loads the tables run at (up to about 75%
+        // when a step is sized from the growth trend) has clusters of a few
+        // hundred slots among billions of keys: a long probe limit keeps
+        // those keys out of the spill list, whose contents cost a whole
+        // table growth
+        d.max_probe = 4096;
         d.spill = spill.p;
         d.spill_cap = (uint32_t)std::min<uint64_t>(spill.cap / 2, kSpillCap);
         d.c32 = c32.p;

@@ -93,6 +93,40 @@ __device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g,
     return true;
 }
 
+// viable() for pieces at most 5 rows tall: closed regions can only lie in
+// the first c = max_height - 1 <= 4 window rows (32 bits), and a cell there
+// reaches the seed rows exactly when it reaches a cell of row c - 1 that is
+// open above an open cell of row c (a path into the seed rows enters them
+// through row c), so the flood fill runs on 32-bit words.  Same result as
+// viable(E, rows >= c, g, min_size).
+__device__ __forceinline__ uint32_t grow4_32(uint32_t x) {
+    return x | (x << 8) | (x >> 8) | ((x << 1) & ~0x01010101u) | ((x >> 1) & ~0x80808080u);
+}
+__device__ __forceinline__ bool viable32(uint64_t E, uint32_t c, uint32_t g, uint32_t min_size) {
+    if (c == 0) return true;
+    const uint32_t Ec = (uint32_t)E & (c >= 4 ? 0xffffffffu : (1u << (8 * c)) - 1u);
+    const uint32_t below = (uint32_t)(E >> (8 * c)) & 0xffu;
+    uint32_t x = Ec & (below << (8 * (c - 1)));
+    for (;;) {
+        const uint32_t y = grow4_32(x) & Ec;
+        if (y == x) break;
+        x = y;
+    }
+    uint32_t closed = Ec & ~x;
+    while (closed) {
+        uint32_t comp = closed & (~closed + 1u);
+        for (;;) {
+            const uint32_t y = grow4_32(comp) & closed;
+            if (y == comp) break;
+            comp = y;
+        }
+        const uint32_t area = (uint32_t)__builtin_popcount(comp);
+        if (area < min_size || area % g) return false;
+        closed &= ~comp;
+    }
+    return true;
+}
+
 // The dedup table interleaves each slot's key and count (16 B per slot:
 // tab_key[2h], tab_cnt[2h] with tab_cnt = tab_key + 1), so an insert touches
 // one cache line.  A table has any number m of slots (the host sizes it to

@@ -153,6 +153,9 @@ struct DedupArgs {
     uint64_t occ_mask;
     uint64_t seed_mask;
     uint32_t gcd, min_size;
+    uint32_t closed_rows;    // max_height - 1: window rows that can hold closed regions
+    uint32_t n_inst;         // instances (the children's board windows go to LDS when few)
+    const uint32_t *hgt;
     unsigned long long *tab;  // {key, count} slots
     uint64_t tab_size;
     uint32_t max_probe;

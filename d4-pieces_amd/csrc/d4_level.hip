@@ -289,8 +289,46 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     for (uint64_t j = wc.pos + lane; j < wc.end; j += 64) a.ch_key[j] = EMPTY_KEY;
 }
 
-template <bool PRUNE>
+// Board windows of the children's row, per instance, staged in LDS when the
+// batch has at most this many instances.
+constexpr uint32_t DEDUP_LDS_INST = 1024;
+constexpr uint32_t DEDUP_UNROLL = 4;  // children per thread per pass (their probes overlap)
+
+// Merges one child into the table, its first probe already loaded (cur):
+// returns 1 new, 0 merged, -1 no slot within the probe limit.  A slot
+// holding the key only takes the count (one atomic add, no CAS); an empty
+// slot is claimed by CAS; keys only ever change EMPTY -> key, so a load that
+// saw another key lets the probe move on.
+__device__ __forceinline__ int dedup_insert(const DedupArgs &a, uint64_t key, uint64_t cnt,
+                                            uint64_t h, uint64_t cur) {
+    for (uint32_t p = 0;;) {
+        if (cur == EMPTY_KEY) {
+            cur = atomicCAS(a.tab + 2 * h, (unsigned long long)EMPTY_KEY, (unsigned long long)key);
+            if (cur == EMPTY_KEY) {
+                atomicAdd(a.tab + 2 * h + 1, (unsigned long long)cnt);
+                return 1;
+            }
+        }
+        if (cur == key) {
+            atomicAdd(a.tab + 2 * h + 1, (unsigned long long)cnt);
+            return 0;
+        }
+        if (++p >= a.max_probe) return -1;
+        h = next_slot(h, a.tab_size);
+        cur = __hip_atomic_load(a.tab + 2 * h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <bool PRUNE, bool LDSW>
 __global__ __launch_bounds__(256) void k_dedup(DedupArgs a) {
+    __shared__ uint64_t s_win[LDSW ? DEDUP_LDS_INST : 1];
+    if (LDSW) {
+        for (uint32_t i = threadIdx.x; i < a.n_inst; i += 256) {
+            const uint32_t rr = a.row + 1 - a.start[i];
+            s_win[i] = rr < a.bw_stride ? a.bw[(size_t)i * a.bw_stride + rr] : 0ull;
+        }
+        __syncthreads();
+    }
     uint64_t n = a.c64[C64_CHW];
     if (n > a.ch_cap) n = a.ch_cap;
     // a chunk whose level kernels overflowed is rerun by the host: nothing
@@ -299,43 +337,59 @@ __global__ __launch_bounds__(256) void k_dedup(DedupArgs a) {
     else if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.c32 + C32_LVDONE, 1u);
     uint64_t kept = 0, fresh = 0, pruned = 0;
     bool failed = false;
-    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-        const uint64_t key = a.ch_key[i];
-        if (key == EMPTY_KEY) continue;
-        const uint64_t cnt = a.ch_cnt[i];
-        if (PRUNE) {
-            const uint32_t inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
-            const uint32_t rr = a.row + 1 - a.start[inst];
-            const uint64_t E = a.bw[(size_t)inst * a.bw_stride + rr] & ~(key & a.occ_mask);
-            if (!viable(E, a.seed_mask, a.gcd, a.min_size)) {
-                pruned++;
-                continue;
+    const uint64_t G = (uint64_t)gridDim.x * 256;
+    for (uint64_t i0 = blockIdx.x * 256ull + threadIdx.x; i0 < n; i0 += DEDUP_UNROLL * G) {
+        uint64_t key[DEDUP_UNROLL], cnt[DEDUP_UNROLL], h[DEDUP_UNROLL], cur[DEDUP_UNROLL];
+        bool live[DEDUP_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < DEDUP_UNROLL; u++) {
+            const uint64_t i = i0 + u * G;
+            key[u] = i < n ? a.ch_key[i] : EMPTY_KEY;
+            cnt[u] = i < n ? a.ch_cnt[i] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < DEDUP_UNROLL; u++) {
+            live[u] = key[u] != EMPTY_KEY;
+            if (PRUNE && live[u]) {
+                const uint32_t inst = a.inst_shift < 64 ? (uint32_t)(key[u] >> a.inst_shift) : 0u;
+                uint64_t w;
+                if (LDSW) {
+                    w = s_win[inst];
+                } else {
+                    const uint32_t rr = a.row + 1 - a.start[inst];
+                    w = a.bw[(size_t)inst * a.bw_stride + rr];
+                }
+                const uint64_t E = w & ~(key[u] & a.occ_mask);
+                const bool ok = a.closed_rows <= 4 ? viable32(E, a.closed_rows, a.gcd, a.min_size)
+                                                   : viable(E, a.seed_mask, a.gcd, a.min_size);
+                if (!ok) {
+                    pruned++;
+                    live[u] = false;
+                }
+            }
+            if (live[u]) {
+                h[u] = home_slot(key[u], a.tab_size);
+                cur[u] = __hip_atomic_load(a.tab + 2 * h[u], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        kept++;
-        uint64_t h = home_slot(key, a.tab_size);
-        int r = -1;
-        for (uint32_t p = 0; p < a.max_probe; p++) {
-            const uint64_t cur =
-                atomicCAS(a.tab + 2 * h, (unsigned long long)EMPTY_KEY, (unsigned long long)key);
-            if (cur == EMPTY_KEY || cur == key) {
-                atomicAdd(a.tab + 2 * h + 1, (unsigned long long)cnt);
-                r = cur == EMPTY_KEY ? 1 : 0;
-                break;
+#pragma unroll
+        for (uint32_t u = 0; u < DEDUP_UNROLL; u++) {
+            if (!live[u]) continue;
+            kept++;
+            int r = dedup_insert(a, key[u], cnt[u], h[u], cur[u]);
+            if (r < 0 && a.spill_cap) {
+                // no slot within the probe limit: park it for the host's table growth
+                const uint32_t s = atomicAdd(a.c32 + C32_SPILL, 1u);
+                if (s < a.spill_cap) {
+                    a.spill[2 * s] = key[u];
+                    a.spill[2 * s + 1] = cnt[u];
+                    r = 0;
+                }
             }
-            h = next_slot(h, a.tab_size);
+            if (r < 0) failed = true;
+            fresh += r > 0;
         }
-        if (r < 0 && a.spill_cap) {
-            // no slot within the probe limit: park it for the host's table growth
-            const uint32_t s = atomicAdd(a.c32 + C32_SPILL, 1u);
-            if (s < a.spill_cap) {
-                a.spill[2 * s] = key;
-                a.spill[2 * s + 1] = cnt;
-                r = 0;
-            }
-        }
-        if (r < 0) failed = true;
-        fresh += r > 0;
     }
     if (failed) atomicOr(a.c32 + C32_OVERFLOW, 1u);
     kept = wave_sum_u64(kept);
@@ -377,8 +431,9 @@ void launch_level(const LevelArgs &a, int n_cu, hipStream_t st) {
 
 void launch_dedup(const DedupArgs &a, bool prune, int n_cu, hipStream_t st) {
     const dim3 g((uint32_t)n_cu * 8);
-    if (prune) hipLaunchKernelGGL(k_dedup<true>, g, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_dedup<false>, g, dim3(256), 0, st, a);
+    if (!prune) hipLaunchKernelGGL((k_dedup<false, false>), g, dim3(256), 0, st, a);
+    else if (a.n_inst <= DEDUP_LDS_INST) hipLaunchKernelGGL((k_dedup<true, true>), g, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_dedup<true, false>), g, dim3(256), 0, st, a);
 }
 
 }  // namespace d4
