@@ -467,6 +467,10 @@ def main():
                     help="count-mode row-step kernel: level-synchronous (level), partial-row "
                          "snapshots on a per-wave stack (expand) or one depth-first search "
                          "per lane (dfs)")
+    ap.add_argument("--step-mode", choices=("general", "fused", "graph"), default="fused",
+                    help="how count-mode level row steps are issued: host round trips "
+                         "between their parts, one queued chain, or that chain replayed "
+                         "from a HIP graph")
     ap.add_argument("--tail-rounds", type=int, default=256,
                     help="expand kernel: rounds after a launch's records run out before "
                          "a wave's partial rows move to a follow-up launch")
@@ -508,6 +512,9 @@ def main():
     kernel = {"level": d4.KERNEL_LEVEL, "expand": d4.KERNEL_EXPAND,
               "dfs": d4.KERNEL_DFS}[args.kernel]
     lib.set_kernel(kernel, args.tail_rounds)
+    step_mode = {"general": d4.STEP_GENERAL, "fused": d4.STEP_FUSED,
+                 "graph": d4.STEP_GRAPH}[args.step_mode]
+    lib.set_step_mode(step_mode)
     if args.launch and not args.tune and not args.no_tune:
         f, t, r, b = (int(x) for x in args.launch.split(","))
         lib.set_launch(f, t, r)
@@ -590,6 +597,7 @@ def main():
             "pieces": "12 free pentominoes x1, D4",
             "mode": "count",
             "kernel": args.kernel,
+            "step_mode": args.step_mode,
             "parallelism": "instances batched in one frontier, 1 GPU",
             "launch": dict(zip(("fit_rows", "threads", "refill"), lib.get_launch()),
                            budget=lib.get_budget(), autotuned=tuned is not None),
@@ -602,6 +610,8 @@ def main():
         "table_reruns_per_step": tot("retries") / args.steps,
         "chunk_reruns_per_step": tot("chunk_reruns") / args.steps,
         "host_syncs_per_row_step": tot("host_syncs") / rows,
+        "fused_steps_per_step": tot("fused_steps") / args.steps,
+        "fused_fallbacks_per_step": tot("fused_fallbacks") / args.steps,
         "table_growths_per_step": tot("rehashes") / args.steps,
         "latency_ms": {"batched_p50": dt / args.steps * 1e3,
                        "batched_p99": dt / args.steps * 1e3},
@@ -652,6 +662,7 @@ def main():
         # a library with its own measured launch shape
         slib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
         slib.set_kernel(kernel, args.tail_rounds)
+        slib.set_step_mode(step_mode)
         f, t, r, bud = (int(x) for x in args.serve_launch.split(","))
         slib.set_launch(f, t, r)
         slib.set_budget(bud)

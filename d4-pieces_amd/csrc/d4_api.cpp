@@ -329,8 +329,25 @@ struct Engine {
     // buffer, and per-parent sizes seen on the last row step
     DevBuf<uint64_t> lv_a, lv_b, ch_k, ch_c;
     double lv_ratio = 0.0, ch_ratio = 0.0;  // max level / children written per parent
+    double pp_bound = 0.0;                   // children bound per parent (this step)
+    double seen_lv = 0.0, seen_ch = 0.0;     // the library's sizes for this step index
     uint64_t lv_cap = 0, ch_cap = 0;  // last sizes used
     std::vector<hipEvent_t> lv_ev;  // chunk boundaries: levels / dedup timing
+    // fused count-mode row step: the whole chain queued, one host round trip
+    uint32_t *pin32 = nullptr;              // pinned copies of the counters
+    unsigned long long *pin64 = nullptr;
+    static constexpr uint64_t kFusedMaxSlots = 1ull << 30;
+    // graph-replayed row steps: two recorded steps (frontier buffer 0 -> 1
+    // and 1 -> 0) whose kernels read their sizes from a device control block
+    hipGraphExec_t gx[2] = {nullptr, nullptr};
+    DevBuf<uint64_t> gk[2], gc[2];       // the frontier buffers they alternate between
+    DevBuf<RowCtl> ctl_d;
+    RowCtl *ctl_h = nullptr;             // pinned
+    std::vector<uint64_t> g_sig;         // buffers and sizes the graphs were recorded with
+    uint64_t g_cap = 0, g_lv = 0, g_ch = 0;
+    uint32_t g_fail = 0;                 // the last replay's failure flags
+    bool g_off_once = false;             // the next row step does not replay
+    static constexpr uint64_t kGraphMaxSlots = 1ull << 25;
     uint64_t n_front = 0, new_roots = 0;
     double grow_unique = 4.0, grow_children = 8.0, grow_unique_raw = 4.0;
     double prev_unique_raw = 0.0;  // the step before's unique ratio (0: none yet)
@@ -349,6 +366,13 @@ struct Engine {
     std::vector<std::unique_ptr<Part>> parked;
     uint64_t parked_bytes = 0;
     bool trace = getenv("D4_TRACE") != nullptr;  // per-step lines of run() on stderr
+    // scale of the count-mode dedup table estimate (D4_TABLE_SCALE, default
+    // 1): below 1 the tables overflow and spill, which exercises the reruns
+    double table_scale = [] {
+        const char *v = getenv("D4_TABLE_SCALE");
+        const double x = v ? atof(v) : 1.0;
+        return x > 0 ? std::min(x, 16.0) : 1.0;
+    }();
     static constexpr uint64_t kExactCountMax = 1u << 16;
     // children per record allowed for when a first row is sized without a
     // count (count mode; more spill into a grown table)
@@ -468,6 +492,10 @@ struct Engine {
         for (hipEvent_t e : {ev0, ev1, evx0, evx1, evc0})
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : lv_ev) (void)hipEventDestroy(e);
+        if (pin32) (void)hipHostFree(pin32);
+        if (pin64) (void)hipHostFree(pin64);
+        drop_graphs();
+        if (ctl_h) (void)hipHostFree(ctl_h);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -630,8 +658,10 @@ struct Engine {
         return std::max<uint64_t>(std::min<uint64_t>(cap, 16ull << 30), 1ull << 20);
     }
     // per-parent estimates at the start of a step
-    double lv_pp0() const { return std::max(lv_ratio, 2.0); }
-    double ch_pp0() const { return std::max(ch_ratio, 4.0); }
+    // (pp_bound: the step's children bound per parent where it was computed;
+    // the widest level has held well under a quarter of the children)
+    double lv_pp0() const { return std::max({lv_ratio, 2.0, 0.25 * pp_bound, 1.1 * seen_lv}); }
+    double ch_pp0() const { return std::max({ch_ratio, 4.0, pp_bound, 1.1 * seen_ch}); }
     uint64_t level_min_cap() const {
         return std::min<uint64_t>(std::max<uint64_t>(level_budget() / 1024, 4096), 1u << 20);
     }
@@ -661,14 +691,10 @@ struct Engine {
     // dedup of each completed chunk counts itself in c32[C32_LVDONE]).
     // Terminal records (instances that finish at this step) are counted by
     // one pass of their own, so a rerun chunk adds nothing twice.
-    void run_levels(uint64_t tab_m, bool prune) {
+    // The level kernels' and the dedup kernel's arguments for this step.
+    void level_args(uint64_t tab_m, LevelArgs &a, DedupArgs &d) {
         if (!spill.ext) spill.reserve(2 * kSpillCap);
-        bool any_term = false;
-        for (uint32_t i = 0; i < S; i++)
-            if (active[i] && start[i] + heights[i] == g) any_term = true;
-        if (any_term)
-            launch_terminal(fk.p, fc.p, n_front, g, dstart.p, hgt.p, inst_shift, result.p, lib->n_cu, st);
-        LevelArgs a{};
+        a = LevelArgs{};
         a.fit = lib->d_fit;
         a.placeinfo = lib->d_placeinfo;
         a.xavail = lib->d_xavail;
@@ -685,7 +711,7 @@ struct Engine {
         a.occ_mask = occ_mask;
         a.c32 = c32.p;
         a.c64 = c64.p;
-        DedupArgs d{};
+        d = DedupArgs{};
         d.bw = bw.p;
         d.bw_stride = bw_stride;
         d.start = dstart.p;
@@ -711,53 +737,108 @@ struct Engine {
         d.spill_cap = (uint32_t)std::min<uint64_t>(spill.cap / 2, kSpillCap);
         d.c32 = c32.p;
         d.c64 = c64.p;
+    }
+
+    // Queues the terminal pass when an instance finishes at this step.
+    void queue_terminal() {
+        bool any_term = false;
+        for (uint32_t i = 0; i < S; i++)
+            if (active[i] && start[i] + heights[i] == g) any_term = true;
+        if (any_term)
+            launch_terminal(fk.p, fc.p, n_front, g, dstart.p, hgt.p, inst_shift, result.p,
+                            lib->n_cu, st);
+    }
+
+    // Sizes the level buffers for chunks of `chunk` parents.
+    void level_buffers(uint64_t chunk, double lv_pp, double ch_pp, LevelArgs &a, DedupArgs &d) {
+        lv_cap = level_cap(chunk, lv_pp);
+        ch_cap = level_cap(chunk, ch_pp);
+        if (noalloc) {
+            lv_cap = std::min<uint64_t>(lv_cap, lv_a.cap / LVL_FIELDS);
+            ch_cap = std::min<uint64_t>(ch_cap, ch_k.cap);
+        }
+        lv_a.reserve(LVL_FIELDS * lv_cap);
+        lv_b.reserve(LVL_FIELDS * lv_cap);
+        ch_k.reserve(ch_cap);
+        ch_c.reserve(ch_cap);
+        a.ch_key = ch_k.p;
+        a.ch_cnt = ch_c.p;
+        d.ch_key = ch_k.p;
+        d.ch_cnt = ch_c.p;
+        a.ch_cap = d.ch_cap = ch_cap;
+        a.in_cap = a.out_cap = lv_cap;
+    }
+
+    // Queues chunks b0, b0 + chunk, ... of the frontier: per chunk the level
+    // launches and the dedup, with events around them when timed.
+    uint32_t queue_levels(LevelArgs &a, const DedupArgs &d, uint64_t b0, uint64_t chunk,
+                          bool prune, bool timed) {
+        const uint32_t nch = (uint32_t)((n_front - b0 + chunk - 1) / chunk);
+        while (timed && lv_ev.size() < 2 * (size_t)nch + 1) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            lv_ev.push_back(e);
+        }
+        launch_reset_levels(c32.p, c64.p, true, st);
+        if (timed) HIPCHK(hipEventRecord(lv_ev[0], st));
+        for (uint32_t j = 0; j < nch; j++) {
+            const uint64_t b = b0 + (uint64_t)j * chunk;
+            if (j) launch_reset_levels(c32.p, c64.p, false, st);
+            a.in_key = fk.p + b;
+            a.in_cnt = fc.p + b;
+            a.n_rec = (uint32_t)std::min<uint64_t>(chunk, n_front - b);
+            for (uint32_t L = 0; L < LVL_MAX; L++) {
+                a.level = L;
+                a.in_st = (L & 1) ? lv_a.p : lv_b.p;
+                a.out_st = (L & 1) ? lv_b.p : lv_a.p;
+                launch_level(a, lib->n_cu, st);
+            }
+            if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 1], st));
+            launch_dedup(d, prune, lib->n_cu, st);
+            if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 2], st));
+        }
+        return nch;
+    }
+
+    // Kernel times of the chunks queued last (after the stream is synced).
+    void level_times(uint32_t nch) {
+        for (uint32_t j = 0; j < nch; j++) {
+            float lms = 0, dms = 0;
+            HIPCHK(hipEventElapsedTime(&lms, j ? lv_ev[2 * j] : lv_ev[0], lv_ev[2 * j + 1]));
+            HIPCHK(hipEventElapsedTime(&dms, lv_ev[2 * j + 1], lv_ev[2 * j + 2]));
+            stats.extend_ms += lms;
+            stats.dedup_ms += dms;
+            stats.dedup_launches++;
+            stats.extend_launches++;
+        }
+    }
+
+    // Per-parent sizes of a completed chunk of `c` parents (its counters are
+    // still set) for the next steps' chunking.
+    void level_sizes(const unsigned long long *h64, uint64_t c) {
+        if (!c) return;
+        uint64_t m = 0;
+        for (uint32_t L = 0; L < LVL_MAX; L++) m = std::max<uint64_t>(m, h64[C64_LVL + L]);
+        const double lv = (double)m / (double)c, ch = (double)h64[C64_CHW] / (double)c;
+        lib->note_sizes(stats.rows_done, lv, ch);
+        if (c < 4096) return;
+        lv_ratio = lv;
+        ch_ratio = ch;
+    }
+
+    void run_levels(uint64_t tab_m, bool prune) {
+        LevelArgs a;
+        DedupArgs d;
+        level_args(tab_m, a, d);
+        queue_terminal();
         uint64_t b0 = 0;
         uint32_t reruns = 0;
-        uint64_t mx_lv = 0, mx_ch = 0, mx_chunk = 0;
         double lv_pp = lv_pp0(), ch_pp = ch_pp0();
         for (;;) {
             const uint64_t chunk = level_chunk(n_front - b0, lv_pp, ch_pp);
-            lv_cap = level_cap(chunk, lv_pp);
-            ch_cap = level_cap(chunk, ch_pp);
-            if (noalloc) {
-                lv_cap = std::min<uint64_t>(lv_cap, lv_a.cap / LVL_FIELDS);
-                ch_cap = std::min<uint64_t>(ch_cap, ch_k.cap);
-            }
-            lv_a.reserve(LVL_FIELDS * lv_cap);
-            lv_b.reserve(LVL_FIELDS * lv_cap);
-            ch_k.reserve(ch_cap);
-            ch_c.reserve(ch_cap);
-            a.ch_key = ch_k.p;
-            a.ch_cnt = ch_c.p;
-            d.ch_key = ch_k.p;
-            d.ch_cnt = ch_c.p;
-            a.ch_cap = d.ch_cap = ch_cap;
-            a.in_cap = a.out_cap = lv_cap;
-            const uint32_t nch = (uint32_t)((n_front - b0 + chunk - 1) / chunk);
+            level_buffers(chunk, lv_pp, ch_pp, a, d);
             const bool timed = !noalloc;  // (no events are made for the scratch's chunks)
-            while (timed && lv_ev.size() < 2 * (size_t)nch + 1) {
-                hipEvent_t e;
-                HIPCHK(hipEventCreate(&e));
-                lv_ev.push_back(e);
-            }
-            launch_reset_levels(c32.p, c64.p, true, st);
-            if (timed) HIPCHK(hipEventRecord(lv_ev[0], st));
-            for (uint32_t j = 0; j < nch; j++) {
-                const uint64_t b = b0 + (uint64_t)j * chunk;
-                if (j) launch_reset_levels(c32.p, c64.p, false, st);
-                a.in_key = fk.p + b;
-                a.in_cnt = fc.p + b;
-                a.n_rec = (uint32_t)std::min<uint64_t>(chunk, n_front - b);
-                for (uint32_t L = 0; L < LVL_MAX; L++) {
-                    a.level = L;
-                    a.in_st = (L & 1) ? lv_a.p : lv_b.p;
-                    a.out_st = (L & 1) ? lv_b.p : lv_a.p;
-                    launch_level(a, lib->n_cu, st);
-                }
-                if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 1], st));
-                launch_dedup(d, prune, lib->n_cu, st);
-                if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 2], st));
-            }
+            const uint32_t nch = queue_levels(a, d, b0, chunk, prune, timed);
             HIPCHK(hipGetLastError());
             uint32_t h32[C32_N];
             unsigned long long h64[C64_N];
@@ -765,27 +846,10 @@ struct Engine {
             HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
             sync_host();
             const uint32_t done = std::min<uint32_t>(h32[C32_LVDONE], nch);
-            for (uint32_t j = 0; timed && j < nch; j++) {
-                float lms = 0, dms = 0;
-                HIPCHK(hipEventElapsedTime(&lms, j ? lv_ev[2 * j] : lv_ev[0], lv_ev[2 * j + 1]));
-                HIPCHK(hipEventElapsedTime(&dms, lv_ev[2 * j + 1], lv_ev[2 * j + 2]));
-                stats.extend_ms += lms;
-                stats.dedup_ms += dms;
-                stats.dedup_launches++;
-                stats.extend_launches++;
-            }
+            if (timed) level_times(nch);
             if (h32[C32_OVERFLOW]) return;  // the table: the row step reruns larger
-            // sizes of the last completed chunk (its counters are still set)
-            if (done && !h32[C32_XOVF]) {
-                uint64_t m = 0;
-                for (uint32_t L = 0; L < LVL_MAX; L++) m = std::max<uint64_t>(m, h64[C64_LVL + L]);
-                const uint64_t last = std::min<uint64_t>(chunk, n_front - (b0 + (uint64_t)(nch - 1) * chunk));
-                if (last >= mx_chunk) {
-                    mx_chunk = last;
-                    mx_lv = m;
-                    mx_ch = h64[C64_CHW];
-                }
-            }
+            if (done == nch && !h32[C32_XOVF])
+                level_sizes(h64, std::min<uint64_t>(chunk, n_front - (b0 + (uint64_t)(nch - 1) * chunk)));
             if (trace)
                 fprintf(stderr,
                         "levels step %u: parents %llu from %llu, chunk %llu, caps %llu / %llu, "
@@ -812,10 +876,6 @@ struct Engine {
                 throw Error(D4_ERANGE, noalloc ? "allocation-free search: a row step's partial "
                                                  "rows exceed the scratch"
                                                : "a row step's partial rows outgrow every buffer");
-        }
-        if (mx_chunk >= 4096) {
-            lv_ratio = (double)mx_lv / (double)mx_chunk;
-            ch_ratio = (double)mx_ch / (double)mx_chunk;
         }
     }
 
@@ -870,13 +930,17 @@ struct Engine {
         // slots (a batch admits all its boards as one run)
         std::vector<uint32_t> order(pending.begin(), pending.end());
         std::sort(order.begin(), order.end());
-        std::vector<uint64_t> win;
-        std::vector<unsigned long long> zero;
+        // (all copies are queued, then one wait: the host buffers live until it)
+        std::deque<std::vector<uint64_t>> wins;
+        std::deque<std::vector<unsigned long long>> zeros;
+        auto copy_async = [&](void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+            HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+        };
         for (size_t i = 0; i < order.size();) {
             size_t j = i + 1;
             while (j < order.size() && order[j] == order[j - 1] + 1) j++;
             const uint32_t first = order[i], run = (uint32_t)(j - i);
-            win.assign((size_t)run * bw_stride, 0);
+            std::vector<uint64_t> &win = wins.emplace_back((size_t)run * bw_stride, 0);
             for (uint32_t q = 0; q < run; q++) {
                 const uint8_t *bd = &boards[(size_t)(first + q) * (MAX_ROWS + 8)];
                 for (uint32_t r = 0; r < bw_stride; r++) {
@@ -886,28 +950,32 @@ struct Engine {
                     win[(size_t)q * bw_stride + r] = w;
                 }
             }
-            copy(bw.p + (size_t)first * bw_stride, win.data(), win.size() * 8,
-                 hipMemcpyHostToDevice);
-            zero.assign(run, 0);
-            copy(result.p + first, zero.data(), (size_t)run * 8, hipMemcpyHostToDevice);
+            copy_async(bw.p + (size_t)first * bw_stride, win.data(), win.size() * 8,
+                       hipMemcpyHostToDevice);
+            std::vector<unsigned long long> &zero = zeros.emplace_back(run, 0);
+            copy_async(result.p + first, zero.data(), (size_t)run * 8, hipMemcpyHostToDevice);
             i = j;
         }
-        copy(hgt.p, heights.data(), (size_t)S * 4, hipMemcpyHostToDevice);
-        copy(dstart.p, start.data(), (size_t)S * 4, hipMemcpyHostToDevice);
+        copy_async(hgt.p, heights.data(), (size_t)S * 4, hipMemcpyHostToDevice);
+        copy_async(dstart.p, start.data(), (size_t)S * 4, hipMemcpyHostToDevice);
         const uint64_t n = n_front + k0.size();
         if (n > fk.cap || n > fc.cap) {  // grow, keeping the current frontier
             DevBuf<uint64_t> k2, c2;
             k2.reserve(n);
             c2.reserve(n);
             if (n_front) {
-                copy(k2.p, fk.p, n_front * 8, hipMemcpyDeviceToDevice);
-                copy(c2.p, fc.p, n_front * 8, hipMemcpyDeviceToDevice);
+                copy_async(k2.p, fk.p, n_front * 8, hipMemcpyDeviceToDevice);
+                copy_async(c2.p, fc.p, n_front * 8, hipMemcpyDeviceToDevice);
             }
+            // (the old buffers go back to the allocator's cache only after
+            // the copies out of them have ended)
+            sync_host();
             fk.swap(k2);
             fc.swap(c2);
         }
-        copy(fk.p + n_front, k0.data(), k0.size() * 8, hipMemcpyHostToDevice);
-        copy(fc.p + n_front, c0.data(), c0.size() * 8, hipMemcpyHostToDevice);
+        copy_async(fk.p + n_front, k0.data(), k0.size() * 8, hipMemcpyHostToDevice);
+        copy_async(fc.p + n_front, c0.data(), c0.size() * 8, hipMemcpyHostToDevice);
+        sync_host();
         n_front = n;
         new_roots = k0.size();
         stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, n_front);
@@ -945,6 +1013,264 @@ struct Engine {
             while (n_active) step();
     }
 
+    void drop_graphs() {
+        for (hipGraphExec_t &x : gx)
+            if (x) {
+                (void)hipGraphExecDestroy(x);
+                x = nullptr;
+            }
+        g_sig.clear();
+    }
+
+    // What a recorded step depends on: every buffer address and capacity it
+    // was recorded with (a buffer the general path grows moves, and the
+    // graphs are recorded again).
+    std::vector<uint64_t> graph_sig(bool prune) const {
+        std::vector<uint64_t> v;
+        for (const void *q : {(const void *)tk.p, (const void *)lv_a.p, (const void *)lv_b.p,
+                              (const void *)ch_k.p, (const void *)ch_c.p, (const void *)spill.p,
+                              (const void *)result.p, (const void *)result_save.p,
+                              (const void *)bw.p, (const void *)hgt.p, (const void *)dstart.p,
+                              (const void *)c32.p, (const void *)c64.p, (const void *)blk.p,
+                              (const void *)gk[0].p, (const void *)gk[1].p,
+                              (const void *)gc[0].p, (const void *)gc[1].p,
+                              (const void *)ctl_d.p})
+            v.push_back((uint64_t)(uintptr_t)q);
+        for (uint64_t x : {tk.cap, lv_a.cap, lv_b.cap, ch_k.cap, ch_c.cap, spill.cap, g_cap, g_lv,
+                           g_ch, (uint64_t)S, (uint64_t)prune, (uint64_t)bw_stride})
+            v.push_back(x);
+        return v;
+    }
+
+    // Records the row step that reads frontier buffer p and writes 1 - p:
+    // result save, table fill, counter resets, terminal pass, the eight level
+    // launches, the dedup, compaction and the close kernel, all sized from
+    // the control block (launch arguments are the capacities).
+    void record_step(uint32_t p, bool prune) {
+        LevelArgs a;
+        DedupArgs d;
+        level_args(g_cap, a, d);
+        a.ch_key = ch_k.p;
+        a.ch_cnt = ch_c.p;
+        d.ch_key = ch_k.p;
+        d.ch_cnt = ch_c.p;
+        a.ch_cap = d.ch_cap = g_ch;
+        a.in_cap = a.out_cap = g_lv;
+        a.ctl = d.ctl = ctl_d.p;
+        a.in_key = gk[p].p;
+        a.in_cnt = gc[p].p;
+        a.n_rec = (uint32_t)std::min<uint64_t>(g_cap, 0xffffffffull);
+        const RowCtl *ctl = ctl_d.p;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        launch_copy_u64(result_save.p, result.p, S, st, ctl);
+        launch_fill_table(tk.p, tk.p + 1, g_cap, st, ctl);
+        launch_reset_counters(c32.p, c64.p, st, ctl);
+        launch_terminal(gk[p].p, gc[p].p, g_cap, g, dstart.p, hgt.p, inst_shift, result.p,
+                        lib->n_cu, st, ctl);
+        launch_reset_levels(c32.p, c64.p, true, st, ctl);
+        for (uint32_t L = 0; L < LVL_MAX; L++) {
+            a.level = L;
+            a.in_st = (L & 1) ? lv_a.p : lv_b.p;
+            a.out_st = (L & 1) ? lv_b.p : lv_a.p;
+            launch_level(a, lib->n_cu, st);
+        }
+        launch_dedup(d, prune, lib->n_cu, st);
+        launch_compact(tk.p, tk.p + 1, g_cap, gk[1 - p].p, gc[1 - p].p, g_cap, c32.p, blk.p,
+                       nullptr, st, ctl);
+        launch_row_close(ctl_d.p, c32.p, c64.p, st);
+        const hipError_t launch_err = hipGetLastError();
+        hipGraph_t graph = nullptr;
+        const hipError_t cap_err = hipStreamEndCapture(st, &graph);
+        HIPCHK(launch_err);
+        HIPCHK(cap_err);
+        const hipError_t inst_err = hipGraphInstantiate(&gx[p], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(inst_err);
+    }
+
+    // Up to kmax count-mode row steps replayed from the recorded graphs, with
+    // one host round trip for all of them.  tab0: the first step's table
+    // slots (0: from the unique states per parent of the step before).
+    // Returns the steps completed, their bookkeeping done (the caller moves
+    // g); a step that failed leaves the frontier before it in fk / fc, its
+    // terminal counts undone, and its flags in g_fail.  Returns 0 with g_fail
+    // clear where replay does not apply.
+    uint32_t graph_rows(uint32_t kmax, uint64_t tab0, bool prune) {
+        g_fail = 0;
+        if (lib->step_mode != D4_STEP_GRAPH || mode != D4_MODE_COUNT || noalloc || !n_front ||
+            !kmax || !level_kernel(false, false) || sample_h || n_front > 0xffffffffull)
+            return 0;
+        if (!tab0)
+            tab0 = round_slots((uint64_t)(1.4 * table_scale *
+                                          ((double)n_front * std::max(grow_unique_raw, 0.01) *
+                                               1.1 + 4096.0)));
+        // (the frontier buffers hold the current frontier too)
+        const uint64_t cap = std::max<uint64_t>(
+            g_cap, next_pow2(std::max<uint64_t>({2 * tab0, n_front, 1u << 16})));
+        if (cap > kGraphMaxSlots) return 0;
+        const uint64_t lv = std::max<uint64_t>(g_lv, level_cap(4 * n_front, lv_pp0()));
+        const uint64_t ch = std::max<uint64_t>(g_ch, level_cap(4 * n_front, ch_pp0()));
+        if (mem_cap && 16ull * cap + 32ull * cap + 2ull * 8 * LVL_FIELDS * lv + 16ull * ch +
+                               16ull * n_front + parked_bytes > mem_cap)
+            return 0;
+        g_cap = cap;
+        g_lv = lv;
+        g_ch = ch;
+        tk.reserve(2 * g_cap);
+        for (int q = 0; q < 2; q++) {
+            gk[q].reserve(g_cap);
+            gc[q].reserve(g_cap);
+        }
+        lv_a.reserve(LVL_FIELDS * g_lv);
+        lv_b.reserve(LVL_FIELDS * g_lv);
+        ch_k.reserve(g_ch);
+        ch_c.reserve(g_ch);
+        lv_cap = g_lv;
+        ch_cap = g_ch;
+        blk.reserve(2 * COMPACT_MAX_BLOCKS);
+        if (!spill.ext) spill.reserve(2 * kSpillCap);
+        result_save.reserve(S);
+        ctl_d.reserve(1);
+        if (!ctl_h) HIPCHK(hipHostMalloc((void **)&ctl_h, sizeof(RowCtl)));
+        const std::vector<uint64_t> sig = graph_sig(prune);
+        if (sig != g_sig) {
+            drop_graphs();
+            record_step(0, prune);
+            record_step(1, prune);
+            g_sig = sig;
+        }
+        const uint32_t k = std::min<uint32_t>(kmax, CTL_LOG);
+        HIPCHK(hipMemcpyAsync(gk[0].p, fk.p, n_front * 8, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(gc[0].p, fc.p, n_front * 8, hipMemcpyDeviceToDevice, st));
+        ctl_h->n_front = n_front;
+        ctl_h->tab_m = std::min(tab0, g_cap);
+        ctl_h->tab_cap = g_cap;
+        ctl_h->row = g;
+        ctl_h->fail = 0;
+        ctl_h->done = 0;
+        HIPCHK(hipMemcpyAsync(ctl_d.p, ctl_h, offsetof(RowCtl, log), hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(ev0, st));
+        for (uint32_t j = 0; j < k; j++) HIPCHK(hipGraphLaunch(gx[j & 1], st));
+        HIPCHK(hipEventRecord(ev1, st));
+        HIPCHK(hipMemcpyAsync(ctl_h, ctl_d.p, sizeof(RowCtl), hipMemcpyDeviceToHost, st));
+        sync_host();
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.gpu_ms += ms;
+        stats.graph_replays += k;
+        const uint32_t done = std::min(ctl_h->done, k);
+        g_fail = ctl_h->fail;
+        for (uint32_t j = 0; j < done; j++) {
+            const StepLog &l = ctl_h->log[j];
+            const uint64_t n = l.n_front, u = l.fresh;
+            check_row_states(u);
+            check_compacted(l.out, u);
+            stats.children += l.children;
+            stats.pruned += l.pruned;
+            stats.records_extended += n;
+            stats.unique += u;
+            stats.extend_bytes += 16ull * n + 16ull * l.children;
+            stats.dedup_bytes += 16ull * (l.children + l.pruned) + 16ull * u;
+            stats.compact_bytes += 8ull * l.tab_m + 24ull * u;
+            stats.compact_launches++;
+            stats.extend_launches++;
+            stats.dedup_launches++;
+            if (n) {
+                prev_unique_raw = stats.rows_done ? grow_unique_raw : 0.0;
+                grow_unique = std::max(1.0, (double)u / (double)n);
+                grow_unique_raw = std::max(0.01, (double)u / (double)n);
+                grow_children = std::max(1.0, (double)l.children / (double)n);
+                lib->note_sizes(stats.rows_done, (double)l.lv_max / (double)n,
+                                (double)l.chw / (double)n);
+                if (n >= 4096) {
+                    lv_ratio = (double)l.lv_max / (double)n;
+                    ch_ratio = (double)l.chw / (double)n;
+                }
+            }
+            stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, u);
+            stats.rows_done++;
+            if (trace)
+                fprintf(stderr, "graph step %u: parents %llu, table %llu, children %llu, new %llu\n",
+                        g + j, (unsigned long long)n, (unsigned long long)l.tab_m,
+                        (unsigned long long)l.chw, (unsigned long long)u);
+        }
+        // the frontier after the last completed step (the input of a failed one)
+        const uint32_t q = done & 1;
+        n_front = ctl_h->n_front;
+        fk.reserve(std::max<uint64_t>(n_front, 1));
+        fc.reserve(std::max<uint64_t>(n_front, 1));
+        HIPCHK(hipMemcpyAsync(fk.p, gk[q].p, n_front * 8, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(fc.p, gc[q].p, n_front * 8, hipMemcpyDeviceToDevice, st));
+        if (g_fail) {
+            // undo the failed step's terminal counts (its save is the last one)
+            stats.fused_fallbacks++;
+            HIPCHK(hipMemcpyAsync(result.p, result_save.p, (size_t)S * 8,
+                                  hipMemcpyDeviceToDevice, st));
+            if (trace)
+                fprintf(stderr, "graph step %u: failed (flags %u), rerun on the general path\n",
+                        g + done, g_fail);
+        }
+        if (done) new_roots = 0;
+        return done;
+    }
+
+    // The count-mode row step as one queued chain with one host round trip
+    // (level kernels, the frontier in a single chunk): table fill, counter
+    // resets, terminal pass, the level launches, the dedup, and compaction
+    // into next-frontier buffers sized to the table (the unique count is not
+    // known before the chain ends), then the counters.  Returns false, with
+    // the terminal counts restored, when a buffer overflowed or children
+    // spilled: the general path then runs the step.
+    bool fused_step(uint64_t tab_m, bool prune, uint32_t *h32, unsigned long long *h64) {
+        if (!pin32) {
+            HIPCHK(hipHostMalloc((void **)&pin32, sizeof(uint32_t) * C32_N));
+            HIPCHK(hipHostMalloc((void **)&pin64, sizeof(unsigned long long) * C64_N));
+        }
+        nk.reserve(tab_m);
+        nc.reserve(tab_m);
+        blk.reserve(2 * COMPACT_MAX_BLOCKS);
+        LevelArgs a;
+        DedupArgs d;
+        level_args(tab_m, a, d);
+        const double lv_pp = lv_pp0(), ch_pp = ch_pp0();
+        level_buffers(n_front, lv_pp, ch_pp, a, d);
+        HIPCHK(hipEventRecord(ev0, st));
+        HIPCHK(hipMemcpyAsync(result_save.p, result.p, (size_t)S * 8, hipMemcpyDeviceToDevice, st));
+        launch_fill_table(tk.p, tk.p + 1, tab_m, st);
+        launch_reset_counters(c32.p, c64.p, st);
+        queue_terminal();
+        queue_levels(a, d, 0, n_front, prune, true);
+        HIPCHK(hipEventRecord(evc0, st));
+        launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, tab_m, c32.p, blk.p, nullptr, st);
+        HIPCHK(hipEventRecord(ev1, st));
+        HIPCHK(hipMemcpyAsync(pin32, c32.p, sizeof(uint32_t) * C32_N, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(pin64, c64.p, sizeof(unsigned long long) * C64_N,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipGetLastError());
+        sync_host();
+        std::memcpy(h32, pin32, sizeof(uint32_t) * C32_N);
+        std::memcpy(h64, pin64, sizeof(unsigned long long) * C64_N);
+        level_times(1);
+        stats.fused_steps++;
+        stats.donation_full += h32[C32_DONFULL];
+        stats.extend_bytes += 16ull * n_front + 16ull * h64[C64_CHILDREN];
+        stats.dedup_bytes += 16ull * (h64[C64_CHILDREN] + h64[C64_PRUNED]) + 16ull * h64[C64_NEW];
+        if (trace)
+            fprintf(stderr,
+                    "fused step %u: parents %llu, table %llu, caps %llu / %llu, children %llu, "
+                    "new %llu, level overflow %u, table overflow %u, spilled %u\n",
+                    g, (unsigned long long)n_front, (unsigned long long)tab_m,
+                    (unsigned long long)lv_cap, (unsigned long long)ch_cap, h64[C64_CHW],
+                    h64[C64_NEW], h32[C32_XOVF], h32[C32_OVERFLOW], h32[C32_SPILL]);
+        if (h32[C32_XOVF] || h32[C32_OVERFLOW] || h32[C32_SPILL]) {
+            copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
+            stats.fused_fallbacks++;
+            return false;
+        }
+        level_sizes(h64, n_front);
+        return true;
+    }
+
     // One row step over the whole frontier.  With a memory cap (count mode)
     // it returns false, with nothing changed, when it would not fit.
     bool row_step(bool may_split = false) {
@@ -966,8 +1292,14 @@ struct Engine {
             (uint64_t)(n_front * std::min(std::max(grow_children, 1.0), 64.0) * 1.5) + 4096;
         const uint64_t g_u =
             std::min<uint64_t>(g_c, (uint64_t)(n_front * std::max(grow_unique, 0.25) * 1.5) + 4096);
-        if (n_front <= (1u << 20)) {
+        // (count mode past its first rows sizes from the growth estimate, as
+        // the spill and table growth catch a short table: the bound, one more
+        // host round trip, is taken only where new roots make growth unknown)
+        pp_bound = 0.0;
+        lib->seen_sizes(stats.rows_done, seen_lv, seen_ch);
+        if (n_front <= (1u << 20) && (en || stats.rows_done < 2 || new_roots)) {
             const uint64_t bound = next_size();
+            if (n_front && bound <= (1ull << 26)) pp_bound = (double)bound / (double)n_front;
             if (bound <= (1ull << 26)) {
                 est_c = est_u = bound;
                 exact = true;
@@ -1002,7 +1334,7 @@ struct Engine {
         } else {
             uint64_t t = est_u;
             if (!exact) t = std::min<uint64_t>(g_c, (uint64_t)(n_front * grow_unique_raw * 1.1) + 4096);
-            tab_m = round_slots((uint64_t)(1.4 * (double)t));
+            tab_m = round_slots((uint64_t)(1.4 * table_scale * (double)t));
             if (!spill.ext) spill.reserve(2 * kSpillCap);
         }
         uint64_t est_e = en ? est_c + 64 : 0;
@@ -1011,6 +1343,7 @@ struct Engine {
         uint32_t h32[C32_N];
         unsigned long long h64[C64_N];
         bool trend = false;
+        bool fused_ok = false, fused_skip = false;
         for (;;) {
             if (bounded && (tab_m > (1ull << 34) ||
                             16ull * tab_m + 16ull * std::min(est_u, tab_m) + 16ull * n_front +
@@ -1050,6 +1383,51 @@ struct Engine {
                 e_parent.reserve(est_e);
                 e_desc.reserve(est_e);
                 e_slot.reserve(est_e);
+            }
+            if (!fused_skip && !en && lib->step_mode != D4_STEP_GENERAL && !noalloc &&
+                n_front &&
+                level_kernel(en, false) && tab_m <= kFusedMaxSlots &&
+                level_chunk(n_front, lv_pp0(), ch_pp0()) >= n_front &&
+                (!mem_cap || 48ull * tab_m + 16ull * n_front + level_bytes(n_front) +
+                                     parked_bytes <= mem_cap)) {
+                if (lib->step_mode == D4_STEP_GRAPH && !fused_skip && !g_off_once) {
+                    // one replayed step; where replay does not apply, the fused chain
+                    if (graph_rows(1, tab_m, prune)) return true;
+                    if (g_fail) {
+                        // the general path reruns it, with a larger table if that was short
+                        fused_skip = true;
+                        if (g_fail & 6) {
+                            stats.retries++;
+                            tab_m *= 4;
+                        }
+                        continue;
+                    }
+                }
+                bool done = false;
+                try {
+                    done = fused_step(tab_m, prune, h32, h64);
+                } catch (const Error &e) {
+                    // no room for next-frontier buffers sized to the table:
+                    // the general path sizes them to the unique count
+                    if (e.code != D4_ENOMEM) throw;
+                    h32[C32_OVERFLOW] = h32[C32_SPILL] = 0;
+                }
+                if (done) {
+                    fused_ok = true;
+                    break;
+                }
+                // a table overflow, a spill or a level buffer overflow: the
+                // general path below reruns the step (a bigger table for the
+                // first two, chunk reruns for the last)
+                fused_skip = true;
+                if (h32[C32_OVERFLOW] || h32[C32_SPILL]) {
+                    stats.retries++;
+                    tab_m = h32[C32_OVERFLOW]
+                                ? 4 * tab_m
+                                : round_slots(std::max<uint64_t>(
+                                      2 * tab_m, 2 * ((uint64_t)h64[C64_NEW] + h32[C32_SPILL])));
+                    continue;  // (the table is reserved again at its new size)
+                }
             }
             HIPCHK(hipEventRecord(ev0, st));
             // terminal counts are added during the step: keep a copy so a
@@ -1136,13 +1514,15 @@ struct Engine {
         }
         const uint64_t U = h64[C64_NEW];
         check_row_states(U);
-        try {
-            nk.reserve(U);
-            nc.reserve(U);
-        } catch (const Error &e) {
-            if (e.code != D4_ENOMEM || !bounded) throw;
-            copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
-            return false;
+        if (!fused_ok) {
+            try {
+                nk.reserve(U);
+                nc.reserve(U);
+            } catch (const Error &e) {
+                if (e.code != D4_ENOMEM || !bounded) throw;
+                copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
+                return false;
+            }
         }
         stats.children += h64[C64_CHILDREN];
         stats.pruned += h64[C64_PRUNED];
@@ -1155,21 +1535,25 @@ struct Engine {
             grow_unique_raw = std::max(0.01, (double)U / (double)n_front);
             grow_children = std::max(1.0, (double)h64[C64_CHILDREN] / (double)n_front);
         }
-        if (en) slot_uid.reserve(tab_m);
-        HIPCHK(hipEventRecord(evc0, st));
-        blk.reserve(2 * COMPACT_MAX_BLOCKS);
-        launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, U, c32.p, blk.p,
-                       en ? slot_uid.p : nullptr, st);
         uint32_t n_out = 0;
-        HIPCHK(hipMemcpyAsync(&n_out, c32.p + C32_OUT, 4, hipMemcpyDeviceToHost, st));
-        uint32_t n_edges = en ? h32[C32_EDGES] : 0;
-        if (en && n_edges) {
-            e_child.reserve(n_edges);
-            launch_remap_edges(e_slot.p, slot_uid.p, e_child.p, n_edges, st);
+        const uint32_t n_edges = en ? h32[C32_EDGES] : 0;
+        if (fused_ok) {
+            n_out = h32[C32_OUT];  // (compacted inside the fused chain)
+        } else {
+            if (en) slot_uid.reserve(tab_m);
+            HIPCHK(hipEventRecord(evc0, st));
+            blk.reserve(2 * COMPACT_MAX_BLOCKS);
+            launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, U, c32.p, blk.p,
+                           en ? slot_uid.p : nullptr, st);
+            HIPCHK(hipMemcpyAsync(&n_out, c32.p + C32_OUT, 4, hipMemcpyDeviceToHost, st));
+            if (en && n_edges) {
+                e_child.reserve(n_edges);
+                launch_remap_edges(e_slot.p, slot_uid.p, e_child.p, n_edges, st);
+            }
+            HIPCHK(hipEventRecord(ev1, st));
+            HIPCHK(hipGetLastError());
+            sync_host();
         }
-        HIPCHK(hipEventRecord(ev1, st));
-        HIPCHK(hipGetLastError());
-        sync_host();
         check_compacted(n_out, U);
         float ms = 0, cms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -1257,7 +1641,23 @@ struct Engine {
             if (active[i]) gend = std::max(gend, start[i] + heights[i]);
         for (;;) {
             while (n_front && g <= gend) {
-                if (row_step(true)) {
+                // past the first rows (whose growth is unknown), replay many
+                // steps per host round trip; a failed step takes the path below
+                if (stats.rows_done >= 2 && !new_roots) {
+                    const uint32_t k = graph_rows(gend - g + 1, 0, !(flags & D4_FLAG_NO_PRUNE));
+                    if (k) {
+                        g += k;
+                        if (trace)
+                            fprintf(stderr, "run step %u: frontier %llu, %zu parts parked\n",
+                                    g - 1, (unsigned long long)n_front, parked.size());
+                        continue;
+                    }
+                    // (a step that failed is not replayed again)
+                    g_off_once = g_fail != 0;
+                }
+                const bool stepped = row_step(true);
+                g_off_once = false;
+                if (stepped) {
                     g++;
                     take_sample();
                     g--;
@@ -1856,6 +2256,21 @@ int d4_library_set_kernel(d4_library *lib, uint32_t kernel, uint32_t tail_rounds
         if (kernel > D4_KERNEL_LEVEL) throw Error(D4_EINVAL, "unknown kernel");
         lib->xp_kernel = kernel;
         lib->xp_budget = tail_rounds;
+    });
+}
+
+int d4_library_set_step_mode(d4_library *lib, uint32_t mode) {
+    return guard([&] {
+        if (!lib) throw Error(D4_EINVAL, "null library");
+        if (mode > D4_STEP_GRAPH) throw Error(D4_EINVAL, "unknown step mode");
+        lib->step_mode = mode;
+    });
+}
+
+int d4_library_get_step_mode(const d4_library *lib, uint32_t *mode) {
+    return guard([&] {
+        if (!lib || !mode) throw Error(D4_EINVAL, "null argument");
+        *mode = lib->step_mode;
     });
 }
 

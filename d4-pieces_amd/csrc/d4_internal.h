@@ -116,6 +116,26 @@ struct ExtendArgs {
 // buffer of `cap` entries per field; children are {key, count} SoA.
 constexpr uint32_t LVL_FIELDS = 5;
 constexpr uint32_t LVL_MAX = 8;  // placements per row (one level each)
+// Control block of row steps replayed from a HIP graph: the kernels of a
+// recorded step read the step's sizes here instead of from their launch
+// arguments, and the step's last kernel (k_row_close) writes the next step's.
+// A step that overflows a buffer sets `fail` and leaves the rest unchanged, so
+// the steps queued after it do nothing and its input frontier stays intact.
+constexpr uint32_t CTL_LOG = 64;  // steps the host may queue between two looks
+struct StepLog {
+    unsigned long long n_front, tab_m, children, pruned, fresh, chw, lv_max, out;
+};
+struct RowCtl {
+    unsigned long long n_front;  // records of the next step's frontier
+    unsigned long long tab_m;    // its dedup table slots (<= tab_cap)
+    unsigned long long tab_cap;  // table slots and next-frontier records allocated
+    uint32_t row;                // its row
+    uint32_t fail;               // 1 level buffer, 2 table, 4 spill, 8 sizes
+    uint32_t done;               // steps completed
+    uint32_t pad;
+    StepLog log[CTL_LOG];        // per completed step, at done % CTL_LOG
+};
+
 struct LevelArgs {
     // level 0 reads records; later levels read the states level - 1 wrote
     const uint64_t *in_key, *in_cnt;
@@ -141,6 +161,7 @@ struct LevelArgs {
     uint64_t occ_mask;
     uint32_t *c32;
     unsigned long long *c64;
+    const RowCtl *ctl;       // set: n_rec and row come from it (graph replay)
 };
 struct DedupArgs {
     const uint64_t *ch_key, *ch_cnt;
@@ -163,6 +184,7 @@ struct DedupArgs {
     uint32_t spill_cap;
     uint32_t *c32;
     unsigned long long *c64;
+    const RowCtl *ctl;       // set: row and tab_size come from it (graph replay)
 };
 void launch_level(const LevelArgs &a, int n_cu, hipStream_t st);
 void launch_dedup(const DedupArgs &a, bool prune, int n_cu, hipStream_t st);
@@ -229,19 +251,31 @@ struct MultiArgs {
 };
 
 // Launchers (d4_kernels.hip, d4_multi.hip).
+// (A non-null ctl, on the kernels of a graph-replayed row step, supplies the
+// sizes -- n, m, row -- from device memory, the arguments being the
+// capacities, and makes the kernel do nothing after a failed step.)
 void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
-                       uint64_t n, hipStream_t st);
+                       uint64_t n, hipStream_t st, const RowCtl *ctl = nullptr);
 void launch_reset_counters(uint32_t *c32, unsigned long long *c64,
-                           hipStream_t st);
+                           hipStream_t st, const RowCtl *ctl = nullptr);
+// dst[i] = src[i], i < n
+void launch_copy_u64(unsigned long long *dst, const unsigned long long *src, uint64_t n,
+                     hipStream_t st, const RowCtl *ctl = nullptr);
+// a graph-replayed row step's last kernel: checks the step's flags, logs it
+// and sets the next step's frontier size, table size and row
+void launch_row_close(RowCtl *ctl, const uint32_t *c32, const unsigned long long *c64,
+                      hipStream_t st);
 void launch_reset_work(unsigned long long *c64, hipStream_t st);
 void launch_zero_u32(uint32_t *p, uint64_t n, hipStream_t st);
 // zero the level kernels' per-chunk counters; first: also the overflow flag
 // and the completed-chunk count
-void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st);
+void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st,
+                         const RowCtl *ctl = nullptr);
 // tiling counts of the records whose instance finishes at this step
 void launch_terminal(const uint64_t *key, const uint64_t *cnt, uint64_t n, uint32_t row,
                      const uint32_t *start, const uint32_t *hgt, uint32_t inst_shift,
-                     unsigned long long *result, int n_cu, hipStream_t st);
+                     unsigned long long *result, int n_cu, hipStream_t st,
+                     const RowCtl *ctl = nullptr);
 void launch_reset_donations(uint32_t *c32, unsigned long long *c64, hipStream_t st);
 // reset one donation counter pair (C32_DON or C32_DON1) and the work counter
 void launch_reset_donation_pair(uint32_t *c32, unsigned long long *c64, uint32_t pair,
@@ -266,7 +300,8 @@ constexpr uint32_t COMPACT_MAX_BLOCKS = 65536;
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
                     uint64_t *out_key, uint64_t *out_cnt, uint64_t out_cap, uint32_t *c32,
-                    uint32_t *blk, uint32_t *slot_uid, hipStream_t st);
+                    uint32_t *blk, uint32_t *slot_uid, hipStream_t st,
+                    const RowCtl *ctl = nullptr);
 // multi-library batch: 4-word table slots, two-word keys
 void launch_fill_table4(unsigned long long *tab, uint64_t n, hipStream_t st);
 void launch_compact4(const unsigned long long *tab, uint64_t m, uint64_t *out_k0,

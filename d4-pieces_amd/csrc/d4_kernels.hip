@@ -520,7 +520,11 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
 
 __global__ __launch_bounds__(256) void k_fill_table(unsigned long long *key,
                                                     unsigned long long *cnt,
-                                                    uint64_t n) {
+                                                    uint64_t n, const RowCtl *ctl) {
+    if (ctl) {
+        if (ctl->fail) return;
+        n = min(n, (uint64_t)ctl->tab_m);
+    }
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         key[2 * i] = EMPTY_KEY;
@@ -540,9 +544,58 @@ __global__ __launch_bounds__(256) void k_fill_table4(unsigned long long *tab, ui
     }
 }
 
-__global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64) {
+__global__ void k_reset_counters(uint32_t *c32, unsigned long long *c64, const RowCtl *ctl) {
+    if (ctl && ctl->fail) return;
     if (threadIdx.x < C32_N) c32[threadIdx.x] = 0;
     if (threadIdx.x < C64_N) c64[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_copy_u64(unsigned long long *dst,
+                                                  const unsigned long long *src, uint64_t n,
+                                                  const RowCtl *ctl) {
+    if (ctl && ctl->fail) return;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+// The last kernel of a graph-replayed row step (one lane): a step whose level
+// buffers, table or spill list overflowed, or whose compaction count differs
+// from its new states, fails and stays put (the frontier it read is intact
+// and the steps after it do nothing); otherwise it is logged and the next
+// step's frontier, row and table size are set -- the table sized as the host
+// sizes a count-mode step from the unique states per parent, capped at the
+// allocation.
+__global__ void k_row_close(RowCtl *ctl, const uint32_t *c32, const unsigned long long *c64) {
+    if (threadIdx.x != 0 || ctl->fail) return;
+    const uint64_t n = ctl->n_front, u = c64[C64_NEW];
+    uint32_t f = (c32[C32_XOVF] ? 1u : 0u) | (c32[C32_OVERFLOW] ? 2u : 0u) |
+                 (c32[C32_SPILL] ? 4u : 0u);
+    if (!f && ((uint64_t)c32[C32_OUT] != u || u > ctl->tab_cap)) f = 8u;
+    if (f) {
+        ctl->fail = f;
+        return;
+    }
+    StepLog &l = ctl->log[ctl->done % CTL_LOG];
+    uint64_t lv = 0;
+    for (uint32_t L = 0; L < LVL_MAX; L++) lv = max(lv, (uint64_t)c64[C64_LVL + L]);
+    l.n_front = n;
+    l.tab_m = ctl->tab_m;
+    l.children = c64[C64_CHILDREN];
+    l.pruned = c64[C64_PRUNED];
+    l.fresh = u;
+    l.chw = c64[C64_CHW];
+    l.lv_max = lv;
+    l.out = c32[C32_OUT];
+    double r = n ? (double)u / (double)n : 1.0;
+    if (r < 0.01) r = 0.01;
+    const uint64_t t = (uint64_t)(1.4 * ((double)u * r * 1.1 + 4096.0));
+    uint64_t m = (t + 1023) & ~1023ull;
+    if (m < 1024) m = 1024;
+    if (m > ctl->tab_cap) m = ctl->tab_cap;
+    ctl->tab_m = m;
+    ctl->n_front = u;
+    ctl->row = ctl->row + 1;
+    ctl->done = ctl->done + 1;
 }
 
 __global__ void k_reset_work(unsigned long long *c64) { c64[C64_WORK] = 0; }
@@ -557,7 +610,12 @@ __global__ __launch_bounds__(256) void k_zero_u32(uint32_t *p, uint64_t n) {
 __global__ __launch_bounds__(256) void k_terminal(const uint64_t *key, const uint64_t *cnt,
                                                   uint64_t n, uint32_t row, const uint32_t *start,
                                                   const uint32_t *hgt, uint32_t inst_shift,
-                                                  unsigned long long *result) {
+                                                  unsigned long long *result, const RowCtl *ctl) {
+    if (ctl) {
+        if (ctl->fail) return;
+        n = min(n, (uint64_t)ctl->n_front);
+        row = ctl->row;
+    }
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
         const uint64_t k = key[i];
         const uint32_t inst = inst_shift < 64 ? (uint32_t)(k >> inst_shift) : 0u;
@@ -567,7 +625,9 @@ __global__ __launch_bounds__(256) void k_terminal(const uint64_t *key, const uin
 
 // the level kernels' per-chunk counters (children written, partial rows per
 // level); at a step's first chunk also the overflow flag and the chunk count
-__global__ void k_reset_levels(uint32_t *c32, unsigned long long *c64, uint32_t first) {
+__global__ void k_reset_levels(uint32_t *c32, unsigned long long *c64, uint32_t first,
+                               const RowCtl *ctl) {
+    if (ctl && ctl->fail) return;
     // after an overflow the counters keep the failed chunk's sizes for the host
     if (!first && c32[C32_XOVF]) return;
     if (threadIdx.x <= LVL_MAX) c64[C64_CHW + threadIdx.x] = 0;
@@ -640,8 +700,12 @@ __global__ void k_reset_donations(uint32_t *c32, unsigned long long *c64) {
 template <int SW>  // table words per slot; the key's first word leads the slot
 __global__ __launch_bounds__(CMP_THREADS) void k_compact_count(
     const unsigned long long *tab_key, uint64_t m, uint64_t per_block,
-    uint32_t *blk_cnt) {
+    uint32_t *blk_cnt, const RowCtl *ctl) {
     __shared__ uint32_t s_w[CMP_THREADS / 64];
+    if (ctl) {
+        if (ctl->fail) return;
+        m = min(m, (uint64_t)ctl->tab_m);
+    }
     const uint64_t lo = blockIdx.x * per_block;
     const uint64_t hi = min(m, lo + per_block);
     uint32_t n = 0;
@@ -659,8 +723,10 @@ __global__ __launch_bounds__(CMP_THREADS) void k_compact_count(
 
 // Exclusive scan of n block counts by one workgroup; total -> c32[C32_OUT].
 __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t *blk_cnt, uint32_t n,
-                                                      uint32_t *blk_off, uint32_t *c32) {
+                                                      uint32_t *blk_off, uint32_t *c32,
+                                                      const RowCtl *ctl) {
     __shared__ uint32_t s_part[1024];
+    if (ctl && ctl->fail) return;
     const uint32_t per = (n + 1023) / 1024;
     const uint32_t lo = threadIdx.x * per, hi = min(n, lo + per);
     uint32_t t = 0;
@@ -689,8 +755,12 @@ template <int SW>
 __global__ __launch_bounds__(CMP_THREADS) void k_compact_write(
     const unsigned long long *tab_key, const unsigned long long *tab_cnt, uint64_t m,
     uint64_t per_block, const uint32_t *blk_off, uint64_t *out_key, uint64_t *out_k1,
-    uint64_t *out_cnt, uint32_t *slot_uid, uint64_t out_cap) {
+    uint64_t *out_cnt, uint32_t *slot_uid, uint64_t out_cap, const RowCtl *ctl) {
     __shared__ uint32_t s_w[CMP_THREADS / 64];
+    if (ctl) {
+        if (ctl->fail) return;
+        m = min(m, (uint64_t)ctl->tab_m);
+    }
     const uint64_t lo = blockIdx.x * per_block;
     const uint64_t hi = min(m, lo + per_block);
     const uint32_t wv = threadIdx.x >> 6;
@@ -883,27 +953,40 @@ size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool enum_mode) {
 }
 
 void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
-                       uint64_t n, hipStream_t st) {
+                       uint64_t n, hipStream_t st, const RowCtl *ctl) {
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (!blocks) blocks = 1;
-    hipLaunchKernelGGL(k_fill_table, dim3((uint32_t)blocks), dim3(256), 0, st, key, cnt, n);
+    hipLaunchKernelGGL(k_fill_table, dim3((uint32_t)blocks), dim3(256), 0, st, key, cnt, n, ctl);
 }
 
-void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c32, c64);
+void launch_reset_counters(uint32_t *c32, unsigned long long *c64, hipStream_t st,
+                           const RowCtl *ctl) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c32, c64, ctl);
 }
 
-void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset_levels, dim3(1), dim3(64), 0, st, c32, c64, (uint32_t)first);
+void launch_copy_u64(unsigned long long *dst, const unsigned long long *src, uint64_t n,
+                     hipStream_t st, const RowCtl *ctl) {
+    const uint64_t b = std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), 1024);
+    hipLaunchKernelGGL(k_copy_u64, dim3((uint32_t)b), dim3(256), 0, st, dst, src, n, ctl);
+}
+
+void launch_row_close(RowCtl *ctl, const uint32_t *c32, const unsigned long long *c64,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_row_close, dim3(1), dim3(64), 0, st, ctl, c32, c64);
+}
+
+void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st,
+                         const RowCtl *ctl) {
+    hipLaunchKernelGGL(k_reset_levels, dim3(1), dim3(64), 0, st, c32, c64, (uint32_t)first, ctl);
 }
 
 void launch_terminal(const uint64_t *key, const uint64_t *cnt, uint64_t n, uint32_t row,
                      const uint32_t *start, const uint32_t *hgt, uint32_t inst_shift,
-                     unsigned long long *result, int n_cu, hipStream_t st) {
+                     unsigned long long *result, int n_cu, hipStream_t st, const RowCtl *ctl) {
     const uint64_t b = std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), (uint64_t)n_cu * 8);
     hipLaunchKernelGGL(k_terminal, dim3((uint32_t)b), dim3(256), 0, st, key, cnt, n, row, start,
-                       hgt, inst_shift, result);
+                       hgt, inst_shift, result, ctl);
 }
 
 void launch_zero_u32(uint32_t *p, uint64_t n, hipStream_t st) {
@@ -1005,18 +1088,18 @@ void launch_extend(const ExtendArgs &a, bool enum_mode, bool prune, bool count_o
 void launch_compact(const unsigned long long *tab_key,
                     const unsigned long long *tab_cnt, uint64_t m,
                     uint64_t *out_key, uint64_t *out_cnt, uint64_t out_cap, uint32_t *c32,
-                    uint32_t *blk, uint32_t *slot_uid, hipStream_t st) {
+                    uint32_t *blk, uint32_t *slot_uid, hipStream_t st, const RowCtl *ctl) {
     // blk holds 2 * COMPACT_MAX_BLOCKS words: counts, then offsets
     uint64_t per_block = 4096;
     while ((m + per_block - 1) / per_block > COMPACT_MAX_BLOCKS) per_block *= 2;
     const uint32_t nb = (uint32_t)((m + per_block - 1) / per_block);
     hipLaunchKernelGGL(k_compact_count<2>, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, m,
-                       per_block, blk);
+                       per_block, blk, ctl);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blk, nb,
-                       blk + COMPACT_MAX_BLOCKS, c32);
+                       blk + COMPACT_MAX_BLOCKS, c32, ctl);
     hipLaunchKernelGGL(k_compact_write<2>, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, tab_cnt,
                        m, per_block, blk + COMPACT_MAX_BLOCKS, out_key, nullptr, out_cnt,
-                       slot_uid, out_cap);
+                       slot_uid, out_cap, ctl);
 }
 
 void launch_fill_table4(unsigned long long *tab, uint64_t n, hipStream_t st) {
@@ -1031,12 +1114,12 @@ void launch_compact4(const unsigned long long *tab, uint64_t m, uint64_t *out_k0
     while ((m + per_block - 1) / per_block > COMPACT_MAX_BLOCKS) per_block *= 2;
     const uint32_t nb = (uint32_t)((m + per_block - 1) / per_block);
     hipLaunchKernelGGL(k_compact_count<4>, dim3(nb), dim3(CMP_THREADS), 0, st, tab, m,
-                       per_block, blk);
+                       per_block, blk, (const RowCtl *)nullptr);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, blk, nb,
-                       blk + COMPACT_MAX_BLOCKS, c32);
+                       blk + COMPACT_MAX_BLOCKS, c32, (const RowCtl *)nullptr);
     hipLaunchKernelGGL(k_compact_write<4>, dim3(nb), dim3(CMP_THREADS), 0, st, tab, tab + 2, m,
                        per_block, blk + COMPACT_MAX_BLOCKS, out_k0, out_k1, out_cnt, nullptr,
-                       out_cap);
+                       out_cap, (const RowCtl *)nullptr);
 }
 
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,

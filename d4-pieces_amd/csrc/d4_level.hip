@@ -98,6 +98,12 @@ __device__ __forceinline__ uint64_t place(Writer &w, uint64_t m, unsigned long l
 template <bool FROM_REC, int FR>
 __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     extern __shared__ uint64_t lds[];
+    if (a.ctl) {
+        // graph replay: the launch carries capacities, the step's sizes are here
+        if (a.ctl->fail) return;
+        a.n_rec = (uint32_t)min((uint64_t)a.ctl->n_front, (uint64_t)a.n_rec);
+        a.row = a.ctl->row;
+    }
     const uint32_t nfit = 8u * a.fit_rows * 256u;
     uint64_t *s_fit = lds;
     const PlaceInfo *s_pc = reinterpret_cast<const PlaceInfo *>(s_fit + nfit);
@@ -322,6 +328,11 @@ __device__ __forceinline__ int dedup_insert(const DedupArgs &a, uint64_t key, ui
 template <bool PRUNE, bool LDSW>
 __global__ __launch_bounds__(256) void k_dedup(DedupArgs a) {
     __shared__ uint64_t s_win[LDSW ? DEDUP_LDS_INST : 1];
+    if (a.ctl) {
+        if (a.ctl->fail) return;
+        a.row = a.ctl->row;
+        a.tab_size = min((uint64_t)a.ctl->tab_m, a.tab_size);
+    }
     if (LDSW) {
         for (uint32_t i = threadIdx.x; i < a.n_inst; i += 256) {
             const uint32_t rr = a.row + 1 - a.start[i];

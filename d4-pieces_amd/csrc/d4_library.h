@@ -2,6 +2,7 @@
 // and, for the single-library kernel, its LDS tables.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <mutex>
 #include <vector>
@@ -32,8 +33,31 @@ struct d4_library {
     // k_dedup), D4_KERNEL_EXPAND (k_expand: snapshots on a per-wave stack) or
     // D4_KERNEL_DFS (k_extend: one depth-first search per lane)
     uint32_t xp_kernel = D4_KERNEL_LEVEL;
+    // count-mode level row steps: D4_STEP_GENERAL / _FUSED / _GRAPH
+    uint32_t step_mode = D4_STEP_FUSED;
     uint32_t xp_budget = 256;     // k_expand: tail rounds before a wave's snapshots move on
     uint32_t don_items = 1u << 20;  // donated work items / spilled partial rows per buffer
+    // Level-buffer sizes seen by earlier searches on this library, per row
+    // step index of a search (widest level and children per parent): the
+    // first rows' fan-out is unknown to a search's own history, and a level
+    // buffer sized below it reruns its chunk.  Shared by engines on several
+    // threads, hence the lock.
+    static constexpr uint32_t kSizeRows = 64;
+    mutable std::mutex size_mu;
+    mutable float seen_lv[kSizeRows] = {}, seen_ch[kSizeRows] = {};
+    void note_sizes(uint64_t step, double lv, double ch) const {
+        if (step >= kSizeRows) return;
+        std::lock_guard<std::mutex> lk(size_mu);
+        seen_lv[step] = std::max(seen_lv[step], (float)lv);
+        seen_ch[step] = std::max(seen_ch[step], (float)ch);
+    }
+    void seen_sizes(uint64_t step, double &lv, double &ch) const {
+        lv = ch = 0.0;
+        if (step >= kSizeRows) return;
+        std::lock_guard<std::mutex> lk(size_mu);
+        lv = seen_lv[step];
+        ch = seen_ch[step];
+    }
     // single-library tables (at most MAX_CAND candidates: one u64 mask),
     // built and uploaded on first use by an engine
     std::mutex fast_mu;

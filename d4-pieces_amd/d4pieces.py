@@ -33,6 +33,7 @@ D4_OK, D4_EINVAL, D4_ENOMEM, D4_EDEVICE, D4_ESTATE, D4_ERANGE = 0, -1, -2, -3, -
 MODE_COUNT, MODE_ENUM = 0, 1
 FLAG_NO_PRUNE = 1
 KERNEL_DFS, KERNEL_EXPAND, KERNEL_LEVEL = 0, 1, 2
+STEP_GENERAL, STEP_FUSED, STEP_GRAPH = 0, 1, 2
 
 
 class D4Error(RuntimeError):
@@ -68,6 +69,9 @@ class Stats(ctypes.Structure):
         ("dedup_bytes", ctypes.c_uint64),
         ("chunk_reruns", ctypes.c_uint64),
         ("donation_full", ctypes.c_uint64),
+        ("fused_steps", ctypes.c_uint64),
+        ("fused_fallbacks", ctypes.c_uint64),
+        ("graph_replays", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -100,6 +104,8 @@ SIGNATURES = [
     ("d4_library_set_budget", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_library_set_kernel", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     ("d4_library_set_donation_capacity", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_library_set_step_mode", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_library_get_step_mode", ctypes.c_int, [_P, _U32P]),
     ("d4_searcher_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
     ("d4_searcher_set_sample", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint64]),
     ("d4_searcher_sample", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
@@ -373,14 +379,25 @@ class PieceLibrary:
         return v.value
 
     def set_kernel(self, kernel: int, tail_rounds: int = 256) -> None:
-        """Count-mode row-step kernel: KERNEL_EXPAND (default) or KERNEL_DFS;
-        tail_rounds: k_expand rounds after the records run out before a wave's
-        partial rows move to a follow-up launch (0: never)."""
+        """Count-mode row-step kernel: KERNEL_LEVEL (default), KERNEL_EXPAND or
+        KERNEL_DFS; tail_rounds: k_expand rounds after the records run out
+        before a wave's partial rows move to a follow-up launch (0: never)."""
         _check(load().d4_library_set_kernel(self._h, kernel, tail_rounds))
 
     def set_donation_capacity(self, items: int) -> None:
         """Items per donation buffer (default 1 Mi)."""
         _check(load().d4_library_set_donation_capacity(self._h, items))
+
+    def set_step_mode(self, mode: int) -> None:
+        """How count-mode level row steps are issued: STEP_GENERAL (host round
+        trips between the parts), STEP_FUSED (default: one queued chain, one
+        round trip) or STEP_GRAPH (that chain replayed from a HIP graph)."""
+        _check(load().d4_library_set_step_mode(self._h, mode))
+
+    def get_step_mode(self) -> int:
+        m = ctypes.c_uint32()
+        _check(load().d4_library_get_step_mode(self._h, ctypes.byref(m)))
+        return m.value
 
     def get_kernel(self) -> Tuple[int, int]:
         k, t = ctypes.c_uint32(), ctypes.c_uint32()

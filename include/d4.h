@@ -85,6 +85,11 @@ typedef struct d4_stats {
     uint64_t dedup_bytes;      /* 16 per child read + 16 per new state      */
     uint64_t chunk_reruns;     /* level-kernel chunks rerun with larger buffers */
     uint64_t donation_full;    /* donations / spills refused by a full buffer */
+    uint64_t fused_steps;      /* row steps queued as one chain with a single
+                                  host round trip (level kernels, count mode) */
+    uint64_t fused_fallbacks;  /* of those, steps rerun on the general path
+                                  (a buffer overflowed or children spilled) */
+    uint64_t graph_replays;    /* fused row steps replayed from a HIP graph  */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -166,6 +171,19 @@ int d4_library_get_budget(const d4_library *lib, uint32_t *steps);
 enum { D4_KERNEL_DFS = 0, D4_KERNEL_EXPAND = 1, D4_KERNEL_LEVEL = 2 };
 int d4_library_set_kernel(d4_library *lib, uint32_t kernel, uint32_t tail_rounds);
 int d4_library_get_kernel(const d4_library *lib, uint32_t *kernel, uint32_t *tail_rounds);
+/* How a count-mode row step on the level kernels is issued (no counterpart in
+ * the reference, whose launch loop is host-driven per row: src/device.cu:39-162):
+ *   D4_STEP_GENERAL: host round trips between the search, the table checks and
+ *     the compaction (about four per row step);
+ *   D4_STEP_FUSED (the default): the whole row step -- table fill, counter
+ *     resets, terminal pass, level launches, dedup, compaction -- queued as one
+ *     chain with one host round trip; a step whose buffers overflow or whose
+ *     children spill is rerun on the general path;
+ *   D4_STEP_GRAPH: the fused chain captured once into a HIP graph and replayed
+ *     row after row, its sizes and buffer choice read from device memory. */
+enum { D4_STEP_GENERAL = 0, D4_STEP_FUSED = 1, D4_STEP_GRAPH = 2 };
+int d4_library_set_step_mode(d4_library *lib, uint32_t mode);
+int d4_library_get_step_mode(const d4_library *lib, uint32_t *mode);
 /* Items per donation buffer of the depth-first and snapshot kernels (default
  * 1 Mi; at least 64).  A lane or wave that finds the buffer full keeps its
  * work (counted in d4_stats.donation_full); small values exercise that path. */

@@ -360,3 +360,69 @@ def test_large_sample_states_match_oracle(gpu, p5, oracle):
         want = oracle.count(p5, left, sub)
         sl = gpu.PieceLibrary(p5, left, 0xFF, 8)
         assert gpu_count(gpu, sl, sub) == want
+
+
+@pytest.mark.parametrize("mode", ["general", "fused", "graph"])
+@pytest.mark.parametrize("scale", ["1", "0.02"])
+def test_step_modes_agree(gpu, p5, oracle, monkeypatch, mode, scale):
+    """Count-mode level row steps issued with host round trips between their
+    parts, as one queued chain with a single round trip, or replayed from
+    recorded HIP graphs (many steps per round trip), give the oracle's
+    counts: the 256 seeded 8x8 boards (fixture), 6x10 and random small
+    instances with multiplicities -- every replay of the graphs, with host
+    work and waits between them, checked through the counts.  With the dedup
+    table estimate scaled to 2% the tables overflow and spill, so fused and
+    replayed steps fail and rerun on the general path; the counts stay the
+    same."""
+    import json
+    import os
+    monkeypatch.setenv("D4_TABLE_SCALE", scale)
+    m = {"general": gpu.STEP_GENERAL, "fused": gpu.STEP_FUSED, "graph": gpu.STEP_GRAPH}[mode]
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    lib.set_step_mode(m)
+    assert lib.get_step_mode() == m
+    b = gpu.Batch(lib, fx["boards"])
+    b.run()
+    assert b.counts() == fx["counts"]
+    st = b.stats()
+    if mode == "fused":
+        assert st["fused_steps"] > 0
+        if scale != "1":
+            assert st["fused_fallbacks"] > 0
+        else:
+            # fewer host round trips than the same search on the general path
+            lg = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+            lg.set_step_mode(gpu.STEP_GENERAL)
+            bg = gpu.Batch(lg, fx["boards"])
+            bg.run()
+            assert bg.counts() == fx["counts"]
+            assert st["host_syncs"] < bg.stats()["host_syncs"]
+    elif mode == "graph":
+        assert st["graph_replays"] > 0
+        if scale != "1":
+            assert st["fused_fallbacks"] > 0
+    else:
+        assert st["fused_steps"] == 0 and st["graph_replays"] == 0
+    lib6 = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 6)
+    lib6.set_step_mode(m)
+    for _ in range(2):  # (the second search replays the graphs the first recorded)
+        s = gpu.CudaSearcher(lib6, rows=rect(6, 10))
+        s.run()
+        assert s.count() == 9356
+    if mode == "graph":
+        assert s.stats()["graph_replays"] > 0
+    rng = random.Random(12)
+    for _ in range(6):
+        shapes = rng.sample(gpu.free_polyominoes(4) + gpu.free_polyominoes(3), 4)
+        counts = [rng.randint(1, 2) for _ in shapes]
+        w = rng.randint(3, 6)
+        area = sum(bin(s).count("1") * c for s, c in zip(shapes, counts))
+        h = -(-area // w)
+        rows = [(1 << w) - 1] * h
+        extra = w * h - area
+        rows[-1] &= ((1 << w) - 1) >> extra
+        lb = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+        lb.set_step_mode(m)
+        assert gpu_count(gpu, lb, rows) == oracle.count(shapes, counts, rows)
