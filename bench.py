@@ -33,6 +33,7 @@ import argparse
 import json
 import os
 import sys
+import statistics
 import time
 from concurrent.futures import ThreadPoolExecutor
 
@@ -375,6 +376,72 @@ def sub_board(rows, g, key, maxh, layout, counts):
     return sub, left
 
 
+def step_modes_section(d4, workloads, boards, expected, serve_slots, n_req=300, rate=200.0):
+    """The same work with count-mode row steps issued three ways: host round
+    trips between the parts of a step, one queued chain per step, and steps
+    replayed from recorded HIP graphs (many per round trip).  Per mode: the
+    batched 256-board solve, the 6x10 board alone (latency), and a served
+    stream (the count-mode requests' engine steps that way)."""
+    shapes = workloads.pentominoes()
+    s6, c6, r6, _ = workloads.plain_6x10()
+    reqs = workloads.request_stream(n_req, rate=rate, seed=5)
+    out = {}
+    for name, mode in (("general", d4.STEP_GENERAL), ("fused", d4.STEP_FUSED),
+                       ("graph", d4.STEP_GRAPH)):
+        lib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
+        lib.set_step_mode(mode)
+        ts, st = [], None
+        for it in range(4):
+            b = d4.Batch(lib, boards)
+            t = time.perf_counter()
+            b.run()
+            dt = time.perf_counter() - t
+            if b.counts() != expected:
+                raise SystemExit(f"step mode {name}: batched counts differ")
+            if it:
+                ts.append(dt * 1e3)
+                st = b.stats()
+        lib6 = d4.PieceLibrary(s6, c6, 0xFF, 6)
+        lib6.set_step_mode(mode)
+        t6, st6 = [], None
+        for it in range(23):
+            sr = d4.CudaSearcher(lib6, rows=r6)
+            t = time.perf_counter()
+            sr.run()
+            n6 = sr.count()
+            dt = time.perf_counter() - t
+            if n6 != 9356:
+                raise SystemExit(f"step mode {name}: 6x10 count {n6}")
+            if it >= 3:
+                t6.append(dt * 1e3)
+                st6 = sr.stats()
+        slib = d4.PieceLibrary(shapes, [1] * 12, 0xFF, 8)
+        slib.set_step_mode(mode)
+        lat, wall, res, sst, step_s, steps = serve_run(d4, slib, reqs, serve_slots)
+        rows = st["rows_done"]
+        out[name] = {
+            "batched_ms": statistics.median(ts),
+            "batched_host_syncs_per_row_step": st["host_syncs"] / rows,
+            "batched_gpu_ms": st["gpu_ms"],
+            "plain_6x10_p50_ms": percentile(t6, 0.5),
+            "plain_6x10_p99_ms": percentile(t6, 0.99),
+            "plain_6x10_gpu_ms": st6["gpu_ms"],
+            "plain_6x10_host_syncs": st6["host_syncs"],
+            "plain_6x10_graph_replays": st6["graph_replays"],
+            "served_rate": rate,
+            "served_p50_ms": percentile(list(lat.values()), 0.5),
+            "served_p99_ms": percentile(list(lat.values()), 0.99),
+            "served_requests_per_s": len(res) / wall,
+            "served_count_engine_host_syncs_per_step":
+                sst["all"]["host_syncs"] / max(1, sst["all"]["rows_done"]),
+        }
+        log(f"step mode {name}: batched {out[name]['batched_ms']:.1f} ms "
+            f"({out[name]['batched_host_syncs_per_row_step']:.2f} syncs/row step), 6x10 p50 "
+            f"{out[name]['plain_6x10_p50_ms']:.2f} ms, served p50 "
+            f"{out[name]['served_p50_ms']:.1f} / p99 {out[name]['served_p99_ms']:.1f} ms")
+    return out
+
+
 def large_section(d4, workloads, oracle, sample_height, n_sample, cpu_seconds):
     """12 pentominoes x2 on 8x15: one count-mode search under the default
     memory cap (80% of free device memory; a row that would not fit parks
@@ -471,6 +538,8 @@ def main():
                     help="how count-mode level row steps are issued: host round trips "
                          "between their parts, one queued chain, or that chain replayed "
                          "from a HIP graph")
+    ap.add_argument("--no-step-modes", action="store_true",
+                    help="skip the general / fused / graph-replay comparison")
     ap.add_argument("--tail-rounds", type=int, default=256,
                     help="expand kernel: rounds after a launch's records run out before "
                          "a wave's partial rows move to a follow-up launch")
@@ -680,6 +749,9 @@ def main():
         out["latency_ms"]["served_p99"] = head["latency_ms_p99"]
         out["served_vs_plain"] = sat["served_vs_plain"]
         out["served_saturated_requests_per_s"] = sat["requests_per_s"]
+
+    if not args.no_step_modes and kernel == d4.KERNEL_LEVEL:
+        out["step_modes"] = step_modes_section(d4, workloads, boards, counts, args.serve_slots)
 
     if args.uneven:
         out["uneven"] = uneven_section(d4, workloads, args.uneven, args.uneven_budget,

@@ -1086,6 +1086,8 @@ struct Engine {
         const hipError_t inst_err = hipGraphInstantiate(&gx[p], graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIPCHK(inst_err);
+        // (uploaded now, so that the first replay does not pay for it)
+        HIPCHK(hipGraphUpload(gx[p], st));
     }
 
     // Up to kmax count-mode row steps replayed from the recorded graphs, with
@@ -1104,12 +1106,32 @@ struct Engine {
             tab0 = round_slots((uint64_t)(1.4 * table_scale *
                                           ((double)n_front * std::max(grow_unique_raw, 0.01) *
                                                1.1 + 4096.0)));
-        // (the frontier buffers hold the current frontier too)
+        // Capacities for every step of the run: the table and frontier
+        // buffers with room for eight times the first step's table (capped;
+        // the frontier buffers hold the current frontier too), and level
+        // buffers for that many parents at the widest sizes per parent seen
+        // so far (this search's and the library's for the rows ahead).  A
+        // step that outgrows them fails and reruns; spare room costs memory
+        // only, as the kernels read the step's sizes from the control block.
+        if (2 * tab0 > kGraphMaxSlots || n_front > kGraphMaxSlots) return 0;
         const uint64_t cap = std::max<uint64_t>(
-            g_cap, next_pow2(std::max<uint64_t>({2 * tab0, n_front, 1u << 16})));
-        if (cap > kGraphMaxSlots) return 0;
-        const uint64_t lv = std::max<uint64_t>(g_lv, level_cap(4 * n_front, lv_pp0()));
-        const uint64_t ch = std::max<uint64_t>(g_ch, level_cap(4 * n_front, ch_pp0()));
+            g_cap, std::min<uint64_t>(kGraphMaxSlots,
+                                      next_pow2(std::max<uint64_t>({8 * tab0, n_front, 1u << 16}))));
+        // (the first step's own sizes per parent include its children bound,
+        // which applies to its parents only)
+        double lpp = std::max(lv_ratio, 2.0), cpp = std::max(ch_ratio, 4.0);
+        for (uint64_t r = stats.rows_done; r < stats.rows_done + kmax && r < lib->kSizeRows; r++) {
+            double a = 0, b = 0;
+            lib->seen_sizes(r, a, b);
+            lpp = std::max(lpp, 1.1 * a);
+            cpp = std::max(cpp, 1.1 * b);
+        }
+        const uint64_t par = std::min<uint64_t>(cap, std::max<uint64_t>(8 * n_front, 1u << 16));
+        const uint64_t lv = std::max<uint64_t>(
+            {g_lv, level_cap(par, lpp), level_cap(n_front, lv_pp0())});
+        const uint64_t ch = std::max<uint64_t>(
+            {g_ch, level_cap(par, cpp), level_cap(n_front, ch_pp0())});
+        if (2ull * 8 * LVL_FIELDS * lv + 16ull * ch > level_budget()) return 0;
         if (mem_cap && 16ull * cap + 32ull * cap + 2ull * 8 * LVL_FIELDS * lv + 16ull * ch +
                                16ull * n_front + parked_bytes > mem_cap)
             return 0;
@@ -1146,6 +1168,7 @@ struct Engine {
         ctl_h->tab_m = std::min(tab0, g_cap);
         ctl_h->tab_cap = g_cap;
         ctl_h->row = g;
+        ctl_h->ratio = stats.rows_done ? grow_unique_raw : 0.0;
         ctl_h->fail = 0;
         ctl_h->done = 0;
         HIPCHK(hipMemcpyAsync(ctl_d.p, ctl_h, offsetof(RowCtl, log), hipMemcpyHostToDevice, st));

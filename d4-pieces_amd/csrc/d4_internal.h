@@ -129,6 +129,7 @@ struct RowCtl {
     unsigned long long n_front;  // records of the next step's frontier
     unsigned long long tab_m;    // its dedup table slots (<= tab_cap)
     unsigned long long tab_cap;  // table slots and next-frontier records allocated
+    double ratio;                // unique states per parent of the step before
     uint32_t row;                // its row
     uint32_t fail;               // 1 level buffer, 2 table, 4 spill, 8 sizes
     uint32_t done;               // steps completed

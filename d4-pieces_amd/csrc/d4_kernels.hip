@@ -586,9 +586,12 @@ __global__ void k_row_close(RowCtl *ctl, const uint32_t *c32, const unsigned lon
     l.chw = c64[C64_CHW];
     l.lv_max = lv;
     l.out = c32[C32_OUT];
+    // (growth that speeds up from step to step is extrapolated once more)
     double r = n ? (double)u / (double)n : 1.0;
     if (r < 0.01) r = 0.01;
-    const uint64_t t = (uint64_t)(1.4 * ((double)u * r * 1.1 + 4096.0));
+    const double trend = ctl->ratio > 0.0 && r > ctl->ratio ? r / ctl->ratio : 1.0;
+    const uint64_t t = (uint64_t)(1.4 * ((double)u * r * trend * 1.1 + 4096.0));
+    ctl->ratio = r;
     uint64_t m = (t + 1023) & ~1023ull;
     if (m < 1024) m = 1024;
     if (m > ctl->tab_cap) m = ctl->tab_cap;
@@ -955,7 +958,8 @@ size_t extend_lds_bytes(uint32_t fit_rows, uint32_t threads, bool enum_mode) {
 void launch_fill_table(unsigned long long *key, unsigned long long *cnt,
                        uint64_t n, hipStream_t st, const RowCtl *ctl) {
     uint64_t blocks = (n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
+    // (n is a capacity under a control block: fewer blocks, each striding)
+    if (blocks > (ctl ? 2048 : 65536)) blocks = ctl ? 2048 : 65536;
     if (!blocks) blocks = 1;
     hipLaunchKernelGGL(k_fill_table, dim3((uint32_t)blocks), dim3(256), 0, st, key, cnt, n, ctl);
 }
@@ -1090,8 +1094,10 @@ void launch_compact(const unsigned long long *tab_key,
                     uint64_t *out_key, uint64_t *out_cnt, uint64_t out_cap, uint32_t *c32,
                     uint32_t *blk, uint32_t *slot_uid, hipStream_t st, const RowCtl *ctl) {
     // blk holds 2 * COMPACT_MAX_BLOCKS words: counts, then offsets
+    // (m is a capacity under a control block: at most 2048 blocks, so that
+    // the blocks past the step's table end cost little)
     uint64_t per_block = 4096;
-    while ((m + per_block - 1) / per_block > COMPACT_MAX_BLOCKS) per_block *= 2;
+    while ((m + per_block - 1) / per_block > (ctl ? 2048u : COMPACT_MAX_BLOCKS)) per_block *= 2;
     const uint32_t nb = (uint32_t)((m + per_block - 1) / per_block);
     hipLaunchKernelGGL(k_compact_count<2>, dim3(nb), dim3(CMP_THREADS), 0, st, tab_key, m,
                        per_block, blk, ctl);
