@@ -214,6 +214,18 @@ struct MLib {
     uint32_t gcd, min_size, n_pieces, n_cand;
 };
 
+// What the device needs to build one library's MLib (k_build_mlib): the
+// candidates and pieces in MLib's own encoding, plus the candidates' pieces,
+// the board width and the pruning constants.
+struct MLibSrc {
+    uint64_t cand[MAX_LIB_CAND][2];  // as MLib::cand
+    uint64_t piece[MAX_PIECES];      // as MLib::piece
+    uint8_t cpiece[MAX_LIB_CAND];    // piece of each candidate
+    uint32_t W, maxh, gcd, min_size, n_pieces, n_cand;
+};
+// Builds dst[i] from src[i], i < n, one workgroup per library (d4_tables.hip).
+void launch_build_mlib(const MLibSrc *src, MLib *dst, uint32_t n, hipStream_t st);
+
 // Multi-library state key: word 0 = occupancy | instance << MKEY_INST_SHIFT,
 // word 1 = use counters.
 constexpr uint32_t MKEY_INST_SHIFT = 48;

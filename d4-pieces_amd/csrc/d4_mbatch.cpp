@@ -23,7 +23,38 @@
 namespace d4 {
 namespace {
 
-// Device tables of one library (layout in d4_internal.h).
+// The candidates and pieces of a library in MLib's encoding, with what the
+// device table builder needs besides (k_build_mlib, d4_tables.hip).
+void mlib_src(const d4_library &L, MLibSrc &S) {
+    std::memset(&S, 0, sizeof S);
+    const uint32_t k = L.ncand;
+    for (uint32_t j = 0; j < k; j++) {
+        S.cand[j][0] = L.cnormal[j];
+        S.cand[j][1] = (uint64_t)(L.cinfo[j] & 0xffffu) | ((uint64_t)(L.cinfo[j] >> 16) << 16) |
+                       ((uint64_t)L.cminc[j] << 32) | ((uint64_t)L.clo[j] << 40) |
+                       ((uint64_t)L.chi[j] << 48);
+        S.cpiece[j] = L.cpiece[j];
+    }
+    for (uint32_t i = 0; i < L.n; i++) {
+        uint64_t lo = 0, hi = 0;
+        for (uint32_t j = 0; j < k; j++)
+            if (L.cpiece[j] == i) {
+                lo = L.clo[j];
+                hi = L.chi[j];
+                break;
+            }
+        S.piece[i] = (uint64_t)L.off[i] | ((uint64_t)((1u << L.fbits[i]) - 1u) << 8) |
+                     ((uint64_t)L.counts[i] << 16) | (lo << 32) | (hi << 40);
+    }
+    S.W = L.W;
+    S.maxh = L.maxh;
+    S.gcd = L.gcd;
+    S.min_size = L.min_size;
+    S.n_pieces = L.n;
+    S.n_cand = k;
+}
+
+// Device tables of one library built on the host (layout in d4_internal.h).
 void build_mlib(const d4_library &L, MLib &M) {
     std::memset(&M, 0, sizeof M);
     const uint32_t k = L.ncand;
@@ -121,6 +152,7 @@ struct d4_mbatch {
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evx0 = nullptr, evx1 = nullptr, evc0 = nullptr;
     DevBuf<MLib> libs;
+    std::vector<uint32_t> lib_index;  // instance -> its library's tables in libs
     DevBuf<uint32_t> inst_lib, hgt;
     DevBuf<uint64_t> bw;
     DevBuf<unsigned long long> result, result_save;
@@ -654,7 +686,8 @@ static std::unique_ptr<d4_mbatch> make_mbatch(uint32_t n_inst, const d4_library 
     {
         if (!n_inst || n_inst > MAX_MULTI_INST)
             throw Error(D4_EINVAL, "need 1..32768 instances");
-        if (flags & ~(uint32_t)D4_FLAG_NO_PRUNE) throw Error(D4_EINVAL, "unknown flags");
+        if (flags & ~(uint32_t)(D4_FLAG_NO_PRUNE | D4_FLAG_HOST_TABLES))
+            throw Error(D4_EINVAL, "unknown flags");
         auto B = std::make_unique<d4_mbatch>();
         B->N = n_inst;
         B->flags = flags;
@@ -689,11 +722,28 @@ static std::unique_ptr<d4_mbatch> make_mbatch(uint32_t n_inst, const d4_library 
         HIPCHK(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking));
         for (hipEvent_t *e : {&B->ev0, &B->ev1, &B->evx0, &B->evx1, &B->evc0})
             HIPCHK(hipEventCreate(e));
-        std::vector<MLib> ml(uniq.size());
-        for (size_t j = 0; j < uniq.size(); j++) build_mlib(*uniq[j], ml[j]);
-        B->libs.reserve(ml.size());
-        HIPCHK(hipMemcpyAsync(B->libs.p, ml.data(), sizeof(MLib) * ml.size(),
-                              hipMemcpyHostToDevice, B->st));
+        // per-library tables: built on the device from each library's
+        // candidates (2.4 KB sent per library instead of 92 KB of tables),
+        // or on the host (D4_FLAG_HOST_TABLES)
+        B->libs.reserve(uniq.size());
+        std::vector<MLib> ml;
+        std::vector<MLibSrc> ms;
+        DevBuf<MLibSrc> msrc;
+        if (flags & D4_FLAG_HOST_TABLES) {
+            ml.resize(uniq.size());
+            for (size_t j = 0; j < uniq.size(); j++) build_mlib(*uniq[j], ml[j]);
+            HIPCHK(hipMemcpyAsync(B->libs.p, ml.data(), sizeof(MLib) * ml.size(),
+                                  hipMemcpyHostToDevice, B->st));
+        } else {
+            ms.resize(uniq.size());
+            for (size_t j = 0; j < uniq.size(); j++) mlib_src(*uniq[j], ms[j]);
+            msrc.reserve(ms.size());
+            HIPCHK(hipMemcpyAsync(msrc.p, ms.data(), sizeof(MLibSrc) * ms.size(),
+                                  hipMemcpyHostToDevice, B->st));
+            launch_build_mlib(msrc.p, B->libs.p, (uint32_t)ms.size(), B->st);
+            HIPCHK(hipGetLastError());
+        }
+        B->lib_index = il;
         B->inst_lib.reserve(n_inst);
         HIPCHK(hipMemcpyAsync(B->inst_lib.p, il.data(), 4ull * n_inst, hipMemcpyHostToDevice,
                               B->st));
@@ -829,3 +879,27 @@ int d4_mbatch_terminals(const d4_mbatch *b, uint32_t *inst, uint64_t *counters, 
 void d4_mbatch_destroy(d4_mbatch *b) { delete b; }
 
 }  // extern "C"
+
+uint64_t d4_mlib_table_bytes(void) { return sizeof(MLib); }
+
+int d4_mbatch_tables(const d4_mbatch *b, uint32_t inst, void *out, uint64_t bytes) {
+    return guard([&] {
+        if (!b || !out) throw Error(D4_EINVAL, "null argument");
+        if (inst >= b->N) throw Error(D4_EINVAL, "instance out of range");
+        if (bytes < sizeof(MLib)) throw Error(D4_ERANGE, "buffer smaller than the tables");
+        HIPCHK(hipSetDevice(b->device));
+        HIPCHK(hipMemcpyAsync(out, b->libs.p + b->lib_index[inst], sizeof(MLib),
+                              hipMemcpyDeviceToHost, b->st));
+        HIPCHK(hipStreamSynchronize(b->st));
+    });
+}
+
+int d4_library_host_tables(const d4_library *lib, void *out, uint64_t bytes) {
+    return guard([&] {
+        if (!lib || !out) throw Error(D4_EINVAL, "null argument");
+        if (bytes < sizeof(MLib)) throw Error(D4_ERANGE, "buffer smaller than the tables");
+        auto m = std::make_unique<MLib>();
+        build_mlib(*lib, *m);
+        std::memcpy(out, m.get(), sizeof(MLib));
+    });
+}

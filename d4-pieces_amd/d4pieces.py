@@ -32,6 +32,7 @@ LIB_PATH = os.path.join(_HERE, "libd4.so")
 D4_OK, D4_EINVAL, D4_ENOMEM, D4_EDEVICE, D4_ESTATE, D4_ERANGE = 0, -1, -2, -3, -4, -5
 MODE_COUNT, MODE_ENUM = 0, 1
 FLAG_NO_PRUNE = 1
+FLAG_HOST_TABLES = 2
 KERNEL_DFS, KERNEL_EXPAND, KERNEL_LEVEL = 0, 1, 2
 STEP_GENERAL, STEP_FUSED, STEP_GRAPH = 0, 1, 2
 
@@ -105,6 +106,9 @@ SIGNATURES = [
     ("d4_library_set_kernel", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     ("d4_library_set_donation_capacity", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_library_set_step_mode", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_mlib_table_bytes", ctypes.c_uint64, []),
+    ("d4_mbatch_tables", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
+    ("d4_library_host_tables", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_uint64]),
     ("d4_library_get_step_mode", ctypes.c_int, [_P, _U32P]),
     ("d4_searcher_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
     ("d4_searcher_set_sample", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint64]),
@@ -388,6 +392,13 @@ class PieceLibrary:
         """Items per donation buffer (default 1 Mi)."""
         _check(load().d4_library_set_donation_capacity(self._h, items))
 
+    def host_tables(self) -> bytes:
+        """This library's multi-library batch tables built on the host."""
+        n = load().d4_mlib_table_bytes()
+        buf = ctypes.create_string_buffer(n)
+        _check(load().d4_library_host_tables(self._h, buf, n))
+        return buf.raw
+
     def set_step_mode(self, mode: int) -> None:
         """How count-mode level row steps are issued: STEP_GENERAL (host round
         trips between the parts), STEP_FUSED (default: one queued chain, one
@@ -614,7 +625,7 @@ class MultiBatch:
     together in one frontier (d4_mbatch_*); count mode."""
 
     def __init__(self, libraries: Sequence[PieceLibrary], boards: Sequence[Sequence[int]],
-                 prune: bool = True):
+                 prune: bool = True, host_tables: bool = False):
         lib = load()
         if len(libraries) != len(boards):
             raise ValueError("one library per board")
@@ -623,9 +634,17 @@ class MultiBatch:
         flat, stride, heights = _flatten(boards)
         handles = (_P * self.n)(*[L._h for L in libraries])
         h = _P()
+        flags = (0 if prune else FLAG_NO_PRUNE) | (FLAG_HOST_TABLES if host_tables else 0)
         _check(lib.d4_mbatch_create(self.n, handles, _u8arr(flat), stride, _u32arr(heights),
-                                    0 if prune else FLAG_NO_PRUNE, ctypes.byref(h)))
+                                    flags, ctypes.byref(h)))
         self._h = h
+
+    def tables(self, inst: int) -> bytes:
+        """Instance inst's library tables as the batch holds them on the device."""
+        n = load().d4_mlib_table_bytes()
+        buf = ctypes.create_string_buffer(n)
+        _check(load().d4_mbatch_tables(self._h, inst, buf, n))
+        return buf.raw
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -48,6 +48,9 @@ enum {
 
 enum {
     D4_FLAG_NO_PRUNE = 1, /* disable region-connectivity pruning           */
+    D4_FLAG_HOST_TABLES = 2, /* multi-library batch: build the per-library
+                                placement tables on the host (default: on
+                                the device)                                */
 };
 
 typedef struct d4_library d4_library;
@@ -335,6 +338,16 @@ typedef struct d4_mbatch d4_mbatch;
  * may hold up to 128 (piece, transform) pairs; rows are within libs[i]'s
  * width), boards[i*stride + r] = row r, heights[i] = row count.  Count mode
  * (all tilings).  At most 32768 instances. */
+/* Bytes of one library's placement tables in a multi-library batch. */
+uint64_t d4_mlib_table_bytes(void);
+/* Instance inst's library tables as the batch holds them on the device, and
+ * the same tables built by the host from a library, for comparing the two
+ * builds.  The reference builds its row tables on the host once per run and
+ * copies them over (compute_frow_on_cpu, src/frow.cpp:39-166;
+ * transfer_frow_to_gpu, src/frow.cu:7); a batch builds one table set per
+ * distinct library on the device (D4_FLAG_HOST_TABLES: on the host). */
+int d4_mbatch_tables(const d4_mbatch *b, uint32_t inst, void *out, uint64_t bytes);
+int d4_library_host_tables(const d4_library *lib, void *out, uint64_t bytes);
 int d4_mbatch_create(uint32_t n_inst, const d4_library *const *libs,
                      const uint8_t *boards, uint32_t stride,
                      const uint32_t *heights, uint32_t flags, d4_mbatch **out);

@@ -245,3 +245,46 @@ def test_donation_buffer_full(gpu, uneven, uneven_fixture):
     checked = [i for i in pinned if fin[i]]
     assert len(checked) > 100
     assert all(got[i] == pinned[i] for i in checked)
+
+
+def test_device_tables_equal_host_tables(gpu, uneven, oracle):
+    """The per-library tables a batch builds on the device equal, byte for
+    byte, the host build of the same libraries: all distinct libraries of the
+    4096 uneven puzzles (sizes 3-6, up to 14 kinds, widths 4-8), libraries
+    with more than 64 placements, one-sided pieces, zero counts and
+    multiplicities up to 3.  Batches built either way count alike, and like
+    the oracle on small boards."""
+    cache, libs, boards = {}, [], []
+    for shapes, counts, rows, w in uneven:
+        key = (tuple(shapes), tuple(counts), w)
+        if key not in cache:
+            cache[key] = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+            libs.append(cache[key])
+            boards.append(rows)
+    rng = random.Random(5)
+    pool = gpu.free_polyominoes(3) + gpu.free_polyominoes(4) + gpu.free_polyominoes(5)
+    extra = []
+    for sym in (0xFF, 0x69, 0x01):
+        for _ in range(20):
+            shapes = sorted(rng.sample(pool, rng.randint(1, 9)))
+            counts = [rng.randint(0, 3) for _ in shapes]
+            w = rng.randint(3, 8)
+            lib = gpu.PieceLibrary(shapes, counts, sym, w)
+            libs.append(lib)
+            boards.append(rect(w, 2))
+            extra.append((shapes, counts, sym, w, lib))
+    b = gpu.MultiBatch(libs, boards)
+    for i, lib in enumerate(libs):
+        assert b.tables(i) == lib.host_tables(), i
+    bh = gpu.MultiBatch(libs, boards, host_tables=True)
+    for i in range(0, len(libs), 97):
+        assert bh.tables(i) == b.tables(i)
+    small = [(s, c, sym, w, lib) for s, c, sym, w, lib in extra if sym == 0xFF][:12]
+    sb = [rect(w, 3) for _, _, _, w, _ in small]
+    bd = gpu.MultiBatch([x[4] for x in small], sb)
+    bd.run()
+    bh2 = gpu.MultiBatch([x[4] for x in small], sb, host_tables=True)
+    bh2.run()
+    assert bd.counts() == bh2.counts()
+    for (s, c, _, w, _), rows, n in zip(small, sb, bd.counts()):
+        assert n == oracle.count(s, c, rows)
