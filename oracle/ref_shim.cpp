@@ -60,4 +60,41 @@ size_t ref_known(int one_sided, unsigned n, uint64_t *out, size_t cap) {
     return cnt[n];
 }
 
+// Shape<11> (121 bits, two 64-bit halves in and out): the same functions
+// on the reference's 11x11 frame (src/Shape.cpp:158-176 instantiates them).
+static Shape<11> s11(uint64_t lo, uint64_t hi) {
+    return Shape<11>{(static_cast<__uint128_t>(hi) << 64) | lo};
+}
+static void put11(Shape<11> s, uint64_t *out) {
+    const __uint128_t v = s.get_value();
+    out[0] = static_cast<uint64_t>(v);
+    out[1] = static_cast<uint64_t>(v >> 64);
+}
+
+void ref11_transforms(uint64_t lo, uint64_t hi, int norm, uint64_t *out) {
+    const auto t = s11(lo, hi).transforms(norm != 0);
+    for (int i = 0; i < 8; i++) put11(t[i], out + 2 * i);
+}
+
+void ref11_geometry(uint64_t lo, uint64_t hi, uint64_t *out) {
+    const Shape<11> sh = s11(lo, hi);
+    out[0] = sh.left();
+    out[1] = sh.width();
+    out[2] = sh.right();
+    out[3] = sh.top();
+    out[4] = sh.height();
+    out[5] = sh.bottom();
+}
+
+void ref11_canonical(uint64_t lo, uint64_t hi, unsigned forms, uint64_t *out) {
+    put11(s11(lo, hi).canonical_form(forms), out);
+}
+
+unsigned ref11_symmetry(uint64_t lo, uint64_t hi) { return s11(lo, hi).symmetry(); }
+
+void ref11_normalize(uint64_t lo, uint64_t hi, uint64_t *out) { put11(s11(lo, hi).normalize(), out); }
+
+// shape_at<11>(n, i) (src/known.hpp:15-18)
+void ref11_shape_at(unsigned n, size_t i, uint64_t *out) { put11(shape_at<11>(n, i), out); }
+
 }  // extern "C"

@@ -362,3 +362,110 @@ def test_step_matrix_maps_transform_back(oracle):
         m = (ctypes.c_int * 4)()
         oracle.L.or_step_matrix(t, m)
         assert tuple(d4pieces._STEP_MATRIX[t]) == tuple(m)
+
+
+def _solve_l():
+    import sys
+    from conftest import ROOT
+    p = os.path.join(ROOT, "oracle")
+    if p not in sys.path:
+        sys.path.insert(0, p)
+    import solve_l
+    return solve_l
+
+
+def _to_frame(v8, L):
+    """An 8x8 value (bit = 8r + c) in the L x L frame."""
+    out = 0
+    for b in range(64):
+        if v8 >> b & 1:
+            out |= 1 << (L * (b // 8) + b % 8)
+    return out
+
+
+def test_solve_l_matches_oracle(oracle):
+    """The L x L restatement (oracle/solve_l.py) gives the C oracle's
+    solutions, steps and order on boards up to 8 wide, in the 8x8 and the
+    11x11 frame: random piece sets of 1-4 cells, multiplicities 1-2, symmetry
+    masks D4 / C4 / D1_X / C1, and per-placement enabled masks that switch
+    duplicates on and placements off."""
+    S = _solve_l()
+    rng = random.Random(17)
+    pool = [1, 3, 7, 0x103] + [s for s in _free(oracle, 4)]
+    checked = 0
+    for _ in range(60):
+        shapes = sorted(rng.sample(pool, rng.randint(1, 4)))
+        counts = [rng.randint(1, 2) for _ in shapes]
+        sym = rng.choice([0xFF, 0x69, 0x03, 0x01])
+        nplace = bin(sym).count("1")
+        enabled = [rng.getrandbits(nplace) | 1 for _ in shapes] if rng.random() < 0.5 else None
+        w, h = rng.randint(2, 8), rng.randint(1, 4)
+        rows = [rng.getrandbits(w) | (rng.getrandbits(w) if rng.random() < 0.7 else 0)
+                for _ in range(h)]
+        rows = [r | ((1 << w) - 1) * (rng.random() < 0.5) for r in rows]
+        expect = oracle.solve(shapes, counts, rows, sym=sym, enabled=enabled)
+        for L in (8, 11):
+            lib = []
+            for i, (s, c) in enumerate(zip(shapes, counts)):
+                p = S.Piece(_to_frame(s, L), L, sym, c)
+                if enabled is not None:
+                    for q, pl in enumerate(p.placements):
+                        pl[2] = bool(enabled[i] >> q & 1)
+                lib.append(p)
+            board = sum(_to_frame(r, L) << (L * i) for i, r in enumerate(rows))
+            got = [[(pid, trs, y, x) for pid, trs, x, y, _ in sol]
+                   for sol in S.solve(lib, board, L)]
+            assert got == expect, (shapes, counts, sym, enabled, rows, L)
+            first = S.solve(lib, board, L, single=True)
+            assert [[(p_, t, y, x) for p_, t, x, y, _ in s] for s in first] == expect[:1]
+        checked += len(expect)
+    assert checked > 200
+
+
+def _free(oracle, n):
+    import d4pieces as d4
+    return d4.free_polyominoes(n)
+
+
+def test_solve_l_closed_forms_11_wide():
+    """On 11-wide boards: dominoes on 2x11 tile F(12) = 144 ways, straight
+    trominoes on 3x11 a(11) = 41 ways (a(n) = a(n-1) + a(n-3), a(0..2) = 1),
+    and each tiling covers the board exactly once."""
+    S = _solve_l()
+    L = 11
+    full = (1 << 11) - 1
+    sols = S.solve([S.Piece(0b11, L, 0xFF, 11)], full | full << 11, L)
+    assert len(sols) == 144
+    for sol in sols:
+        cover = 0
+        for _, _, _, _, placed in sol:
+            assert not cover & placed
+            cover |= placed
+        assert cover == full | full << 11
+    a = [1, 1, 1]
+    for n in range(3, 12):
+        a.append(a[-1] + a[-3])
+    sols = S.solve([S.Piece(0b111, L, 0xFF, 11)], full | full << 11 | full << 22, L)
+    assert len(sols) == a[11] == 41
+
+
+def test_solve_l_transposition():
+    """A board and its transpose (11x11 frame) have equally many tilings
+    under D4, and the transposed tilings are the transposes of the tilings."""
+    S = _solve_l()
+    L = 11
+    rng = random.Random(23)
+    shapes = [0b111, 0b11 | 0b1 << 11, 0b11 | 0b11 << 11]  # I3, L3, O4
+    for _ in range(12):
+        w, h = rng.randint(6, 11), rng.randint(2, 4)
+        board = 0
+        for r in range(h):
+            for c in range(w):
+                if rng.random() < 0.85:
+                    board |= 1 << (r * L + c)
+        tb = S.transform(board, 4, L, norm=False)
+        lib = [S.Piece(s, L, 0xFF, 3) for s in shapes]
+        a = {frozenset(p for *_, p in sol) for sol in S.solve(lib, board, L)}
+        b = {frozenset(S.transform(p, 4, L, norm=False) for *_, p in sol)
+             for sol in S.solve(lib, tb, L)}
+        assert a == b

@@ -184,3 +184,80 @@ def test_embind_piece_and_groups(ref, oracle):
     assert pa.groupProduct(pa.SymmetryGroup.C4, pa.SymmetryGroup.D1_P) == pa.SymmetryGroup.D4
     assert [pa.shape_count(n) for n in range(1, 9)] == [1, 1, 2, 5, 12, 35, 108, 369]
     assert pa.shape_at8(5, 0).value == known(ref, 0, 5)[0]
+
+
+def _ref11(ref):
+    u64, vp = ctypes.c_uint64, ctypes.c_void_p
+    ref.ref11_transforms.argtypes = [u64, u64, ctypes.c_int, vp]
+    ref.ref11_geometry.argtypes = [u64, u64, vp]
+    ref.ref11_canonical.argtypes = [u64, u64, ctypes.c_uint, vp]
+    ref.ref11_symmetry.restype = ctypes.c_uint
+    ref.ref11_symmetry.argtypes = [u64, u64]
+    ref.ref11_normalize.argtypes = [u64, u64, vp]
+    ref.ref11_shape_at.argtypes = [ctypes.c_uint, ctypes.c_size_t, vp]
+    return ref
+
+
+def _split(v):
+    return v & ((1 << 64) - 1), v >> 64
+
+
+def _join(buf, i=0):
+    return buf[2 * i] | buf[2 * i + 1] << 64
+
+
+def sample_shapes11():
+    """Free polyominoes up to 6 cells placed at random offsets in the 11x11
+    frame, shapes touching the right and bottom edges, full rows and
+    columns, and random (often disconnected) 121-bit masks."""
+    import pieces_api as pa
+    rng = random.Random(11)
+    out = [1, (1 << 11) - 1, sum(1 << (11 * i) for i in range(11)), (1 << 121) - 1,
+           1 << 120, 1 << 10, 1 << 110]
+    for n in range(1, 7):
+        for i in range(pa.shape_count(n)):
+            s = pa.shape_at11(n, i)
+            for t in range(8):
+                v = s._t(t, True).value
+                out.append(v)
+                out.append(v << (11 * rng.randint(0, 5) + rng.randint(0, 5)) & ((1 << 121) - 1))
+    for _ in range(300):
+        out.append(rng.getrandbits(121) & rng.getrandbits(121) & rng.getrandbits(121))
+    return [v for v in out if v]
+
+
+def test_shape11_against_reference(ref):
+    """Shape11's transforms (normalised and in place), normalize, margins and
+    extents (including the whole-word shifts of a shape spanning every
+    column), canonical forms and symmetry equal the compiled Shape<11>."""
+    import pieces_api as pa
+    ref = _ref11(ref)
+    buf = (ctypes.c_uint64 * 16)()
+    g = (ctypes.c_uint64 * 6)()
+    two = (ctypes.c_uint64 * 2)()
+    for v in sample_shapes11():
+        lo, hi = _split(v)
+        sh = pa.Shape11(v)
+        for norm in (1, 0):
+            ref.ref11_transforms(lo, hi, norm, buf)
+            assert [sh._t(t, bool(norm)).value for t in range(8)] == \
+                [_join(buf, t) for t in range(8)], (hex(v), norm)
+        ref.ref11_normalize(lo, hi, two)
+        assert sh.normalize().value == _join(two)
+        ref.ref11_geometry(lo, hi, g)
+        assert [sh.left, sh.width, sh.right, sh.top, sh.height, sh.bottom] == list(g), hex(v)
+        assert sh.symmetry == ref.ref11_symmetry(lo, hi)
+        for forms in (0, 1, 0x0F, 0x69, 0xFF, 0x90):
+            ref.ref11_canonical(lo, hi, forms, two)
+            assert sh.canonical_form(forms).value == _join(two), (hex(v), forms)
+
+
+def test_shape_at11_against_reference(ref):
+    """shape_at11 equals shape_at<11> for every free polyomino of 1-8 cells."""
+    import pieces_api as pa
+    ref = _ref11(ref)
+    two = (ctypes.c_uint64 * 2)()
+    for n in range(1, 9):
+        for i in range(pa.shape_count(n)):
+            ref.ref11_shape_at(n, i, two)
+            assert pa.shape_at11(n, i).value == _join(two), (n, i)
