@@ -356,15 +356,32 @@ struct Engine {
     static constexpr uint64_t kSpillCap = 1u << 22;
     // bounded memory (count mode, run()): frontier halves parked for later
     uint64_t mem_cap = 0;  // 0: unbounded
+    // A parked part lies in device memory, or, with a host budget set, in
+    // pinned host memory (keys then counts), so that the frontier of a row
+    // can exceed the device: halves stream out over the host link and back
+    // when their turn comes.
     struct Part {
         uint32_t g = 0;
         uint64_t n = 0;
         double grow_unique = 1.0, grow_children = 1.0, grow_unique_raw = 1.0;
         double prev_unique_raw = 0.0;
         DevBuf<uint64_t> k, c;
+        uint64_t *host = nullptr;  // [2][n] when parked on the host
+        ~Part() {
+            if (host) (void)hipHostFree(host);
+        }
     };
     std::vector<std::unique_ptr<Part>> parked;
-    uint64_t parked_bytes = 0;
+    uint64_t parked_bytes = 0;       // device bytes of the parked parts
+    uint64_t host_cap = 0;           // pinned host bytes parts may take (0: none)
+    uint64_t host_parked = 0;
+    bool stream_ok = true;           // rows that do not fit stream (else halves are parked)
+    static constexpr uint32_t kMaxPartitions = 64;
+    // The frontier in parts (device or pinned host memory) as a streamed row
+    // step left it when it was too large to gather into fk / fc; empty:
+    // fk / fc hold the frontier.
+    std::vector<std::unique_ptr<Part>> fparts;
+    DevBuf<uint64_t> sk, sc;         // a host part's chunk, copied in
     bool trace = getenv("D4_TRACE") != nullptr;  // per-step lines of run() on stderr
     // scale of the count-mode dedup table estimate (D4_TABLE_SCALE, default
     // 1): below 1 the tables overflow and spill, which exercises the reruns
@@ -740,13 +757,13 @@ struct Engine {
     }
 
     // Queues the terminal pass when an instance finishes at this step.
-    void queue_terminal() {
+    void queue_terminal() { queue_terminal(fk.p, fc.p, n_front); }
+    void queue_terminal(const uint64_t *k, const uint64_t *c, uint64_t n) {
         bool any_term = false;
         for (uint32_t i = 0; i < S; i++)
             if (active[i] && start[i] + heights[i] == g) any_term = true;
-        if (any_term)
-            launch_terminal(fk.p, fc.p, n_front, g, dstart.p, hgt.p, inst_shift, result.p,
-                            lib->n_cu, st);
+        if (any_term && n)
+            launch_terminal(k, c, n, g, dstart.p, hgt.p, inst_shift, result.p, lib->n_cu, st);
     }
 
     // Sizes the level buffers for chunks of `chunk` parents.
@@ -773,7 +790,12 @@ struct Engine {
     // launches and the dedup, with events around them when timed.
     uint32_t queue_levels(LevelArgs &a, const DedupArgs &d, uint64_t b0, uint64_t chunk,
                           bool prune, bool timed) {
-        const uint32_t nch = (uint32_t)((n_front - b0 + chunk - 1) / chunk);
+        return queue_levels(a, d, fk.p, fc.p, n_front, b0, chunk, prune, timed);
+    }
+    uint32_t queue_levels(LevelArgs &a, const DedupArgs &d, const uint64_t *ik,
+                          const uint64_t *ic, uint64_t n_in, uint64_t b0, uint64_t chunk,
+                          bool prune, bool timed) {
+        const uint32_t nch = (uint32_t)((n_in - b0 + chunk - 1) / chunk);
         while (timed && lv_ev.size() < 2 * (size_t)nch + 1) {
             hipEvent_t e;
             HIPCHK(hipEventCreate(&e));
@@ -784,9 +806,9 @@ struct Engine {
         for (uint32_t j = 0; j < nch; j++) {
             const uint64_t b = b0 + (uint64_t)j * chunk;
             if (j) launch_reset_levels(c32.p, c64.p, false, st);
-            a.in_key = fk.p + b;
-            a.in_cnt = fc.p + b;
-            a.n_rec = (uint32_t)std::min<uint64_t>(chunk, n_front - b);
+            a.in_key = ik + b;
+            a.in_cnt = ic + b;
+            a.n_rec = (uint32_t)std::min<uint64_t>(chunk, n_in - b);
             for (uint32_t L = 0; L < LVL_MAX; L++) {
                 a.level = L;
                 a.in_st = (L & 1) ? lv_a.p : lv_b.p;
@@ -827,18 +849,29 @@ struct Engine {
     }
 
     void run_levels(uint64_t tab_m, bool prune) {
+        run_levels(fk.p, fc.p, n_front, tab_m, prune, 0, 0, true);
+    }
+    // The level launches and dedups over n_in parents at ik / ic, in chunks
+    // that fit the level buffers, rerunning a chunk whose buffers overflowed.
+    // Only children whose key hash has (hash & part_mask) == part_id enter
+    // the table (a streamed row step's partition); terminal: queue the
+    // terminal pass over these parents first.
+    void run_levels(const uint64_t *ik, const uint64_t *ic, uint64_t n_in, uint64_t tab_m,
+                    bool prune, uint32_t part_mask, uint32_t part_id, bool terminal) {
         LevelArgs a;
         DedupArgs d;
         level_args(tab_m, a, d);
-        queue_terminal();
+        d.part_mask = part_mask;
+        d.part_id = part_id;
+        if (terminal) queue_terminal(ik, ic, n_in);
         uint64_t b0 = 0;
         uint32_t reruns = 0;
         double lv_pp = lv_pp0(), ch_pp = ch_pp0();
         for (;;) {
-            const uint64_t chunk = level_chunk(n_front - b0, lv_pp, ch_pp);
+            const uint64_t chunk = level_chunk(n_in - b0, lv_pp, ch_pp);
             level_buffers(chunk, lv_pp, ch_pp, a, d);
             const bool timed = !noalloc;  // (no events are made for the scratch's chunks)
-            const uint32_t nch = queue_levels(a, d, b0, chunk, prune, timed);
+            const uint32_t nch = queue_levels(a, d, ik, ic, n_in, b0, chunk, prune, timed);
             HIPCHK(hipGetLastError());
             uint32_t h32[C32_N];
             unsigned long long h64[C64_N];
@@ -849,13 +882,13 @@ struct Engine {
             if (timed) level_times(nch);
             if (h32[C32_OVERFLOW]) return;  // the table: the row step reruns larger
             if (done == nch && !h32[C32_XOVF])
-                level_sizes(h64, std::min<uint64_t>(chunk, n_front - (b0 + (uint64_t)(nch - 1) * chunk)));
+                level_sizes(h64, std::min<uint64_t>(chunk, n_in - (b0 + (uint64_t)(nch - 1) * chunk)));
             if (trace)
                 fprintf(stderr,
                         "levels step %u: parents %llu from %llu, chunk %llu, caps %llu / %llu, "
                         "done %u of %u, overflow %u, children %llu, level rows %llu %llu %llu %llu "
                         "%llu %llu %llu %llu\n",
-                        g, (unsigned long long)n_front, (unsigned long long)b0,
+                        g, (unsigned long long)n_in, (unsigned long long)b0,
                         (unsigned long long)chunk, (unsigned long long)lv_cap,
                         (unsigned long long)ch_cap, done, nch, h32[C32_XOVF], h64[C64_CHW],
                         h64[C64_LVL], h64[C64_LVL + 1], h64[C64_LVL + 2], h64[C64_LVL + 3],
@@ -866,7 +899,7 @@ struct Engine {
             // its capacity)
             stats.chunk_reruns++;
             b0 += (uint64_t)done * chunk;
-            const uint64_t c_fail = std::min<uint64_t>(chunk, n_front - b0);
+            const uint64_t c_fail = std::min<uint64_t>(chunk, n_in - b0);
             bool lv_over = false;
             for (uint32_t L = 0; L < LVL_MAX; L++) lv_over |= h64[C64_LVL + L] > lv_cap;
             if (lv_over) lv_pp = 4.0 * std::max(lv_pp, (double)lv_cap / (double)c_fail);
@@ -1610,6 +1643,205 @@ struct Engine {
         return true;
     }
 
+    // One row step whose dedup table is split into P partitions by key hash
+    // (low bits of the hash; the home slot takes the high ones): pass p
+    // reads every parent, in chunks (host parts copied in), and keeps only
+    // the children of partition p, so each pass's table holds about a P-th
+    // of the unique states while equal states still meet in one table.  Each
+    // partition is compacted into a part of the next frontier, in device
+    // memory or, with a host budget, in pinned host memory.  Unlike parking
+    // halves of the frontier, nothing is searched twice.  Returns false,
+    // with nothing changed, when no partition count fits.
+    bool streamed_step(bool prune) {
+        struct Src {
+            const uint64_t *k, *c, *h;  // device keys / counts, or host [keys, counts]
+            uint64_t n;
+        };
+        std::vector<Src> src;
+        uint64_t dev_src = 0, max_host = 0;
+        if (fparts.empty()) {
+            src.push_back({fk.p, fc.p, nullptr, n_front});
+            dev_src = 16 * n_front;
+        } else {
+            for (auto &q : fparts) {
+                if (q->host) {
+                    src.push_back({nullptr, nullptr, q->host, q->n});
+                    max_host = std::max(max_host, q->n);
+                } else {
+                    src.push_back({q->k.p, q->c.p, nullptr, q->n});
+                    dev_src += 16 * q->n;
+                }
+            }
+        }
+        // unique children: the last step's ratio, extrapolated once more
+        // when growth speeds up
+        double r = std::max(grow_unique_raw, 0.01);
+        if (prev_unique_raw > 0 && r > prev_unique_raw) r = r * r / prev_unique_raw;
+        const double est_u = (double)n_front * r * 1.1 + 4096.0;
+        const uint64_t cap = mem_cap ? mem_cap : (uint64_t)(0.8 * (double)available_device_bytes());
+        const uint64_t stage_n = std::min<uint64_t>(max_host, 1ull << 26);
+        const uint64_t fixed = dev_src + parked_bytes + level_budget() + 16 * stage_n +
+                               (host_cap ? 0 : (uint64_t)(16.0 * est_u));
+        if (fixed >= cap) return false;
+        const double avail = (double)(cap - fixed);
+        // (every pass reads all parents: past 64 partitions, parking halves
+        // of the frontier costs less)
+        uint32_t P = 2;
+        while (P < kMaxPartitions && 2.0 * 38.4 * est_u / P > avail) P *= 2;
+        if (2.0 * 38.4 * est_u / P > avail) return false;
+        if (host_cap && host_parked + (uint64_t)(16.0 * est_u) > host_cap + (host_cap >> 1))
+            return false;
+        sk.reserve(std::max<uint64_t>(stage_n, 1));
+        sc.reserve(std::max<uint64_t>(stage_n, 1));
+        result_save.reserve(S);
+        if (!spill.ext) spill.reserve(2 * kSpillCap);
+        HIPCHK(hipMemcpyAsync(result_save.p, result.p, (size_t)S * 8, hipMemcpyDeviceToDevice, st));
+        std::vector<std::unique_ptr<Part>> out;
+        uint64_t U = 0, children = 0, pruned = 0;
+        for (uint32_t p = 0; p < P; p++) {
+            for (uint32_t attempt = 0;; attempt++) {
+                if (attempt > 8) throw Error(D4_ERANGE, "a streamed partition outgrows its table");
+                const uint64_t tab_m = round_slots((uint64_t)(1.4 * table_scale * est_u / P) << attempt);
+                tk.reserve(2 * tab_m);
+                launch_fill_table(tk.p, tk.p + 1, tab_m, st);
+                launch_reset_counters(c32.p, c64.p, st);
+                bool ovf = false;
+                for (const Src &x : src) {
+                    if (!x.h) {
+                        run_levels(x.k, x.c, x.n, tab_m, prune, P - 1, p, p == 0);
+                    } else {
+                        for (uint64_t b = 0; b < x.n && !ovf; b += stage_n) {
+                            const uint64_t m = std::min(stage_n, x.n - b);
+                            HIPCHK(hipMemcpyAsync(sk.p, x.h + b, 8 * m, hipMemcpyHostToDevice, st));
+                            HIPCHK(hipMemcpyAsync(sc.p, x.h + x.n + b, 8 * m,
+                                                  hipMemcpyHostToDevice, st));
+                            run_levels(sk.p, sc.p, m, tab_m, prune, P - 1, p, p == 0);
+                            uint32_t f = 0;
+                            HIPCHK(hipMemcpyAsync(&f, c32.p + C32_OVERFLOW, 4, hipMemcpyDeviceToHost, st));
+                            sync_host();
+                            ovf = f != 0;
+                        }
+                    }
+                    uint32_t h[C32_N];
+                    HIPCHK(hipMemcpyAsync(h, c32.p, sizeof h, hipMemcpyDeviceToHost, st));
+                    sync_host();
+                    ovf = h[C32_OVERFLOW] || h[C32_SPILL];
+                    if (ovf) break;
+                }
+                if (ovf) {
+                    // a larger table; the first pass's terminal counts are undone
+                    stats.retries++;
+                    if (p == 0)
+                        copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
+                    continue;
+                }
+                unsigned long long h64[C64_N];
+                HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
+                sync_host();
+                const uint64_t u = h64[C64_NEW];
+                check_row_states(U + u);
+                nk.reserve(std::max<uint64_t>(u, 1));
+                nc.reserve(std::max<uint64_t>(u, 1));
+                blk.reserve(2 * COMPACT_MAX_BLOCKS);
+                launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, u, c32.p, blk.p, nullptr, st);
+                uint32_t n_out = 0;
+                HIPCHK(hipMemcpyAsync(&n_out, c32.p + C32_OUT, 4, hipMemcpyDeviceToHost, st));
+                sync_host();
+                check_compacted(n_out, u);
+                auto q = std::make_unique<Part>();
+                q->g = g + 1;
+                q->n = u;
+                if (host_cap && u) {
+                    if (host_parked + 16 * u > host_cap)
+                        throw Error(D4_ENOMEM, "frontier parts exceed the host memory budget");
+                    HIPCHK(hipHostMalloc((void **)&q->host, 16 * u));
+                    HIPCHK(hipMemcpyAsync(q->host, nk.p, 8 * u, hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipMemcpyAsync(q->host + u, nc.p, 8 * u, hipMemcpyDeviceToHost, st));
+                    sync_host();
+                    host_parked += 16 * u;
+                    stats.host_parked_peak = std::max<uint64_t>(stats.host_parked_peak, host_parked);
+                } else {
+                    q->k.swap(nk);
+                    q->c.swap(nc);
+                }
+                out.push_back(std::move(q));
+                U += u;
+                children += h64[C64_CHILDREN];
+                pruned += h64[C64_PRUNED];
+                stats.stream_passes++;
+                if (trace)
+                    fprintf(stderr, "streamed step %u: partition %u of %u, table %llu, new %llu\n",
+                            g, p, P, (unsigned long long)tab_m, (unsigned long long)u);
+                break;
+            }
+        }
+        // the parents are done with; the partitions are the next frontier
+        for (auto &q : fparts)
+            if (q->host) host_parked -= 16 * q->n;
+        fparts.clear();
+        tk.release();
+        stats.children += children;
+        stats.pruned += pruned;
+        stats.records_extended += n_front;
+        stats.unique += U;
+        if (n_front) {
+            prev_unique_raw = stats.rows_done ? grow_unique_raw : 0.0;
+            grow_unique = std::max(1.0, (double)U / (double)n_front);
+            grow_unique_raw = std::max(0.01, (double)U / (double)n_front);
+            grow_children = std::max(1.0, (double)children / (double)n_front);
+        }
+        stats.peak_frontier = std::max<uint64_t>(stats.peak_frontier, U);
+        stats.rows_done++;
+        stats.streamed_steps++;
+        new_roots = 0;
+        n_front = U;
+        fparts = std::move(out);
+        // gathered into fk / fc when the frontier fits with room to spare
+        if (16 * U <= cap / 4) {
+            fk.release();
+            fc.release();
+            fk.reserve(std::max<uint64_t>(U, 1));
+            fc.reserve(std::max<uint64_t>(U, 1));
+            uint64_t at = 0;
+            for (auto &q : fparts) {
+                if (q->host) {
+                    HIPCHK(hipMemcpyAsync(fk.p + at, q->host, 8 * q->n, hipMemcpyHostToDevice, st));
+                    HIPCHK(hipMemcpyAsync(fc.p + at, q->host + q->n, 8 * q->n,
+                                          hipMemcpyHostToDevice, st));
+                } else if (q->n) {
+                    HIPCHK(hipMemcpyAsync(fk.p + at, q->k.p, 8 * q->n, hipMemcpyDeviceToDevice, st));
+                    HIPCHK(hipMemcpyAsync(fc.p + at, q->c.p, 8 * q->n, hipMemcpyDeviceToDevice, st));
+                }
+                at += q->n;
+            }
+            sync_host();
+            for (auto &q : fparts)
+                if (q->host) host_parked -= 16 * q->n;
+            fparts.clear();
+        } else {
+            fk.release();
+            fc.release();
+        }
+        return true;
+    }
+
+    // The parts of a frontier left by a streamed step, parked (as halves
+    // are) and the last one resumed: depth first from here on.
+    void park_parts() {
+        for (auto &q : fparts) {
+            q->g = g;
+            q->grow_unique = grow_unique;
+            q->grow_unique_raw = grow_unique_raw;
+            q->prev_unique_raw = prev_unique_raw;
+            q->grow_children = grow_children;
+            if (!q->host) parked_bytes += 16 * q->n;
+            parked.push_back(std::move(q));
+            stats.splits++;
+        }
+        fparts.clear();
+        resume();
+    }
+
     // Park the upper half of the frontier, to resume at this step later.
     void split() {
         // the failed step's table and next-frontier buffers hold no state:
@@ -1626,12 +1858,23 @@ struct Engine {
         p->grow_unique_raw = grow_unique_raw;
         p->prev_unique_raw = prev_unique_raw;
         p->grow_children = grow_children;
-        p->k.reserve(m);
-        p->c.reserve(m);
-        HIPCHK(hipMemcpyAsync(p->k.p, fk.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipMemcpyAsync(p->c.p, fc.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
-        sync_host();
-        parked_bytes += 16 * m;
+        if (host_cap) {
+            if (host_parked + 16 * m > host_cap)
+                throw Error(D4_ENOMEM, "parked frontier parts exceed the host memory budget");
+            HIPCHK(hipHostMalloc((void **)&p->host, 16 * m));
+            HIPCHK(hipMemcpyAsync(p->host, fk.p + keep, 8 * m, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(p->host + m, fc.p + keep, 8 * m, hipMemcpyDeviceToHost, st));
+            sync_host();
+            host_parked += 16 * m;
+            stats.host_parked_peak = std::max<uint64_t>(stats.host_parked_peak, host_parked);
+        } else {
+            p->k.reserve(m);
+            p->c.reserve(m);
+            HIPCHK(hipMemcpyAsync(p->k.p, fk.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(p->c.p, fc.p + keep, 8 * m, hipMemcpyDeviceToDevice, st));
+            sync_host();
+            parked_bytes += 16 * m;
+        }
         parked.push_back(std::move(p));
         new_roots = std::min<uint64_t>(new_roots, keep);
         n_front = keep;
@@ -1641,9 +1884,18 @@ struct Engine {
     void resume() {
         std::unique_ptr<Part> p = std::move(parked.back());
         parked.pop_back();
-        parked_bytes -= 16 * p->n;
-        fk.swap(p->k);
-        fc.swap(p->c);
+        if (p->host) {
+            fk.reserve(std::max<uint64_t>(p->n, 1));
+            fc.reserve(std::max<uint64_t>(p->n, 1));
+            HIPCHK(hipMemcpyAsync(fk.p, p->host, 8 * p->n, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(fc.p, p->host + p->n, 8 * p->n, hipMemcpyHostToDevice, st));
+            sync_host();
+            host_parked -= 16 * p->n;
+        } else {
+            parked_bytes -= 16 * p->n;
+            fk.swap(p->k);
+            fc.swap(p->c);
+        }
         n_front = p->n;
         g = p->g;
         grow_unique = p->grow_unique;
@@ -1664,6 +1916,21 @@ struct Engine {
             if (active[i]) gend = std::max(gend, start[i] + heights[i]);
         for (;;) {
             while (n_front && g <= gend) {
+                const bool prune_ = !(flags & D4_FLAG_NO_PRUNE);
+                if (!fparts.empty()) {
+                    // a frontier left in parts streams through the next step
+                    // too; where no partitioning fits, its parts are parked
+                    // and finished one after the other
+                    if (!streamed_step(prune_)) {
+                        park_parts();
+                        continue;
+                    }
+                    if (trace)
+                        fprintf(stderr, "run step %u: frontier %llu in %zu parts\n", g,
+                                (unsigned long long)n_front, fparts.size());
+                    g++;
+                    continue;
+                }
                 // past the first rows (whose growth is unknown), replay many
                 // steps per host round trip; a failed step takes the path below
                 if (stats.rows_done >= 2 && !new_roots) {
@@ -1690,6 +1957,16 @@ struct Engine {
                     g++;
                     continue;
                 }
+                // the row does not fit: stream it through partitioned tables
+                // (level kernels), else park half of the frontier
+                if (stream_ok && level_kernel(false, false) && n_front >= 2 &&
+                    streamed_step(prune_)) {
+                    if (trace)
+                        fprintf(stderr, "run step %u: frontier %llu in %zu parts (streamed)\n", g,
+                                (unsigned long long)n_front, fparts.size());
+                    g++;
+                    continue;
+                }
                 if (n_front < 2) throw Error(D4_ENOMEM, "one state's row step exceeds the memory cap");
                 split();
             }
@@ -1698,6 +1975,8 @@ struct Engine {
         }
         g = gend + 1;
         n_front = 0;
+        fparts.clear();
+        host_parked = 0;
         copy(host_result.data(), result.p, (size_t)S * 8, hipMemcpyDeviceToHost);
         for (uint32_t i = 0; i < S; i++)
             if (active[i]) {
@@ -1774,6 +2053,7 @@ struct Engine {
     uint64_t sample_cap = 0;
     std::vector<uint64_t> sample_keys, sample_cnt;
     void take_sample() {
+        if (!fparts.empty()) return;  // (a frontier in parts is not sampled)
         if (!sample_h || S != 1 || !active[0] || sample_keys.size() >= sample_cap) return;
         if (heights[0] - std::min(heights[0], g - start[0]) != sample_h) return;
         const uint64_t k = std::min<uint64_t>(sample_cap - sample_keys.size(), n_front);
@@ -2414,6 +2694,15 @@ int d4_searcher_set_memory(d4_searcher *s, uint64_t bytes) {
     });
 }
 
+int d4_searcher_set_host_memory(d4_searcher *s, uint64_t bytes) {
+    return guard([&] {
+        if (!s) throw Error(D4_EINVAL, "null searcher");
+        if (s->e->mode != D4_MODE_COUNT) throw Error(D4_ESTATE, "host parking needs count mode");
+        if (s->e->parked.size()) throw Error(D4_ESTATE, "parts are parked");
+        s->e->host_cap = bytes;
+    });
+}
+
 void d4_searcher_destroy(d4_searcher *s) { delete s; }
 
 int d4_searcher_count_noalloc(const d4_library *lib, const uint8_t *rows, uint32_t height,
@@ -2730,6 +3019,15 @@ int d4_batch_set_memory(d4_batch *b, uint64_t bytes) {
         if (b->e->mode != D4_MODE_COUNT) throw Error(D4_ESTATE, "memory cap needs count mode");
         if (bytes < (1u << 20)) throw Error(D4_EINVAL, "memory cap below 1 MiB");
         b->e->mem_cap = bytes;
+    });
+}
+
+int d4_batch_set_host_memory(d4_batch *b, uint64_t bytes) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        if (b->e->mode != D4_MODE_COUNT) throw Error(D4_ESTATE, "host parking needs count mode");
+        if (b->e->parked.size()) throw Error(D4_ESTATE, "parts are parked");
+        b->e->host_cap = bytes;
     });
 }
 

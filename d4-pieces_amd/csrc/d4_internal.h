@@ -186,6 +186,9 @@ struct DedupArgs {
     uint32_t *c32;
     unsigned long long *c64;
     const RowCtl *ctl;       // set: row and tab_size come from it (graph replay)
+    // a streamed row step's partition: only children whose key hash has
+    // (mix64(key) & part_mask) == part_id are counted and inserted
+    uint32_t part_mask, part_id;
 };
 void launch_level(const LevelArgs &a, int n_cu, hipStream_t st);
 void launch_dedup(const DedupArgs &a, bool prune, int n_cu, hipStream_t st);

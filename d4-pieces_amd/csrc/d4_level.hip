@@ -361,6 +361,9 @@ __global__ __launch_bounds__(256) void k_dedup(DedupArgs a) {
 #pragma unroll
         for (uint32_t u = 0; u < DEDUP_UNROLL; u++) {
             live[u] = key[u] != EMPTY_KEY;
+            // (children of other partitions are left to their own pass)
+            if (a.part_mask && live[u] && (mix64(key[u]) & a.part_mask) != a.part_id)
+                live[u] = false;
             if (PRUNE && live[u]) {
                 const uint32_t inst = a.inst_shift < 64 ? (uint32_t)(key[u] >> a.inst_shift) : 0u;
                 uint64_t w;

@@ -73,6 +73,9 @@ class Stats(ctypes.Structure):
         ("fused_steps", ctypes.c_uint64),
         ("fused_fallbacks", ctypes.c_uint64),
         ("graph_replays", ctypes.c_uint64),
+        ("host_parked_peak", ctypes.c_uint64),
+        ("streamed_steps", ctypes.c_uint64),
+        ("stream_passes", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -136,6 +139,8 @@ SIGNATURES = [
     ("d4_searcher_step", ctypes.c_int, [_P]),
     ("d4_searcher_run", ctypes.c_int, [_P]),
     ("d4_searcher_set_memory", ctypes.c_int, [_P, ctypes.c_uint64]),
+    ("d4_searcher_set_host_memory", ctypes.c_int, [_P, ctypes.c_uint64]),
+    ("d4_batch_set_host_memory", ctypes.c_int, [_P, ctypes.c_uint64]),
     ("d4_searcher_count", ctypes.c_int, [_P, _U64P]),
     ("d4_searcher_solutions", ctypes.c_int,
      [_P, ctypes.c_int, _U8P, ctypes.c_size_t, _SZP, _U64P]),
@@ -533,6 +538,11 @@ class CudaSearcher:
         are parked and finished one after the other)."""
         _check(load().d4_searcher_set_memory(self._h, nbytes))
 
+    def set_host_memory(self, nbytes: int) -> None:
+        """Count mode: pinned host bytes parked frontier parts may take (0:
+        they stay in device memory)."""
+        _check(load().d4_searcher_set_host_memory(self._h, nbytes))
+
     def search_GPU(self) -> None:
         _check(load().d4_searcher_step(self._h))
 
@@ -594,6 +604,9 @@ class Batch:
 
     def set_memory(self, nbytes: int) -> None:
         _check(load().d4_batch_set_memory(self._h, nbytes))
+
+    def set_host_memory(self, nbytes: int) -> None:
+        _check(load().d4_batch_set_host_memory(self._h, nbytes))
 
     def run(self) -> "Batch":
         _check(load().d4_batch_run(self._h))

@@ -93,6 +93,10 @@ typedef struct d4_stats {
     uint64_t fused_fallbacks;  /* of those, steps rerun on the general path
                                   (a buffer overflowed or children spilled) */
     uint64_t graph_replays;    /* fused row steps replayed from a HIP graph  */
+    uint64_t host_parked_peak; /* most bytes of parked parts in host memory  */
+    uint64_t streamed_steps;   /* row steps run in hash partitions (too large
+                                  for one dedup table under the cap)        */
+    uint64_t stream_passes;    /* partitions of those steps                 */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -263,6 +267,13 @@ int d4_searcher_run(d4_searcher *s);
  * other half first (depth first), streaming a frontier larger than memory
  * through it; the count is unchanged.  d4_searcher_step is not bounded. */
 int d4_searcher_set_memory(d4_searcher *s, uint64_t bytes);
+/* Count mode: pinned host bytes the parked frontier parts of
+ * d4_searcher_run may take (default 0: parts stay in device memory).  With a
+ * budget, a part is copied out to host memory when parked and back when its
+ * turn comes, so the frontier of a row can exceed the device memory -- the
+ * role of the reference's ring buffer drained to host memory
+ * (src/device.cu:32, :209-230); D4_ENOMEM when the budget runs out. */
+int d4_searcher_set_host_memory(d4_searcher *s, uint64_t bytes);
 /* Allocation-free count: the number of tilings of one board (as
  * d4_searcher_create + d4_searcher_run + d4_searcher_count in count mode),
  * computed without allocating device memory.  No counterpart in the
@@ -318,6 +329,8 @@ void d4_batch_destroy(d4_batch *b);
 int d4_batch_run(d4_batch *b);
 /* Count mode: memory cap of d4_batch_run, as d4_searcher_set_memory. */
 int d4_batch_set_memory(d4_batch *b, uint64_t bytes);
+/* Count mode: host budget for parked parts, as d4_searcher_set_host_memory. */
+int d4_batch_set_host_memory(d4_batch *b, uint64_t bytes);
 /* counts[i] = number of tilings of board i (valid after d4_batch_run). */
 int d4_batch_counts(const d4_batch *b, uint64_t *counts);
 int d4_batch_stats(const d4_batch *b, d4_stats *st);

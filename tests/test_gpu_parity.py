@@ -222,11 +222,82 @@ def test_memory_cap_streams_frontier(gpu, lib6):
     b.set_memory(8 << 20)
     b.run()
     assert b.counts() == fx["counts"][:32]
-    assert b.stats()["splits"] > 0
+    st = b.stats()
+    assert st["splits"] + st["streamed_steps"] > 0
     s = gpu.CudaSearcher(lib6, rows=rect(6, 10))
     s.set_memory(2 << 20)
     s.run()
-    assert s.count() == 9356 and s.stats()["splits"] > 0
+    st = s.stats()
+    assert s.count() == 9356 and st["splits"] + st["streamed_steps"] > 0
+
+
+def test_host_parking_streams_past_device_cap(gpu, lib6, p5, oracle):
+    """With a host budget, parked frontier halves go to pinned host memory
+    and come back when their turn comes: the device holds only the working
+    half, the counts are unchanged (batch, searcher, doubled pentominoes),
+    host memory is used and released, and a host budget too small for the
+    parts fails with D4_ENOMEM instead of exceeding it."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    b = gpu.Batch(lib, fx["boards"][:32])
+    b.set_memory(8 << 20)
+    b.set_host_memory(1 << 30)
+    b.run()
+    assert b.counts() == fx["counts"][:32]
+    st = b.stats()
+    assert st["splits"] + st["streamed_steps"] > 0 and st["host_parked_peak"] > 0
+    s = gpu.CudaSearcher(lib6, rows=rect(6, 10))
+    s.set_memory(2 << 20)
+    s.set_host_memory(1 << 30)
+    s.run()
+    assert s.count() == 9356 and s.stats()["host_parked_peak"] > 0
+    lib2 = gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8)
+    s = gpu.CudaSearcher(lib2, rows=rect(8, 5))
+    s.set_memory(2 << 20)
+    s.set_host_memory(1 << 30)
+    s.run()
+    assert s.count() == 695086 and s.stats()["host_parked_peak"] > 0
+    s = gpu.CudaSearcher(lib2, rows=rect(8, 5))
+    s.set_memory(2 << 20)
+    s.set_host_memory(4096)
+    with pytest.raises(gpu.D4Error) as e:
+        s.run()
+    assert e.value.code == gpu.D4_ENOMEM
+
+
+@pytest.mark.parametrize("host", [0, 1 << 30])
+def test_streamed_row_steps(gpu, p5, host):
+    """Rows too large for one dedup table under the cap run in hash
+    partitions (every pass reads all parents and keeps one partition's
+    children), with the frontier left in parts -- device or host memory --
+    when it is too large to gather: the counts equal the unbounded run's and
+    the oracle-pinned ones (8x5 pentominoes x2: 695,086; 8x6 pentominoes x2
+    on 45 cells of 8x6 with 3 cells blocked), with table estimates scaled
+    down so that partitions overflow and rerun."""
+    lib2 = gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8)
+    free = gpu.CudaSearcher(lib2, rows=rect(8, 6, blocked=((5, 5), (5, 6), (5, 7))))
+    free.run()
+    for cap in (24 << 20, 64 << 20):
+        s = gpu.CudaSearcher(lib2, rows=rect(8, 5))
+        s.set_memory(cap)
+        if host:
+            s.set_host_memory(host)
+        s.run()
+        st = s.stats()
+        assert s.count() == 695086, cap
+        s = gpu.CudaSearcher(lib2, rows=rect(8, 6, blocked=((5, 5), (5, 6), (5, 7))))
+        s.set_memory(cap)
+        if host:
+            s.set_host_memory(host)
+        s.run()
+        st = s.stats()
+        assert s.count() == free.count(), cap
+        assert st["streamed_steps"] > 0 and st["stream_passes"] >= 2 * st["streamed_steps"]
+        if host:
+            assert st["host_parked_peak"] > 0
 
 
 def test_doubled_pentominoes(gpu, oracle):
