@@ -1373,6 +1373,11 @@ struct Engine {
                 // rows are sized from a generous per-record allowance rather
                 // than by searching the row twice
                 est_c = est_u = std::min<uint64_t>(bound, n_front * kFirstRowFanout + 65536);
+                // (earlier searches on the library bound this row's children
+                // per parent, which bound its unique states)
+                if (seen_ch > 0)
+                    est_c = est_u = std::min<uint64_t>(
+                        est_u, (uint64_t)(1.2 * seen_ch * (double)n_front) + 65536);
                 exact = true;
             }
         }
