@@ -648,6 +648,21 @@ def main():
             k["ms_per_row_step"] = tot(mkey) / rows
             k["bytes_per_row_step"] = tot(bkey) / rows
             kernels[name] = k
+    if "dedup" in kernels:
+        # The dedup's table accesses are random: per kept child one coherent
+        # load and one atomic add, per new state a CAS besides.  Its atomics
+        # against the device's measured rate of random 64-bit atomics (which
+        # bounds them before HBM bandwidth does); the loads come on top.
+        ops = tot("children") + tot("unique")
+        rate = d4.atomic_rate()
+        got = ops / (tot("dedup_ms") / 1e3)
+        kernels["dedup"]["atomics_per_row_step"] = ops / rows
+        kernels["dedup"]["random_loads_per_row_step"] = tot("children") / rows
+        kernels["dedup"]["atomics_per_s"] = got
+        kernels["dedup"]["random_atomics_per_s_measured"] = rate
+        kernels["dedup"]["atomics_frac_of_measured_rate"] = got / rate if rate else None
+        log(f"dedup: {got / 1e9:.1f} G atomics/s (and {tot('children') / (tot('dedup_ms') / 1e3) / 1e9:.1f} G "
+            f"random loads/s) against {rate / 1e9:.1f} G random atomics/s measured ({got / rate:.2f})")
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_row_step"])
     roof = dict(kernels[dominant])
     roof["per"] = "row step"

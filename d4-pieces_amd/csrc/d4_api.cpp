@@ -2474,6 +2474,36 @@ int d4_device_count(int *n) {
     });
 }
 
+int d4_device_atomic_rate(double *ops_per_s) {
+    return guard([&] {
+        if (!ops_per_s) throw Error(D4_EINVAL, "null argument");
+        const uint64_t m = 1ull << 27;  // 1 GiB of counters
+        const uint32_t blocks = 8192, iters = 64;
+        DevBuf<unsigned long long> tab;
+        tab.reserve(m);
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        double best = 0;
+        for (int rep = 0; rep < 3; rep++) {
+            HIPCHK(hipEventRecord(e0, st));
+            launch_atomic_rate(tab.p, m, iters, blocks, st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms > 0) best = std::max(best, (double)blocks * 256 * iters / (ms * 1e-3));
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(st);
+        *ops_per_s = best;
+    });
+}
+
 int d4_device_synchronize(void) {
     return guard([&] { HIPCHK(hipDeviceSynchronize()); });
 }

@@ -283,6 +283,9 @@ void launch_row_close(RowCtl *ctl, const uint32_t *c32, const unsigned long long
                       hipStream_t st);
 void launch_reset_work(unsigned long long *c64, hipStream_t st);
 void launch_zero_u32(uint32_t *p, uint64_t n, hipStream_t st);
+// blocks x 256 threads x iters random atomic adds into tab[0..m)
+void launch_atomic_rate(unsigned long long *tab, uint64_t m, uint32_t iters, uint32_t blocks,
+                        hipStream_t st);
 // zero the level kernels' per-chunk counters; first: also the overflow flag
 // and the completed-chunk count
 void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st,

@@ -993,6 +993,21 @@ void launch_terminal(const uint64_t *key, const uint64_t *cnt, uint64_t n, uint3
                        hgt, inst_shift, result, ctl);
 }
 
+// Random 64-bit device-scope atomic adds, iters per thread, at slots spread
+// over the whole buffer by a hash of (thread, i): the device's rate for the
+// access the dedup table makes per child.
+__global__ __launch_bounds__(256) void k_atomic_rate(unsigned long long *tab, uint64_t m,
+                                                     uint32_t iters) {
+    const uint64_t tid = blockIdx.x * 256ull + threadIdx.x;
+    for (uint32_t i = 0; i < iters; i++)
+        atomicAdd(tab + home_slot(tid * 1000003ull + i, m), 1ull);
+}
+
+void launch_atomic_rate(unsigned long long *tab, uint64_t m, uint32_t iters, uint32_t blocks,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_atomic_rate, dim3(blocks), dim3(256), 0, st, tab, m, iters);
+}
+
 void launch_zero_u32(uint32_t *p, uint64_t n, hipStream_t st) {
     const uint64_t b = std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), 1024);
     hipLaunchKernelGGL(k_zero_u32, dim3((uint32_t)b), dim3(256), 0, st, p, n);

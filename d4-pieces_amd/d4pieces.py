@@ -110,6 +110,7 @@ SIGNATURES = [
     ("d4_library_set_donation_capacity", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_library_set_step_mode", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_mlib_table_bytes", ctypes.c_uint64, []),
+    ("d4_device_atomic_rate", ctypes.c_int, [ctypes.POINTER(ctypes.c_double)]),
     ("d4_mbatch_tables", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
     ("d4_library_host_tables", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_uint64]),
     ("d4_library_get_step_mode", ctypes.c_int, [_P, _U32P]),
@@ -239,6 +240,13 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     _check(load().d4_device_count(ctypes.byref(n)))
     return n.value
+
+
+def atomic_rate() -> float:
+    """Measured random 64-bit device-scope atomic adds per second."""
+    v = ctypes.c_double()
+    _check(load().d4_device_atomic_rate(ctypes.byref(v)))
+    return v.value
 
 
 def synchronize() -> None:

@@ -108,6 +108,12 @@ int d4_device_count(int *n);
 /* Wait for all work on the current device (hipDeviceSynchronize). */
 int d4_device_synchronize(void);
 
+/* Measured rate of random 64-bit device-scope atomic adds into 1 GiB of
+ * device memory (best of three launches of 134 M adds): the ceiling of a
+ * dedup that merges each child into a hash table in HBM with one atomic.
+ * No counterpart in the reference (measurement only). */
+int d4_device_atomic_rate(double *ops_per_s);
+
 /* Free polyominoes with n cells (1 <= n <= 8), each as its canonical value
  * (minimum over the 8 D4 transforms of the normalised 8-wide encoding),
  * ascending: the reference's known_shapes[n] tables (src/known1.cpp:3-14,
