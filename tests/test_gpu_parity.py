@@ -275,20 +275,23 @@ def test_streamed_row_steps(gpu, p5, host):
     children), with the frontier left in parts -- device or host memory --
     when it is too large to gather: the counts equal the unbounded run's and
     the oracle-pinned ones (8x5 pentominoes x2: 695,086; 8x6 pentominoes x2
-    on 45 cells of 8x6 with 3 cells blocked), with table estimates scaled
-    down so that partitions overflow and rerun."""
+    on 45 cells of 8x6 with 3 cells blocked).  Each capped search gets a
+    library with no history, so its first rows are sized from the generous
+    per-record allowance (a library that has searched before sizes them from
+    what it saw, and they would fit)."""
     lib2 = gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8)
     free = gpu.CudaSearcher(lib2, rows=rect(8, 6, blocked=((5, 5), (5, 6), (5, 7))))
     free.run()
     for cap in (24 << 20, 64 << 20):
-        s = gpu.CudaSearcher(lib2, rows=rect(8, 5))
+        s = gpu.CudaSearcher(gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8), rows=rect(8, 5))
         s.set_memory(cap)
         if host:
             s.set_host_memory(host)
         s.run()
         st = s.stats()
         assert s.count() == 695086, cap
-        s = gpu.CudaSearcher(lib2, rows=rect(8, 6, blocked=((5, 5), (5, 6), (5, 7))))
+        s = gpu.CudaSearcher(gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8),
+                             rows=rect(8, 6, blocked=((5, 5), (5, 6), (5, 7))))
         s.set_memory(cap)
         if host:
             s.set_host_memory(host)
