@@ -328,6 +328,11 @@ struct Engine {
     // level kernels: partial-row buffers (two, alternating), the children
     // buffer, and per-parent sizes seen on the last row step
     DevBuf<uint64_t> lv_a, lv_b, ch_k, ch_c;
+    // partitioned dedup (level kernels, count mode): children in bucket
+    // order, per (bucket, tile) counts / offsets, scan block sums
+    DevBuf<uint64_t> pbuf, sbuf, pmask;
+    DevBuf<uint32_t> phist, phist2, pbsum;
+    bool last_part = false;  // the last fused step deduped by partitions
     double lv_ratio = 0.0, ch_ratio = 0.0;  // max level / children written per parent
     double pp_bound = 0.0;                   // children bound per parent (this step)
     double seen_lv = 0.0, seen_ch = 0.0;     // the library's sizes for this step index
@@ -345,6 +350,7 @@ struct Engine {
     RowCtl *ctl_h = nullptr;             // pinned
     std::vector<uint64_t> g_sig;         // buffers and sizes the graphs were recorded with
     uint64_t g_cap = 0, g_lv = 0, g_ch = 0;
+    bool g_part = false;  // the recorded steps dedup by partitions
     uint32_t g_fail = 0;                 // the last replay's failure flags
     bool g_off_once = false;             // the next row step does not replay
     static constexpr uint64_t kGraphMaxSlots = 1ull << 25;
@@ -788,13 +794,15 @@ struct Engine {
 
     // Queues chunks b0, b0 + chunk, ... of the frontier: per chunk the level
     // launches and the dedup, with events around them when timed.
+    // (pa set: the chunk -- the whole frontier -- is deduped by partitions
+    // into pa's next frontier instead of the table)
     uint32_t queue_levels(LevelArgs &a, const DedupArgs &d, uint64_t b0, uint64_t chunk,
-                          bool prune, bool timed) {
-        return queue_levels(a, d, fk.p, fc.p, n_front, b0, chunk, prune, timed);
+                          bool prune, bool timed, const PartArgs *pa = nullptr) {
+        return queue_levels(a, d, fk.p, fc.p, n_front, b0, chunk, prune, timed, pa);
     }
     uint32_t queue_levels(LevelArgs &a, const DedupArgs &d, const uint64_t *ik,
                           const uint64_t *ic, uint64_t n_in, uint64_t b0, uint64_t chunk,
-                          bool prune, bool timed) {
+                          bool prune, bool timed, const PartArgs *pa = nullptr) {
         const uint32_t nch = (uint32_t)((n_in - b0 + chunk - 1) / chunk);
         while (timed && lv_ev.size() < 2 * (size_t)nch + 1) {
             hipEvent_t e;
@@ -816,7 +824,8 @@ struct Engine {
                 launch_level(a, lib->n_cu, st);
             }
             if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 1], st));
-            launch_dedup(d, prune, lib->n_cu, st);
+            if (pa) launch_part_dedup(*pa, prune, lib->n_cu, st);
+            else launch_dedup(d, prune, lib->n_cu, st);
             if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 2], st));
         }
         return nch;
@@ -1067,10 +1076,13 @@ struct Engine {
                               (const void *)c32.p, (const void *)c64.p, (const void *)blk.p,
                               (const void *)gk[0].p, (const void *)gk[1].p,
                               (const void *)gc[0].p, (const void *)gc[1].p,
-                              (const void *)ctl_d.p})
+                              (const void *)ctl_d.p, (const void *)pbuf.p, (const void *)sbuf.p,
+                              (const void *)pmask.p, (const void *)phist.p,
+                              (const void *)phist2.p, (const void *)pbsum.p})
             v.push_back((uint64_t)(uintptr_t)q);
         for (uint64_t x : {tk.cap, lv_a.cap, lv_b.cap, ch_k.cap, ch_c.cap, spill.cap, g_cap, g_lv,
-                           g_ch, (uint64_t)S, (uint64_t)prune, (uint64_t)bw_stride})
+                           g_ch, (uint64_t)S, (uint64_t)prune, (uint64_t)bw_stride,
+                           (uint64_t)g_part})
             v.push_back(x);
         return v;
     }
@@ -1094,9 +1106,13 @@ struct Engine {
         a.in_cnt = gc[p].p;
         a.n_rec = (uint32_t)std::min<uint64_t>(g_cap, 0xffffffffull);
         const RowCtl *ctl = ctl_d.p;
+        // (the partitioned dedup writes the next frontier buffer itself)
+        const bool part = g_part;
+        PartArgs pa{};
+        if (part) part_args(d, g_ch, gk[1 - p].p, gc[1 - p].p, g_cap, pa);
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         launch_copy_u64(result_save.p, result.p, S, st, ctl);
-        launch_fill_table(tk.p, tk.p + 1, g_cap, st, ctl);
+        if (!part) launch_fill_table(tk.p, tk.p + 1, g_cap, st, ctl);
         launch_reset_counters(c32.p, c64.p, st, ctl);
         launch_terminal(gk[p].p, gc[p].p, g_cap, g, dstart.p, hgt.p, inst_shift, result.p,
                         lib->n_cu, st, ctl);
@@ -1107,9 +1123,13 @@ struct Engine {
             a.out_st = (L & 1) ? lv_b.p : lv_a.p;
             launch_level(a, lib->n_cu, st);
         }
-        launch_dedup(d, prune, lib->n_cu, st);
-        launch_compact(tk.p, tk.p + 1, g_cap, gk[1 - p].p, gc[1 - p].p, g_cap, c32.p, blk.p,
-                       nullptr, st, ctl);
+        if (part) {
+            launch_part_dedup(pa, prune, lib->n_cu, st);
+        } else {
+            launch_dedup(d, prune, lib->n_cu, st);
+            launch_compact(tk.p, tk.p + 1, g_cap, gk[1 - p].p, gc[1 - p].p, g_cap, c32.p, blk.p,
+                           nullptr, st, ctl);
+        }
         launch_row_close(ctl_d.p, c32.p, c64.p, st);
         const hipError_t launch_err = hipGetLastError();
         hipGraph_t graph = nullptr;
@@ -1171,7 +1191,8 @@ struct Engine {
         g_cap = cap;
         g_lv = lv;
         g_ch = ch;
-        tk.reserve(2 * g_cap);
+        g_part = part_dedup_fits(g_ch, n_front, g_cap, (double)g_ch);
+        if (!g_part) tk.reserve(2 * g_cap);
         for (int q = 0; q < 2; q++) {
             gk[q].reserve(g_cap);
             gc[q].reserve(g_cap);
@@ -1182,7 +1203,19 @@ struct Engine {
         ch_c.reserve(g_ch);
         lv_cap = g_lv;
         ch_cap = g_ch;
-        blk.reserve(2 * COMPACT_MAX_BLOCKS);
+        if (g_part) {
+            const uint64_t nm = std::min<uint64_t>(g_ch, PART_MAX_CHILDREN);
+            uint64_t h1 = 0, h2 = 0;
+            part_hist_entries(nm, h1, h2);
+            pbuf.reserve(2 * nm);
+            sbuf.reserve(2 * nm);
+            pmask.reserve(part_mask_words(nm));
+            phist.reserve(h1);
+            phist2.reserve(h2);
+            pbsum.reserve(part_bsum_entries());
+        } else {
+            blk.reserve(2 * COMPACT_MAX_BLOCKS);
+        }
         if (!spill.ext) spill.reserve(2 * kSpillCap);
         result_save.reserve(S);
         ctl_d.reserve(1);
@@ -1227,8 +1260,12 @@ struct Engine {
             stats.unique += u;
             stats.extend_bytes += 16ull * n + 16ull * l.children;
             stats.dedup_bytes += 16ull * (l.children + l.pruned) + 16ull * u;
-            stats.compact_bytes += 8ull * l.tab_m + 24ull * u;
-            stats.compact_launches++;
+            if (g_part) {
+                stats.part_steps++;
+            } else {
+                stats.compact_bytes += 8ull * l.tab_m + 24ull * u;
+                stats.compact_launches++;
+            }
             stats.extend_launches++;
             stats.dedup_launches++;
             if (n) {
@@ -1270,6 +1307,75 @@ struct Engine {
         return done;
     }
 
+    // Whether the step's children (ch_cap of them) go through the
+    // partitioned dedup: the library asks for it, the children fit its
+    // buckets, and its buffers (children in bucket order, a next frontier
+    // with room for every child, the histogram) fit the memory cap.
+    // (rows expected to have fewer than part_min children keep the table: the
+    // bucket chain's dozen launches cost more than it saves there; 2^20 by
+    // default, D4_PART_MIN_CHILDREN overrides it, so tests can send small
+    // rows through the buckets)
+    double part_min = [] {
+        const char *v = getenv("D4_PART_MIN_CHILDREN");
+        return v ? std::max(0.0, atof(v)) : (double)(1u << 20);
+    }();
+    bool part_dedup_fits(uint64_t chc, uint64_t nf, uint64_t out_cap, double expect) const {
+        if (lib->dedup_mode != D4_DEDUP_PART || noalloc) return false;
+        if (expect < part_min) return false;
+        if (!mem_cap) return true;
+        const uint64_t nm = std::min<uint64_t>(chc, PART_MAX_CHILDREN);
+        uint64_t h1 = 0, h2 = 0;
+        part_hist_entries(nm, h1, h2);
+        return 32ull * nm + nm / 8 + 4ull * (h1 + h2) + 16ull * out_cap + 16ull * nf +
+                   level_bytes(nf) + parked_bytes <= mem_cap;
+    }
+    // The partitioned dedup's arguments (pruning as the table dedup's d).
+    void part_args(const DedupArgs &d, uint64_t chc, uint64_t *ok, uint64_t *oc, uint64_t out_cap,
+                   PartArgs &pa) {
+        const uint64_t nm = std::min<uint64_t>(chc, PART_MAX_CHILDREN);
+        uint64_t h1 = 0, h2 = 0;
+        part_hist_entries(nm, h1, h2);
+        pbuf.reserve(2 * nm);
+        sbuf.reserve(2 * nm);
+        pmask.reserve(part_mask_words(nm));
+        phist.reserve(h1);
+        phist2.reserve(h2);
+        pbsum.reserve(part_bsum_entries());
+        pa = PartArgs{};
+        // (the second radix pass writes the survivors back into the
+        // children buffers, which the level kernels wrote)
+        pa.ch_key = ch_k.p;
+        pa.ch_cnt = ch_c.p;
+        pa.ch_cap = chc;
+        pa.n_max = nm;
+        pa.tile_min = part_tile_min();
+        pa.lp_max = part_lp_max();
+        pa.slots = part_slots();
+        pa.mask = pmask.p;
+        pa.pbuf = pbuf.p;
+        pa.sbuf = sbuf.p;
+        pa.hist = phist.p;
+        pa.hist2 = phist2.p;
+        pa.bsum = pbsum.p;
+        pa.out_key = ok;
+        pa.out_cnt = oc;
+        pa.out_cap = out_cap;
+        pa.bw = d.bw;
+        pa.bw_stride = d.bw_stride;
+        pa.start = d.start;
+        pa.row = d.row;
+        pa.inst_shift = d.inst_shift;
+        pa.occ_mask = d.occ_mask;
+        pa.seed_mask = d.seed_mask;
+        pa.gcd = d.gcd;
+        pa.min_size = d.min_size;
+        pa.closed_rows = d.closed_rows;
+        pa.n_inst = d.n_inst;
+        pa.c32 = d.c32;
+        pa.c64 = d.c64;
+        pa.ctl = d.ctl;
+    }
+
     // The count-mode row step as one queued chain with one host round trip
     // (level kernels, the frontier in a single chunk): table fill, counter
     // resets, terminal pass, the level launches, the dedup, and compaction
@@ -1282,22 +1388,28 @@ struct Engine {
             HIPCHK(hipHostMalloc((void **)&pin32, sizeof(uint32_t) * C32_N));
             HIPCHK(hipHostMalloc((void **)&pin64, sizeof(unsigned long long) * C64_N));
         }
-        nk.reserve(tab_m);
-        nc.reserve(tab_m);
-        blk.reserve(2 * COMPACT_MAX_BLOCKS);
         LevelArgs a;
         DedupArgs d;
         level_args(tab_m, a, d);
         const double lv_pp = lv_pp0(), ch_pp = ch_pp0();
         level_buffers(n_front, lv_pp, ch_pp, a, d);
+        // the partitioned dedup writes the next frontier itself (as many
+        // states as the table would hold)
+        last_part = part_dedup_fits(ch_cap, n_front, tab_m, ch_pp * (double)n_front);
+        nk.reserve(tab_m);
+        nc.reserve(tab_m);
+        PartArgs pa{};
+        if (last_part) part_args(d, ch_cap, nk.p, nc.p, tab_m, pa);
+        else blk.reserve(2 * COMPACT_MAX_BLOCKS);
         HIPCHK(hipEventRecord(ev0, st));
         HIPCHK(hipMemcpyAsync(result_save.p, result.p, (size_t)S * 8, hipMemcpyDeviceToDevice, st));
-        launch_fill_table(tk.p, tk.p + 1, tab_m, st);
+        if (!last_part) launch_fill_table(tk.p, tk.p + 1, tab_m, st);
         launch_reset_counters(c32.p, c64.p, st);
         queue_terminal();
-        queue_levels(a, d, 0, n_front, prune, true);
+        queue_levels(a, d, 0, n_front, prune, true, last_part ? &pa : nullptr);
         HIPCHK(hipEventRecord(evc0, st));
-        launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, tab_m, c32.p, blk.p, nullptr, st);
+        if (!last_part)
+            launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, tab_m, c32.p, blk.p, nullptr, st);
         HIPCHK(hipEventRecord(ev1, st));
         HIPCHK(hipMemcpyAsync(pin32, c32.p, sizeof(uint32_t) * C32_N, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(pin64, c64.p, sizeof(unsigned long long) * C64_N,
@@ -1306,22 +1418,59 @@ struct Engine {
         sync_host();
         std::memcpy(h32, pin32, sizeof(uint32_t) * C32_N);
         std::memcpy(h64, pin64, sizeof(unsigned long long) * C64_N);
+        if (last_part && h32[C32_POVF] == 2u && !h32[C32_XOVF]) {
+            // the unique states outgrew the next frontier: the buckets are
+            // intact, so only the merge runs again, into buffers of the size
+            // it counted
+            nk.reserve(h64[C64_NEW]);
+            nc.reserve(h64[C64_NEW]);
+            pa.out_key = nk.p;
+            pa.out_cnt = nc.p;
+            pa.out_cap = nk.cap;
+            launch_part_merge_again(pa, st);
+            HIPCHK(hipEventRecord(ev1, st));
+            HIPCHK(hipMemcpyAsync(pin32, c32.p, sizeof(uint32_t) * C32_N, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(pin64, c64.p, sizeof(unsigned long long) * C64_N,
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipGetLastError());
+            sync_host();
+            std::memcpy(h32, pin32, sizeof(uint32_t) * C32_N);
+            std::memcpy(h64, pin64, sizeof(unsigned long long) * C64_N);
+            stats.part_remerges++;
+        }
         level_times(1);
         stats.fused_steps++;
         stats.donation_full += h32[C32_DONFULL];
         stats.extend_bytes += 16ull * n_front + 16ull * h64[C64_CHILDREN];
         stats.dedup_bytes += 16ull * (h64[C64_CHILDREN] + h64[C64_PRUNED]) + 16ull * h64[C64_NEW];
+        const bool pfail = last_part && (h32[C32_POVF] || (uint64_t)h32[C32_OUT] != h64[C64_NEW]);
         if (trace)
             fprintf(stderr,
                     "fused step %u: parents %llu, table %llu, caps %llu / %llu, children %llu, "
-                    "new %llu, level overflow %u, table overflow %u, spilled %u\n",
+                    "new %llu, level overflow %u, table overflow %u, spilled %u, partitioned %d "
+                    "(failed %d: flags %u, written %u, kept %llu, count sums %llu / %llu)\n",
                     g, (unsigned long long)n_front, (unsigned long long)tab_m,
                     (unsigned long long)lv_cap, (unsigned long long)ch_cap, h64[C64_CHW],
-                    h64[C64_NEW], h32[C32_XOVF], h32[C32_OVERFLOW], h32[C32_SPILL]);
-        if (h32[C32_XOVF] || h32[C32_OVERFLOW] || h32[C32_SPILL]) {
+                    h64[C64_NEW], h32[C32_XOVF], h32[C32_OVERFLOW], h32[C32_SPILL], (int)last_part,
+                    (int)pfail, h32[C32_POVF], h32[C32_OUT], h64[C64_CHILDREN], h64[C64_PSUM_IN],
+                    h64[C64_PSUM_OUT]);
+        if (h32[C32_XOVF] || h32[C32_OVERFLOW] || h32[C32_SPILL] || pfail) {
             copy(result.p, result_save.p, (size_t)S * 8, hipMemcpyDeviceToDevice);
             stats.fused_fallbacks++;
+            if (pfail) stats.part_fallbacks++;
+            last_part = false;
             return false;
+        }
+        if (last_part) {
+            // the merge keeps the sum of the tiling counts it was given
+            if (h64[C64_PSUM_IN] != h64[C64_PSUM_OUT])
+                throw Error(D4_EDEVICE,
+                            "internal error: partitioned dedup changed the count sum at step " +
+                                std::to_string(g) + " (" + std::to_string(h64[C64_PSUM_IN]) +
+                                " in, " + std::to_string(h64[C64_PSUM_OUT]) + " out, " +
+                                std::to_string(h64[C64_NEW]) + " states, " +
+                                std::to_string(h64[C64_CHW]) + " children written)");
+            stats.part_steps++;
         }
         level_sizes(h64, n_front);
         return true;
@@ -1621,8 +1770,10 @@ struct Engine {
         HIPCHK(hipEventElapsedTime(&cms, evc0, ev1));
         stats.gpu_ms += ms;
         stats.compact_ms += cms;
-        stats.compact_launches++;
-        stats.compact_bytes += 8ull * tab_m + 24ull * U;
+        if (!(fused_ok && last_part)) {
+            stats.compact_launches++;
+            stats.compact_bytes += 8ull * tab_m + 24ull * U;
+        }
         if (en) {
             // the step's frontier and edges stay on the device as its log;
             // the next step gets fresh buffers from the allocator cache
@@ -2609,6 +2760,21 @@ int d4_library_get_step_mode(const d4_library *lib, uint32_t *mode) {
     return guard([&] {
         if (!lib || !mode) throw Error(D4_EINVAL, "null argument");
         *mode = lib->step_mode;
+    });
+}
+
+int d4_library_set_dedup(d4_library *lib, uint32_t mode) {
+    return guard([&] {
+        if (!lib) throw Error(D4_EINVAL, "null library");
+        if (mode > D4_DEDUP_PART) throw Error(D4_EINVAL, "unknown dedup mode");
+        lib->dedup_mode = mode;
+    });
+}
+
+int d4_library_get_dedup(const d4_library *lib, uint32_t *mode) {
+    return guard([&] {
+        if (!lib || !mode) throw Error(D4_EINVAL, "null argument");
+        *mode = lib->dedup_mode;
     });
 }
 

@@ -30,6 +30,30 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t *ctr, bool pred) {
     return base + mbcnt(m);
 }
 
+// A device counter that earlier kernels maintain with atomics (item counts,
+// flags): read with an agent-scope atomic load, which is coherent across the
+// XCDs' L2 caches.  A plain load of such a counter can be served from a stale
+// scalar-cache or L2 line: the partitioned dedup, whose first kernel reads
+// the children count the level kernels just advanced, measured such stale
+// reads (fewer children than written) until its reads were made atomic.
+__device__ __forceinline__ uint64_t ctr_load(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ctr_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroup barrier after LDS atomics: waits for this wave's LDS operations
+// to complete (s_waitcnt lgkmcnt(0)) before s_barrier.  The compiler for
+// gfx950 does not always emit that wait before a barrier (it was missing on
+// the loop-exit path of a histogram kernel), and an LDS atomic still in
+// flight when the other waves pass the barrier is then missed by their
+// reads: measured as lost histogram counts, one wave's worth at a time.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt unconstrained
+    __syncthreads();
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     for (int o = 32; o > 0; o >>= 1) {
         uint32_t lo = __shfl_xor((uint32_t)v, o);

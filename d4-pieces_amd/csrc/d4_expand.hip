@@ -86,7 +86,7 @@ __global__ __launch_bounds__(THREADS) void k_expand(ExtendArgs a) {
     const uint32_t per_rec = 1u << (6 * fix);
     uint32_t n_don = a.n_don;
     if (don_mode && n_don == ~0u) {
-        const uint32_t dn = a.c32[a.don_read], de = a.c32[a.don_read + 1];
+        const uint32_t dn = ctr_load(a.c32 + a.don_read), de = ctr_load(a.c32 + a.don_read + 1);
         n_don = dn <= a.don_cap ? dn : (de ? de - 1 : a.don_cap);
     }
     const uint64_t n_items = don_mode ? (uint64_t)n_don : (uint64_t)a.n_in * per_rec;

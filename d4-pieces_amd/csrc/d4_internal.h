@@ -34,7 +34,8 @@ enum : int {
     C32_XOVF = 11,     // a partial row found no room (k_expand spill, k_level buffers)
     C32_LVDONE = 12,   // chunks the level kernels' dedup completed this step
     C32_DONFULL = 13,  // donations / spills refused because the buffer was full
-    C32_N = 14,
+    C32_POVF = 14,     // a bucket or merge pass of the partitioned dedup did not fit
+    C32_N = 15,
 };
 enum : int {
     C64_CHILDREN = 0,
@@ -45,7 +46,9 @@ enum : int {
     C64_DONATED = 5,   // work items handed to follow-up launches
     C64_CHW = 6,       // children written by the level kernels (k_level, incl. holes)
     C64_LVL = 7,       // C64_LVL + L: partial rows written for level L (k_level)
-    C64_N = 16,
+    C64_PSUM_IN = 15,  // partitioned dedup: sum of the surviving children's counts
+    C64_PSUM_OUT = 16, // and of the unique states' counts it emitted (equal, mod 2^64)
+    C64_N = 17,
 };
 
 // Arguments of the extend kernel (passed by value).
@@ -131,7 +134,8 @@ struct RowCtl {
     unsigned long long tab_cap;  // table slots and next-frontier records allocated
     double ratio;                // unique states per parent of the step before
     uint32_t row;                // its row
-    uint32_t fail;               // 1 level buffer, 2 table, 4 spill, 8 sizes
+    uint32_t fail;               // 1 level buffer, 2 table, 4 spill, 8 sizes,
+                                 // 16 a bucket of the partitioned dedup
     uint32_t done;               // steps completed
     uint32_t pad;
     StepLog log[CTL_LOG];        // per completed step, at done % CTL_LOG
@@ -192,6 +196,54 @@ struct DedupArgs {
 };
 void launch_level(const LevelArgs &a, int n_cu, hipStream_t st);
 void launch_dedup(const DedupArgs &a, bool prune, int n_cu, hipStream_t st);
+
+// Partitioned dedup of a row step's children (d4_part.hip): surviving
+// children (connectivity prune) to hash buckets, each bucket merged in one
+// workgroup's LDS, unique states written to the next frontier (counted in
+// c64[C64_NEW], written in c32[C32_OUT]).  A row with more children than
+// n_max <= PART_MAX_CHILDREN sets c32[C32_POVF] and does nothing.
+constexpr uint64_t PART_MAX_CHILDREN = 1ull << 30;
+struct PartArgs {
+    uint64_t *ch_key, *ch_cnt;  // children; after the second radix pass, the
+                             // survivors in bucket order
+    uint64_t ch_cap;         // children: min(c64[C64_CHW], ch_cap)
+    uint64_t n_max;          // children the buffers below are sized for
+    uint64_t tile_min;       // children per level-1 tile, at least (part_tile_min())
+    uint32_t lp_max;         // log2 of the most buckets (part_lp_max())
+    uint32_t slots;          // LDS table slots of the merge (part_slots())
+    uint64_t *mask;          // [part_mask_words(n_max)] survivor bits
+    uint64_t *pbuf;          // [2 * n_max] {key, count} pairs in coarse-bucket order
+    uint64_t *sbuf;          // [2 * n_max] the same pairs in bucket order (second pass)
+    uint32_t *hist, *hist2;  // [part_hist_entries(n_max)] per (bucket, tile) counts
+    uint32_t *bsum;          // [part_bsum_entries()] scan block sums, then the totals
+    uint64_t *out_key, *out_cnt;
+    uint64_t out_cap;        // next-frontier records allocated
+    // pruning, as DedupArgs
+    const uint64_t *bw;
+    uint32_t bw_stride;
+    const uint32_t *start;
+    uint32_t row;            // the children's row is row + 1
+    uint32_t inst_shift;
+    uint64_t occ_mask;
+    uint64_t seed_mask;
+    uint32_t gcd, min_size;
+    uint32_t closed_rows;
+    uint32_t n_inst;
+    uint32_t *c32;
+    unsigned long long *c64;
+    const RowCtl *ctl;       // set: row comes from it (graph replay)
+};
+void part_hist_entries(uint64_t n_max, uint64_t &level1, uint64_t &level2);
+uint64_t part_bsum_entries();
+uint64_t part_mask_words(uint64_t n_max);
+uint64_t part_tile_min();
+uint32_t part_lp_max();
+uint32_t part_slots();
+// c32[C32_POVF]: bit 0 the buckets did not fit (the step reruns with the
+// table), bit 1 the next frontier was too small (C64_NEW says how large it
+// must be; launch_part_merge_again reruns the merge alone)
+void launch_part_dedup(const PartArgs &a, bool prune, int n_cu, hipStream_t st);
+void launch_part_merge_again(const PartArgs &a, hipStream_t st);
 size_t level_lds_bytes(uint32_t fit_rows);
 
 // Tables of one library in a multi-library batch (device layout, 16-byte

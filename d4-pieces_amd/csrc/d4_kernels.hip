@@ -81,7 +81,7 @@ __global__ __launch_bounds__(THREADS) void k_extend(ExtendArgs a) {
     // count from the counters of the launch before it
     uint32_t n_don = a.n_don;
     if (don_mode && n_don == ~0u) {
-        const uint32_t dn = a.c32[a.don_read], de = a.c32[a.don_read + 1];
+        const uint32_t dn = ctr_load(a.c32 + a.don_read), de = ctr_load(a.c32 + a.don_read + 1);
         n_don = dn <= a.don_cap ? dn : (de ? de - 1 : a.don_cap);
     }
     const uint64_t n_items = don_mode ? (uint64_t)n_don : (uint64_t)a.n_in * per_rec;
@@ -567,25 +567,26 @@ __global__ __launch_bounds__(256) void k_copy_u64(unsigned long long *dst,
 // allocation.
 __global__ void k_row_close(RowCtl *ctl, const uint32_t *c32, const unsigned long long *c64) {
     if (threadIdx.x != 0 || ctl->fail) return;
-    const uint64_t n = ctl->n_front, u = c64[C64_NEW];
-    uint32_t f = (c32[C32_XOVF] ? 1u : 0u) | (c32[C32_OVERFLOW] ? 2u : 0u) |
-                 (c32[C32_SPILL] ? 4u : 0u);
-    if (!f && ((uint64_t)c32[C32_OUT] != u || u > ctl->tab_cap)) f = 8u;
+    const uint64_t n = ctl->n_front, u = ctr_load(c64 + C64_NEW);
+    uint32_t f = (ctr_load(c32 + C32_XOVF) ? 1u : 0u) | (ctr_load(c32 + C32_OVERFLOW) ? 2u : 0u) |
+                 (ctr_load(c32 + C32_SPILL) ? 4u : 0u) | (ctr_load(c32 + C32_POVF) ? 16u : 0u);
+    const uint32_t out = ctr_load(c32 + C32_OUT);
+    if (!f && ((uint64_t)out != u || u > ctl->tab_cap)) f = 8u;
     if (f) {
         ctl->fail = f;
         return;
     }
     StepLog &l = ctl->log[ctl->done % CTL_LOG];
     uint64_t lv = 0;
-    for (uint32_t L = 0; L < LVL_MAX; L++) lv = max(lv, (uint64_t)c64[C64_LVL + L]);
+    for (uint32_t L = 0; L < LVL_MAX; L++) lv = max(lv, ctr_load(c64 + C64_LVL + L));
     l.n_front = n;
     l.tab_m = ctl->tab_m;
-    l.children = c64[C64_CHILDREN];
-    l.pruned = c64[C64_PRUNED];
+    l.children = ctr_load(c64 + C64_CHILDREN);
+    l.pruned = ctr_load(c64 + C64_PRUNED);
     l.fresh = u;
-    l.chw = c64[C64_CHW];
+    l.chw = ctr_load(c64 + C64_CHW);
     l.lv_max = lv;
-    l.out = c32[C32_OUT];
+    l.out = out;
     // (growth that speeds up from step to step is extrapolated once more)
     double r = n ? (double)u / (double)n : 1.0;
     if (r < 0.01) r = 0.01;
@@ -632,7 +633,7 @@ __global__ void k_reset_levels(uint32_t *c32, unsigned long long *c64, uint32_t 
                                const RowCtl *ctl) {
     if (ctl && ctl->fail) return;
     // after an overflow the counters keep the failed chunk's sizes for the host
-    if (!first && c32[C32_XOVF]) return;
+    if (!first && ctr_load(c32 + C32_XOVF)) return;
     if (threadIdx.x <= LVL_MAX) c64[C64_CHW + threadIdx.x] = 0;
     if (first && threadIdx.x == 0) {
         c32[C32_XOVF] = 0;
@@ -839,7 +840,7 @@ __global__ __launch_bounds__(256) void k_buckets(
         atomicAdd(s_n + ea, 1ull);
         atomicAdd(s_c + ea, comp[ea]);
     }
-    __syncthreads();
+    lds_barrier();
     if (s_n[threadIdx.x]) {
         atomicAdd(out + threadIdx.x, s_n[threadIdx.x]);
         atomicAdd(out + 256 + threadIdx.x, s_c[threadIdx.x]);

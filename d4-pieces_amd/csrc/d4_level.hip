@@ -126,7 +126,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     if (FROM_REC) {
         n_in = a.n_rec;
     } else {
-        n_in = a.c64[C64_LVL + a.level - 1];
+        n_in = ctr_load(a.c64 + C64_LVL + a.level - 1);
         if (n_in > a.in_cap) n_in = a.in_cap;
     }
     const uint64_t waves = (uint64_t)gridDim.x * (LT / 64);
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_dedup(DedupArgs a) {
         }
         __syncthreads();
     }
-    uint64_t n = a.c64[C64_CHW];
+    uint64_t n = ctr_load(a.c64 + C64_CHW);
     if (n > a.ch_cap) n = a.ch_cap;
     // a chunk whose level kernels overflowed is rerun by the host: nothing
     // of it enters the table; a complete chunk counts itself once

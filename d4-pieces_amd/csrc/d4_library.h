@@ -35,6 +35,9 @@ struct d4_library {
     uint32_t xp_kernel = D4_KERNEL_LEVEL;
     // count-mode level row steps: D4_STEP_GENERAL / _FUSED / _GRAPH
     uint32_t step_mode = D4_STEP_FUSED;
+    // count-mode level row steps: D4_DEDUP_PART (hash buckets merged in LDS)
+    // or D4_DEDUP_TABLE (one table in device memory)
+    uint32_t dedup_mode = D4_DEDUP_PART;
     uint32_t xp_budget = 256;     // k_expand: tail rounds before a wave's snapshots move on
     uint32_t don_items = 1u << 20;  // donated work items / spilled partial rows per buffer
     // Level-buffer sizes seen by earlier searches on this library, per row

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(THREADS) void k_extend_multi(MultiArgs a) {
     // count from the counters of the launch before it
     uint32_t n_don = a.n_don;
     if (don_mode && n_don == ~0u) {
-        const uint32_t dn = a.c32[a.don_read], de = a.c32[a.don_read + 1];
+        const uint32_t dn = ctr_load(a.c32 + a.don_read), de = ctr_load(a.c32 + a.don_read + 1);
         n_don = dn <= a.don_cap ? dn : (de ? de - 1 : a.don_cap);
     }
     const uint64_t n_items = don_mode ? (uint64_t)n_don : (uint64_t)a.n_in * per_rec;
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(256) void k_inst_hist(const uint64_t *k0, uint64_t 
         if (lds) atomicAdd(bins + i, 1u);
         else atomicAdd(hist + i, 1ull);
     }
-    __syncthreads();
+    lds_barrier();
     if (lds)
         for (uint32_t i = threadIdx.x; i < n_inst; i += blockDim.x)
             if (bins[i]) atomicAdd(hist + i, (unsigned long long)bins[i]);

@@ -35,6 +35,7 @@ FLAG_NO_PRUNE = 1
 FLAG_HOST_TABLES = 2
 KERNEL_DFS, KERNEL_EXPAND, KERNEL_LEVEL = 0, 1, 2
 STEP_GENERAL, STEP_FUSED, STEP_GRAPH = 0, 1, 2
+DEDUP_TABLE, DEDUP_PART = 0, 1
 
 
 class D4Error(RuntimeError):
@@ -76,6 +77,9 @@ class Stats(ctypes.Structure):
         ("host_parked_peak", ctypes.c_uint64),
         ("streamed_steps", ctypes.c_uint64),
         ("stream_passes", ctypes.c_uint64),
+        ("part_steps", ctypes.c_uint64),
+        ("part_fallbacks", ctypes.c_uint64),
+        ("part_remerges", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -114,6 +118,8 @@ SIGNATURES = [
     ("d4_mbatch_tables", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
     ("d4_library_host_tables", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_uint64]),
     ("d4_library_get_step_mode", ctypes.c_int, [_P, _U32P]),
+    ("d4_library_set_dedup", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_library_get_dedup", ctypes.c_int, [_P, _U32P]),
     ("d4_searcher_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
     ("d4_searcher_set_sample", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint64]),
     ("d4_searcher_sample", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
@@ -417,6 +423,16 @@ class PieceLibrary:
         trips between the parts), STEP_FUSED (default: one queued chain, one
         round trip) or STEP_GRAPH (that chain replayed from a HIP graph)."""
         _check(load().d4_library_set_step_mode(self._h, mode))
+
+    def set_dedup(self, mode: int) -> None:
+        """Dedup of count-mode level row steps: DEDUP_PART (default: hash
+        buckets merged in LDS) or DEDUP_TABLE (one table in device memory)."""
+        _check(load().d4_library_set_dedup(self._h, mode))
+
+    def get_dedup(self) -> int:
+        m = ctypes.c_uint32()
+        _check(load().d4_library_get_dedup(self._h, ctypes.byref(m)))
+        return m.value
 
     def get_step_mode(self) -> int:
         m = ctypes.c_uint32()

@@ -97,6 +97,15 @@ typedef struct d4_stats {
     uint64_t streamed_steps;   /* row steps run in hash partitions (too large
                                   for one dedup table under the cap)        */
     uint64_t stream_passes;    /* partitions of those steps                 */
+    uint64_t part_steps;       /* row steps deduped by hash buckets merged in
+                                  LDS                                       */
+    uint64_t part_fallbacks;   /* of those, steps rerun with the device-memory
+                                  table (more children than the buckets' buffers
+                                  were sized for, a coarse bucket past its
+                                  tiles, or a hash sub-range with more unique
+                                  states than an LDS table holds)           */
+    uint64_t part_remerges;    /* bucket merges rerun alone into a larger next
+                                  frontier (more unique states than estimated) */
 } d4_stats;
 
 /* Thread-local text of the last error. */
@@ -197,6 +206,21 @@ int d4_library_get_kernel(const d4_library *lib, uint32_t *kernel, uint32_t *tai
 enum { D4_STEP_GENERAL = 0, D4_STEP_FUSED = 1, D4_STEP_GRAPH = 2 };
 int d4_library_set_step_mode(d4_library *lib, uint32_t mode);
 int d4_library_get_step_mode(const d4_library *lib, uint32_t *mode);
+/* Dedup of count-mode level row steps, fused or replayed (replaces the
+ * reference's host Sorter, src/sorter.cpp:156-214, whose set semantics it
+ * keeps: equal states merge, their tiling counts summed):
+ *   D4_DEDUP_TABLE: every child merged into one open-addressing table in
+ *     device memory (a coherent load, an atomic add, a CAS for a new state),
+ *     then the table compacted into the next frontier;
+ *   D4_DEDUP_PART (the default): children pruned and the survivors moved by
+ *     their key's hash into buckets (a histogram, a scan and a scatter, no
+ *     device atomic per child), each bucket merged in one workgroup's LDS
+ *     hash table and its unique states written to the next frontier; rows
+ *     with more than 2^30 children use the table.
+ * Counts are identical; only the order of the next frontier differs. */
+enum { D4_DEDUP_TABLE = 0, D4_DEDUP_PART = 1 };
+int d4_library_set_dedup(d4_library *lib, uint32_t mode);
+int d4_library_get_dedup(const d4_library *lib, uint32_t *mode);
 /* Items per donation buffer of the depth-first and snapshot kernels (default
  * 1 Mi; at least 64).  A lane or wave that finds the buffer full keeps its
  * work (counted in d4_stats.donation_full); small values exercise that path. */

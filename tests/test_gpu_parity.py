@@ -500,3 +500,68 @@ def test_step_modes_agree(gpu, p5, oracle, monkeypatch, mode, scale):
         lb = gpu.PieceLibrary(shapes, counts, 0xFF, w)
         lb.set_step_mode(m)
         assert gpu_count(gpu, lb, rows) == oracle.count(shapes, counts, rows)
+
+
+@pytest.mark.parametrize("dedup", ["table", "part", "part_small"])
+def test_dedup_modes_agree(gpu, p5, oracle, monkeypatch, dedup):
+    """The row step's dedup through one table in device memory or through
+    hash buckets merged in LDS gives the oracle's counts: the 256 seeded 8x8
+    boards (fixture), 6x10 alone, each pentomino twice on 8x5 (695,086) and
+    random small instances with multiplicities; the bucket path runs without
+    falling back to the table.  part_small sends every row, however small,
+    through the buckets, among them the 8x5 search under a 2 MiB device cap
+    with host parking, whose small single-bucket rows once lost one wave's
+    histogram counts (an LDS atomic still in flight at a barrier), three
+    times over."""
+    import json
+    import os
+    if dedup == "part_small":
+        monkeypatch.setenv("D4_PART_MIN_CHILDREN", "0")
+    m = {"table": gpu.DEDUP_TABLE, "part": gpu.DEDUP_PART, "part_small": gpu.DEDUP_PART}[dedup]
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    lib.set_dedup(m)
+    assert lib.get_dedup() == m
+    b = gpu.Batch(lib, fx["boards"])
+    b.run()
+    assert b.counts() == fx["counts"]
+    st = b.stats()
+    if dedup != "table":
+        assert st["part_steps"] > 0 and st["part_fallbacks"] == 0
+    else:
+        assert st["part_steps"] == 0
+    lib6 = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 6)
+    lib6.set_dedup(m)
+    s = gpu.CudaSearcher(lib6, rows=rect(6, 10))
+    s.run()
+    assert s.count() == 9356
+    lib2 = gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8)
+    lib2.set_dedup(m)
+    s = gpu.CudaSearcher(lib2, rows=rect(8, 5))
+    s.run()
+    assert s.count() == 695086
+    if dedup == "part_small":
+        assert s.stats()["part_steps"] > 0
+        for _ in range(3):
+            lb = gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8)
+            lb.set_dedup(m)
+            s = gpu.CudaSearcher(lb, rows=rect(8, 5))
+            s.set_memory(2 << 20)
+            s.set_host_memory(1 << 30)
+            s.run()
+            st = s.stats()
+            assert s.count() == 695086 and st["part_steps"] > 0 and st["part_fallbacks"] == 0
+    rng = random.Random(21)
+    for _ in range(8):
+        shapes = rng.sample(gpu.free_polyominoes(4) + gpu.free_polyominoes(3), 4)
+        counts = [rng.randint(1, 3) for _ in shapes]
+        w = rng.randint(3, 7)
+        area = sum(bin(x).count("1") * c for x, c in zip(shapes, counts))
+        h = -(-area // w)
+        rows = [(1 << w) - 1] * h
+        extra = w * h - area
+        rows[-1] &= ((1 << w) - 1) >> extra
+        lb = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+        lb.set_dedup(m)
+        assert gpu_count(gpu, lb, rows) == oracle.count(shapes, counts, rows)
