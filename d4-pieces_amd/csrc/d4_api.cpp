@@ -1427,7 +1427,7 @@ struct Engine {
             pa.out_key = nk.p;
             pa.out_cnt = nc.p;
             pa.out_cap = nk.cap;
-            launch_part_merge_again(pa, st);
+            launch_part_merge_again(pa, lib->n_cu, st);
             HIPCHK(hipEventRecord(ev1, st));
             HIPCHK(hipMemcpyAsync(pin32, c32.p, sizeof(uint32_t) * C32_N, hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(pin64, c64.p, sizeof(unsigned long long) * C64_N,

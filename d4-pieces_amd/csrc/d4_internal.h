@@ -243,7 +243,7 @@ uint32_t part_slots();
 // table), bit 1 the next frontier was too small (C64_NEW says how large it
 // must be; launch_part_merge_again reruns the merge alone)
 void launch_part_dedup(const PartArgs &a, bool prune, int n_cu, hipStream_t st);
-void launch_part_merge_again(const PartArgs &a, hipStream_t st);
+void launch_part_merge_again(const PartArgs &a, int n_cu, hipStream_t st);
 size_t level_lds_bytes(uint32_t fit_rows);
 
 // Tables of one library in a multi-library batch (device layout, 16-byte

@@ -25,20 +25,26 @@
 //                   land as a run of consecutive records (one digit of 14
 //                   bits would scatter them as lone 16-byte stores, which
 //                   cost as much as random atomics)
-//   k_part_merge    one workgroup per bucket: an LDS hash table of 4 Ki slots
-//                   per pass over a sub-range of the hash (a pass takes at
-//                   most 3 Ki children; a bucket that has more is read once
-//                   per pass, from the caches after the first), counts summed
-//                   with LDS atomics, the unique states written densely to
-//                   the next frontier with one device atomic per pass.
+//   k_part_merge    as many workgroups as fit at once, each looping over its
+//                   share of the buckets: an LDS hash table of twice the
+//                   pass's survivors (256 to 8 Ki slots, so clearing and
+//                   scanning it costs in proportion to the bucket) per pass
+//                   over a sub-range of the hash (a pass takes at most 6 Ki
+//                   survivors; a bucket that has more is read once per pass,
+//                   from the caches after the first), counts summed with LDS
+//                   atomics, the unique states written densely to the next
+//                   frontier with one device atomic per pass.
 //
 // There is no device atomic per child, and only surviving children (about a
 // third on the 8x8 pentomino boards) are moved.  Every size is derived on the
 // device from c64[C64_CHW], so the chain runs without a host round trip and
 // can be recorded in a graph.  A row with more children than the buffers were
 // sized for, or a pass whose unique states exceed the LDS table (a single
-// sub-range of the hash with more than 4 Ki of them), sets c32[C32_POVF]: the
-// host then reruns the step with the device-memory table.  The set semantics are the
+// sub-range of the hash with more than 8 Ki of them), sets c32[C32_POVF]: the
+// host then reruns the step with the device-memory table; a next frontier too
+// small for the unique states is enlarged and the merge alone rerun.  The
+// dedup checks itself: the counts it was given and the counts it emitted
+// must have the same sum.  The set semantics are the
 // reference's (src/sorter.cpp:184-207: equal states merge, here with their
 // tiling counts summed); its radix dedup prototype (bmark/radix.cu:41-59)
 // partitions by key bits in the same spirit.
@@ -102,8 +108,6 @@ __device__ __forceinline__ uint32_t bucket_of(uint64_t h, uint32_t lp) {
     return lp ? (uint32_t)(h >> (64 - lp)) : 0u;
 }
 __device__ __forceinline__ uint32_t sub_of(uint64_t h) { return (uint32_t)(h >> 32) & (P_SUB - 1); }
-template <uint32_t SL>
-__device__ __forceinline__ uint32_t lslot_of(uint64_t h) { return (uint32_t)h & (SL - 1); }
 
 // Children of the step: none after a level overflow (the host reruns the
 // step), and none when they exceed what the buffers hold (c32[C32_POVF]).
@@ -454,8 +458,10 @@ __global__ __launch_bounds__(PT) void k_part_merge(PartArgs a) {
     constexpr uint32_t PL_FILL = PL_SLOTS / 4 * 3;
     __shared__ unsigned long long s_key[PL_SLOTS];
     __shared__ unsigned long long s_cnt[PL_SLOTS];
+    __shared__ uint32_t s_bs[PT], s_be[PT];     // bounds of this workgroup's next buckets
     __shared__ uint32_t s_sub[P_SUB];
-    __shared__ uint32_t s_pass[P_SUB + 2];  // boundaries, then the pass count
+    __shared__ uint32_t s_pass[P_SUB + 2];      // boundaries, then the pass count
+    __shared__ uint32_t s_pcnt[P_SUB];          // survivors per pass
     __shared__ uint32_t s_w[PT / 64 + 1];
     __shared__ unsigned long long s_base;
     if (a.ctl && a.ctl->fail) return;
@@ -466,114 +472,134 @@ __global__ __launch_bounds__(PT) void k_part_merge(PartArgs a) {
     if (blockIdx.x >= P) return;
     const uint32_t total = a.bsum[PS_MAX_BLOCKS + 1];
     const ulonglong2 *sb = reinterpret_cast<const ulonglong2 *>(a.sbuf);
+    const uint32_t G = gridDim.x;
+    const uint32_t mine = (P - blockIdx.x + G - 1) / G;  // buckets blockIdx.x + k * G
     uint32_t fail = 0;
     uint64_t csum = 0;
-    for (uint32_t b = blockIdx.x; b < P; b += gridDim.x) {
-        const uint32_t s = a.hist2[(uint64_t)b * sh.T2];
-        const uint32_t e = b + 1 < P ? a.hist2[(uint64_t)(b + 1) * sh.T2] : total;
-        if (e <= s) continue;  // (uniform: no survivors in this bucket)
-        uint32_t np = 1;
-        if (e - s > PL_FILL) {
-            // cut the bucket into passes of consecutive sub-bins, at most
-            // PL_FILL survivors each
-            if (threadIdx.x < P_SUB) s_sub[threadIdx.x] = 0;
-            lds_barrier();
-            for (uint32_t i = s + threadIdx.x; i < e; i += PT)
-                atomicAdd(&s_sub[sub_of(mix64(sb[i].x))], 1u);
-            lds_barrier();
-            if (threadIdx.x == 0) {
-                uint32_t k = 0, acc = 0;
-                s_pass[0] = 0;
-                for (uint32_t q = 0; q < P_SUB; q++) {
-                    const uint32_t c = s_sub[q];
-                    if (acc && acc + c > PL_FILL) {
-                        s_pass[++k] = q;
-                        acc = 0;
-                    }
-                    acc += c;
-                }
-                s_pass[++k] = P_SUB;
-                s_pass[P_SUB + 1] = k;
-            }
-            lds_barrier();
-            np = s_pass[P_SUB + 1];
+    for (uint32_t k0 = 0; k0 < mine; k0 += PT) {
+        const uint32_t kn = min(PT, mine - k0);
+        if (threadIdx.x < kn) {
+            const uint32_t b = blockIdx.x + (k0 + threadIdx.x) * G;
+            s_bs[threadIdx.x] = a.hist2[(uint64_t)b * sh.T2];
+            s_be[threadIdx.x] = b + 1 < P ? a.hist2[(uint64_t)(b + 1) * sh.T2] : total;
         }
-        for (uint32_t r = 0; r < np; r++) {
-            const uint32_t lo = np == 1 ? 0u : s_pass[r], hi = np == 1 ? P_SUB : s_pass[r + 1];
-            for (uint32_t i = threadIdx.x; i < PL_SLOTS; i += PT) {
-                s_key[i] = EMPTY_KEY;
-                s_cnt[i] = 0;
-            }
-            lds_barrier();
-            for (uint32_t i0 = s + threadIdx.x; i0 < e; i0 += 4 * PT) {
-                uint64_t key[4], cnt[4];
-#pragma unroll
-                for (uint32_t u = 0; u < 4; u++) {
-                    const uint32_t i = i0 + u * PT;
-                    ulonglong2 v;
-                    v.x = EMPTY_KEY;
-                    v.y = 0;
-                    if (i < e) v = sb[i];
-                    key[u] = v.x;
-                    cnt[u] = v.y;
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < 4; u++) {
-                    if (i0 + u * PT >= e) continue;
-                    const uint64_t h = mix64(key[u]);
-                    const uint32_t q = sub_of(h);
-                    if (q < lo || q >= hi) continue;
-                    uint32_t sl = lslot_of<PL_SLOTS>(h), p = 0;
-                    for (; p < PL_SLOTS; p++) {
-                        const unsigned long long cur = atomicCAS(
-                            &s_key[sl], (unsigned long long)EMPTY_KEY, (unsigned long long)key[u]);
-                        if (cur == EMPTY_KEY || cur == key[u]) {
-                            atomicAdd(&s_cnt[sl], (unsigned long long)cnt[u]);
-                            break;
+        lds_barrier();
+        for (uint32_t k = 0; k < kn; k++) {
+            const uint32_t s = s_bs[k], e = s_be[k];
+            if (e <= s) continue;  // (uniform: no survivors in this bucket)
+            uint32_t np = 1;
+            if (e - s > PL_FILL) {
+                // cut the bucket into passes of consecutive sub-bins, at most
+                // PL_FILL survivors each
+                if (threadIdx.x < P_SUB) s_sub[threadIdx.x] = 0;
+                lds_barrier();
+                for (uint32_t i = s + threadIdx.x; i < e; i += PT)
+                    atomicAdd(&s_sub[sub_of(mix64(sb[i].x))], 1u);
+                lds_barrier();
+                if (threadIdx.x == 0) {
+                    uint32_t q2 = 0, acc = 0;
+                    s_pass[0] = 0;
+                    for (uint32_t q = 0; q < P_SUB; q++) {
+                        const uint32_t c = s_sub[q];
+                        if (acc && acc + c > PL_FILL) {
+                            s_pcnt[q2] = acc;
+                            s_pass[++q2] = q;
+                            acc = 0;
                         }
-                        sl = (sl + 1) & (PL_SLOTS - 1);
+                        acc += c;
                     }
-                    if (p == PL_SLOTS) fail |= 1u;
+                    s_pcnt[q2] = acc;
+                    s_pass[++q2] = P_SUB;
+                    s_pass[P_SUB + 1] = q2;
                 }
+                lds_barrier();
+                np = s_pass[P_SUB + 1];
             }
-            lds_barrier();
-            // this pass's unique states: slots q * PT + lane for q < SPL
-            // (consecutive lanes read consecutive slots: no LDS bank conflicts,
-            // where SPL consecutive slots per lane conflicted 32-fold)
-            constexpr uint32_t SPL = PL_SLOTS / PT;
-            uint32_t live = 0;
-#pragma unroll
-            for (uint32_t q = 0; q < SPL; q++)
-                if (s_key[q * PT + threadIdx.x] != EMPTY_KEY) {
-                    live |= 1u << q;
-                    csum += s_cnt[q * PT + threadIdx.x];
+            for (uint32_t r = 0; r < np; r++) {
+                const uint32_t lo = np == 1 ? 0u : s_pass[r], hi = np == 1 ? P_SUB : s_pass[r + 1];
+                // a table of twice the pass's survivors (at least 256 slots):
+                // clearing and scanning it costs in proportion to the bucket
+                const uint32_t m = np == 1 ? e - s : s_pcnt[r];
+                uint32_t ts = 256;
+                while (ts < 2 * m && ts < PL_SLOTS) ts <<= 1;
+                for (uint32_t i = threadIdx.x; i < ts; i += PT) {
+                    s_key[i] = EMPTY_KEY;
+                    s_cnt[i] = 0;
                 }
-            uint32_t tot;
-            const uint32_t off = block_excl_scan((uint32_t)__builtin_popcount(live), s_w, tot);
-            if (threadIdx.x == 0) {
-                unsigned long long base = 0;
-                if (tot) {
-                    base = atomicAdd(a.c64 + C64_NEW, (unsigned long long)tot);
-                    if (base + tot <= a.out_cap) atomicAdd(a.c32 + C32_OUT, tot);
-                }
-                s_base = base;
-            }
-            lds_barrier();
-            const unsigned long long base = s_base;
-            if (base + tot <= a.out_cap) {
-                uint32_t o = off;
+                lds_barrier();
+                for (uint32_t i0 = s + threadIdx.x; i0 < e; i0 += 4 * PT) {
+                    uint64_t key[4], cnt[4];
 #pragma unroll
-                for (uint32_t q = 0; q < SPL; q++)
-                    if (live >> q & 1u) {
-                        a.out_key[base + o] = s_key[q * PT + threadIdx.x];
-                        a.out_cnt[base + o] = s_cnt[q * PT + threadIdx.x];
-                        o++;
+                    for (uint32_t u = 0; u < 4; u++) {
+                        const uint32_t i = i0 + u * PT;
+                        ulonglong2 v;
+                        v.x = EMPTY_KEY;
+                        v.y = 0;
+                        if (i < e) v = sb[i];
+                        key[u] = v.x;
+                        cnt[u] = v.y;
                     }
-            } else {
-                fail |= 2u;  // (the host can rerun the merge into larger buffers)
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; u++) {
+                        if (i0 + u * PT >= e) continue;
+                        const uint64_t h = mix64(key[u]);
+                        const uint32_t q = sub_of(h);
+                        if (q < lo || q >= hi) continue;
+                        uint32_t sl = (uint32_t)h & (ts - 1), p = 0;
+                        for (; p < ts; p++) {
+                            const unsigned long long cur = atomicCAS(
+                                &s_key[sl], (unsigned long long)EMPTY_KEY, (unsigned long long)key[u]);
+                            if (cur == EMPTY_KEY || cur == key[u]) {
+                                atomicAdd(&s_cnt[sl], (unsigned long long)cnt[u]);
+                                break;
+                            }
+                            sl = (sl + 1) & (ts - 1);
+                        }
+                        if (p == ts) fail |= 1u;
+                    }
+                }
+                lds_barrier();
+                // this pass's unique states: slots q * PT + lane (consecutive
+                // lanes read consecutive slots, free of LDS bank conflicts)
+                constexpr uint32_t SPL = PL_SLOTS / PT;
+                uint32_t live = 0;
+#pragma unroll
+                for (uint32_t q = 0; q < SPL; q++) {
+                    const uint32_t sl = q * PT + threadIdx.x;
+                    if (sl < ts && s_key[sl] != EMPTY_KEY) {
+                        live |= 1u << q;
+                        csum += s_cnt[sl];
+                    }
+                }
+                uint32_t tot;
+                const uint32_t off = block_excl_scan((uint32_t)__builtin_popcount(live), s_w, tot);
+                if (threadIdx.x == 0) {
+                    unsigned long long base = 0;
+                    if (tot) {
+                        base = atomicAdd(a.c64 + C64_NEW, (unsigned long long)tot);
+                        if (base + tot <= a.out_cap) atomicAdd(a.c32 + C32_OUT, tot);
+                    }
+                    s_base = base;
+                }
+                lds_barrier();
+                const unsigned long long base = s_base;
+                if (base + tot <= a.out_cap) {
+                    uint32_t o = off;
+#pragma unroll
+                    for (uint32_t q = 0; q < SPL; q++)
+                        if (live >> q & 1u) {
+                            const uint32_t sl = q * PT + threadIdx.x;
+                            a.out_key[base + o] = s_key[sl];
+                            a.out_cnt[base + o] = s_cnt[sl];
+                            o++;
+                        }
+                } else {
+                    fail |= 2u;  // (the host can rerun the merge into larger buffers)
+                }
+                lds_barrier();  // (the next pass clears the table)
             }
-            lds_barrier();  // (the next pass clears the table)
         }
+        lds_barrier();  // (the next group of bounds overwrites s_bs / s_be)
     }
     if (fail) atomicOr(a.c32 + C32_POVF, fail);
     csum = wave_sum_u64(csum);
@@ -604,8 +630,12 @@ void part_bounds(uint64_t n_max, uint64_t tmin, uint32_t lpmax, uint32_t &P, uin
     T2 = (uint32_t)(2 * n_max / (PT2_TILE << LP1_MAX) + 2);
 }
 
-static void launch_merge(const PartArgs &a, uint32_t P, hipStream_t st) {
-    const dim3 g(std::min<uint32_t>(P, 65535u));
+// The merge runs as many workgroups as fit on the device at once (LDS-bound:
+// one per CU with 8192-slot tables, two with 4096), each looping over its
+// share of the buckets.
+static void launch_merge(const PartArgs &a, uint32_t P, int n_cu, hipStream_t st) {
+    const uint32_t per_cu = a.slots == 8192 ? 1u : 2u;
+    const dim3 g(std::min<uint32_t>(P, per_cu * (uint32_t)n_cu));
     if (a.slots == 8192) hipLaunchKernelGGL(k_part_merge<8192>, g, dim3(PT), 0, st, a);
     else hipLaunchKernelGGL(k_part_merge<4096>, g, dim3(PT), 0, st, a);
 }
@@ -615,8 +645,8 @@ static void launch_merge(const PartArgs &a, uint32_t P, hipStream_t st) {
 uint32_t part_lp_max() {
     static const uint32_t l = [] {
         const char *v = getenv("D4_PART_LP");  // log2 of the most buckets
-        const int x = v ? atoi(v) : 13;
-        return (uint32_t)(x >= LP1_MAX && x <= (int)LP_MAX ? x : 13);
+        const int x = v ? atoi(v) : 14;
+        return (uint32_t)(x >= LP1_MAX && x <= (int)LP_MAX ? x : 14);
     }();
     return l;
 }
@@ -672,14 +702,14 @@ void launch_part_dedup(const PartArgs &a, bool prune, int n_cu, hipStream_t st) 
     hipLaunchKernelGGL(k_part_scan2<2>, dim3(1), dim3(PT), 0, st, a);
     hipLaunchKernelGGL(k_part_scan3<2>, dim3(s2), dim3(PT), 0, st, a);
     hipLaunchKernelGGL(k_part_scatter2, dim3(g2), dim3(PT), 0, st, a);
-    launch_merge(a, P, st);
+    launch_merge(a, P, n_cu, st);
 }
 
-void launch_part_merge_again(const PartArgs &a, hipStream_t st) {
+void launch_part_merge_again(const PartArgs &a, int n_cu, hipStream_t st) {
     uint32_t P, P1, T1, T2;
     part_bounds(a.n_max, a.tile_min, a.lp_max, P, P1, T1, T2);
     hipLaunchKernelGGL(k_part_rearm, dim3(1), dim3(64), 0, st, a.c32, a.c64);
-    launch_merge(a, P, st);
+    launch_merge(a, P, n_cu, st);
 }
 
 }  // namespace d4
