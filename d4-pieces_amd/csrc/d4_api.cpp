@@ -813,7 +813,9 @@ struct Engine {
         if (timed) HIPCHK(hipEventRecord(lv_ev[0], st));
         for (uint32_t j = 0; j < nch; j++) {
             const uint64_t b = b0 + (uint64_t)j * chunk;
-            if (j) launch_reset_levels(c32.p, c64.p, false, st);
+            // (with the partitioned dedup the chunks' children accumulate and
+            // meet in one dedup after the last chunk)
+            if (j) launch_reset_levels(c32.p, c64.p, false, st, nullptr, pa != nullptr);
             a.in_key = ik + b;
             a.in_cnt = ic + b;
             a.n_rec = (uint32_t)std::min<uint64_t>(chunk, n_in - b);
@@ -824,8 +826,8 @@ struct Engine {
                 launch_level(a, lib->n_cu, st);
             }
             if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 1], st));
-            if (pa) launch_part_dedup(*pa, prune, lib->n_cu, st);
-            else launch_dedup(d, prune, lib->n_cu, st);
+            if (!pa) launch_dedup(d, prune, lib->n_cu, st);
+            else if (j + 1 == nch) launch_part_dedup(*pa, prune, lib->n_cu, st);
             if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 2], st));
         }
         return nch;
@@ -846,11 +848,14 @@ struct Engine {
 
     // Per-parent sizes of a completed chunk of `c` parents (its counters are
     // still set) for the next steps' chunking.
-    void level_sizes(const unsigned long long *h64, uint64_t c) {
+    // (c_ch: the parents whose children C64_CHW counts, when the chunks'
+    // children accumulated; 0: c)
+    void level_sizes(const unsigned long long *h64, uint64_t c, uint64_t c_ch = 0) {
         if (!c) return;
         uint64_t m = 0;
         for (uint32_t L = 0; L < LVL_MAX; L++) m = std::max<uint64_t>(m, h64[C64_LVL + L]);
-        const double lv = (double)m / (double)c, ch = (double)h64[C64_CHW] / (double)c;
+        const double lv = (double)m / (double)c,
+                     ch = (double)h64[C64_CHW] / (double)(c_ch ? c_ch : c);
         lib->note_sizes(stats.rows_done, lv, ch);
         if (c < 4096) return;
         lv_ratio = lv;
@@ -1329,6 +1334,23 @@ struct Engine {
         return 32ull * nm + nm / 8 + 4ull * (h1 + h2) + 16ull * out_cap + 16ull * nf +
                    level_bytes(nf) + parked_bytes <= mem_cap;
     }
+    // Whether a row step too large for one chunk of level buffers can still
+    // be fused: its chunks' children accumulate in one buffer for the row and
+    // meet in the partitioned dedup, which must apply and whose buffers
+    // (children, survivors twice, histograms, the next frontier) must fit the
+    // cap, or half the device's free memory without one.
+    bool fused_chunks_ok(uint64_t tab_m) const {
+        const double ch_pp = ch_pp0();
+        const uint64_t chr = level_cap(n_front, ch_pp);
+        if (chr > PART_MAX_CHILDREN) return false;
+        if (!part_dedup_fits(chr, n_front, tab_m, ch_pp * (double)n_front)) return false;
+        const uint64_t c = level_chunk(n_front, lv_pp0(), ch_pp);
+        const uint64_t bytes = 48ull * chr + 16ull * tab_m + 16ull * n_front + parked_bytes +
+                               8ull * 2 * LVL_FIELDS * level_cap(c, lv_pp0());
+        const uint64_t cap = mem_cap ? mem_cap : available_device_bytes() / 2;
+        return bytes <= cap;
+    }
+
     // The partitioned dedup's arguments (pruning as the table dedup's d).
     void part_args(const DedupArgs &d, uint64_t chc, uint64_t *ok, uint64_t *oc, uint64_t out_cap,
                    PartArgs &pa) {
@@ -1392,10 +1414,28 @@ struct Engine {
         DedupArgs d;
         level_args(tab_m, a, d);
         const double lv_pp = lv_pp0(), ch_pp = ch_pp0();
-        level_buffers(n_front, lv_pp, ch_pp, a, d);
+        // a frontier too large for one chunk of level buffers runs in chunks
+        // whose children accumulate in one buffer for the row (partitioned
+        // dedup only: the caller checked that it applies)
+        const uint64_t chunk = level_chunk(n_front, lv_pp, ch_pp);
+        level_buffers(chunk, lv_pp, ch_pp, a, d);
+        if (chunk < n_front) {
+            ch_cap = level_cap(n_front, ch_pp);
+            ch_k.reserve(ch_cap);
+            ch_c.reserve(ch_cap);
+            a.ch_key = ch_k.p;
+            a.ch_cnt = ch_c.p;
+            d.ch_key = ch_k.p;
+            d.ch_cnt = ch_c.p;
+            a.ch_cap = d.ch_cap = ch_cap;
+        }
         // the partitioned dedup writes the next frontier itself (as many
         // states as the table would hold)
         last_part = part_dedup_fits(ch_cap, n_front, tab_m, ch_pp * (double)n_front);
+        if (chunk < n_front && !last_part) {
+            h32[C32_OVERFLOW] = h32[C32_SPILL] = 0;
+            return false;
+        }
         nk.reserve(tab_m);
         nc.reserve(tab_m);
         PartArgs pa{};
@@ -1406,7 +1446,7 @@ struct Engine {
         if (!last_part) launch_fill_table(tk.p, tk.p + 1, tab_m, st);
         launch_reset_counters(c32.p, c64.p, st);
         queue_terminal();
-        queue_levels(a, d, 0, n_front, prune, true, last_part ? &pa : nullptr);
+        const uint32_t nch = queue_levels(a, d, 0, chunk, prune, true, last_part ? &pa : nullptr);
         HIPCHK(hipEventRecord(evc0, st));
         if (!last_part)
             launch_compact(tk.p, tk.p + 1, tab_m, nk.p, nc.p, tab_m, c32.p, blk.p, nullptr, st);
@@ -1438,7 +1478,7 @@ struct Engine {
             std::memcpy(h64, pin64, sizeof(unsigned long long) * C64_N);
             stats.part_remerges++;
         }
-        level_times(1);
+        level_times(nch);
         stats.fused_steps++;
         stats.donation_full += h32[C32_DONFULL];
         stats.extend_bytes += 16ull * n_front + 16ull * h64[C64_CHILDREN];
@@ -1472,7 +1512,8 @@ struct Engine {
                                 std::to_string(h64[C64_CHW]) + " children written)");
             stats.part_steps++;
         }
-        level_sizes(h64, n_front);
+        // (the level counters are the last chunk's)
+        level_sizes(h64, n_front - (uint64_t)(nch - 1) * chunk, n_front);
         return true;
     }
 
@@ -1597,7 +1638,7 @@ struct Engine {
             if (!fused_skip && !en && lib->step_mode != D4_STEP_GENERAL && !noalloc &&
                 n_front &&
                 level_kernel(en, false) && tab_m <= kFusedMaxSlots &&
-                level_chunk(n_front, lv_pp0(), ch_pp0()) >= n_front &&
+                (level_chunk(n_front, lv_pp0(), ch_pp0()) >= n_front || fused_chunks_ok(tab_m)) &&
                 (!mem_cap || 48ull * tab_m + 16ull * n_front + level_bytes(n_front) +
                                      parked_bytes <= mem_cap)) {
                 if (lib->step_mode == D4_STEP_GRAPH && !fused_skip && !g_off_once) {

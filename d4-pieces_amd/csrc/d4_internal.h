@@ -339,9 +339,10 @@ void launch_zero_u32(uint32_t *p, uint64_t n, hipStream_t st);
 void launch_atomic_rate(unsigned long long *tab, uint64_t m, uint32_t iters, uint32_t blocks,
                         hipStream_t st);
 // zero the level kernels' per-chunk counters; first: also the overflow flag
-// and the completed-chunk count
+// and the completed-chunk count; keep_children: not the children count (the
+// chunk's children are appended to the previous chunks')
 void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st,
-                         const RowCtl *ctl = nullptr);
+                         const RowCtl *ctl = nullptr, bool keep_children = false);
 // tiling counts of the records whose instance finishes at this step
 void launch_terminal(const uint64_t *key, const uint64_t *cnt, uint64_t n, uint32_t row,
                      const uint32_t *start, const uint32_t *hgt, uint32_t inst_shift,

@@ -629,12 +629,16 @@ __global__ __launch_bounds__(256) void k_terminal(const uint64_t *key, const uin
 
 // the level kernels' per-chunk counters (children written, partial rows per
 // level); at a step's first chunk also the overflow flag and the chunk count
-__global__ void k_reset_levels(uint32_t *c32, unsigned long long *c64, uint32_t first,
+// mode bit 0: the step's first chunk (also clears the overflow flag and the
+// completed-chunk count); bit 1: keep the children count, so that the next
+// chunk's children follow the previous ones in the children buffer
+__global__ void k_reset_levels(uint32_t *c32, unsigned long long *c64, uint32_t mode,
                                const RowCtl *ctl) {
     if (ctl && ctl->fail) return;
+    const bool first = mode & 1u;
     // after an overflow the counters keep the failed chunk's sizes for the host
     if (!first && ctr_load(c32 + C32_XOVF)) return;
-    if (threadIdx.x <= LVL_MAX) c64[C64_CHW + threadIdx.x] = 0;
+    if (threadIdx.x <= LVL_MAX && (threadIdx.x || !(mode & 2u))) c64[C64_CHW + threadIdx.x] = 0;
     if (first && threadIdx.x == 0) {
         c32[C32_XOVF] = 0;
         c32[C32_LVDONE] = 0;
@@ -982,8 +986,9 @@ void launch_row_close(RowCtl *ctl, const uint32_t *c32, const unsigned long long
 }
 
 void launch_reset_levels(uint32_t *c32, unsigned long long *c64, bool first, hipStream_t st,
-                         const RowCtl *ctl) {
-    hipLaunchKernelGGL(k_reset_levels, dim3(1), dim3(64), 0, st, c32, c64, (uint32_t)first, ctl);
+                         const RowCtl *ctl, bool keep_children) {
+    hipLaunchKernelGGL(k_reset_levels, dim3(1), dim3(64), 0, st, c32, c64,
+                       (uint32_t)first | (keep_children ? 2u : 0u), ctl);
 }
 
 void launch_terminal(const uint64_t *key, const uint64_t *cnt, uint64_t n, uint32_t row,
