@@ -2666,6 +2666,17 @@ int d4_device_count(int *n) {
     });
 }
 
+int d4_device_release_cached(uint64_t *bytes) {
+    return guard([&] {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceSynchronize());
+        const uint64_t n = pool().cached(dev);
+        pool().trim(dev);
+        if (bytes) *bytes = n;
+    });
+}
+
 int d4_device_atomic_rate(double *ops_per_s) {
     return guard([&] {
         if (!ops_per_s) throw Error(D4_EINVAL, "null argument");

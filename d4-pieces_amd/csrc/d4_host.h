@@ -178,6 +178,23 @@ class Pool {
         std::lock_guard<std::mutex> lk(mu_);
         free_.emplace(round(bytes), std::make_pair(dev, p));
     }
+    // Return the blocks cached on a device to the runtime (its pending work
+    // must be finished).
+    void trim(int dev) {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto it = free_.begin(); it != free_.end();) {
+            if (it->second.first == dev) {
+                int cur = 0;
+                (void)hipGetDevice(&cur);
+                (void)hipSetDevice(dev);
+                (void)hipFree(it->second.second);
+                (void)hipSetDevice(cur);
+                it = free_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
     // Bytes cached for reuse on a device: free for the library's purposes,
     // though hipMemGetInfo counts them as in use.
     size_t cached(int dev) {

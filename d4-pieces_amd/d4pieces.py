@@ -118,6 +118,7 @@ SIGNATURES = [
     ("d4_mbatch_tables", ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
     ("d4_library_host_tables", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_uint64]),
     ("d4_library_get_step_mode", ctypes.c_int, [_P, _U32P]),
+    ("d4_device_release_cached", ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64)]),
     ("d4_library_set_dedup", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_library_get_dedup", ctypes.c_int, [_P, _U32P]),
     ("d4_searcher_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
@@ -252,6 +253,13 @@ def atomic_rate() -> float:
     """Measured random 64-bit device-scope atomic adds per second."""
     v = ctypes.c_double()
     _check(load().d4_device_atomic_rate(ctypes.byref(v)))
+    return v.value
+
+
+def release_cached() -> int:
+    """Return the allocator's cached device memory to the runtime; bytes."""
+    v = ctypes.c_uint64()
+    _check(load().d4_device_release_cached(ctypes.byref(v)))
     return v.value
 
 

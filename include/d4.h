@@ -117,6 +117,15 @@ int d4_device_count(int *n);
 /* Wait for all work on the current device (hipDeviceSynchronize). */
 int d4_device_synchronize(void);
 
+/* Return the device memory the library's caching allocator holds for reuse
+ * on the current device to the runtime (after the device's pending work);
+ * bytes (may be NULL) receives how much.  Handles keep their own buffers.
+ * Useful between workloads of very different size: a later allocation that
+ * finds no cached block of its size would otherwise release the whole cache
+ * at once, a multi-second pause after a search of hundreds of GB.  No
+ * counterpart in the reference (its buffers come from cudaMalloc per row). */
+int d4_device_release_cached(uint64_t *bytes);
+
 /* Measured rate of random 64-bit device-scope atomic adds into 1 GiB of
  * device memory (best of three launches of 134 M adds): the ceiling of a
  * dedup that merges each child into a hash table in HBM with one atomic.
