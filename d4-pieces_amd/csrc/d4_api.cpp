@@ -760,6 +760,12 @@ struct Engine {
         d.spill_cap = (uint32_t)std::min<uint64_t>(spill.cap / 2, kSpillCap);
         d.c32 = c32.p;
         d.c64 = c64.p;
+        // (the level kernels prune with the dedup's fields when asked to)
+        a.seed_mask = d.seed_mask;
+        a.gcd = d.gcd;
+        a.min_size = d.min_size;
+        a.closed_rows = d.closed_rows;
+        a.n_win = S <= 1024 && level_lds_bytes(lib->fit_rows, S) <= lib->lds_limit ? S : 0;
     }
 
     // Queues the terminal pass when an instance finishes at this step.
@@ -809,6 +815,10 @@ struct Engine {
             HIPCHK(hipEventCreate(&e));
             lv_ev.push_back(e);
         }
+        // the connectivity prune runs in the level kernels, before a child
+        // is written, unless level_prune is off (then in the dedup)
+        a.prune = prune && level_prune;
+        const bool dprune = prune && !a.prune;
         launch_reset_levels(c32.p, c64.p, true, st);
         if (timed) HIPCHK(hipEventRecord(lv_ev[0], st));
         for (uint32_t j = 0; j < nch; j++) {
@@ -826,8 +836,8 @@ struct Engine {
                 launch_level(a, lib->n_cu, st);
             }
             if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 1], st));
-            if (!pa) launch_dedup(d, prune, lib->n_cu, st);
-            else if (j + 1 == nch) launch_part_dedup(*pa, prune, lib->n_cu, st);
+            if (!pa) launch_dedup(d, dprune, lib->n_cu, st);
+            else if (j + 1 == nch) launch_part_dedup(*pa, dprune, lib->n_cu, st);
             if (timed) HIPCHK(hipEventRecord(lv_ev[2 * j + 2], st));
         }
         return nch;
@@ -1110,6 +1120,8 @@ struct Engine {
         a.in_key = gk[p].p;
         a.in_cnt = gc[p].p;
         a.n_rec = (uint32_t)std::min<uint64_t>(g_cap, 0xffffffffull);
+        a.prune = prune && level_prune;
+        const bool dprune = prune && !a.prune;
         const RowCtl *ctl = ctl_d.p;
         // (the partitioned dedup writes the next frontier buffer itself)
         const bool part = g_part;
@@ -1129,9 +1141,9 @@ struct Engine {
             launch_level(a, lib->n_cu, st);
         }
         if (part) {
-            launch_part_dedup(pa, prune, lib->n_cu, st);
+            launch_part_dedup(pa, dprune, lib->n_cu, st);
         } else {
-            launch_dedup(d, prune, lib->n_cu, st);
+            launch_dedup(d, dprune, lib->n_cu, st);
             launch_compact(tk.p, tk.p + 1, g_cap, gk[1 - p].p, gc[1 - p].p, g_cap, c32.p, blk.p,
                            nullptr, st, ctl);
         }
@@ -1264,7 +1276,7 @@ struct Engine {
             stats.records_extended += n;
             stats.unique += u;
             stats.extend_bytes += 16ull * n + 16ull * l.children;
-            stats.dedup_bytes += 16ull * (l.children + l.pruned) + 16ull * u;
+            stats.dedup_bytes += 16ull * (l.children + (level_prune ? 0 : l.pruned)) + 16ull * u;
             if (g_part) {
                 stats.part_steps++;
             } else {
@@ -1320,6 +1332,15 @@ struct Engine {
     // bucket chain's dozen launches cost more than it saves there; 2^20 by
     // default, D4_PART_MIN_CHILDREN overrides it, so tests can send small
     // rows through the buckets)
+    // whether count-mode level kernels prune the children they emit, so the
+    // dedup reads survivors only (D4_LEVEL_PRUNE=1).  Off by default: on
+    // the batched 8x8 boards the flood fill on the emitting lanes alone
+    // (the rest of the wave waits) costs the level kernels 0.71 ms per row
+    // step and saves the dedup 0.48 ms.
+    bool level_prune = [] {
+        const char *e = std::getenv("D4_LEVEL_PRUNE");
+        return e && *e == '1';
+    }();
     double part_min = [] {
         const char *v = getenv("D4_PART_MIN_CHILDREN");
         return v ? std::max(0.0, atof(v)) : (double)(1u << 20);
@@ -1482,7 +1503,8 @@ struct Engine {
         stats.fused_steps++;
         stats.donation_full += h32[C32_DONFULL];
         stats.extend_bytes += 16ull * n_front + 16ull * h64[C64_CHILDREN];
-        stats.dedup_bytes += 16ull * (h64[C64_CHILDREN] + h64[C64_PRUNED]) + 16ull * h64[C64_NEW];
+        stats.dedup_bytes += 16ull * (h64[C64_CHILDREN] + (level_prune ? 0 : h64[C64_PRUNED])) +
+                             16ull * h64[C64_NEW];
         const bool pfail = last_part && (h32[C32_POVF] || (uint64_t)h32[C32_OUT] != h64[C64_NEW]);
         if (trace)
             fprintf(stderr,
@@ -1704,11 +1726,13 @@ struct Engine {
             stats.donation_full += h32[C32_DONFULL];  // (every attempt, reruns included)
             if (n_front) {
                 stats.extend_bytes += 16ull * n_front + 16ull * h64[C64_CHILDREN];
-                // level kernels: the dedup reads every child written (kept or
-                // pruned) and writes the new states
+                // level kernels: the dedup reads every child written (the
+                // kept ones, and the pruned ones unless the level kernels
+                // pruned them) and writes the new states
                 if (lv)
-                    stats.dedup_bytes += 16ull * (h64[C64_CHILDREN] + h64[C64_PRUNED]) +
-                                         16ull * h64[C64_NEW];
+                    stats.dedup_bytes +=
+                        16ull * (h64[C64_CHILDREN] + (level_prune ? 0 : h64[C64_PRUNED])) +
+                        16ull * h64[C64_NEW];
             }
             bool ovf = h32[C32_OVERFLOW] != 0;
             bool eovf = en && h32[C32_EDGE_OVF] != 0;

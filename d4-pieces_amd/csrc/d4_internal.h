@@ -48,7 +48,8 @@ enum : int {
     C64_LVL = 7,       // C64_LVL + L: partial rows written for level L (k_level)
     C64_PSUM_IN = 15,  // partitioned dedup: sum of the surviving children's counts
     C64_PSUM_OUT = 16, // and of the unique states' counts it emitted (equal, mod 2^64)
-    C64_N = 17,
+    C64_LPRUNED = 17,  // children the level kernels pruned (moved to C64_PRUNED by the dedup)
+    C64_N = 18,
 };
 
 // Arguments of the extend kernel (passed by value).
@@ -167,6 +168,12 @@ struct LevelArgs {
     uint32_t *c32;
     unsigned long long *c64;
     const RowCtl *ctl;       // set: n_rec and row come from it (graph replay)
+    // connectivity prune of each child before it is written (as the dedup's,
+    // with its fields); the board windows of row + 1 go to LDS when n_win
+    // (the instances) is not 0
+    uint32_t prune, n_win;
+    uint64_t seed_mask;
+    uint32_t gcd, min_size, closed_rows;
 };
 struct DedupArgs {
     const uint64_t *ch_key, *ch_cnt;
@@ -244,7 +251,7 @@ uint32_t part_slots();
 // must be; launch_part_merge_again reruns the merge alone)
 void launch_part_dedup(const PartArgs &a, bool prune, int n_cu, hipStream_t st);
 void launch_part_merge_again(const PartArgs &a, int n_cu, hipStream_t st);
-size_t level_lds_bytes(uint32_t fit_rows);
+size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win = 0);
 
 // Tables of one library in a multi-library batch (device layout, 16-byte
 // aligned).  Candidate masks are 128 bits: word 0 = candidates 0..63.

@@ -639,6 +639,8 @@ __global__ void k_reset_levels(uint32_t *c32, unsigned long long *c64, uint32_t 
     // after an overflow the counters keep the failed chunk's sizes for the host
     if (!first && ctr_load(c32 + C32_XOVF)) return;
     if (threadIdx.x <= LVL_MAX && (threadIdx.x || !(mode & 2u))) c64[C64_CHW + threadIdx.x] = 0;
+    // (children the level kernels pruned accumulate with the children kept)
+    if (threadIdx.x == LVL_MAX + 1 && !(mode & 2u)) c64[C64_LPRUNED] = 0;
     if (first && threadIdx.x == 0) {
         c32[C32_XOVF] = 0;
         c32[C32_LVDONE] = 0;

@@ -95,6 +95,27 @@ __device__ __forceinline__ uint64_t place(Writer &w, uint64_t m, unsigned long l
     return slot;
 }
 
+// The board window of row + 1 of instance i (0 past the instance's rows,
+// as the dedup stages it).
+__device__ __forceinline__ uint64_t child_window(const LevelArgs &a, uint32_t i) {
+    const uint32_t rr = a.row + 1 - a.start[i];
+    return rr < a.bw_stride ? a.bw[(size_t)i * a.bw_stride + rr] : 0ull;
+}
+
+// Connectivity prune of the child with key bits kb (key without the
+// instance) of instance inst, as the dedup's (k_dedup below): false, and
+// one more pruned, when the open cells it leaves hold a closed region that
+// no set of pieces can fill.
+__device__ __forceinline__ bool child_ok(const LevelArgs &a, const uint64_t *s_win, uint32_t inst,
+                                         uint64_t kb, uint64_t &pruned) {
+    const uint64_t w = inst < a.n_win ? s_win[inst] : child_window(a, inst);
+    const uint64_t E = w & ~(kb & a.occ_mask);
+    const bool ok = a.closed_rows <= 4 ? viable32(E, a.closed_rows, a.gcd, a.min_size)
+                                       : viable(E, a.seed_mask, a.gcd, a.min_size);
+    pruned += !ok;
+    return ok;
+}
+
 template <bool FROM_REC, int FR>
 __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     extern __shared__ uint64_t lds[];
@@ -109,11 +130,15 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     const PlaceInfo *s_pc = reinterpret_cast<const PlaceInfo *>(s_fit + nfit);
     uint64_t *s_xa = s_fit + nfit + 2 * 8 * MAX_CAND;
     uint64_t *s_rng = s_xa + 4 * 256;
+    // (after the waves' own areas: the board windows of row + 1)
+    uint64_t *s_win = s_rng + MAX_CAND + (size_t)(LT / 64) * (OWN * 64 + PCAP / 4);
     for (uint32_t i = threadIdx.x; i < nfit; i += LT) s_fit[i] = a.fit[i];
     for (uint32_t i = threadIdx.x; i < 2 * 8 * MAX_CAND; i += LT) s_fit[nfit + i] = a.placeinfo[i];
     for (uint32_t i = threadIdx.x; i < 4 * 256; i += LT) s_xa[i] = a.xavail[i];
     for (uint32_t i = threadIdx.x; i < MAX_CAND; i += LT)
         s_rng[i] = piece_range(a.placeinfo[2 * i + 1]);
+    if (a.prune)
+        for (uint32_t i = threadIdx.x; i < a.n_win; i += LT) s_win[i] = child_window(a, i);
     __syncthreads();
 
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
@@ -136,6 +161,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     chunk = chunk < 64 ? 64 : chunk > 1024 ? 1024 : chunk;
     Writer ws, wc;
     bool alive = true;
+    uint64_t pruned = 0;
     unsigned long long *ctr_s = a.c64 + C64_LVL + a.level, *ctr_c = a.c64 + C64_CHW;
     uint64_t *oE = a.out_st, *oOk = oE + a.out_cap, *oCnt = oOk + a.out_cap,
              *oCov = oCnt + a.out_cap, *oUi = oCov + a.out_cap;
@@ -163,7 +189,9 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                         const uint32_t c = (uint32_t)__builtin_ctzll(E & 0xffu);
                         ok = fitmask<FR>(s_fit, a.fit_rows, c, E) & avail_of(s_xa, used);
                     } else {
-                        pass = true;  // row already covered from above: its own child
+                        // row already covered from above: its own child
+                        pass = !a.prune || child_ok(a, s_win, inst,
+                                                    cov | ((uint64_t)used << a.occ_bits), pruned);
                     }
                 }
             }
@@ -243,7 +271,9 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                         cov2 = o_cov[o] | (pl >> 8);
                         pc = o_cnt[o];
                         if (!(E2 & 0xffu)) {
-                            emitC = true;
+                            emitC = !a.prune ||
+                                    child_ok(a, s_win, pi, cov2 | ((uint64_t)used2 << a.occ_bits),
+                                             pruned);
                         } else {
                             uint64_t av2 = o_av[o];
                             if (((used2 >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
@@ -293,6 +323,8 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     // unused slots of the wave's last chunks: no candidates / empty keys
     for (uint64_t j = ws.pos + lane; j < ws.end; j += 64) oOk[j] = 0;
     for (uint64_t j = wc.pos + lane; j < wc.end; j += 64) a.ch_key[j] = EMPTY_KEY;
+    pruned = wave_sum_u64(pruned);
+    if (lane == 0 && pruned) atomicAdd(a.c64 + C64_LPRUNED, (unsigned long long)pruned);
 }
 
 // Board windows of the children's row, per instance, staged in LDS when the
@@ -344,8 +376,17 @@ __global__ __launch_bounds__(256) void k_dedup(DedupArgs a) {
     if (n > a.ch_cap) n = a.ch_cap;
     // a chunk whose level kernels overflowed is rerun by the host: nothing
     // of it enters the table; a complete chunk counts itself once
-    if (__hip_atomic_load(a.c32 + C32_XOVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) n = 0;
-    else if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.c32 + C32_LVDONE, 1u);
+    if (__hip_atomic_load(a.c32 + C32_XOVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        n = 0;
+    } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(a.c32 + C32_LVDONE, 1u);
+        // the children the level kernels pruned count now that the chunk is
+        // complete (once per row step when a streamed step reruns the level
+        // kernels for each partition of the hash)
+        const uint64_t lp = ctr_load(a.c64 + C64_LPRUNED);
+        if (lp && (!a.part_mask || a.part_id == 0))
+            atomicAdd(a.c64 + C64_PRUNED, (unsigned long long)lp);
+    }
     uint64_t kept = 0, fresh = 0, pruned = 0;
     bool failed = false;
     const uint64_t G = (uint64_t)gridDim.x * 256;
@@ -428,13 +469,14 @@ static void launch_lv(const LevelArgs &a, dim3 g, size_t lds, hipStream_t st) {
 
 }  // namespace
 
-size_t level_lds_bytes(uint32_t fit_rows) {
+size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win) {
     return sizeof(uint64_t) * (8u * fit_rows * 256u + 2u * 8u * MAX_CAND + 4u * 256u + MAX_CAND) +
-           (sizeof(uint64_t) * OWN * 64 + sizeof(uint16_t) * PCAP) * (LT / 64);
+           (sizeof(uint64_t) * OWN * 64 + sizeof(uint16_t) * PCAP) * (LT / 64) +
+           sizeof(uint64_t) * n_win;
 }
 
 void launch_level(const LevelArgs &a, int n_cu, hipStream_t st) {
-    const size_t lds = level_lds_bytes(a.fit_rows);
+    const size_t lds = level_lds_bytes(a.fit_rows, a.prune ? a.n_win : 0);
     // one workgroup per CU (LDS-bound); level 0 knows its record count
     uint64_t blocks = (uint64_t)n_cu;
     if (a.level == 0) blocks = std::min<uint64_t>(blocks, ((uint64_t)a.n_rec + LT - 1) / LT);

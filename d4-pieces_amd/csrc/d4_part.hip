@@ -183,9 +183,12 @@ __global__ __launch_bounds__(PT) void k_part_hist1(PartArgs a) {
     if (a.ctl && a.ctl->fail) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint64_t n0 = ctr_load(a.c64 + C64_CHW);
-        if (!__hip_atomic_load(a.c32 + C32_XOVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
-            (n0 < a.ch_cap ? n0 : a.ch_cap) > a.n_max)
-            atomicOr(a.c32 + C32_POVF, 1u);
+        if (!__hip_atomic_load(a.c32 + C32_XOVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if ((n0 < a.ch_cap ? n0 : a.ch_cap) > a.n_max) atomicOr(a.c32 + C32_POVF, 1u);
+            // (children the level kernels pruned)
+            const uint64_t lp = ctr_load(a.c64 + C64_LPRUNED);
+            if (lp) atomicAdd(a.c64 + C64_PRUNED, (unsigned long long)lp);
+        }
     }
     const uint64_t n = part_n(a);
     const Shape sh = part_shape(n, a.tile_min, a.lp_max);

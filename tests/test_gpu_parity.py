@@ -502,6 +502,59 @@ def test_step_modes_agree(gpu, p5, oracle, monkeypatch, mode, scale):
         assert gpu_count(gpu, lb, rows) == oracle.count(shapes, counts, rows)
 
 
+def test_level_prune_matches_dedup_prune(gpu, p5, monkeypatch):
+    """The connectivity prune in the level kernels (before a child is
+    written) and in the dedup (after) keep the same children: equal counts,
+    and equal children kept, pruned and unique states, in every step mode
+    and both dedups on the 256 seeded 8x8 boards (fixture); and on 8x6 with
+    three cells blocked under a 24 MiB cap, whose large rows run in hash
+    partitions that each rerun the level kernels, equal counts and the same
+    children kept and pruned within 1% (the capped run parks frontier parts
+    where its size estimates say so, and those follow the children per
+    parent, which the level prune lowers; pruned children counted once per
+    partition would be twice as many or more)."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    keys = ("children", "pruned", "unique", "rows_done")
+
+    def batch(level_prune, mode, dedup):
+        monkeypatch.setenv("D4_LEVEL_PRUNE", "1" if level_prune else "0")
+        lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+        lib.set_step_mode(mode)
+        lib.set_dedup(dedup)
+        b = gpu.Batch(lib, fx["boards"])
+        b.run()
+        assert b.counts() == fx["counts"], (level_prune, mode, dedup)
+        st = b.stats()
+        return {k: st[k] for k in keys}
+
+    for mode in (gpu.STEP_GENERAL, gpu.STEP_FUSED, gpu.STEP_GRAPH):
+        for dedup in (gpu.DEDUP_TABLE, gpu.DEDUP_PART):
+            on, off = batch(True, mode, dedup), batch(False, mode, dedup)
+            assert on == off, (mode, dedup, on, off)
+            assert on["pruned"] > 0
+
+    def streamed(level_prune):
+        monkeypatch.setenv("D4_LEVEL_PRUNE", "1" if level_prune else "0")
+        s = gpu.CudaSearcher(gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8),
+                             rows=rect(8, 6, blocked=((5, 5), (5, 6), (5, 7))))
+        s.set_memory(24 << 20)
+        s.run()
+        st = s.stats()
+        assert st["streamed_steps"] > 0
+        return s.count(), {k: st[k] for k in keys}
+
+    free = gpu.CudaSearcher(gpu.PieceLibrary(p5, [2] * 12, 0xFF, 8),
+                            rows=rect(8, 6, blocked=((5, 5), (5, 6), (5, 7))))
+    free.run()
+    (c1, s1), (c0, s0) = streamed(True), streamed(False)
+    assert c1 == c0 == free.count()
+    for k in ("children", "pruned", "unique"):
+        assert abs(s1[k] - s0[k]) <= 0.01 * s0[k], (k, s1, s0)
+
+
 @pytest.mark.parametrize("dedup", ["table", "part", "part_small"])
 def test_dedup_modes_agree(gpu, p5, oracle, monkeypatch, dedup):
     """The row step's dedup through one table in device memory or through
