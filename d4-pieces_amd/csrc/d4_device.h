@@ -54,6 +54,21 @@ __device__ __forceinline__ void lds_barrier() {
     __syncthreads();
 }
 
+// Loads of data a kernel reads exactly once (streaming passes): marked
+// nontemporal, so they do not push out of the caches the partly written
+// lines of the pass's own scattered output.
+__device__ __forceinline__ uint64_t ld_once(const uint64_t *p) {
+    return __builtin_nontemporal_load(p);
+}
+typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ ulonglong2 ld_once(const ulonglong2 *p) {
+    const u64x2_t v = __builtin_nontemporal_load(reinterpret_cast<const u64x2_t *>(p));
+    ulonglong2 r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     for (int o = 32; o > 0; o >>= 1) {
         uint32_t lo = __shfl_xor((uint32_t)v, o);

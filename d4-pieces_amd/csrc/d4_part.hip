@@ -341,14 +341,14 @@ __global__ __launch_bounds__(PT) void k_part_scatter1(PartArgs a) {
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) {
                 const uint64_t i0 = j0 + u * PT + (threadIdx.x & ~63u);
-                m[u] = i0 < t1 ? a.mask[i0 >> 6] : 0ull;
+                m[u] = i0 < t1 ? ld_once(a.mask + (i0 >> 6)) : 0ull;
             }
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) {
                 const uint64_t i = j0 + u * PT + threadIdx.x;
                 const bool live = (m[u] >> lane) & 1u;
-                k[u] = live ? a.ch_key[i] : EMPTY_KEY;
-                c[u] = live ? a.ch_cnt[i] : 0ull;
+                k[u] = live ? ld_once(a.ch_key + i) : EMPTY_KEY;
+                c[u] = live ? ld_once(a.ch_cnt + i) : 0ull;
             }
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++)
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(PT) void k_part_scatter2(PartArgs a) {
 #pragma unroll
                 for (uint32_t u = 0; u < 4; u++) {
                     const uint32_t i = i0 + u * PT;
-                    if (i < r1) v[u] = pb[i];
+                    if (i < r1) v[u] = ld_once(pb + i);
                     else v[u].x = v[u].y = 0;
                 }
 #pragma unroll
