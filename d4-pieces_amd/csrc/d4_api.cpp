@@ -765,7 +765,9 @@ struct Engine {
         a.gcd = d.gcd;
         a.min_size = d.min_size;
         a.closed_rows = d.closed_rows;
-        a.n_win = S <= 1024 && level_lds_bytes(lib->fit_rows, S) <= lib->lds_limit ? S : 0;
+        a.n_win0 = S <= 1024 && level_lds_bytes(lib->fit_rows, 0, S) <= lib->lds_limit ? S : 0;
+        a.n_win =
+            S <= 1024 && level_lds_bytes(lib->fit_rows, S, a.n_win0) <= lib->lds_limit ? S : 0;
     }
 
     // Queues the terminal pass when an instance finishes at this step.

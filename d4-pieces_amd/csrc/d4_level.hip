@@ -130,8 +130,13 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     const PlaceInfo *s_pc = reinterpret_cast<const PlaceInfo *>(s_fit + nfit);
     uint64_t *s_xa = s_fit + nfit + 2 * 8 * MAX_CAND;
     uint64_t *s_rng = s_xa + 4 * 256;
-    // (after the waves' own areas: the board windows of row + 1)
+    // (after the waves' own areas: the board windows of row + 1 for the
+    // prune, then at level 0 each instance's window at the row and whether
+    // the row is one of its rows)
     uint64_t *s_win = s_rng + MAX_CAND + (size_t)(LT / 64) * (OWN * 64 + PCAP / 4);
+    uint64_t *s_w0 = s_win + (a.prune ? a.n_win : 0u);
+    const uint32_t nw0 = FROM_REC ? a.n_win0 : 0u;
+    uint8_t *s_t0 = reinterpret_cast<uint8_t *>(s_w0 + nw0);
     for (uint32_t i = threadIdx.x; i < nfit; i += LT) s_fit[i] = a.fit[i];
     for (uint32_t i = threadIdx.x; i < 2 * 8 * MAX_CAND; i += LT) s_fit[nfit + i] = a.placeinfo[i];
     for (uint32_t i = threadIdx.x; i < 4 * 256; i += LT) s_xa[i] = a.xavail[i];
@@ -139,6 +144,12 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
         s_rng[i] = piece_range(a.placeinfo[2 * i + 1]);
     if (a.prune)
         for (uint32_t i = threadIdx.x; i < a.n_win; i += LT) s_win[i] = child_window(a, i);
+    for (uint32_t i = threadIdx.x; i < nw0; i += LT) {
+        const uint32_t rr = a.row - a.start[i];
+        const bool in = rr < a.hgt[i];
+        s_w0[i] = in ? a.bw[(size_t)i * a.bw_stride + rr] : 0ull;
+        s_t0[i] = in;
+    }
     __syncthreads();
 
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
@@ -166,24 +177,55 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     uint64_t *oE = a.out_st, *oOk = oE + a.out_cap, *oCnt = oOk + a.out_cap,
              *oCov = oCnt + a.out_cap, *oUi = oCov + a.out_cap;
 
-    for (uint64_t b0 = wid * 64; alive && b0 < n_in; b0 += waves * 64) {
+    // the next batch's inputs are loaded while the current one is processed
+    const uint64_t stride = waves * 64;
+    uint64_t nx0 = 0, nx1 = 0, nx2 = 0, nx3 = 0, nx4 = 0;
+    auto fetch = [&](uint64_t b) {
+        const uint64_t i = b + lane;
+        if (i < n_in) {
+            if (FROM_REC) {
+                nx0 = a.in_key[i];
+                nx1 = a.in_cnt[i];
+            } else {
+                const uint64_t *s = a.in_st;
+                nx0 = s[i];
+                nx1 = s[a.in_cap + i];
+                nx2 = s[2 * a.in_cap + i];
+                nx3 = s[3 * a.in_cap + i];
+                nx4 = s[4 * a.in_cap + i];
+            }
+        }
+    };
+    if (wid * 64 < n_in) fetch(wid * 64);
+    for (uint64_t b0 = wid * 64; alive && b0 < n_in; b0 += stride) {
         const uint64_t i = b0 + lane;
         const bool valid = i < n_in;
+        const uint64_t c0 = nx0, c1 = nx1, c2 = nx2, c3 = nx3, c4 = nx4;
+        if (b0 + stride < n_in) fetch(b0 + stride);
         uint64_t E = 0, ok = 0, cnt = 0, cov = 0;
         uint32_t used = 0, inst = 0;
         bool pass = false;
         if (FROM_REC) {
             if (valid) {
-                const uint64_t key = a.in_key[i];
-                cnt = a.in_cnt[i];
+                const uint64_t key = c0;
+                cnt = c1;
                 inst = a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
                 used = (uint32_t)(key >> a.occ_bits) & a.used_mask;
-                const uint32_t rr = a.row - a.start[inst];
                 // a terminal state (its instance finishes at this step) is
                 // counted by the step's terminal pass and has no children
-                if (rr < a.hgt[inst]) {
+                bool in;
+                uint64_t w;
+                if (inst < nw0) {
+                    in = s_t0[inst] != 0;
+                    w = s_w0[inst];
+                } else {
+                    const uint32_t rr = a.row - a.start[inst];
+                    in = rr < a.hgt[inst];
+                    w = in ? a.bw[(size_t)inst * a.bw_stride + rr] : 0ull;
+                }
+                if (in) {
                     const uint64_t occ = key & a.occ_mask;
-                    E = a.bw[(size_t)inst * a.bw_stride + rr] & ~occ;
+                    E = w & ~occ;
                     cov = occ >> 8;
                     if (E & 0xffu) {
                         const uint32_t c = (uint32_t)__builtin_ctzll(E & 0xffu);
@@ -196,12 +238,11 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                 }
             }
         } else if (valid) {
-            const uint64_t *s = a.in_st;
-            E = s[i];
-            ok = s[a.in_cap + i];
-            cnt = s[2 * a.in_cap + i];
-            cov = s[3 * a.in_cap + i];
-            const uint64_t ui = s[4 * a.in_cap + i];
+            E = c0;
+            ok = c1;
+            cnt = c2;
+            cov = c3;
+            const uint64_t ui = c4;
             used = (uint32_t)ui;
             inst = (uint32_t)(ui >> 32);
         }
@@ -469,14 +510,15 @@ static void launch_lv(const LevelArgs &a, dim3 g, size_t lds, hipStream_t st) {
 
 }  // namespace
 
-size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win) {
+size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win, uint32_t n_win0) {
     return sizeof(uint64_t) * (8u * fit_rows * 256u + 2u * 8u * MAX_CAND + 4u * 256u + MAX_CAND) +
            (sizeof(uint64_t) * OWN * 64 + sizeof(uint16_t) * PCAP) * (LT / 64) +
-           sizeof(uint64_t) * n_win;
+           sizeof(uint64_t) * n_win + (sizeof(uint64_t) + 1) * n_win0;
 }
 
 void launch_level(const LevelArgs &a, int n_cu, hipStream_t st) {
-    const size_t lds = level_lds_bytes(a.fit_rows, a.prune ? a.n_win : 0);
+    const size_t lds =
+        level_lds_bytes(a.fit_rows, a.prune ? a.n_win : 0, a.level == 0 ? a.n_win0 : 0);
     // one workgroup per CU (LDS-bound); level 0 knows its record count
     uint64_t blocks = (uint64_t)n_cu;
     if (a.level == 0) blocks = std::min<uint64_t>(blocks, ((uint64_t)a.n_rec + LT - 1) / LT);

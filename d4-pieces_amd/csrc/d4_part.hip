@@ -665,8 +665,8 @@ uint32_t part_slots() {
 uint64_t part_tile_min() {
     static const uint64_t t = [] {
         const char *v = getenv("D4_PART_TILE");  // log2 of the smallest level-1 tile
-        const int l = v ? atoi(v) : 16;
-        return (uint64_t)1 << (l >= 12 && l <= 24 ? l : 16);
+        const int l = v ? atoi(v) : 0;
+        return l >= 12 && l <= 24 ? (uint64_t)1 << l : PT_TILE;
     }();
     return t;
 }
