@@ -1347,6 +1347,21 @@ struct Engine {
         const char *v = getenv("D4_PART_MIN_CHILDREN");
         return v ? std::max(0.0, atof(v)) : (double)(1u << 20);
     }();
+    // Limits of the partitioned dedup below its built-in ones, which reach
+    // its recovery paths on small searches (tests): the children its
+    // buffers take (D4_PART_MAX_CHILDREN; a row with more falls back to the
+    // table) and the next-frontier states the merge is given room for
+    // (D4_PART_OUT_CAP; finding more, it merges again into a frontier of
+    // the size it counted).
+    uint64_t part_n_cap = [] {
+        const char *v = getenv("D4_PART_MAX_CHILDREN");
+        const unsigned long long x = v ? strtoull(v, nullptr, 10) : 0ull;
+        return x ? std::min<uint64_t>(x, PART_MAX_CHILDREN) : PART_MAX_CHILDREN;
+    }();
+    uint64_t part_out_cap = [] {
+        const char *v = getenv("D4_PART_OUT_CAP");
+        return (uint64_t)(v ? strtoull(v, nullptr, 10) : 0ull);
+    }();
     bool part_dedup_fits(uint64_t chc, uint64_t nf, uint64_t out_cap, double expect) const {
         if (lib->dedup_mode != D4_DEDUP_PART || noalloc) return false;
         if (expect < part_min) return false;
@@ -1392,7 +1407,7 @@ struct Engine {
         pa.ch_key = ch_k.p;
         pa.ch_cnt = ch_c.p;
         pa.ch_cap = chc;
-        pa.n_max = nm;
+        pa.n_max = std::min<uint64_t>(nm, part_n_cap);
         pa.tile_min = part_tile_min();
         pa.lp_max = part_lp_max();
         pa.slots = part_slots();
@@ -1404,7 +1419,7 @@ struct Engine {
         pa.bsum = pbsum.p;
         pa.out_key = ok;
         pa.out_cnt = oc;
-        pa.out_cap = out_cap;
+        pa.out_cap = part_out_cap ? std::min<uint64_t>(out_cap, part_out_cap) : out_cap;
         pa.bw = d.bw;
         pa.bw_stride = d.bw_stride;
         pa.start = d.start;

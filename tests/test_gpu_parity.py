@@ -555,6 +555,41 @@ def test_level_prune_matches_dedup_prune(gpu, p5, monkeypatch):
         assert abs(s1[k] - s0[k]) <= 0.01 * s0[k], (k, s1, s0)
 
 
+@pytest.mark.parametrize("mode", ["fused", "graph"])
+def test_part_dedup_recovery_paths(gpu, p5, monkeypatch, mode):
+    """The partitioned dedup's two ways out, forced on every row of the 256
+    seeded 8x8 boards (fixture) by limits far below the built-in ones: a
+    next frontier with room for fewer unique states than the merge finds
+    (it merges again into one of the size it counted) and more children than
+    its buffers take (the step reruns with the table).  Counts equal the
+    fixture's either way."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    monkeypatch.setenv("D4_PART_MIN_CHILDREN", "0")
+    m = {"fused": gpu.STEP_FUSED, "graph": gpu.STEP_GRAPH}[mode]
+
+    def run():
+        lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+        lib.set_step_mode(m)
+        b = gpu.Batch(lib, fx["boards"])
+        b.run()
+        assert b.counts() == fx["counts"]
+        return b.stats()
+
+    monkeypatch.setenv("D4_PART_OUT_CAP", "1000")
+    st = run()
+    assert st["part_remerges"] > 0 and st["part_fallbacks"] == 0, st
+    monkeypatch.delenv("D4_PART_OUT_CAP")
+    monkeypatch.setenv("D4_PART_MAX_CHILDREN", "1000")
+    st = run()
+    assert st["part_fallbacks"] > 0, st
+    monkeypatch.delenv("D4_PART_MAX_CHILDREN")
+    st = run()
+    assert st["part_steps"] > 0 and st["part_fallbacks"] == 0, st
+
+
 @pytest.mark.parametrize("dedup", ["table", "part", "part_small"])
 def test_dedup_modes_agree(gpu, p5, oracle, monkeypatch, dedup):
     """The row step's dedup through one table in device memory or through
