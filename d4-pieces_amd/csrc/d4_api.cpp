@@ -768,6 +768,7 @@ struct Engine {
         a.n_win0 = S <= 1024 && level_lds_bytes(lib->fit_rows, 0, S) <= lib->lds_limit ? S : 0;
         a.n_win =
             S <= 1024 && level_lds_bytes(lib->fit_rows, S, a.n_win0) <= lib->lds_limit ? S : 0;
+        a.stage = level_lds_bytes(lib->fit_rows, a.n_win, a.n_win0, true) <= lib->lds_limit;
     }
 
     // Queues the terminal pass when an instance finishes at this step.
@@ -818,7 +819,8 @@ struct Engine {
             lv_ev.push_back(e);
         }
         // the connectivity prune runs in the level kernels, before a child
-        // is written, unless level_prune is off (then in the dedup)
+        // is written, or in the dedup
+        level_prune = level_prunes(a);
         a.prune = prune && level_prune;
         const bool dprune = prune && !a.prune;
         launch_reset_levels(c32.p, c64.p, true, st);
@@ -1122,6 +1124,7 @@ struct Engine {
         a.in_key = gk[p].p;
         a.in_cnt = gc[p].p;
         a.n_rec = (uint32_t)std::min<uint64_t>(g_cap, 0xffffffffull);
+        level_prune = level_prunes(a);
         a.prune = prune && level_prune;
         const bool dprune = prune && !a.prune;
         const RowCtl *ctl = ctl_d.p;
@@ -1326,6 +1329,23 @@ struct Engine {
         return done;
     }
 
+    // Where count-mode row steps run the connectivity prune.  In the level
+    // kernels (before a child is written, so that the dedup reads survivors
+    // only) when each wave's staging area fits in LDS next to the tables:
+    // the wave's completed children are gathered there and tested 64 at a
+    // time.  Otherwise in the dedup: tested on the completing lanes alone
+    // (the rest of the wave idle), the flood fill costs the level kernels
+    // more than the dedup saves.  D4_LEVEL_PRUNE=0 keeps it in the dedup,
+    // =1 puts it in the level kernels either way.
+    int level_prune_mode = [] {
+        const char *e = std::getenv("D4_LEVEL_PRUNE");
+        return e && *e == '0' ? 0 : e && *e == '1' ? 1 : -1;
+    }();
+    bool level_prune = false;  // (the last row step's choice)
+    bool level_prunes(const LevelArgs &a) const {
+        return level_prune_mode == 1 || (level_prune_mode < 0 && a.stage);
+    }
+
     // Whether the step's children (ch_cap of them) go through the
     // partitioned dedup: the library asks for it, the children fit its
     // buckets, and its buffers (children in bucket order, a next frontier
@@ -1334,15 +1354,6 @@ struct Engine {
     // bucket chain's dozen launches cost more than it saves there; 2^20 by
     // default, D4_PART_MIN_CHILDREN overrides it, so tests can send small
     // rows through the buckets)
-    // whether count-mode level kernels prune the children they emit, so the
-    // dedup reads survivors only (D4_LEVEL_PRUNE=1).  Off by default: on
-    // the batched 8x8 boards the flood fill on the emitting lanes alone
-    // (the rest of the wave waits) costs the level kernels 0.71 ms per row
-    // step and saves the dedup 0.48 ms.
-    bool level_prune = [] {
-        const char *e = std::getenv("D4_LEVEL_PRUNE");
-        return e && *e == '1';
-    }();
     double part_min = [] {
         const char *v = getenv("D4_PART_MIN_CHILDREN");
         return v ? std::max(0.0, atof(v)) : (double)(1u << 20);

@@ -174,6 +174,9 @@ struct LevelArgs {
     uint32_t prune, n_win;
     // level 0: instances whose board window at the row is staged in LDS
     uint32_t n_win0;
+    // with the prune: children gathered per wave in LDS and pruned 64 at a
+    // time (else each on its own lane)
+    uint32_t stage;
     uint64_t seed_mask;
     uint32_t gcd, min_size, closed_rows;
 };
@@ -253,7 +256,8 @@ uint32_t part_slots();
 // must be; launch_part_merge_again reruns the merge alone)
 void launch_part_dedup(const PartArgs &a, bool prune, int n_cu, hipStream_t st);
 void launch_part_merge_again(const PartArgs &a, int n_cu, hipStream_t st);
-size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win = 0, uint32_t n_win0 = 0);
+size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win = 0, uint32_t n_win0 = 0,
+                       bool stage = false);
 
 // Tables of one library in a multi-library batch (device layout, 16-byte
 // aligned).  Candidate masks are 128 bits: word 0 = candidates 0..63.

@@ -137,6 +137,10 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     uint64_t *s_w0 = s_win + (a.prune ? a.n_win : 0u);
     const uint32_t nw0 = FROM_REC ? a.n_win0 : 0u;
     uint8_t *s_t0 = reinterpret_cast<uint8_t *>(s_w0 + nw0);
+    // (then, when the prune runs on gathered children, each wave's staging
+    // area: 64 child keys and their 64 counts)
+    const bool stage = a.prune && a.stage;
+    uint64_t *s_stg = s_w0 + nw0 + (nw0 + 7) / 8;
     for (uint32_t i = threadIdx.x; i < nfit; i += LT) s_fit[i] = a.fit[i];
     for (uint32_t i = threadIdx.x; i < 2 * 8 * MAX_CAND; i += LT) s_fit[nfit + i] = a.placeinfo[i];
     for (uint32_t i = threadIdx.x; i < 4 * 256; i += LT) s_xa[i] = a.xavail[i];
@@ -156,6 +160,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     uint64_t *o_E = s_rng + MAX_CAND + (size_t)wv * (OWN * 64 + PCAP / 4);
     uint64_t *o_cov = o_E + 64, *o_cnt = o_cov + 64, *o_ui = o_cnt + 64, *o_av = o_ui + 64;
     uint16_t *plist = reinterpret_cast<uint16_t *>(o_av + 64);
+    uint64_t *st_key = s_stg + (size_t)wv * 128, *st_cnt = st_key + 64;
 
     if (__hip_atomic_load(a.c32 + C32_XOVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     uint64_t n_in;
@@ -176,6 +181,64 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     unsigned long long *ctr_s = a.c64 + C64_LVL + a.level, *ctr_c = a.c64 + C64_CHW;
     uint64_t *oE = a.out_st, *oOk = oE + a.out_cap, *oCnt = oOk + a.out_cap,
              *oCov = oCnt + a.out_cap, *oUi = oCov + a.out_cap;
+
+    // Children to the children buffer: lanes with e hold one {key, count}.
+    // With the prune, either each child is tested on its own lane before it
+    // is written, or (stage) the wave's children are gathered in its
+    // staging area and tested 64 at a time, so that the flood fill runs on
+    // every lane.  False when the buffer is full.
+    uint32_t n_s = 0;  // children staged (wave-uniform)
+    auto write_children = [&](bool keep, uint64_t key, uint64_t cnt) -> bool {
+        const uint64_t km = __ballot(keep);
+        if (!km) return true;
+        bool okw = true;
+        const uint64_t sl = place(wc, km, ctr_c, chunk, a.ch_cap, a.c32, okw);
+        if (!okw) return false;
+        if (keep) {
+            a.ch_key[sl] = key;
+            a.ch_cnt[sl] = cnt;
+        }
+        return true;
+    };
+    auto inst_of = [&](uint64_t key) -> uint32_t {
+        return a.inst_shift < 64 ? (uint32_t)(key >> a.inst_shift) : 0u;
+    };
+    auto flush_staged = [&](uint32_t n) -> bool {
+        __builtin_amdgcn_wave_barrier();
+        bool keep = false;
+        uint64_t key = 0, cnt = 0;
+        if (lane < n) {
+            key = st_key[lane];
+            cnt = st_cnt[lane];
+            keep = child_ok(a, s_win, inst_of(key), key, pruned);
+        }
+        __builtin_amdgcn_wave_barrier();
+        return write_children(keep, key, cnt);
+    };
+    auto emit_children = [&](bool e, uint64_t key, uint64_t cnt) -> bool {
+        if (!stage) return write_children(e && (!a.prune || child_ok(a, s_win, inst_of(key), key, pruned)),
+                                          key, cnt);
+        const uint64_t m = __ballot(e);
+        if (!m) return true;
+        const uint32_t nn = (uint32_t)__builtin_popcountll(m);
+        const uint32_t slot = n_s + mbcnt(m);
+        __builtin_amdgcn_wave_barrier();
+        if (e && slot < 64) {
+            st_key[slot] = key;
+            st_cnt[slot] = cnt;
+        }
+        if (n_s + nn < 64) {
+            n_s += nn;
+            return true;
+        }
+        if (!flush_staged(64)) return false;
+        if (e && slot >= 64) {
+            st_key[slot - 64] = key;
+            st_cnt[slot - 64] = cnt;
+        }
+        n_s = n_s + nn - 64;
+        return true;
+    };
 
     // the next batch's inputs are loaded while the current one is processed
     const uint64_t stride = waves * 64;
@@ -231,9 +294,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                         const uint32_t c = (uint32_t)__builtin_ctzll(E & 0xffu);
                         ok = fitmask<FR>(s_fit, a.fit_rows, c, E) & avail_of(s_xa, used);
                     } else {
-                        // row already covered from above: its own child
-                        pass = !a.prune || child_ok(a, s_win, inst,
-                                                    cov | ((uint64_t)used << a.occ_bits), pruned);
+                        pass = true;  // row already covered from above: its own child
                     }
                 }
             }
@@ -247,20 +308,11 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
             inst = (uint32_t)(ui >> 32);
         }
         if (FROM_REC) {
-            const uint64_t pm = __ballot(pass);
-            if (pm) {
-                bool okw = true;
-                const uint64_t sl = place(wc, pm, ctr_c, chunk, a.ch_cap, a.c32, okw);
-                if (!okw) {
-                    alive = false;
-                    break;
-                }
-                if (pass) {
-                    uint64_t key2 = cov | ((uint64_t)used << a.occ_bits);
-                    if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
-                    a.ch_key[sl] = key2;
-                    a.ch_cnt[sl] = cnt;
-                }
+            uint64_t key2 = cov | ((uint64_t)used << a.occ_bits);
+            if (a.inst_shift < 64) key2 |= (uint64_t)inst << a.inst_shift;
+            if (!emit_children(pass, key2, cnt)) {
+                alive = false;
+                break;
             }
         }
         // the loaded rows go to LDS; their pairs are listed pass by pass
@@ -312,9 +364,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                         cov2 = o_cov[o] | (pl >> 8);
                         pc = o_cnt[o];
                         if (!(E2 & 0xffu)) {
-                            emitC = !a.prune ||
-                                    child_ok(a, s_win, pi, cov2 | ((uint64_t)used2 << a.occ_bits),
-                                             pruned);
+                            emitC = true;
                         } else {
                             uint64_t av2 = o_av[o];
                             if (((used2 >> (ci & 0xffu)) & ((ci >> 8) & 0xffu)) == (ci >> 16))
@@ -325,19 +375,12 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                         }
                     }
                 }
-                const uint64_t cm = __ballot(emitC);
-                if (cm) {
-                    bool okw = true;
-                    const uint64_t sl = place(wc, cm, ctr_c, chunk, a.ch_cap, a.c32, okw);
-                    if (!okw) {
+                {
+                    uint64_t key2 = cov2 | ((uint64_t)used2 << a.occ_bits);
+                    if (a.inst_shift < 64) key2 |= (uint64_t)pi << a.inst_shift;
+                    if (!emit_children(emitC, key2, pc)) {
                         alive = false;
                         break;
-                    }
-                    if (emitC) {
-                        uint64_t key2 = cov2 | ((uint64_t)used2 << a.occ_bits);
-                        if (a.inst_shift < 64) key2 |= (uint64_t)pi << a.inst_shift;
-                        a.ch_key[sl] = key2;
-                        a.ch_cnt[sl] = pc;
                     }
                 }
                 const uint64_t sm = __ballot(emitS);
@@ -360,6 +403,8 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
             __builtin_amdgcn_wave_barrier();
         }
     }
+    // (the children still staged)
+    if (alive && stage && n_s && !flush_staged(n_s)) alive = false;
     if (!alive) return;  // the step is rerun
     // unused slots of the wave's last chunks: no candidates / empty keys
     for (uint64_t j = ws.pos + lane; j < ws.end; j += 64) oOk[j] = 0;
@@ -510,15 +555,16 @@ static void launch_lv(const LevelArgs &a, dim3 g, size_t lds, hipStream_t st) {
 
 }  // namespace
 
-size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win, uint32_t n_win0) {
+size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win, uint32_t n_win0, bool stage) {
     return sizeof(uint64_t) * (8u * fit_rows * 256u + 2u * 8u * MAX_CAND + 4u * 256u + MAX_CAND) +
            (sizeof(uint64_t) * OWN * 64 + sizeof(uint16_t) * PCAP) * (LT / 64) +
-           sizeof(uint64_t) * n_win + (sizeof(uint64_t) + 1) * n_win0;
+           sizeof(uint64_t) * (n_win + n_win0 + (n_win0 + 7) / 8) +
+           (stage ? sizeof(uint64_t) * 128 * (LT / 64) : 0);
 }
 
 void launch_level(const LevelArgs &a, int n_cu, hipStream_t st) {
-    const size_t lds =
-        level_lds_bytes(a.fit_rows, a.prune ? a.n_win : 0, a.level == 0 ? a.n_win0 : 0);
+    const size_t lds = level_lds_bytes(a.fit_rows, a.prune ? a.n_win : 0,
+                                       a.level == 0 ? a.n_win0 : 0, a.prune && a.stage);
     // one workgroup per CU (LDS-bound); level 0 knows its record count
     uint64_t blocks = (uint64_t)n_cu;
     if (a.level == 0) blocks = std::min<uint64_t>(blocks, ((uint64_t)a.n_rec + LT - 1) / LT);

@@ -506,7 +506,9 @@ def test_level_prune_matches_dedup_prune(gpu, p5, monkeypatch):
     """The connectivity prune in the level kernels (before a child is
     written) and in the dedup (after) keep the same children: equal counts,
     and equal children kept, pruned and unique states, in every step mode
-    and both dedups on the 256 seeded 8x8 boards (fixture); and on 8x6 with
+    and both dedups on the 256 seeded 8x8 boards (fixture; children gathered
+    per wave and pruned 64 at a time), on those boards four times over
+    (each child pruned on its own lane); and on 8x6 with
     three cells blocked under a 24 MiB cap, whose large rows run in hash
     partitions that each rerun the level kernels, equal counts and the same
     children kept and pruned within 1% (the capped run parks frontier parts
@@ -535,6 +537,19 @@ def test_level_prune_matches_dedup_prune(gpu, p5, monkeypatch):
             on, off = batch(True, mode, dedup), batch(False, mode, dedup)
             assert on == off, (mode, dedup, on, off)
             assert on["pruned"] > 0
+    # 1,024 boards (the fixture's four times): the waves' staging areas no
+    # longer fit in LDS beside the instances' windows, so the level kernels
+    # prune each child on its own lane
+    monkeypatch.setenv("D4_LEVEL_PRUNE", "1")
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    b1 = gpu.Batch(lib, fx["boards"] * 4)
+    b1.run()
+    monkeypatch.setenv("D4_LEVEL_PRUNE", "0")
+    b0 = gpu.Batch(lib, fx["boards"] * 4)
+    b0.run()
+    assert b1.counts() == b0.counts() == fx["counts"] * 4
+    s1, s0 = b1.stats(), b0.stats()
+    assert {k: s1[k] for k in keys} == {k: s0[k] for k in keys}
 
     def streamed(level_prune):
         monkeypatch.setenv("D4_LEVEL_PRUNE", "1" if level_prune else "0")
