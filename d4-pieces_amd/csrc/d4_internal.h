@@ -115,10 +115,12 @@ struct ExtendArgs {
 };
 
 // Level-synchronous count-mode row step (d4_level.hip).  A partial row
-// (state) is {open window E, remaining candidates ok, tiling count, occupancy
-// of the rows below cov, use counters | instance << 32}, stored SoA in a
-// buffer of `cap` entries per field; children are {key, count} SoA.
-constexpr uint32_t LVL_FIELDS = 5;
+// (state) is {tiling count, occupancy of the rows below cov | the row's open
+// cells << 56, use counters | instance << 32}, stored SoA in a buffer of
+// `cap` entries per field (open cells 0: an unused slot); its open window
+// and candidates are rebuilt from these and the instance's board window when
+// it is loaded.  Children are {key, count} SoA.
+constexpr uint32_t LVL_FIELDS = 3;
 constexpr uint32_t LVL_MAX = 8;  // placements per row (one level each)
 // Control block of row steps replayed from a HIP graph: the kernels of a
 // recorded step read the step's sizes here instead of from their launch
@@ -172,7 +174,7 @@ struct LevelArgs {
     // with its fields); the board windows of row + 1 go to LDS when n_win
     // (the instances) is not 0
     uint32_t prune, n_win;
-    // level 0: instances whose board window at the row is staged in LDS
+    // instances whose board window at the row is staged in LDS
     uint32_t n_win0;
     // with the prune: children gathered per wave in LDS and pruned 64 at a
     // time (else each on its own lane)

@@ -131,11 +131,11 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     uint64_t *s_xa = s_fit + nfit + 2 * 8 * MAX_CAND;
     uint64_t *s_rng = s_xa + 4 * 256;
     // (after the waves' own areas: the board windows of row + 1 for the
-    // prune, then at level 0 each instance's window at the row and whether
-    // the row is one of its rows)
+    // prune, then each instance's window at the row and whether the row is
+    // one of its rows)
     uint64_t *s_win = s_rng + MAX_CAND + (size_t)(LT / 64) * (OWN * 64 + PCAP / 4);
     uint64_t *s_w0 = s_win + (a.prune ? a.n_win : 0u);
-    const uint32_t nw0 = FROM_REC ? a.n_win0 : 0u;
+    const uint32_t nw0 = a.n_win0;
     uint8_t *s_t0 = reinterpret_cast<uint8_t *>(s_w0 + nw0);
     // (then, when the prune runs on gathered children, each wave's staging
     // area: 64 child keys and their 64 counts)
@@ -179,8 +179,11 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     bool alive = true;
     uint64_t pruned = 0;
     unsigned long long *ctr_s = a.c64 + C64_LVL + a.level, *ctr_c = a.c64 + C64_CHW;
-    uint64_t *oE = a.out_st, *oOk = oE + a.out_cap, *oCnt = oOk + a.out_cap,
-             *oCov = oCnt + a.out_cap, *oUi = oCov + a.out_cap;
+    uint64_t *oCnt = a.out_st, *oCw = oCnt + a.out_cap, *oUi = oCw + a.out_cap;
+    // the board window of instance i at the row
+    auto row_window = [&](uint32_t i) -> uint64_t {
+        return i < nw0 ? s_w0[i] : a.bw[(size_t)i * a.bw_stride + (a.row - a.start[i])];
+    };
 
     // Children to the children buffer: lanes with e hold one {key, count}.
     // With the prune, either each child is tested on its own lane before it
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
 
     // the next batch's inputs are loaded while the current one is processed
     const uint64_t stride = waves * 64;
-    uint64_t nx0 = 0, nx1 = 0, nx2 = 0, nx3 = 0, nx4 = 0;
+    uint64_t nx0 = 0, nx1 = 0, nx2 = 0;
     auto fetch = [&](uint64_t b) {
         const uint64_t i = b + lane;
         if (i < n_in) {
@@ -254,8 +257,6 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                 nx0 = s[i];
                 nx1 = s[a.in_cap + i];
                 nx2 = s[2 * a.in_cap + i];
-                nx3 = s[3 * a.in_cap + i];
-                nx4 = s[4 * a.in_cap + i];
             }
         }
     };
@@ -263,9 +264,9 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     for (uint64_t b0 = wid * 64; alive && b0 < n_in; b0 += stride) {
         const uint64_t i = b0 + lane;
         const bool valid = i < n_in;
-        const uint64_t c0 = nx0, c1 = nx1, c2 = nx2, c3 = nx3, c4 = nx4;
+        const uint64_t c0 = nx0, c1 = nx1, c2 = nx2;
         if (b0 + stride < n_in) fetch(b0 + stride);
-        uint64_t E = 0, ok = 0, cnt = 0, cov = 0;
+        uint64_t E = 0, ok = 0, cnt = 0, cov = 0, av = 0;
         uint32_t used = 0, inst = 0;
         bool pass = false;
         if (FROM_REC) {
@@ -292,20 +293,28 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                     cov = occ >> 8;
                     if (E & 0xffu) {
                         const uint32_t c = (uint32_t)__builtin_ctzll(E & 0xffu);
-                        ok = fitmask<FR>(s_fit, a.fit_rows, c, E) & avail_of(s_xa, used);
+                        av = avail_of(s_xa, used);
+                        ok = fitmask<FR>(s_fit, a.fit_rows, c, E) & av;
                     } else {
                         pass = true;  // row already covered from above: its own child
                     }
                 }
             }
         } else if (valid) {
-            E = c0;
-            ok = c1;
-            cnt = c2;
-            cov = c3;
-            const uint64_t ui = c4;
+            cnt = c0;
+            const uint64_t ui = c2;
             used = (uint32_t)ui;
             inst = (uint32_t)(ui >> 32);
+            const uint32_t open = (uint32_t)(c1 >> 56);
+            if (open) {
+                // the open window and the candidates at its lowest open cell,
+                // rebuilt (as the level before computed them)
+                cov = c1 & ((1ull << 56) - 1);
+                E = (row_window(inst) & ~(cov << 8) & ~0xffull) | open;
+                const uint32_t c = (uint32_t)__builtin_ctz(open);
+                av = avail_of(s_xa, used);
+                ok = fitmask<FR>(s_fit, a.fit_rows, c, E) & av;
+            }
         }
         if (FROM_REC) {
             uint64_t key2 = cov | ((uint64_t)used << a.occ_bits);
@@ -321,7 +330,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
         o_cov[lane] = cov;
         o_cnt[lane] = cnt;
         o_ui[lane] = (uint64_t)used | ((uint64_t)inst << 32);
-        o_av[lane] = ok ? avail_of(s_xa, used) : 0ull;
+        o_av[lane] = ok ? av : 0ull;
         const uint32_t n = (uint32_t)__builtin_popcountll(ok);
         const uint32_t incl = wave_incl_scan(n);
         const uint32_t excl = incl - n;
@@ -392,10 +401,8 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
                         break;
                     }
                     if (emitS) {
-                        oE[sl] = E2;
-                        oOk[sl] = ok2;
                         oCnt[sl] = pc;
-                        oCov[sl] = cov2;
+                        oCw[sl] = cov2 | ((E2 & 0xffull) << 56);
                         oUi[sl] = (uint64_t)used2 | ((uint64_t)pi << 32);
                     }
                 }
@@ -407,7 +414,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     if (alive && stage && n_s && !flush_staged(n_s)) alive = false;
     if (!alive) return;  // the step is rerun
     // unused slots of the wave's last chunks: no candidates / empty keys
-    for (uint64_t j = ws.pos + lane; j < ws.end; j += 64) oOk[j] = 0;
+    for (uint64_t j = ws.pos + lane; j < ws.end; j += 64) oCw[j] = 0;
     for (uint64_t j = wc.pos + lane; j < wc.end; j += 64) a.ch_key[j] = EMPTY_KEY;
     pruned = wave_sum_u64(pruned);
     if (lane == 0 && pruned) atomicAdd(a.c64 + C64_LPRUNED, (unsigned long long)pruned);
@@ -563,8 +570,8 @@ size_t level_lds_bytes(uint32_t fit_rows, uint32_t n_win, uint32_t n_win0, bool 
 }
 
 void launch_level(const LevelArgs &a, int n_cu, hipStream_t st) {
-    const size_t lds = level_lds_bytes(a.fit_rows, a.prune ? a.n_win : 0,
-                                       a.level == 0 ? a.n_win0 : 0, a.prune && a.stage);
+    const size_t lds =
+        level_lds_bytes(a.fit_rows, a.prune ? a.n_win : 0, a.n_win0, a.prune && a.stage);
     // one workgroup per CU (LDS-bound); level 0 knows its record count
     uint64_t blocks = (uint64_t)n_cu;
     if (a.level == 0) blocks = std::min<uint64_t>(blocks, ((uint64_t)a.n_rec + LT - 1) / LT);
