@@ -13,16 +13,21 @@
 //             time, one per lane, so no lane idles on a row with few
 //             candidates or waits on a sibling's deeper subtree.  A placement
 //             that completes the row yields a child {key, count}; one that
-//             leaves it open yields a partial row of the next level together
-//             with its candidate mask -- or nothing, when that mask is empty.
-//             Outputs are written densely (ballot + mbcnt) into chunks that
-//             each wave reserves with one atomic per chunk, not per record.
-//   k_dedup   prunes each child (connectivity) and merges it into the
-//             open-addressing table (CAS, counts summed), with nothing else
-//             on its lanes' path, so many probes are in flight at once.
+//             leaves it open yields a partial row of the next level (24 B:
+//             its window and candidates are rebuilt when it is loaded) -- or
+//             nothing, when it has no candidate.  Completed rows are gathered
+//             in the wave's LDS staging area and pruned 64 at a time (the
+//             connectivity test on every lane), and only the survivors are
+//             written.  Outputs are written densely (ballot + mbcnt) into
+//             chunks that each wave reserves with one atomic per chunk, not
+//             per record.
+//   k_dedup   merges each child into the open-addressing table (CAS, counts
+//             summed; pruning it first when the level kernels did not), with
+//             nothing else on its lanes' path, so many probes are in flight
+//             at once.  Large rows use the partitioned dedup (d4_part.hip).
 //
-// The children and the table are the ones k_extend produces (each child is
-// one placement sequence; only the order of discovery differs).  The
+// The surviving children are the ones k_extend keeps (each child is one
+// placement sequence; only the order of discovery differs).  The
 // reference's counterparts are tiled_row_search / impl<H>
 // (src/kernel.cuh:79-160) and the host Sorter (src/sorter.cpp:156-214).
 #include "d4_internal.h"

@@ -11,10 +11,11 @@
 // in streaming passes, into buckets small enough for one workgroup's LDS,
 // and merged there:
 //
-//   k_part_hist1    per tile (at least 64 Ki children): the connectivity
-//                   prune of each child (a survivor bit per child, one 64-bit
-//                   word per wave), survivors counted per coarse bucket (the
-//                   top <= 8 bits of the key's hash) with LDS atomics
+//   k_part_hist1    per tile (at least 64 Ki children): a survivor bit per
+//                   child (one 64-bit word per wave; the connectivity prune
+//                   runs here when the level kernels did not run it),
+//                   survivors counted per coarse bucket (the top <= 8 bits of
+//                   the key's hash) with LDS atomics
 //   k_part_scan*    exclusive scan of the counts in bucket-major order: every
 //                   (bucket, tile) pair gets its range of the output
 //   k_part_scatter1 per tile: each survivor to its coarse bucket's range (LDS
@@ -36,7 +37,7 @@
 //                   frontier with one device atomic per pass.
 //
 // There is no device atomic per child, and only surviving children (about a
-// third on the 8x8 pentomino boards) are moved.  Every size is derived on the
+// third on the 8x8 pentomino boards) are moved past the first pass.  Every size is derived on the
 // device from c64[C64_CHW], so the chain runs without a host round trip and
 // can be recorded in a graph.  A row with more children than the buffers were
 // sized for, or a pass whose unique states exceed the LDS table (a single
