@@ -322,9 +322,54 @@ __global__ __launch_bounds__(PT) void k_part_scan3(PartArgs a) {
     }
 }
 
+// A workgroup's block of up to 4 * PT records, scattered to their buckets'
+// runs (bucket b's next free position is s_cur[b]) through LDS: the block is
+// first placed there in bucket order, so that consecutive lanes then store
+// consecutive records of one run -- a wave's stores form a few runs of whole
+// lines instead of 64 lone 16-byte stores to as many lines.  Called by every
+// thread of the workgroup (it has barriers); P <= 256 buckets.
+struct StageLds {
+    ulonglong2 rec[4 * PT];
+    uint32_t cnt[256], off[256];
+    uint32_t w[PT / 64 + 1];
+    uint8_t bkt[4 * PT];
+};
+static_assert((1u << LP1_MAX) <= 256 && (1u << (LP_MAX - LP1_MAX)) <= 256,
+              "bucket ids of a staged block are bytes");
+
+__device__ __forceinline__ void staged_scatter(StageLds &S, uint32_t *s_cur, uint32_t P,
+                                               const ulonglong2 (&v)[4], const uint32_t (&b)[4],
+                                               const bool (&live)[4], ulonglong2 *out) {
+    for (uint32_t i = threadIdx.x; i < P; i += PT) S.cnt[i] = 0;
+    lds_barrier();
+    uint32_t r[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++) r[u] = live[u] ? atomicAdd(&S.cnt[b[u]], 1u) : 0u;
+    lds_barrier();
+    uint32_t total;
+    const uint32_t o = block_excl_scan(threadIdx.x < P ? S.cnt[threadIdx.x] : 0u, S.w, total);
+    if (threadIdx.x < P) S.off[threadIdx.x] = o;
+    lds_barrier();
+#pragma unroll
+    for (uint32_t u = 0; u < 4; u++)
+        if (live[u]) {
+            const uint32_t sl = S.off[b[u]] + r[u];
+            S.rec[sl] = v[u];
+            S.bkt[sl] = (uint8_t)b[u];
+        }
+    lds_barrier();
+    for (uint32_t i = threadIdx.x; i < total; i += PT) {
+        const uint32_t bb = S.bkt[i];
+        out[s_cur[bb] + (i - S.off[bb])] = S.rec[i];
+    }
+    lds_barrier();
+    if (threadIdx.x < P) s_cur[threadIdx.x] += S.cnt[threadIdx.x];
+}
+
 // Level 1: survivors to their coarse bucket's range of pbuf.
 __global__ __launch_bounds__(PT) void k_part_scatter1(PartArgs a) {
     __shared__ uint32_t s_cur[1u << LP1_MAX];
+    __shared__ StageLds S;
     if (a.ctl && a.ctl->fail) return;
     const uint64_t n = part_n(a);
     const Shape sh = part_shape(n, a.tile_min, a.lp_max);
@@ -337,8 +382,11 @@ __global__ __launch_bounds__(PT) void k_part_scatter1(PartArgs a) {
         lds_barrier();
         const uint64_t t0 = (uint64_t)t * sh.tile, t1 = min(t0 + sh.tile, n);
         for (uint64_t j0 = t0; j0 < t1; j0 += 4 * PT) {
-            // (all loads of the batch in flight before any dependent work)
-            uint64_t m[4], k[4], c[4];
+            // (all loads of the block in flight before any dependent work)
+            uint64_t m[4];
+            ulonglong2 v[4];
+            uint32_t bk[4];
+            bool live[4];
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) {
                 const uint64_t i0 = j0 + u * PT + (threadIdx.x & ~63u);
@@ -347,20 +395,16 @@ __global__ __launch_bounds__(PT) void k_part_scatter1(PartArgs a) {
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) {
                 const uint64_t i = j0 + u * PT + threadIdx.x;
-                const bool live = (m[u] >> lane) & 1u;
-                k[u] = live ? ld_once(a.ch_key + i) : EMPTY_KEY;
-                c[u] = live ? ld_once(a.ch_cnt + i) : 0ull;
+                live[u] = (m[u] >> lane) & 1u;
+                v[u].x = live[u] ? ld_once(a.ch_key + i) : EMPTY_KEY;
+                v[u].y = live[u] ? ld_once(a.ch_cnt + i) : 0ull;
             }
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++)
-                if (k[u] != EMPTY_KEY) {
-                    const uint32_t pos = atomicAdd(&s_cur[bucket_of(mix64(k[u]), sh.lp1)], 1u);
-                    ulonglong2 v;
-                    v.x = k[u];
-                    v.y = c[u];
-                    pb[pos] = v;
-                    csum += c[u];
-                }
+            for (uint32_t u = 0; u < 4; u++) {
+                bk[u] = live[u] ? bucket_of(mix64(v[u].x), sh.lp1) : 0u;
+                csum += v[u].y;
+            }
+            staged_scatter(S, s_cur, P1, v, bk, live, pb);
         }
         lds_barrier();
     }
@@ -415,10 +459,10 @@ __global__ __launch_bounds__(PT) void k_part_hist2(PartArgs a) {
     }
 }
 
-// Level 2: survivors from pbuf to their fine bucket's range of the children
-// buffers (keys and counts apart).
+// Level 2: survivors from pbuf to their fine bucket's range of sbuf.
 __global__ __launch_bounds__(PT) void k_part_scatter2(PartArgs a) {
     __shared__ uint32_t s_cur[1u << (LP_MAX - LP1_MAX)];
+    __shared__ StageLds S;
     if (a.ctl && a.ctl->fail) return;
     const uint64_t n = part_n(a);
     const Shape sh = part_shape(n, a.tile_min, a.lp_max);
@@ -433,21 +477,21 @@ __global__ __launch_bounds__(PT) void k_part_scatter2(PartArgs a) {
             s_cur[threadIdx.x] = a.hist2[((uint64_t)c * P2 + threadIdx.x) * sh.T2 + t2];
         lds_barrier();
         if (!big) {
-            for (uint32_t i0 = r0 + threadIdx.x; i0 < r1; i0 += 4 * PT) {
+            for (uint32_t j0 = r0; j0 < r1; j0 += 4 * PT) {
                 ulonglong2 v[4];
+                uint32_t bk[4];
+                bool live[4];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; u++) {
-                    const uint32_t i = i0 + u * PT;
-                    if (i < r1) v[u] = ld_once(pb + i);
+                    const uint32_t i = j0 + u * PT + threadIdx.x;
+                    live[u] = i < r1;
+                    if (live[u]) v[u] = ld_once(pb + i);
                     else v[u].x = v[u].y = 0;
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < 4; u++)
-                    if (i0 + u * PT < r1) {
-                        const uint32_t pos =
-                            atomicAdd(&s_cur[bucket_of(mix64(v[u].x), sh.lp) & (P2 - 1)], 1u);
-                        sb[pos] = v[u];
-                    }
+                    bk[u] = live[u] ? bucket_of(mix64(v[u].x), sh.lp) & (P2 - 1) : 0u;
+                staged_scatter(S, s_cur, P2, v, bk, live, sb);
             }
         }
         lds_barrier();
