@@ -746,6 +746,7 @@ struct Engine {
         d.gcd = lib->gcd;
         d.min_size = lib->min_size;
         d.closed_rows = lib->maxh >= 1 ? lib->maxh - 1 : 0;
+        d.area_ok = area_mask(lib->gcd, lib->min_size);
         d.n_inst = S;
         d.hgt = hgt.p;
         d.tab = tk.p;
@@ -765,6 +766,7 @@ struct Engine {
         a.gcd = d.gcd;
         a.min_size = d.min_size;
         a.closed_rows = d.closed_rows;
+        a.area_ok = d.area_ok;
         a.n_win0 = S <= 1024 && level_lds_bytes(lib->fit_rows, 0, S) <= lib->lds_limit ? S : 0;
         a.n_win =
             S <= 1024 && level_lds_bytes(lib->fit_rows, S, a.n_win0) <= lib->lds_limit ? S : 0;
@@ -1441,6 +1443,7 @@ struct Engine {
         pa.gcd = d.gcd;
         pa.min_size = d.min_size;
         pa.closed_rows = d.closed_rows;
+        pa.area_ok = d.area_ok;
         pa.n_inst = d.n_inst;
         pa.c32 = d.c32;
         pa.c64 = d.c64;

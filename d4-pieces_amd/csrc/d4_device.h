@@ -109,8 +109,24 @@ __device__ __forceinline__ uint64_t grow4(uint64_t x) {
 // can complete it.  (Shape::extend1 / connected / sml_island semantics,
 // src/Shape.cpp:81-123, applied to the live board at run time instead of the
 // reference's static table-time island test, src/frow.cpp:78-100.)
-__device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g,
-                                       uint32_t min_size) {
+//
+// Whether a closed region's area is allowed is a functor: AreaDiv tests it
+// (a division), AreaMask looks it up in a mask of allowed areas made once
+// (bit a: a >= min_size and a multiple of g; regions have at most 56 cells).
+// The areas of the closed regions add up to the closed cells: when their
+// count is not allowed, some region is not, and the flood fill of each
+// region is skipped.
+struct AreaDiv {
+    uint32_t g, min_size;
+    __device__ bool operator()(uint32_t a) const { return a >= min_size && a % g == 0; }
+};
+struct AreaMask {
+    uint64_t m;
+    __device__ bool operator()(uint32_t a) const { return (m >> a) & 1u; }
+};
+
+template <class AreaOk>
+__device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, AreaOk area_ok) {
     uint64_t x = E & seed;
     for (;;) {
         uint64_t y = grow4(x) & E;
@@ -118,6 +134,8 @@ __device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g,
         x = y;
     }
     uint64_t closed = E & ~x;
+    if (!closed) return true;
+    if (!area_ok((uint32_t)__builtin_popcountll(closed))) return false;
     while (closed) {
         uint64_t comp = closed & (~closed + 1ull);
         for (;;) {
@@ -125,11 +143,13 @@ __device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g,
             if (y == comp) break;
             comp = y;
         }
-        uint32_t area = (uint32_t)__builtin_popcountll(comp);
-        if (area < min_size || area % g) return false;
+        if (!area_ok((uint32_t)__builtin_popcountll(comp))) return false;
         closed &= ~comp;
     }
     return true;
+}
+__device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g, uint32_t min_size) {
+    return viable(E, seed, AreaDiv{g, min_size});
 }
 
 // viable() for pieces at most 5 rows tall: closed regions can only lie in
@@ -137,11 +157,12 @@ __device__ __forceinline__ bool viable(uint64_t E, uint64_t seed, uint32_t g,
 // reaches the seed rows exactly when it reaches a cell of row c - 1 that is
 // open above an open cell of row c (a path into the seed rows enters them
 // through row c), so the flood fill runs on 32-bit words.  Same result as
-// viable(E, rows >= c, g, min_size).
+// viable(E, rows >= c, ...).
 __device__ __forceinline__ uint32_t grow4_32(uint32_t x) {
     return x | (x << 8) | (x >> 8) | ((x << 1) & ~0x01010101u) | ((x >> 1) & ~0x80808080u);
 }
-__device__ __forceinline__ bool viable32(uint64_t E, uint32_t c, uint32_t g, uint32_t min_size) {
+template <class AreaOk>
+__device__ __forceinline__ bool viable32(uint64_t E, uint32_t c, AreaOk area_ok) {
     if (c == 0) return true;
     const uint32_t Ec = (uint32_t)E & (c >= 4 ? 0xffffffffu : (1u << (8 * c)) - 1u);
     const uint32_t below = (uint32_t)(E >> (8 * c)) & 0xffu;
@@ -152,6 +173,8 @@ __device__ __forceinline__ bool viable32(uint64_t E, uint32_t c, uint32_t g, uin
         x = y;
     }
     uint32_t closed = Ec & ~x;
+    if (!closed) return true;
+    if (!area_ok((uint32_t)__builtin_popcount(closed))) return false;
     while (closed) {
         uint32_t comp = closed & (~closed + 1u);
         for (;;) {
@@ -159,11 +182,13 @@ __device__ __forceinline__ bool viable32(uint64_t E, uint32_t c, uint32_t g, uin
             if (y == comp) break;
             comp = y;
         }
-        const uint32_t area = (uint32_t)__builtin_popcount(comp);
-        if (area < min_size || area % g) return false;
+        if (!area_ok((uint32_t)__builtin_popcount(comp))) return false;
         closed &= ~comp;
     }
     return true;
+}
+__device__ __forceinline__ bool viable32(uint64_t E, uint32_t c, uint32_t g, uint32_t min_size) {
+    return viable32(E, c, AreaDiv{g, min_size});
 }
 
 // The dedup table interleaves each slot's key and count (16 B per slot:

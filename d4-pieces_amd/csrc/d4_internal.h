@@ -122,6 +122,14 @@ struct ExtendArgs {
 // it is loaded.  Children are {key, count} SoA.
 constexpr uint32_t LVL_FIELDS = 3;
 constexpr uint32_t LVL_MAX = 8;  // placements per row (one level each)
+// Closed-region areas the connectivity prune allows, as a mask (bit a: at
+// least min_size cells and a multiple of g; regions have at most 56 cells).
+inline uint64_t area_mask(uint32_t g, uint32_t min_size) {
+    uint64_t m = 0;
+    for (uint32_t a = 1; a < 64; a++)
+        if (a >= min_size && g && a % g == 0) m |= 1ull << a;
+    return m;
+}
 // Control block of row steps replayed from a HIP graph: the kernels of a
 // recorded step read the step's sizes here instead of from their launch
 // arguments, and the step's last kernel (k_row_close) writes the next step's.
@@ -176,6 +184,7 @@ struct LevelArgs {
     uint32_t prune, n_win;
     // instances whose board window at the row is staged in LDS
     uint32_t n_win0;
+    uint64_t area_ok;        // closed-region areas allowed (bit a: area a)
     // with the prune: children gathered per wave in LDS and pruned 64 at a
     // time (else each on its own lane)
     uint32_t stage;
@@ -195,6 +204,7 @@ struct DedupArgs {
     uint32_t gcd, min_size;
     uint32_t closed_rows;    // max_height - 1: window rows that can hold closed regions
     uint32_t n_inst;         // instances (the children's board windows go to LDS when few)
+    uint64_t area_ok;        // closed-region areas allowed (bit a: area a)
     const uint32_t *hgt;
     unsigned long long *tab;  // {key, count} slots
     uint64_t tab_size;
@@ -243,6 +253,7 @@ struct PartArgs {
     uint32_t gcd, min_size;
     uint32_t closed_rows;
     uint32_t n_inst;
+    uint64_t area_ok;        // closed-region areas allowed (bit a: area a)
     uint32_t *c32;
     unsigned long long *c64;
     const RowCtl *ctl;       // set: row comes from it (graph replay)

@@ -115,8 +115,8 @@ __device__ __forceinline__ bool child_ok(const LevelArgs &a, const uint64_t *s_w
                                          uint64_t kb, uint64_t &pruned) {
     const uint64_t w = inst < a.n_win ? s_win[inst] : child_window(a, inst);
     const uint64_t E = w & ~(kb & a.occ_mask);
-    const bool ok = a.closed_rows <= 4 ? viable32(E, a.closed_rows, a.gcd, a.min_size)
-                                       : viable(E, a.seed_mask, a.gcd, a.min_size);
+    const bool ok = a.closed_rows <= 4 ? viable32(E, a.closed_rows, AreaMask{a.area_ok})
+                                       : viable(E, a.seed_mask, AreaMask{a.area_ok});
     pruned += !ok;
     return ok;
 }
@@ -513,8 +513,8 @@ __global__ __launch_bounds__(256) void k_dedup(DedupArgs a) {
                     w = a.bw[(size_t)inst * a.bw_stride + rr];
                 }
                 const uint64_t E = w & ~(key[u] & a.occ_mask);
-                const bool ok = a.closed_rows <= 4 ? viable32(E, a.closed_rows, a.gcd, a.min_size)
-                                                   : viable(E, a.seed_mask, a.gcd, a.min_size);
+                const bool ok = a.closed_rows <= 4 ? viable32(E, a.closed_rows, AreaMask{a.area_ok})
+                                                   : viable(E, a.seed_mask, AreaMask{a.area_ok});
                 if (!ok) {
                     pruned++;
                     live[u] = false;

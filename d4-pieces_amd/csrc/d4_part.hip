@@ -172,8 +172,8 @@ __device__ __forceinline__ bool child_viable(const PartArgs &a, uint32_t row, co
         w = a.bw[(size_t)inst * a.bw_stride + rr];
     }
     const uint64_t E = w & ~(k & a.occ_mask);
-    return a.closed_rows <= 4 ? viable32(E, a.closed_rows, a.gcd, a.min_size)
-                              : viable(E, a.seed_mask, a.gcd, a.min_size);
+    return a.closed_rows <= 4 ? viable32(E, a.closed_rows, AreaMask{a.area_ok})
+                              : viable(E, a.seed_mask, AreaMask{a.area_ok});
 }
 
 // Level 1: prune, survivor bits, survivors per coarse bucket and tile.

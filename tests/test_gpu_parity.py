@@ -341,6 +341,55 @@ def test_calendar_puzzle_batch(gpu, oracle):
 
 
 
+def test_prune_statistics_agree_across_kernels(gpu, p5, oracle):
+    """The connectivity prune keeps and drops the same children in the
+    depth-first kernel (64-bit flood fill, areas tested by division) and in
+    the level kernels (32-bit flood fill, areas looked up in a mask, closed
+    cells' total tested first): equal children kept and pruned, and the
+    oracle's counts, for pieces of gcd 5 (pentominoes: the 256 seeded 8x8
+    boards), gcd 2 (tetrominoes with hexominoes) and gcd 1 (trominoes with
+    tetrominoes)."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+
+    def stats_of(k, make):
+        r = make(k)
+        st = r.stats()
+        return st["children"], st["pruned"]
+
+    def batch(k):
+        lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+        lib.set_kernel(k, 256)
+        b = gpu.Batch(lib, fx["boards"])
+        b.run()
+        assert b.counts() == fx["counts"]
+        return b
+
+    dfs, lvl = stats_of(gpu.KERNEL_DFS, batch), stats_of(gpu.KERNEL_LEVEL, batch)
+    assert dfs == lvl and lvl[1] > 0, (dfs, lvl)
+    p4, p6, p3 = gpu.free_polyominoes(4), gpu.free_polyominoes(6), gpu.free_polyominoes(3)
+    for shapes, counts, w in ((p4 + p6[:2], [1] * 5 + [1, 1], 8),
+                              (p3 + p4, [2, 2, 1, 1, 1, 1, 1], 6)):
+        area = sum(bin(x).count("1") * c for x, c in zip(shapes, counts))
+        h = -(-area // w)
+        rows = [(1 << w) - 1] * h
+        rows[-1] &= ((1 << w) - 1) >> (w * h - area)
+        want = oracle.count(shapes, counts, rows)
+
+        def one(k, shapes=shapes, counts=counts, w=w, rows=rows, want=want):
+            lib = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+            lib.set_kernel(k, 256)
+            sr = gpu.CudaSearcher(lib, rows=rows)
+            sr.run()
+            assert sr.count() == want
+            return sr
+
+        dfs, lvl = stats_of(gpu.KERNEL_DFS, one), stats_of(gpu.KERNEL_LEVEL, one)
+        assert dfs == lvl, (shapes, dfs, lvl)
+
+
 @pytest.mark.parametrize("kernel", ["dfs", "expand", "level"])
 def test_row_step_kernels_agree(gpu, p5, oracle, kernel):
     """The three count-mode row-step kernels (depth-first per lane, partial
