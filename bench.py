@@ -196,11 +196,12 @@ def serve_section(d4, lib, workloads, rates, n_req, slots):
     s = None
     out["plain_requests_per_s"] = plain_rps
     bands = ((30, 39), (40, 49), (50, 64))
-    # untimed warmup at the highest rate: the server's buffers grow to their
+    # untimed warmup, the whole stream at the highest rate (all its requests
+    # in one burst: the largest batches): the server's buffers grow to their
     # working sizes once, and later servers take them from the allocator's
     # cache instead of allocating inside the measured stream
     if rates:
-        warm = workloads.request_stream(min(200, n_req), rate=max(rates), seed=6)
+        warm = workloads.request_stream(n_req, rate=max(rates), seed=6)
         t = time.perf_counter()
         serve_run(d4, lib, warm, slots)
         log(f"serving: warmup of {len(warm)} requests in {time.perf_counter() - t:.2f} s")
