@@ -796,6 +796,11 @@ def main():
         out["step_modes"] = step_modes_section(d4, workloads, boards, counts, args.serve_slots)
 
     if args.uneven:
+        # (a clean device for the multi-library batch: it splits its frontier
+        # when an allocation fails, and blocks cached by the sections before
+        # it, sized for their own buffers, can leave too little room)
+        freed = d4.release_cached()
+        log(f"uneven: {freed / 2**30:.1f} GiB of cached device memory released first")
         out["uneven"] = uneven_section(d4, workloads, args.uneven, args.uneven_budget,
                                        args.uneven_budget2, args.uneven_plain,
                                        tune=not args.no_tune)
