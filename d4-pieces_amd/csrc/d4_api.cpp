@@ -1230,7 +1230,7 @@ struct Engine {
         if (g_part) {
             const uint64_t nm = std::min<uint64_t>(g_ch, PART_MAX_CHILDREN);
             uint64_t h1 = 0, h2 = 0;
-            part_hist_entries(nm, h1, h2);
+            part_hist_entries(nm, part_tile_min(), part_lp_max(), h1, h2);
             pbuf.reserve(2 * nm);
             sbuf.reserve(2 * nm);
             pmask.reserve(part_mask_words(nm));
@@ -1381,7 +1381,7 @@ struct Engine {
         if (!mem_cap) return true;
         const uint64_t nm = std::min<uint64_t>(chc, PART_MAX_CHILDREN);
         uint64_t h1 = 0, h2 = 0;
-        part_hist_entries(nm, h1, h2);
+        part_hist_entries(nm, part_tile_min(), part_lp_max(), h1, h2);
         return 32ull * nm + nm / 8 + 4ull * (h1 + h2) + 16ull * out_cap + 16ull * nf +
                    level_bytes(nf) + parked_bytes <= mem_cap;
     }
@@ -1406,8 +1406,11 @@ struct Engine {
     void part_args(const DedupArgs &d, uint64_t chc, uint64_t *ok, uint64_t *oc, uint64_t out_cap,
                    PartArgs &pa) {
         const uint64_t nm = std::min<uint64_t>(chc, PART_MAX_CHILDREN);
+        // (the bucket shape is read once: the buffers are sized for it)
+        const uint64_t tile_min = part_tile_min();
+        const uint32_t lp_max = part_lp_max();
         uint64_t h1 = 0, h2 = 0;
-        part_hist_entries(nm, h1, h2);
+        part_hist_entries(nm, tile_min, lp_max, h1, h2);
         pbuf.reserve(2 * nm);
         sbuf.reserve(2 * nm);
         pmask.reserve(part_mask_words(nm));
@@ -1421,8 +1424,8 @@ struct Engine {
         pa.ch_cnt = ch_c.p;
         pa.ch_cap = chc;
         pa.n_max = std::min<uint64_t>(nm, part_n_cap);
-        pa.tile_min = part_tile_min();
-        pa.lp_max = part_lp_max();
+        pa.tile_min = tile_min;
+        pa.lp_max = lp_max;
         pa.slots = part_slots();
         pa.mask = pmask.p;
         pa.pbuf = pbuf.p;

@@ -258,7 +258,8 @@ struct PartArgs {
     unsigned long long *c64;
     const RowCtl *ctl;       // set: row comes from it (graph replay)
 };
-void part_hist_entries(uint64_t n_max, uint64_t &level1, uint64_t &level2);
+void part_hist_entries(uint64_t n_max, uint64_t tile_min, uint32_t lp_max, uint64_t &level1,
+                       uint64_t &level2);
 uint64_t part_bsum_entries();
 uint64_t part_mask_words(uint64_t n_max);
 uint64_t part_tile_min();

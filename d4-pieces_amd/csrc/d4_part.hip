@@ -690,35 +690,30 @@ static void launch_merge(const PartArgs &a, uint32_t P, int n_cu, hipStream_t st
 
 }  // namespace
 
+// The bucket shape, read from the environment at each call (a row step
+// reads it once, in part_args): tests reach the merge's multi-pass and
+// small-table paths on small searches.
 uint32_t part_lp_max() {
-    static const uint32_t l = [] {
-        const char *v = getenv("D4_PART_LP");  // log2 of the most buckets
-        const int x = v ? atoi(v) : 14;
-        return (uint32_t)(x >= LP1_MAX && x <= (int)LP_MAX ? x : 14);
-    }();
-    return l;
+    const char *v = getenv("D4_PART_LP");  // log2 of the most buckets
+    const int x = v ? atoi(v) : 14;
+    return (uint32_t)(x >= LP1_MAX && x <= (int)LP_MAX ? x : 14);
 }
 
 uint32_t part_slots() {
-    static const uint32_t l = [] {
-        const char *v = getenv("D4_PART_SLOTS");  // LDS table slots of the merge
-        return (uint32_t)(v && atoi(v) == 4096 ? 4096 : 8192);
-    }();
-    return l;
+    const char *v = getenv("D4_PART_SLOTS");  // LDS table slots of the merge
+    return (uint32_t)(v && atoi(v) == 4096 ? 4096 : 8192);
 }
 
 uint64_t part_tile_min() {
-    static const uint64_t t = [] {
-        const char *v = getenv("D4_PART_TILE");  // log2 of the smallest level-1 tile
-        const int l = v ? atoi(v) : 0;
-        return l >= 12 && l <= 24 ? (uint64_t)1 << l : PT_TILE;
-    }();
-    return t;
+    const char *v = getenv("D4_PART_TILE");  // log2 of the smallest level-1 tile
+    const int l = v ? atoi(v) : 0;
+    return l >= 12 && l <= 24 ? (uint64_t)1 << l : PT_TILE;
 }
 
-void part_hist_entries(uint64_t ch_cap, uint64_t &h1, uint64_t &h2) {
+void part_hist_entries(uint64_t ch_cap, uint64_t tile_min, uint32_t lp_max, uint64_t &h1,
+                       uint64_t &h2) {
     uint32_t P, P1, T1, T2;
-    part_bounds(std::min<uint64_t>(ch_cap, PART_MAX_CHILDREN), part_tile_min(), part_lp_max(), P, P1, T1, T2);
+    part_bounds(std::min<uint64_t>(ch_cap, PART_MAX_CHILDREN), tile_min, lp_max, P, P1, T1, T2);
     h1 = (uint64_t)P1 * T1;
     h2 = (uint64_t)P * T2;
 }

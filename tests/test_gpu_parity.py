@@ -619,6 +619,32 @@ def test_level_prune_matches_dedup_prune(gpu, p5, monkeypatch):
         assert abs(s1[k] - s0[k]) <= 0.01 * s0[k], (k, s1, s0)
 
 
+@pytest.mark.parametrize("lp,slots,tile", [(8, 8192, 16), (8, 4096, 16), (16, 4096, 12)])
+def test_part_dedup_shapes(gpu, p5, monkeypatch, lp, slots, tile):
+    """Bucket shapes far from the default, every row through the buckets:
+    2^8 buckets (about 90 Ki survivors each on the large rows of the 256
+    seeded 8x8 boards, so the merge cuts each into passes over sub-ranges of
+    the hash, with 8 Ki- and 4 Ki-slot LDS tables) and 2^16 buckets with
+    4 Ki-slot tables and 4 Ki-child first-pass tiles.  Counts equal the
+    fixture's, with no fallback to the table."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "seeded_8x8_seed2.json")))
+    monkeypatch.setenv("D4_PART_MIN_CHILDREN", "0")
+    monkeypatch.setenv("D4_PART_LP", str(lp))
+    monkeypatch.setenv("D4_PART_SLOTS", str(slots))
+    monkeypatch.setenv("D4_PART_TILE", str(tile))
+    for mode in (gpu.STEP_FUSED, gpu.STEP_GRAPH):
+        lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+        lib.set_step_mode(mode)
+        b = gpu.Batch(lib, fx["boards"])
+        b.run()
+        assert b.counts() == fx["counts"], (lp, slots, tile, mode)
+        st = b.stats()
+        assert st["part_steps"] > 0 and st["part_fallbacks"] == 0, st
+
+
 @pytest.mark.parametrize("mode", ["fused", "graph"])
 def test_part_dedup_recovery_paths(gpu, p5, monkeypatch, mode):
     """The partitioned dedup's two ways out, forced on every row of the 256
