@@ -19,13 +19,15 @@
 //   k_part_scan*    exclusive scan of the counts in bucket-major order: every
 //                   (bucket, tile) pair gets its range of the output
 //   k_part_scatter1 per tile: each survivor to its coarse bucket's range (LDS
-//                   cursors), {key, count} pairs written with 16-byte stores
+//                   cursors); each block of 4 Ki {key, count} records is
+//                   first sorted by bucket in LDS, so that a wave's 16-byte
+//                   stores form runs of whole lines
 //   k_part_hist2 /  the same within each coarse bucket, by the next bits of
-//   k_part_scatter2 the hash, back into the children buffers: a radix pass
-//                   of two digits, so that a tile's survivors of one bucket
-//                   land as a run of consecutive records (one digit of 14
-//                   bits would scatter them as lone 16-byte stores, which
-//                   cost as much as random atomics)
+//   k_part_scatter2 the hash, into a second buffer: a radix pass of two
+//                   digits, so that a tile's survivors of one bucket land as
+//                   a run of consecutive records (one digit of 14 bits would
+//                   scatter them as lone 16-byte stores, which cost as much
+//                   as random atomics)
 //   k_part_merge    as many workgroups as fit at once, each looping over its
 //                   share of the buckets: an LDS hash table of twice the
 //                   pass's survivors (256 to 8 Ki slots, so clearing and
