@@ -144,6 +144,30 @@ def test_edge_cases(gpu, lib8, p5):
     assert gpu_count(gpu, lib8, [0x1F]) == 1
 
 
+@pytest.mark.parametrize("dedup", ["part", "table"])
+def test_edge_cases_every_row_bucketed(gpu, p5, oracle, monkeypatch, dedup):
+    """The edge cases above with every row step's children, however few,
+    through the hash buckets (and, for comparison, through the table), alone
+    and in one batch with ordinary boards: empty rows, rows whose children
+    are all pruned, rows with one child, and boards whose frontier dies, with
+    the oracle's counts."""
+    monkeypatch.setenv("D4_PART_MIN_CHILDREN", "0")
+    lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
+    lib.set_dedup(gpu.DEDUP_PART if dedup == "part" else gpu.DEDUP_TABLE)
+    boards = [rect(3, 3), [0xFF, 0x00, 0xFF], [0x1F], rect(5, 4), rect(5, 6), rect(8, 5)]
+    boards += workloads.seeded_8x8(2, seed=3)
+    want = [oracle.count(p5, [1] * 12, rows) for rows in boards]
+    assert want[:3] == [0, 0, 1]
+    s = gpu.CudaSearcher(lib, rows=[])
+    s.run()
+    assert s.count() == 1
+    for rows, w in zip(boards, want):
+        assert gpu_count(gpu, lib, rows) == w, rows
+    b = gpu.Batch(lib, boards)
+    b.run()
+    assert b.counts() == want
+
+
 def test_next_size_bounds_children(gpu, lib6):
     s = gpu.CudaSearcher(lib6, rows=rect(6, 10))
     while s.get_height():
