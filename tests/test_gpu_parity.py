@@ -368,11 +368,12 @@ def test_calendar_puzzle_batch(gpu, oracle):
 def test_prune_statistics_agree_across_kernels(gpu, p5, oracle):
     """The connectivity prune keeps and drops the same children in the
     depth-first kernel (64-bit flood fill, areas tested by division) and in
-    the level kernels (32-bit flood fill, areas looked up in a mask, closed
-    cells' total tested first): equal children kept and pruned, and the
+    the level kernels (32-bit flood fill for pieces at most 5 rows tall,
+    areas looked up in a mask, closed cells' total tested first), and in the
+    bucket dedup's first pass: equal children kept and pruned, and the
     oracle's counts, for pieces of gcd 5 (pentominoes: the 256 seeded 8x8
-    boards), gcd 2 (tetrominoes with hexominoes) and gcd 1 (trominoes with
-    tetrominoes)."""
+    boards), gcd 2 (tetrominoes with hexominoes, 6 rows tall: the 64-bit
+    fill) and gcd 1 (trominoes with tetrominoes)."""
     import json
     import os
     fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
@@ -412,6 +413,14 @@ def test_prune_statistics_agree_across_kernels(gpu, p5, oracle):
 
         dfs, lvl = stats_of(gpu.KERNEL_DFS, one), stats_of(gpu.KERNEL_LEVEL, one)
         assert dfs == lvl, (shapes, dfs, lvl)
+        # the prune in the bucket dedup's first pass (every row bucketed, the
+        # level kernels not pruning): the 64-bit fill for the hexominoes
+        with pytest.MonkeyPatch.context() as mp:
+            mp.setenv("D4_LEVEL_PRUNE", "0")
+            mp.setenv("D4_PART_MIN_CHILDREN", "0")
+            st = one(gpu.KERNEL_LEVEL).stats()
+        assert st["part_steps"] > 0 and st["part_fallbacks"] == 0, st
+        assert (st["children"], st["pruned"]) == dfs, (shapes, dfs, st)
 
 
 @pytest.mark.parametrize("kernel", ["dfs", "expand", "level"])
