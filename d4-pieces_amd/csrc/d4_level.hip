@@ -130,6 +130,16 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
         a.n_rec = (uint32_t)min((uint64_t)a.ctl->n_front, (uint64_t)a.n_rec);
         a.row = a.ctl->row;
     }
+    // the level's rows, written by the launch before (every thread reads the
+    // same count): an empty level leaves before staging the tables
+    uint64_t n_in;
+    if (FROM_REC) {
+        n_in = a.n_rec;
+    } else {
+        n_in = ctr_load(a.c64 + C64_LVL + a.level - 1);
+        if (n_in > a.in_cap) n_in = a.in_cap;
+    }
+    if (!n_in) return;
     const uint32_t nfit = 8u * a.fit_rows * 256u;
     uint64_t *s_fit = lds;
     const PlaceInfo *s_pc = reinterpret_cast<const PlaceInfo *>(s_fit + nfit);
@@ -168,13 +178,6 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
     uint64_t *st_key = s_stg + (size_t)wv * 128, *st_cnt = st_key + 64;
 
     if (__hip_atomic_load(a.c32 + C32_XOVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    uint64_t n_in;
-    if (FROM_REC) {
-        n_in = a.n_rec;
-    } else {
-        n_in = ctr_load(a.c64 + C64_LVL + a.level - 1);
-        if (n_in > a.in_cap) n_in = a.in_cap;
-    }
     const uint64_t waves = (uint64_t)gridDim.x * (LT / 64);
     const uint64_t wid = (uint64_t)blockIdx.x * (LT / 64) + wv;
     // chunk size: small levels get small chunks (fewer holes)
