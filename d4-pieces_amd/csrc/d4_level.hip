@@ -131,7 +131,9 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
         a.row = a.ctl->row;
     }
     // the level's rows, written by the launch before (every thread reads the
-    // same count): an empty level leaves before staging the tables
+    // same count): a workgroup with no batch of them -- its first row, the
+    // first of its first wave, lies past the level's end -- leaves before
+    // staging the tables
     uint64_t n_in;
     if (FROM_REC) {
         n_in = a.n_rec;
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(LT) void k_level(LevelArgs a) {
         n_in = ctr_load(a.c64 + C64_LVL + a.level - 1);
         if (n_in > a.in_cap) n_in = a.in_cap;
     }
-    if (!n_in) return;
+    if ((uint64_t)blockIdx.x * LT >= n_in) return;
     const uint32_t nfit = 8u * a.fit_rows * 256u;
     uint64_t *s_fit = lds;
     const PlaceInfo *s_pc = reinterpret_cast<const PlaceInfo *>(s_fit + nfit);
