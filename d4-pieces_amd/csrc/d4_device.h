@@ -93,6 +93,62 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, (unsigned)o);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+// Dense output of one stream (wave-uniform state): lanes with `pred` get
+// consecutive positions of the wave's current chunk; a chunk is reserved
+// with one atomic on the stream's u64 counter.  Returns false when the
+// buffer is exhausted (C32_XOVF is set; the host reruns the step larger).
+struct Writer {
+    uint64_t pos = 0, end = 0;
+};
+
+__device__ __forceinline__ bool reserve_chunk(unsigned long long *ctr, uint64_t chunk, uint64_t cap,
+                                              uint32_t *c32, uint64_t &base) {
+    uint64_t b = 0;
+    if (lane_id() == 0) b = atomicAdd(ctr, (unsigned long long)chunk);
+    b = shfl64(b, 0);
+    if (b + chunk > cap) {
+        if (lane_id() == 0) atomicOr(c32 + C32_XOVF, 1u);
+        return false;
+    }
+    base = b;
+    return true;
+}
+
+// Positions for the lanes in m: returns each lane's slot; `ok` false on
+// overflow.  At most one chunk switch per call (a call places <= 64 items
+// and chunks hold >= 64).
+__device__ __forceinline__ uint64_t place(Writer &w, uint64_t m, unsigned long long *ctr,
+                                          uint64_t chunk, uint64_t cap, uint32_t *c32, bool &ok) {
+    const uint32_t n = (uint32_t)__builtin_popcountll(m);
+    const uint32_t q = mbcnt(m);
+    uint64_t slot = w.pos + q;
+    if (w.pos + n > w.end) {
+        uint64_t base = 0;
+        if (!reserve_chunk(ctr, chunk, cap, c32, base)) {
+            ok = false;
+            return 0;
+        }
+        const uint64_t room = w.end - w.pos;  // slots left in the old chunk
+        if (q >= room) slot = base + (q - room);
+        w.pos = base + (n - room);
+        w.end = base + chunk;
+        return slot;
+    }
+    w.pos += n;
+    return slot;
+}
+
+
 constexpr uint64_t COL0 = 0x0101010101010101ull;
 constexpr uint64_t COL7 = 0x8080808080808080ull;
 

@@ -5,7 +5,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -159,13 +162,22 @@ class Pool {
                 }
         }
         void *p = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
         hipError_t e = hipMalloc(&p, bytes);
+        bool trimmed = false;
         if (e == hipErrorOutOfMemory) {
             (void)hipGetLastError();
             std::lock_guard<std::mutex> lk(mu_);
             trim_locked();
+            trimmed = true;
             e = hipMalloc(&p, bytes);
         }
+        static const bool trace = getenv("D4_TRACE") != nullptr;
+        if (trace)
+            fprintf(stderr, "pool: hipMalloc %.3f GiB in %.1f ms%s\n", (double)bytes / (1u << 30),
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                        .count(),
+                    trimmed ? " (cache released first)" : "");
         if (e != hipSuccess)
             throw Error(e == hipErrorOutOfMemory ? D4_ENOMEM : D4_EDEVICE,
                         std::string("hipMalloc: ") + hipGetErrorString(e));

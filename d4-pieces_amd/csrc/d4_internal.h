@@ -327,7 +327,7 @@ struct MultiArgs {
     const uint8_t *dropped;    // [n_inst] nonzero: instance left the search
     unsigned long long *result;
     unsigned long long *tab;   // 4 words per slot: k0, k1, count, unused
-    uint64_t tab_mask;
+    uint64_t tab_size;         // slots (any number; home slot = mulhi(hash, size))
     uint32_t max_probe;
     uint32_t *c32;
     unsigned long long *c64;
@@ -410,6 +410,43 @@ void launch_extend_multi(const MultiArgs &a, bool prune, bool count_only, uint32
                          int n_cu, hipStream_t st);
 void launch_inst_hist(const uint64_t *k0, uint64_t n, uint32_t n_inst, unsigned long long *hist,
                       int n_cu, hipStream_t st);
+
+// Level-synchronous row step of the multi-library batch (d4_mlevel.hip):
+// the single-library level kernels' structure with each row's library read
+// through the caches.  A partial row is {tiling count, covered cells below
+// >> 8 | the row's open cells << 56, counters | instance << 32} (3 u64, SoA);
+// children are {k0, k1, count} SoA, survivors of the prune only (k0 ==
+// EMPTY_KEY: an unused slot), counted in c64[C64_CHW].
+struct MLevelArgs {
+    const uint64_t *in_k0, *in_k1, *in_cnt;  // level 0: the frontier's records
+    uint32_t n_rec;
+    const uint64_t *in_st;                   // levels >= 1: [LVL_FIELDS][in_cap]
+    uint64_t in_cap;
+    uint32_t level;
+    uint64_t *out_st;                        // [LVL_FIELDS][out_cap]
+    uint64_t out_cap;
+    uint64_t *ch_k0, *ch_k1, *ch_cnt;
+    uint64_t ch_cap;
+    const MLib *libs;
+    const uint32_t *inst_lib;
+    const uint64_t *bw;
+    uint32_t bw_stride;
+    const uint32_t *hgt;
+    const uint8_t *dropped;
+    uint32_t row;
+    uint32_t prune;
+    uint32_t *c32;
+    unsigned long long *c64;
+};
+void launch_mlevel(const MLevelArgs &a, int n_cu, hipStream_t st);
+// Terminal records of the frontier (instances finished at this row): their
+// counts to the results and the sink (a's table is not used).
+void launch_mterminal(const MultiArgs &a, int n_cu, hipStream_t st);
+// The children of a completed chunk into a's table (count_only: counted);
+// nothing after a level overflow.
+void launch_mdedup(const MultiArgs &a, const uint64_t *ch_k0, const uint64_t *ch_k1,
+                   const uint64_t *ch_cnt, uint64_t ch_cap, bool count_only, int n_cu,
+                   hipStream_t st);
 void launch_remap_edges(const uint64_t *e_slot, const uint32_t *slot_uid,
                         uint32_t *e_child, uint32_t n, hipStream_t st);
 // out[0..255] = records per current-row pattern, out[256..511] = their

@@ -45,61 +45,6 @@ constexpr int LT = 1024;          // threads per workgroup (16 waves: 4 per SIMD
 constexpr uint32_t PCAP = 1024;   // listed (row, candidate) pairs per wave and pass
 constexpr uint32_t OWN = 5;       // per-wave LDS words per loaded row: E, cov, cnt, ui, avail
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, (unsigned)o);
-        if (lane >= (uint32_t)o) v += u;
-    }
-    return v;
-}
-
-// Dense output of one stream (wave-uniform state): lanes with `pred` get
-// consecutive positions of the wave's current chunk; a chunk is reserved
-// with one atomic on the stream's u64 counter.  Returns false when the
-// buffer is exhausted (C32_XOVF is set; the host reruns the step larger).
-struct Writer {
-    uint64_t pos = 0, end = 0;
-};
-
-__device__ __forceinline__ bool reserve_chunk(unsigned long long *ctr, uint64_t chunk, uint64_t cap,
-                                              uint32_t *c32, uint64_t &base) {
-    uint64_t b = 0;
-    if (lane_id() == 0) b = atomicAdd(ctr, (unsigned long long)chunk);
-    b = shfl64(b, 0);
-    if (b + chunk > cap) {
-        if (lane_id() == 0) atomicOr(c32 + C32_XOVF, 1u);
-        return false;
-    }
-    base = b;
-    return true;
-}
-
-// Positions for the lanes in m: returns each lane's slot; `ok` false on
-// overflow.  At most one chunk switch per call (a call places <= 64 items
-// and chunks hold >= 64).
-__device__ __forceinline__ uint64_t place(Writer &w, uint64_t m, unsigned long long *ctr,
-                                          uint64_t chunk, uint64_t cap, uint32_t *c32, bool &ok) {
-    const uint32_t n = (uint32_t)__builtin_popcountll(m);
-    const uint32_t q = mbcnt(m);
-    uint64_t slot = w.pos + q;
-    if (w.pos + n > w.end) {
-        uint64_t base = 0;
-        if (!reserve_chunk(ctr, chunk, cap, c32, base)) {
-            ok = false;
-            return 0;
-        }
-        const uint64_t room = w.end - w.pos;  // slots left in the old chunk
-        if (q >= room) slot = base + (q - room);
-        w.pos = base + (n - room);
-        w.end = base + chunk;
-        return slot;
-    }
-    w.pos += n;
-    return slot;
-}
-
 // The board window of row + 1 of instance i (0 past the instance's rows,
 // as the dedup stages it).
 __device__ __forceinline__ uint64_t child_window(const LevelArgs &a, uint32_t i) {

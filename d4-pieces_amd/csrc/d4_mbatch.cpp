@@ -195,6 +195,7 @@ struct d4_mbatch {
     uint64_t sink_n = 0, sink_cap = 0, sink_init = 0;  // entries
     std::vector<uint64_t> h_sink;
     static constexpr uint64_t kExactCountMax = 1u << 16;
+    static constexpr uint64_t kSampleCount = 1u << 20;  // parents of a sampled count
     bool done = false;
     bool trace = getenv("D4_TRACE") != nullptr;  // per-row lines on stderr
     d4_stats stats{};
@@ -248,7 +249,7 @@ struct d4_mbatch {
         a.hgt = hgt.p;
         a.result = result.p;
         a.tab = tab.p;
-        a.tab_mask = tab_m - 1;
+        a.tab_size = tab_m;
         a.max_probe = 256;
         a.c32 = c32.p;
         a.c64 = c64.p;
@@ -309,10 +310,131 @@ struct d4_mbatch {
         }
     }
 
-    // Children the frontier will emit after pruning (search only).
-    uint64_t count_children(uint32_t row) {
+    // ---- level kernels (D4_KERNEL_LEVEL, the default) ---------------------
+    // The row step as the single-library level kernels run it: one launch
+    // per placement level over the frontier (in chunks of at most 8 Mi
+    // parents whose buffers fit an eighth of the memory cap), each chunk's
+    // surviving children inserted
+    // after its levels; instances finishing at this row counted first by a
+    // pass of their own.  A chunk whose buffers overflow is rerun alone with
+    // four times the room (the chunks before it are complete and inserted).
+    bool use_level = true;  // D4_KERNEL_LEVEL (false: D4_KERNEL_DFS)
+    DevBuf<uint64_t> lv_a, lv_b, ch0, ch1, chc;
+    // partial rows of the widest level, and children, per parent
+    double lv_pp = 8.0, ch_pp = 16.0;
+    uint64_t level_budget() const { return mem_cap / 8; }
+    // bytes per parent of the level buffers (two of LVL_FIELDS words per
+    // partial row of the widest level, three words per child, 1.5x room)
+    double level_bytes_pp(double lpp, double cpp) const {
+        return 8.0 * 1.5 * (2.0 * LVL_FIELDS * lpp + 3.0 * cpp);
+    }
+    // Parents per chunk: within the budget, and at most 8 Mi (far more rows
+    // than the level kernels need to fill the device; larger chunks only
+    // make the buffers larger)
+    uint64_t level_chunk(uint64_t n, double per) const {
+        const uint64_t fit = (uint64_t)((double)level_budget() / per);
+        return std::max<uint64_t>(1, std::min<uint64_t>({n, fit, 8ull << 20}));
+    }
+    // the level buffers a row step of n parents plans with (their floor of
+    // 64 Ki entries each is left out: splitting the frontier cannot shrink it)
+    uint64_t level_need(uint64_t n) const {
+        if (!n) return 0;
+        const double per = level_bytes_pp(lv_pp, ch_pp);
+        return (uint64_t)(per * (double)level_chunk(n, per));
+    }
+
+    // (nf: the first nf parents of the frontier; a count pass may take a
+    // prefix, which is a uniform sample since the frontier is in hash order)
+    void extend_levels(uint32_t row, uint64_t tab_m, bool prune, bool count_only, uint64_t nf) {
+        const MultiArgs m = args(row, tab_m);
+        if (!count_only && n_front) launch_mterminal(m, n_cu, st);
+        if (!nf) return;
+        MLevelArgs a{};
+        a.libs = libs.p;
+        a.inst_lib = inst_lib.p;
+        a.bw = bw.p;
+        a.bw_stride = bw_stride;
+        a.hgt = hgt.p;
+        a.dropped = budget ? dropped.p : nullptr;
+        a.row = row;
+        a.prune = prune;
+        a.c32 = c32.p;
+        a.c64 = c64.p;
+        double lpp = lv_pp, cpp = ch_pp;
+        uint64_t b0 = 0;
+        for (uint32_t reruns = 0;;) {
+            const double per = level_bytes_pp(lpp, cpp);
+            const uint64_t rest = nf - b0;
+            const uint64_t chunk = level_chunk(rest, per);
+            // (sized exactly, the device pool rounds to 2 MiB; at least 64 Ki
+            // entries, room for the waves' partly filled write reservations)
+            const uint64_t lcap =
+                std::max<uint64_t>((uint64_t)((double)chunk * lpp * 1.5), 1u << 16);
+            const uint64_t ccap =
+                std::max<uint64_t>((uint64_t)((double)chunk * cpp * 1.5), 1u << 16);
+            lv_a.reserve(LVL_FIELDS * lcap);
+            lv_b.reserve(LVL_FIELDS * lcap);
+            ch0.reserve(ccap);
+            ch1.reserve(ccap);
+            chc.reserve(ccap);
+            a.in_cap = a.out_cap = lcap;
+            a.ch_k0 = ch0.p;
+            a.ch_k1 = ch1.p;
+            a.ch_cnt = chc.p;
+            a.ch_cap = ccap;
+            const uint32_t nch = (uint32_t)((rest + chunk - 1) / chunk);
+            launch_reset_levels(c32.p, c64.p, true, st);
+            for (uint32_t j = 0; j < nch; j++) {
+                const uint64_t b = b0 + (uint64_t)j * chunk;
+                if (j) launch_reset_levels(c32.p, c64.p, false, st);
+                a.in_k0 = f0.p + b;
+                a.in_k1 = f1.p + b;
+                a.in_cnt = fc.p + b;
+                a.n_rec = (uint32_t)std::min<uint64_t>(chunk, nf - b);
+                for (uint32_t L = 0; L < LVL_MAX; L++) {
+                    a.level = L;
+                    a.in_st = (L & 1) ? lv_a.p : lv_b.p;
+                    a.out_st = (L & 1) ? lv_b.p : lv_a.p;
+                    launch_mlevel(a, n_cu, st);
+                }
+                launch_mdedup(m, ch0.p, ch1.p, chc.p, ccap, count_only, n_cu, st);
+            }
+            HIPCHK(hipGetLastError());
+            uint32_t h32[C32_N];
+            unsigned long long h64[C64_N];
+            HIPCHK(hipMemcpyAsync(h32, c32.p, sizeof h32, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (h32[C32_OVERFLOW]) return;  // the table: the step is rerun larger
+            const uint32_t done = std::min<uint32_t>(h32[C32_LVDONE], nch);
+            const uint64_t c_last = std::min<uint64_t>(chunk, nf - (b0 + (uint64_t)(nch - 1) * chunk));
+            if (!h32[C32_XOVF]) {
+                // sizes per parent of the last chunk, for the next steps
+                if (c_last >= 4096) {
+                    uint64_t mx = 0;
+                    for (uint32_t L = 0; L < LVL_MAX; L++) mx = std::max<uint64_t>(mx, h64[C64_LVL + L]);
+                    lv_pp = std::max(1.0, (double)mx / (double)c_last);
+                    ch_pp = std::max(1.0, (double)h64[C64_CHW] / (double)c_last);
+                }
+                return;
+            }
+            stats.chunk_reruns++;
+            b0 += (uint64_t)done * chunk;
+            const uint64_t c_fail = std::min<uint64_t>(chunk, nf - b0);
+            bool lv_over = false;
+            for (uint32_t L = 0; L < LVL_MAX; L++) lv_over |= h64[C64_LVL + L] > lcap;
+            if (lv_over) lpp = 4.0 * std::max(lpp, (double)lcap / (double)c_fail);
+            if (h64[C64_CHW] > ccap || !lv_over) cpp = 4.0 * std::max(cpp, (double)ccap / (double)c_fail);
+            if (++reruns > 32) throw Error(D4_ERANGE, "a row step's partial rows outgrow every buffer");
+        }
+    }
+
+    // Children the frontier (the level kernels: its first nf parents) will
+    // emit after pruning (search only).
+    uint64_t count_children(uint32_t row, uint64_t nf) {
         launch_reset_counters(c32.p, c64.p, st);
-        extend(row, 1, !(flags & D4_FLAG_NO_PRUNE), true);
+        if (use_level) extend_levels(row, 1, !(flags & D4_FLAG_NO_PRUNE), true, nf);
+        else extend(row, 1, !(flags & D4_FLAG_NO_PRUNE), true);
         unsigned long long h64[C64_N];
         HIPCHK(hipMemcpyAsync(h64, c64.p, sizeof h64, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -324,7 +446,8 @@ struct d4_mbatch {
     uint64_t need(uint64_t tab_m, uint64_t u) const {
         // (the terminal-state sink, 16 B per state kept, is output and
         // outside the cap: splitting the frontier would not shrink it)
-        return 32ull * tab_m + 24ull * u + 24ull * n_front + parked_bytes;
+        return 32ull * tab_m + 24ull * u + 24ull * n_front + parked_bytes +
+               (use_level ? level_need(n_front) : 0);
     }
 
     // Park the upper half of the frontier (resumed at this row later).
@@ -381,16 +504,24 @@ struct d4_mbatch {
         if (n_front > 0xffffffffull) throw Error(D4_ERANGE, "frontier exceeds 2^32 records");
         const bool prune = !(flags & D4_FLAG_NO_PRUNE);
         // Table size: unique children <= children.  A small frontier gets an
-        // exact children count from a search-only pass; a large one an
-        // estimate from the previous step's ratios (first rows grow by
-        // thousands per record, later ones by a few), with a rerun at 4x the
-        // size if the table still overflows.  (The estimate errs large after
-        // the first rows; sized from a sampled count instead, the tables are
-        // 4x smaller but inserts collide more on shared lines, and the step
-        // is no faster.)
+        // exact children count from a search-only pass.  On the level
+        // kernels a large one gets the children of its first 1 Mi parents
+        // counted (a uniform sample: the frontier is in hash order), scaled;
+        // the count pass also sizes the level buffers.  On the depth-first
+        // kernel a large one gets an estimate from the previous step's ratios
+        // (first rows grow by thousands per record, later ones by a few).
+        // Either way the step is rerun at 4x the size if the table still
+        // overflows.  (The ratio estimate errs large, up to 20x on the
+        // uneven puzzles' early rows: such tables crowd the device, and an
+        // allocation that must first release the cached blocks waits
+        // seconds for the memory.)
         uint64_t est_u;
         if (n_front <= kExactCountMax) {
-            est_u = count_children(row);
+            est_u = count_children(row, n_front);
+        } else if (use_level) {
+            const uint64_t ns = std::min<uint64_t>(n_front, kSampleCount);
+            const uint64_t c = count_children(row, ns);
+            est_u = (uint64_t)((double)c * ((double)n_front / (double)ns) * 1.1) + 4096;
         } else {
             const uint64_t est_c =
                 (uint64_t)((double)n_front * std::min(std::max(grow_children, 1.0), 64.0) * 1.5) +
@@ -398,7 +529,9 @@ struct d4_mbatch {
             est_u = std::min<uint64_t>(
                 est_c, (uint64_t)((double)n_front * std::max(grow_unique, 0.25) * 1.5) + 4096);
         }
-        uint64_t tab_m = next_pow2(std::max<uint64_t>(2 * est_u, 1024));
+        // (a table has any number of slots: the sampled count sizes it to
+        // its estimate; the ratio estimate keeps a power of two's slack)
+        uint64_t tab_m = use_level ? round_slots(2 * est_u) : next_pow2(std::max<uint64_t>(2 * est_u, 1024));
         uint32_t h32[C32_N];
         unsigned long long h64[C64_N];
         bool trend = false;
@@ -413,7 +546,8 @@ struct d4_mbatch {
                     trend = true;
                     const double r = unique_raw * unique_raw / prev_unique_raw;
                     const uint64_t u2 = (uint64_t)((double)n_front * r * 1.5) + 4096;
-                    const uint64_t m2 = next_pow2(std::max<uint64_t>(2 * u2, 1024));
+                    const uint64_t m2 =
+                        use_level ? round_slots(2 * u2) : next_pow2(std::max<uint64_t>(2 * u2, 1024));
                     if (m2 < tab_m) {
                         tab_m = m2;
                         est_u = std::min(est_u, u2);
@@ -435,7 +569,8 @@ struct d4_mbatch {
             launch_reset_counters(c32.p, c64.p, st);
             HIPCHK(hipEventRecord(evx0, st));
             if (n_front) {
-                extend(row, tab_m, prune, false);
+                if (use_level) extend_levels(row, tab_m, prune, false, n_front);
+                else extend(row, tab_m, prune, false);
                 HIPCHK(hipGetLastError());
             }
             HIPCHK(hipEventRecord(evx1, st));
@@ -664,6 +799,7 @@ int d4_mbatch_tune(d4_mbatch *b, uint32_t sample, uint32_t *threads, uint32_t *r
                 s->mem_cap = b->mem_cap;
                 s->threads = t;
                 s->refill = r;
+                s->use_level = false;  // the launch shape is the depth-first kernel's
                 s->run();
                 const double ms = s->stats.extend_ms;
                 if (best < 0 || ms < best) {
@@ -799,6 +935,15 @@ int d4_mbatch_set_donation_capacity(d4_mbatch *b, uint32_t items) {
         if (!b) throw Error(D4_EINVAL, "null batch");
         if (items < 64) throw Error(D4_EINVAL, "donation capacity below 64 items");
         b->don_items = items;
+    });
+}
+
+int d4_mbatch_set_kernel(d4_mbatch *b, uint32_t kernel) {
+    return guard([&] {
+        if (!b) throw Error(D4_EINVAL, "null batch");
+        if (kernel != D4_KERNEL_LEVEL && kernel != D4_KERNEL_DFS)
+            throw Error(D4_EINVAL, "a multi-library batch runs the level or the depth-first kernel");
+        b->use_level = kernel == D4_KERNEL_LEVEL;
     });
 }
 

@@ -129,6 +129,7 @@ SIGNATURES = [
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("d4_searcher_next_size_bucket", ctypes.c_uint64, [_P, ctypes.c_uint]),
     ("d4_mbatch_set_donation_capacity", ctypes.c_int, [_P, ctypes.c_uint32]),
+    ("d4_mbatch_set_kernel", ctypes.c_int, [_P, ctypes.c_uint32]),
     ("d4_searcher_count_noalloc", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_uint32,
                                                  ctypes.c_uint32, ctypes.c_void_p]),
     ("d4_library_get_kernel", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
@@ -670,7 +671,7 @@ class MultiBatch:
     together in one frontier (d4_mbatch_*); count mode."""
 
     def __init__(self, libraries: Sequence[PieceLibrary], boards: Sequence[Sequence[int]],
-                 prune: bool = True, host_tables: bool = False):
+                 prune: bool = True, host_tables: bool = False, kernel: Optional[int] = None):
         lib = load()
         if len(libraries) != len(boards):
             raise ValueError("one library per board")
@@ -683,6 +684,8 @@ class MultiBatch:
         _check(lib.d4_mbatch_create(self.n, handles, _u8arr(flat), stride, _u32arr(heights),
                                     flags, ctypes.byref(h)))
         self._h = h
+        if kernel is not None:
+            self.set_kernel(kernel)
 
     def tables(self, inst: int) -> bytes:
         """Instance inst's library tables as the batch holds them on the device."""
@@ -698,6 +701,11 @@ class MultiBatch:
 
     def set_launch(self, threads: int, refill: int = 16) -> None:
         _check(load().d4_mbatch_set_launch(self._h, threads, refill))
+
+    def set_kernel(self, kernel: int) -> None:
+        """Row-step kernel: KERNEL_LEVEL (default) or KERNEL_DFS (the one
+        set_launch, tune, set_budget_steps and set_donation_capacity shape)."""
+        _check(load().d4_mbatch_set_kernel(self._h, kernel))
 
     def set_memory(self, nbytes: int) -> None:
         _check(load().d4_mbatch_set_memory(self._h, nbytes))

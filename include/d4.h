@@ -417,6 +417,19 @@ int d4_mbatch_set_budget_steps(d4_mbatch *b, uint32_t steps);
 /* Items per donation buffer (default 1 Mi; at least 64), as
  * d4_library_set_donation_capacity. */
 int d4_mbatch_set_donation_capacity(d4_mbatch *b, uint32_t items);
+/* Kernel of the batch's row steps (replacing tiled_row_search,
+ * src/kernel.cuh:79-160, for instances of different libraries):
+ *   D4_KERNEL_LEVEL (the default): level-synchronous -- one launch per
+ *     placement level over every partial row of the frontier (each row with
+ *     its instance's library), children written densely and pruned per
+ *     wave, then a separate insert into the two-word dedup table; the
+ *     frontier goes through in chunks sized to an eighth of the memory cap.
+ *   D4_KERNEL_DFS: one depth-first search per lane with donation of the
+ *     rest of a long search to follow-up launches (d4_mbatch_set_budget_steps,
+ *     d4_mbatch_set_donation_capacity; d4_mbatch_set_launch and
+ *     d4_mbatch_tune set its launch shape).
+ * Counts are identical; D4_KERNEL_EXPAND is refused (D4_EINVAL). */
+int d4_mbatch_set_kernel(d4_mbatch *b, uint32_t kernel);
 /* Device bytes the frontier, its dedup table and parked frontier parts may
  * take (default: 80% of the free device memory when the batch is created).
  * A row step that would exceed it parks half of the frontier and finishes

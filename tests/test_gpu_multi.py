@@ -36,7 +36,14 @@ def uneven_fixture():
     return json.load(open(os.path.join(GOLDEN, "uneven_4096_seed3.json")))
 
 
-def run_multi(gpu, insts, prune=True, cache=None):
+@pytest.fixture(params=["level", "dfs"])
+def mk(request, gpu):
+    """Both row-step kernels of a multi-library batch: level-synchronous
+    (the default) and depth-first per lane."""
+    return {"level": gpu.KERNEL_LEVEL, "dfs": gpu.KERNEL_DFS}[request.param]
+
+
+def run_multi(gpu, insts, prune=True, cache=None, kernel=None):
     cache = {} if cache is None else cache
     libs = []
     for shapes, counts, rows, w in insts:
@@ -44,12 +51,12 @@ def run_multi(gpu, insts, prune=True, cache=None):
         if key not in cache:
             cache[key] = gpu.PieceLibrary(shapes, counts, 0xFF, w)
         libs.append(cache[key])
-    b = gpu.MultiBatch(libs, [x[2] for x in insts], prune=prune)
+    b = gpu.MultiBatch(libs, [x[2] for x in insts], prune=prune, kernel=kernel)
     b.run()
     return b.counts(), b.stats()
 
 
-def test_random_small_mixed(gpu, oracle):
+def test_random_small_mixed(gpu, oracle, mk):
     """Distinct piece sets, multiplicities and widths in one batch."""
     rng = random.Random(31)
     pool = [1, 3, 7, 259] + gpu.free_polyominoes(4) + gpu.free_polyominoes(5)
@@ -60,16 +67,16 @@ def test_random_small_mixed(gpu, oracle):
         w, h = rng.randint(2, 8), rng.randint(1, 6)
         blocked = [(rng.randrange(h), rng.randrange(w)) for _ in range(rng.randint(0, 3))]
         insts.append((shapes, counts, rect(w, h, blocked), w))
-    got, st = run_multi(gpu, insts)
+    got, st = run_multi(gpu, insts, kernel=mk)
     want = [oracle.count(s, c, r) for s, c, r, _ in insts]
     assert got == want
     assert sum(want) > 0 and st["retries"] >= 0
     # pruning changes the work, never the counts
-    got2, _ = run_multi(gpu, insts, prune=False)
+    got2, _ = run_multi(gpu, insts, prune=False, kernel=mk)
     assert got2 == want
 
 
-def test_more_than_64_candidates(gpu, oracle):
+def test_more_than_64_candidates(gpu, oracle, mk):
     """Libraries whose (piece, transform) pairs need a 128-bit mask: 14 free
     hexominoes (subset tilings of small boards)."""
     p6 = gpu.free_polyominoes(6)
@@ -81,7 +88,7 @@ def test_more_than_64_candidates(gpu, oracle):
         insts.append((shapes, [1] * 14, rect(w, h), w))
     libs = [gpu.PieceLibrary(s, c, 0xFF, w) for s, c, _, w in insts]
     assert max(L.n_cand for L in libs) > 64
-    got, _ = run_multi(gpu, insts)
+    got, _ = run_multi(gpu, insts, kernel=mk)
     want = [oracle.count(s, c, r) for s, c, r, _ in insts]
     assert got == want
     assert sum(1 for x in want if x) >= 3
@@ -91,44 +98,44 @@ def test_more_than_64_candidates(gpu, oracle):
         gpu.CudaSearcher(big, rows=rect(big.width, 2))
 
 
-def test_single_library_batch_agrees(gpu):
+def test_single_library_batch_agrees(gpu, mk):
     """The 256 seeded 8x8 boards through the multi-library kernel equal the
     oracle's counts (fixture written by tests/golden/make_seeded_counts.py)."""
     fx = json.load(open(os.path.join(GOLDEN, "seeded_8x8_seed2.json")))
     p5 = gpu.free_polyominoes(5)
     insts = [(p5, [1] * 12, rows, 8) for rows in fx["boards"]]
-    got, st = run_multi(gpu, insts)
+    got, st = run_multi(gpu, insts, kernel=mk)
     assert got == fx["counts"]
     assert st["rows_done"] == 9
 
 
-def test_memory_cap_splits_frontier(gpu):
+def test_memory_cap_splits_frontier(gpu, mk):
     """A memory cap far below the frontier's size parks frontier halves and
     finishes them one after the other: counts are unchanged."""
     fx = json.load(open(os.path.join(GOLDEN, "seeded_8x8_seed2.json")))
     p5 = gpu.free_polyominoes(5)
     lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
     n = 16
-    b = gpu.MultiBatch([lib] * n, fx["boards"][:n])
+    b = gpu.MultiBatch([lib] * n, fx["boards"][:n], kernel=mk)
     b.set_memory(24 << 20)
     b.run()
     assert b.counts() == fx["counts"][:n]
     assert b.stats()["splits"] > 0
 
 
-def test_budget_drops_instances(gpu):
+def test_budget_drops_instances(gpu, mk):
     """Instances whose frontier share exceeds the state budget leave the
     search and are reported unfinished; the others keep exact counts."""
     fx = json.load(open(os.path.join(GOLDEN, "seeded_8x8_seed2.json")))
     p5 = gpu.free_polyominoes(5)
     lib = gpu.PieceLibrary(p5, [1] * 12, 0xFF, 8)
     n = 32
-    b = gpu.MultiBatch([lib] * n, fx["boards"][:n])
+    b = gpu.MultiBatch([lib] * n, fx["boards"][:n], kernel=mk)
     b.run()
     peak, fin = b.instances()
     assert all(fin) and b.counts() == fx["counts"][:n]
     budget = sorted(peak)[n // 2]
-    b2 = gpu.MultiBatch([lib] * n, fx["boards"][:n])
+    b2 = gpu.MultiBatch([lib] * n, fx["boards"][:n], kernel=mk)
     b2.set_budget(budget)
     b2.run()
     peak2, fin2 = b2.instances()
@@ -160,7 +167,7 @@ def test_tune_keeps_counts(gpu, oracle):
     assert b.counts() == [oracle.count(s, c, r) for s, c, r, _ in insts]
 
 
-def test_edge_cases(gpu, oracle):
+def test_edge_cases(gpu, oracle, mk):
     p4 = gpu.free_polyominoes(4)
     insts = [
         ([3], [1], [], 2),                 # empty board: one empty tiling
@@ -170,13 +177,13 @@ def test_edge_cases(gpu, oracle):
         (p4, [1] * 5, rect(4, 5), 4),      # 5 tetrominoes on 4x5
         ([1], [64], rect(8, 8), 8),        # monominoes, 7-bit counter
     ]
-    got, _ = run_multi(gpu, insts)
+    got, _ = run_multi(gpu, insts, kernel=mk)
     want = [oracle.count(s, c, r) for s, c, r, _ in insts]
     assert want[0] == 1 and want[1] == 1 and want[5] == 1
     assert got == want
 
 
-def test_uneven_instances(gpu, uneven, uneven_fixture):
+def test_uneven_instances(gpu, uneven, uneven_fixture, mk):
     """The uneven puzzles (first 1024 of the 4096) in one batch with a state
     budget of 2^20 per instance: every instance that finishes and that the
     oracle finished within its node budget (fixture written by
@@ -199,7 +206,7 @@ def test_uneven_instances(gpu, uneven, uneven_fixture):
             if key not in cache:
                 cache[key] = gpu.PieceLibrary(shapes, counts, 0xFF, w)
             libs.append(cache[key])
-        b = gpu.MultiBatch(libs, [x[2] for x in batch])
+        b = gpu.MultiBatch(libs, [x[2] for x in batch], kernel=mk)
         b.set_budget(budget)
         b.run()
         return b.counts(), b.instances()[1]
@@ -233,7 +240,7 @@ def test_donation_buffer_full(gpu, uneven, uneven_fixture):
         if key not in cache:
             cache[key] = gpu.PieceLibrary(shapes, counts, 0xFF, w)
         libs.append(cache[key])
-    b = gpu.MultiBatch(libs, [x[2] for x in insts])
+    b = gpu.MultiBatch(libs, [x[2] for x in insts], kernel=gpu.KERNEL_DFS)
     b.set_budget(budget)
     b.set_budget_steps(8)
     b.set_donation_capacity(64)
@@ -288,3 +295,38 @@ def test_device_tables_equal_host_tables(gpu, uneven, oracle):
     assert bd.counts() == bh2.counts()
     for (s, c, _, w, _), rows, n in zip(small, sb, bd.counts()):
         assert n == oracle.count(s, c, rows)
+
+
+def test_kernels_agree_on_work(gpu, uneven, uneven_fixture):
+    """The level and depth-first kernels walk the same search: on 512 uneven
+    puzzles with a state budget, the same instances finish with the same
+    counts (the oracle's where it has them), and the rows extended, children
+    kept, children pruned and unique states match exactly.  The expand
+    kernel is refused."""
+    n, budget = 512, 1 << 20
+    insts = uneven[:n]
+    cache, libs = {}, []
+    for shapes, counts, _, w in insts:
+        key = (tuple(shapes), tuple(counts), w)
+        if key not in cache:
+            cache[key] = gpu.PieceLibrary(shapes, counts, 0xFF, w)
+        libs.append(cache[key])
+    res = {}
+    for k in (gpu.KERNEL_LEVEL, gpu.KERNEL_DFS):
+        b = gpu.MultiBatch(libs, [x[2] for x in insts], kernel=k)
+        b.set_budget(budget)
+        b.run()
+        res[k] = (b.counts(), b.instances()[1], b.stats())
+        b = None
+    (cl, fl, sl), (cd, fd, sd) = res[gpu.KERNEL_LEVEL], res[gpu.KERNEL_DFS]
+    assert fl == fd and cl == cd
+    assert sl["splits"] == 0 and sd["splits"] == 0
+    for f in ("records_extended", "children", "pruned", "unique", "rows_done"):
+        assert sl[f] == sd[f], f
+    assert sl["donated"] == 0
+    pinned = {int(k): v for k, v in uneven_fixture["counts"].items() if int(k) < n}
+    checked = [i for i in pinned if fl[i]]
+    assert len(checked) >= 150 and all(cl[i] == pinned[i] for i in checked)
+    b = gpu.MultiBatch(libs[:4], [x[2] for x in insts[:4]])
+    with pytest.raises(gpu.D4Error):
+        b.set_kernel(gpu.KERNEL_EXPAND)
