@@ -259,7 +259,7 @@ def multi_libraries(d4, insts):
 
 
 def uneven_section(d4, workloads, n, budget, budget2, plain_sample, tune=True,
-                   pass2_seconds=60.0):
+                   pass2_seconds=60.0, kernel="level"):
     """The uneven puzzles (own piece set and width per instance) in one
     multi-library batch with a per-instance state budget; the instances that
     budget drops run again in a second batch with budget2; the rest is
@@ -278,19 +278,41 @@ def uneven_section(d4, workloads, n, budget, budget2, plain_sample, tune=True,
     t_create_host = time.perf_counter() - t
     bh = None
     t = time.perf_counter()
-    b = d4.MultiBatch(libs, [x[2] for x in insts])
+    kid = d4.KERNEL_LEVEL if kernel == "level" else d4.KERNEL_DFS
+    b = d4.MultiBatch(libs, [x[2] for x in insts], kernel=kid)
     t_create = time.perf_counter() - t
     log(f"uneven: {len(set(map(id, libs)))} libraries built on the host in {t_lib:.2f} s, "
         f"batch created with device-built tables in {t_create:.3f} s "
         f"(host-built tables: {t_create_host:.3f} s)")
     b.set_budget(budget)
-    if tune:
+    # (the launch shape is the depth-first kernel's; the level kernel's
+    # launches are sized from the frontier)
+    tuned = None
+    if kernel == "dfs" and tune:
         t = time.perf_counter()
         tuned = b.tune(64)  # measured launch shape on a sample (outside the timing)
         log(f"uneven autotune {tuned} in {time.perf_counter() - t:.1f} s")
-    else:
+    elif kernel == "dfs":
         tuned = {"threads": 256, "refill": 16}
         b.set_launch(tuned["threads"], tuned["refill"])
+
+    def shape(x):
+        if tuned:
+            x.set_launch(tuned["threads"], tuned["refill"])
+    # an untimed cold pass of the same batch first (reported beside): it
+    # allocates the device blocks of the frontiers, tables and level buffers,
+    # which the timed pass takes from the library's cache (device memory just
+    # released to the driver, as by the cache release before this section,
+    # becomes available to a new allocation only seconds later)
+    bw = d4.MultiBatch(libs, [x[2] for x in insts], kernel=kid)
+    bw.set_budget(budget)
+    shape(bw)
+    t = time.perf_counter()
+    bw.run()
+    d4.synchronize()
+    cold = time.perf_counter() - t
+    bw = None
+    log(f"uneven cold pass (allocating): {cold:.1f} s")
     t = time.perf_counter()
     b.run()
     d4.synchronize()
@@ -300,9 +322,10 @@ def uneven_section(d4, workloads, n, budget, budget2, plain_sample, tune=True,
     st = b.stats()
     b = None
     out = {"instances": n, "state_budget": budget, "finished": sum(fin), "wall_s": wall,
+           "cold_wall_s": cold,
            "library_build_s": t_lib, "batch_create_s": t_create,
            "batch_create_host_tables_s": t_create_host,
-           "launch": tuned,
+           "launch": {"kernel": kernel, **(tuned or {})},
            "records_per_s": st["records_extended"] / wall,
            "solutions_per_s_finished": sum(counts) / wall,
            "records_extended": st["records_extended"], "peak_frontier": st["peak_frontier"],
@@ -321,9 +344,10 @@ def uneven_section(d4, workloads, n, budget, budget2, plain_sample, tune=True,
             if time.perf_counter() - t > pass2_seconds:
                 break
             part = rest[c0:c0 + 128]
-            b2 = d4.MultiBatch([libs[i] for i in part], [insts[i][2] for i in part])
+            b2 = d4.MultiBatch([libs[i] for i in part], [insts[i][2] for i in part],
+                               kernel=kid)
             b2.set_budget(budget2)
-            b2.set_launch(tuned["threads"], tuned["refill"])
+            shape(b2)
             b2.run()
             c2 = b2.counts()
             peak2, fin2 = b2.instances()
@@ -362,9 +386,9 @@ def uneven_section(d4, workloads, n, budget, budget2, plain_sample, tune=True,
         t = time.perf_counter()
         prec = 0
         for i in range(m):
-            pb = d4.MultiBatch([libs[i]], [insts[i][2]])
+            pb = d4.MultiBatch([libs[i]], [insts[i][2]], kernel=kid)
             pb.set_budget(budget)
-            pb.set_launch(tuned["threads"], tuned["refill"])
+            shape(pb)
             pb.run()
             prec += pb.stats()["records_extended"]
         d4.synchronize()
@@ -553,6 +577,9 @@ def main():
                     help="count-mode row-step kernel: level-synchronous (level), partial-row "
                          "snapshots on a per-wave stack (expand) or one depth-first search "
                          "per lane (dfs)")
+    ap.add_argument("--mbatch-kernel", choices=("level", "dfs"), default="level",
+                    help="row-step kernel of the multi-library batches: level-synchronous "
+                         "(level) or one depth-first search per lane (dfs)")
     ap.add_argument("--step-mode", choices=("general", "fused", "graph"), default="fused",
                     help="how count-mode level row steps are issued: host round trips "
                          "between their parts, one queued chain, or that chain replayed "
@@ -803,7 +830,7 @@ def main():
         log(f"uneven: {freed / 2**30:.1f} GiB of cached device memory released first")
         out["uneven"] = uneven_section(d4, workloads, args.uneven, args.uneven_budget,
                                        args.uneven_budget2, args.uneven_plain,
-                                       tune=not args.no_tune)
+                                       tune=not args.no_tune, kernel=args.mbatch_kernel)
 
 
     if not args.no_cpu:
