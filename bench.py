@@ -426,6 +426,11 @@ def step_modes_section(d4, workloads, boards, expected, serve_slots, n_req=300, 
     shapes = workloads.pentominoes()
     s6, c6, r6, _ = workloads.plain_6x10()
     reqs = workloads.request_stream(n_req, rate=rate, seed=5)
+    # (the serving sweep before this section leaves the allocator's cache
+    # full of its servers' blocks: released here, so that the untimed first
+    # solve of each mode allocates what the timed ones reuse, instead of a
+    # timed one releasing the cache and waiting for the freed memory)
+    d4.release_cached()
     out = {}
     for name, mode in (("general", d4.STEP_GENERAL), ("fused", d4.STEP_FUSED),
                        ("graph", d4.STEP_GRAPH)):
