@@ -301,8 +301,9 @@ def test_kernels_agree_on_work(gpu, uneven, uneven_fixture):
     """The level and depth-first kernels walk the same search: on 512 uneven
     puzzles with a state budget, the same instances finish with the same
     counts (the oracle's where it has them), and the rows extended, children
-    kept, children pruned and unique states match exactly.  The expand
-    kernel is refused."""
+    kept, children pruned and unique states match exactly, although the
+    level kernels rerun the chunks whose buffers overflow (the first row's
+    2 Mi children exceed the initial room).  The expand kernel is refused."""
     n, budget = 512, 1 << 20
     insts = uneven[:n]
     cache, libs = {}, []
@@ -323,7 +324,7 @@ def test_kernels_agree_on_work(gpu, uneven, uneven_fixture):
     assert sl["splits"] == 0 and sd["splits"] == 0
     for f in ("records_extended", "children", "pruned", "unique", "rows_done"):
         assert sl[f] == sd[f], f
-    assert sl["donated"] == 0
+    assert sl["donated"] == 0 and sl["chunk_reruns"] > 0 and sd["chunk_reruns"] == 0
     pinned = {int(k): v for k, v in uneven_fixture["counts"].items() if int(k) < n}
     checked = [i for i in pinned if fl[i]]
     assert len(checked) >= 150 and all(cl[i] == pinned[i] for i in checked)
